@@ -1,0 +1,78 @@
+// bnb_args.hpp -- layouts shared by libgpugo.so (hipcc) and the hipRTC
+// objective modules.  Plain structs, no pointers to host memory.
+#pragma once
+#ifndef GPUGO_BNB_ARGS_HPP
+#define GPUGO_BNB_ARGS_HPP
+
+namespace gpugo {
+
+// Per-problem state kept in device memory for the whole solve.  One
+// branch-and-bound step is: select (k_select), split+bound (objective
+// module), compact (k_compact).  The reference keeps the same state as host
+// NumPy arrays and scalars (1-Ackley/ackley50.py:971-1006).
+struct DevState {
+  unsigned long long ub_key;     // incumbent upper bound, order-preserving key (atomicMin)
+  double ub_snap;                // upper bound at the start of the step (pruning)
+  double lb_global;              // smallest live lower bound at the last select
+  double out_min_lb;             // smallest lower bound among output boxes
+  double width;                  // ub - lb rounded up, set when the step stops
+  long long q_count;             // queue entries incl. this step's appends
+  long long compact_count;       // live entries after this step's compaction
+  long long parents_used;        // parent records written
+  long long first_parent;        // first parent record split in this step
+  long long iterations;          // steps that split boxes
+  long long bounded;             // children bounded
+  long long pruned_value;
+  long long pruned_kkt;
+  long long n_out;               // output boxes (x-width rule)
+  long long queue_peak;
+  long long compactions;         // compaction passes run (selects the live buffer)
+  int n_split;                   // boxes split in this step
+  int status;                    // gpugo_status
+  int overflow;                  // 1: queue append, 2: parent arena, 4: output arena
+  int compact_final;             // 1: the stop step's compaction still has to run
+};
+
+// Arguments of the module kernel gpugo_bound_{exact,fast}.
+struct BoundArgs {
+  const double *plo, *phi;       // parent arena, [max_parents][d]
+  const double *pw;              // (hi - lo) / wid per parent coordinate
+  const int *pwin;               // first split coordinate of each parent
+  const double *params;          // objective coefficients
+  const double *dom_lo, *dom_hi; // initial box (monotonicity boundary rule)
+  double *q_lb;                  // queue, structure of arrays
+  unsigned int *q_par, *q_pos;
+  DevState *st;
+  long long q_cap;
+  int d, wid, c, n_samples;
+  unsigned int W;                // wid^c children per parent
+  int pad0;
+  double inv_wid, half_inv_wid;  // exact digit extraction: q = floor(r*inv + half)
+};
+
+// Arguments of the module kernel gpugo_eval_{exact,fast} (parity hook).
+struct EvalArgs {
+  const double *params;
+  const double *lo, *hi;         // [n][d]
+  const double *dom_lo, *dom_hi;
+  double *flo, *fhi, *glo, *ghi, *ub;
+  int *kkt;
+  long long n;
+  int d, n_samples;
+};
+
+__host__ __device__ inline unsigned long long order_key(double x) {
+  unsigned long long b;
+  __builtin_memcpy(&b, &x, 8);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
+}
+__host__ __device__ inline double key_value(unsigned long long k) {
+  unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffULL) : ~k;
+  double x;
+  __builtin_memcpy(&x, &b, 8);
+  return x;
+}
+
+}  // namespace gpugo
+
+#endif  // GPUGO_BNB_ARGS_HPP

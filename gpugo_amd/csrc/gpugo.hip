@@ -1,0 +1,1364 @@
+// gpugo.hip -- libgpugo.so: C ABI (include/gpugo.h), objective-independent
+// kernels (interval op batch, select, compact, box materialisation) and the
+// host side of the branch-and-bound loop.  Objective-specific kernels live in
+// hipRTC modules built from text by gpugo_amd.emit.
+//
+// Reference correspondence (paths relative to GPUGO_1.1/):
+//   k_ia_eval        IA device functions ("Updated Interval Arithmetic
+//                    Operations/IA.py":57-570)
+//   k_select         list prune + argmin pop + completeRegion + cycle update
+//                    (1-Ackley/ackley50.py:1154-1220, :1270-1370, :820-948),
+//                    generalised to the K smallest boxes per step
+//   module kernel    sampling + ruleOutGPU (1-Ackley/ackley50.py:602-766)
+//   k_compact        constructFlagArr + mask compaction (:768-810, :1217-1242)
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gpugo.h"
+#include "ia.hpp"
+#include "bnb_args.hpp"
+
+using namespace gpugo;
+
+#define GPUGO_VERSION 100  // 0.1.0
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int set_err(int code, const char *fmt, ...) {
+  char buf[2048];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHECK(x)                                                              \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess)                                                        \
+      return set_err(e_ == hipErrorOutOfMemory ? GPUGO_E_NOMEM : GPUGO_E_HIP,    \
+                     "%s: %s", #x, hipGetErrorString(e_));                       \
+  } while (0)
+
+extern "C" const char *gpugo_last_error(void) { return g_err.c_str(); }
+extern "C" int gpugo_version(void) { return GPUGO_VERSION; }
+
+extern "C" int gpugo_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" int gpugo_init(int device) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return set_err(GPUGO_E_NODEV, "no HIP device available");
+  if (device < 0 || device >= n) return set_err(GPUGO_E_INVALID, "device %d out of range [0,%d)", device, n);
+  HIPCHECK(hipSetDevice(device));
+  HIPCHECK(hipFree(nullptr));
+  return GPUGO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched interval ops
+// ---------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ T to_dn(double x);
+template <typename T> __device__ __forceinline__ T to_up(double x);
+template <> __device__ __forceinline__ double to_dn<double>(double x) { return x; }
+template <> __device__ __forceinline__ double to_up<double>(double x) { return x; }
+template <> __device__ __forceinline__ float to_dn<float>(double x) {
+  float f = (float)x;
+  return ((double)f > x) ? pred<float>(f) : f;
+}
+template <> __device__ __forceinline__ float to_up<float>(double x) {
+  float f = (float)x;
+  return ((double)f < x) ? succ<float>(f) : f;
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_ia_eval(int op, long long n, const double *__restrict__ alo,
+                                                 const double *__restrict__ ahi, const double *__restrict__ blo,
+                                                 const double *__restrict__ bhi, double s,
+                                                 double *__restrict__ olo, double *__restrict__ ohi) {
+  typedef IA<T, R> A;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  iv<T> a = mk<T>(to_dn<T>(alo[i]), to_up<T>(ahi[i]));
+  iv<T> b = blo ? mk<T>(to_dn<T>(blo[i]), to_up<T>(bhi[i])) : mk<T>(T(0), T(0));
+  // scalar operand as an interval (exact for FP64; outward for FP32)
+  iv<T> S = mk<T>(to_dn<T>(s), to_up<T>(s));
+  bool s_exact = S.lo == S.hi;
+  iv<T> r;
+  switch (op) {
+    case GPUGO_OP_ADD: r = A::add(a, b); break;
+    case GPUGO_OP_SUB: r = A::sub(a, b); break;
+    case GPUGO_OP_MUL: r = A::mul(a, b); break;
+    case GPUGO_OP_DIV: r = A::div(a, b); break;
+    case GPUGO_OP_NEG: r = A::neg(a); break;
+    case GPUGO_OP_SCALE: r = s_exact ? A::scale(a, S.lo) : A::mul(a, S); break;
+    case GPUGO_OP_ADDC: r = s_exact ? A::addc(a, S.lo) : A::add(a, S); break;
+    case GPUGO_OP_RSUBC: r = s_exact ? A::rsubc(S.lo, a) : A::sub(S, a); break;
+    case GPUGO_OP_DIVC: r = s_exact ? A::divc(a, S.lo) : A::div(a, S); break;
+    case GPUGO_OP_SQR: r = A::sqr(a); break;
+    case GPUGO_OP_POWN: r = A::pown(a, (int)s); break;
+    case GPUGO_OP_SQRT: r = A::sqrt(a); break;
+    case GPUGO_OP_EXP: r = A::exp(a); break;
+    case GPUGO_OP_SIN: r = A::sin(a); break;
+    case GPUGO_OP_COS: r = A::cos(a); break;
+    default: r = mk<T>(T(0), T(0)); break;
+  }
+  olo[i] = (double)r.lo;
+  ohi[i] = (double)r.hi;
+}
+
+static bool is_unary(int op) {
+  return op == GPUGO_OP_NEG || op == GPUGO_OP_SQR || op == GPUGO_OP_SQRT || op == GPUGO_OP_EXP ||
+         op == GPUGO_OP_SIN || op == GPUGO_OP_COS;
+}
+static bool is_binary(int op) {
+  return op == GPUGO_OP_ADD || op == GPUGO_OP_SUB || op == GPUGO_OP_MUL || op == GPUGO_OP_DIV;
+}
+
+static int check_intervals(const double *lo, const double *hi, int64_t n, const char *name) {
+  for (int64_t i = 0; i < n; ++i)
+    if (!(lo[i] <= hi[i]))
+      return set_err(GPUGO_E_DOMAIN, "%s[%lld] = [%g, %g] is NaN or inverted", name, (long long)i, lo[i], hi[i]);
+  return GPUGO_OK;
+}
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 8); }
+  template <typename T> T *as() const { return (T *)p; }
+};
+
+extern "C" int gpugo_ia_eval(int op, int prec, int rounding, int64_t n, const double *alo, const double *ahi,
+                             const double *blo, const double *bhi, double s, double *olo, double *ohi) {
+  if (op < 0 || op >= GPUGO_OP_COUNT) return set_err(GPUGO_E_INVALID, "unknown op %d", op);
+  if (prec != GPUGO_PREC_F64 && prec != GPUGO_PREC_F32) return set_err(GPUGO_E_INVALID, "unknown precision %d", prec);
+  if (rounding != GPUGO_ROUND_EXACT && rounding != GPUGO_ROUND_FAST)
+    return set_err(GPUGO_E_INVALID, "unknown rounding %d", rounding);
+  if (n < 0) return set_err(GPUGO_E_INVALID, "negative n");
+  if (n == 0) return GPUGO_OK;
+  if (!alo || !ahi || !olo || !ohi) return set_err(GPUGO_E_INVALID, "null pointer");
+  if (is_binary(op) && (!blo || !bhi)) return set_err(GPUGO_E_INVALID, "binary op needs b");
+  if (std::isnan(s)) return set_err(GPUGO_E_DOMAIN, "scalar is NaN");
+  if ((op == GPUGO_OP_DIVC) && s == 0.0) return set_err(GPUGO_E_DOMAIN, "division by scalar zero");
+  if (op == GPUGO_OP_POWN && (s < 0 || s > 1024 || s != std::floor(s)))
+    return set_err(GPUGO_E_DOMAIN, "pown exponent must be an integer in [0,1024]");
+  int rc = check_intervals(alo, ahi, n, "a");
+  if (rc) return rc;
+  if (is_binary(op)) {
+    rc = check_intervals(blo, bhi, n, "b");
+    if (rc) return rc;
+  }
+  size_t bytes = sizeof(double) * (size_t)n;
+  DevBuf d_alo, d_ahi, d_blo, d_bhi, d_olo, d_ohi;
+  HIPCHECK(d_alo.alloc(bytes));
+  HIPCHECK(d_ahi.alloc(bytes));
+  HIPCHECK(d_olo.alloc(bytes));
+  HIPCHECK(d_ohi.alloc(bytes));
+  HIPCHECK(hipMemcpy(d_alo.p, alo, bytes, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_ahi.p, ahi, bytes, hipMemcpyHostToDevice));
+  const double *pb_lo = nullptr, *pb_hi = nullptr;
+  if (is_binary(op)) {
+    HIPCHECK(d_blo.alloc(bytes));
+    HIPCHECK(d_bhi.alloc(bytes));
+    HIPCHECK(hipMemcpy(d_blo.p, blo, bytes, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(d_bhi.p, bhi, bytes, hipMemcpyHostToDevice));
+    pb_lo = d_blo.as<double>();
+    pb_hi = d_bhi.as<double>();
+  }
+  dim3 grid((unsigned)((n + 255) / 256)), block(256);
+  const double *a0 = d_alo.as<double>(), *a1 = d_ahi.as<double>();
+  double *o0 = d_olo.as<double>(), *o1 = d_ohi.as<double>();
+  if (prec == GPUGO_PREC_F64 && rounding == GPUGO_ROUND_EXACT)
+    hipLaunchKernelGGL((k_ia_eval<double, ROUND_EXACT>), grid, block, 0, 0, op, (long long)n, a0, a1, pb_lo, pb_hi, s, o0, o1);
+  else if (prec == GPUGO_PREC_F64)
+    hipLaunchKernelGGL((k_ia_eval<double, ROUND_FAST>), grid, block, 0, 0, op, (long long)n, a0, a1, pb_lo, pb_hi, s, o0, o1);
+  else if (rounding == GPUGO_ROUND_EXACT)
+    hipLaunchKernelGGL((k_ia_eval<float, ROUND_EXACT>), grid, block, 0, 0, op, (long long)n, a0, a1, pb_lo, pb_hi, s, o0, o1);
+  else
+    hipLaunchKernelGGL((k_ia_eval<float, ROUND_FAST>), grid, block, 0, 0, op, (long long)n, a0, a1, pb_lo, pb_hi, s, o0, o1);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpy(olo, d_olo.p, bytes, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(ohi, d_ohi.p, bytes, hipMemcpyDeviceToHost));
+  return GPUGO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// modules
+// ---------------------------------------------------------------------------
+struct gpugo_module_s {
+  hipModule_t mod = nullptr;
+  hipFunction_t bound[2] = {nullptr, nullptr};  // [rounding]
+  hipFunction_t eval[2] = {nullptr, nullptr};
+  int dim = 0, nparams = 0, abi = 0;
+  std::vector<char> code;
+};
+
+#define GPUGO_MODULE_ABI 1
+
+static int module_finish(gpugo_module_t m) {
+  HIPCHECK(hipModuleLoadData(&m->mod, m->code.data()));
+  hipFunction_t info;
+  HIPCHECK(hipModuleGetFunction(&info, m->mod, "gpugo_info"));
+  DevBuf d;
+  HIPCHECK(d.alloc(4 * sizeof(int)));
+  void *args[] = {&d.p};
+  HIPCHECK(hipModuleLaunchKernel(info, 1, 1, 1, 1, 1, 1, 0, nullptr, args, nullptr));
+  int h[4] = {0, 0, 0, 0};
+  HIPCHECK(hipMemcpy(h, d.p, sizeof h, hipMemcpyDeviceToHost));
+  m->dim = h[0];
+  m->nparams = h[1];
+  m->abi = h[2];
+  if (m->abi != GPUGO_MODULE_ABI)
+    return set_err(GPUGO_E_INVALID, "module ABI %d, library expects %d", m->abi, GPUGO_MODULE_ABI);
+  if (h[3] & 1) {
+    HIPCHECK(hipModuleGetFunction(&m->bound[0], m->mod, "gpugo_bound_exact"));
+    HIPCHECK(hipModuleGetFunction(&m->eval[0], m->mod, "gpugo_eval_exact"));
+  }
+  if (h[3] & 2) {
+    HIPCHECK(hipModuleGetFunction(&m->bound[1], m->mod, "gpugo_bound_fast"));
+    HIPCHECK(hipModuleGetFunction(&m->eval[1], m->mod, "gpugo_eval_fast"));
+  }
+  return GPUGO_OK;
+}
+
+static int rtc_compile(const char *source, const char *name, const char *extra_options, std::vector<char> &code) {
+  hiprtcProgram prog;
+  hiprtcResult r = hiprtcCreateProgram(&prog, source, name ? name : "gpugo_objective.hip", 0, nullptr, nullptr);
+  if (r != HIPRTC_SUCCESS) return set_err(GPUGO_E_COMPILE, "hiprtcCreateProgram: %s", hiprtcGetErrorString(r));
+  std::vector<std::string> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+                                   "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  if (extra_options && *extra_options) {
+    std::string e(extra_options), tok;
+    for (size_t i = 0; i <= e.size(); ++i) {
+      if (i == e.size() || e[i] == ' ') {
+        if (!tok.empty()) opts.push_back(tok);
+        tok.clear();
+      } else
+        tok.push_back(e[i]);
+    }
+  }
+  std::vector<const char *> copts;
+  for (auto &o : opts) copts.push_back(o.c_str());
+  r = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
+  if (r != HIPRTC_SUCCESS) {
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    std::string log(ls + 1, '\0');
+    hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    return set_err(GPUGO_E_COMPILE, "hipRTC compile failed: %s", log.c_str());
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  code.resize(cs);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  return GPUGO_OK;
+}
+
+static thread_local std::vector<char> g_rtc_code;
+
+extern "C" int gpugo_rtc_compile(const char *source, const char *name, const char *extra_options,
+                                 size_t *code_size) {
+  if (!source || !code_size) return set_err(GPUGO_E_INVALID, "null pointer");
+  g_rtc_code.clear();
+  int rc = rtc_compile(source, name, extra_options, g_rtc_code);
+  *code_size = g_rtc_code.size();
+  return rc;
+}
+
+extern "C" int gpugo_rtc_code_copy(void *buf, size_t size) {
+  if (!buf) return set_err(GPUGO_E_INVALID, "null pointer");
+  if (size < g_rtc_code.size()) return set_err(GPUGO_E_INVALID, "buffer too small");
+  memcpy(buf, g_rtc_code.data(), g_rtc_code.size());
+  return GPUGO_OK;
+}
+
+extern "C" int gpugo_module_compile(const char *source, const char *name, const char *extra_options,
+                                    gpugo_module_t *out) {
+  if (!source || !out) return set_err(GPUGO_E_INVALID, "null pointer");
+  *out = nullptr;
+  auto *m = new gpugo_module_s;
+  int rc = rtc_compile(source, name, extra_options, m->code);
+  if (rc) {
+    delete m;
+    return rc;
+  }
+  rc = module_finish(m);
+  if (rc) {
+    if (m->mod) (void)hipModuleUnload(m->mod);
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return GPUGO_OK;
+}
+
+extern "C" int gpugo_module_load(const void *code_object, size_t size, gpugo_module_t *out) {
+  if (!code_object || !out || size == 0) return set_err(GPUGO_E_INVALID, "null pointer or empty code");
+  *out = nullptr;
+  auto *m = new gpugo_module_s;
+  m->code.assign((const char *)code_object, (const char *)code_object + size);
+  int rc = module_finish(m);
+  if (rc) {
+    if (m->mod) (void)hipModuleUnload(m->mod);
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return GPUGO_OK;
+}
+
+extern "C" int gpugo_module_code(gpugo_module_t m, void *buf, size_t *size) {
+  if (!m || !size) return set_err(GPUGO_E_INVALID, "null pointer");
+  if (buf) {
+    if (*size < m->code.size()) return set_err(GPUGO_E_INVALID, "buffer too small");
+    memcpy(buf, m->code.data(), m->code.size());
+  }
+  *size = m->code.size();
+  return GPUGO_OK;
+}
+
+extern "C" void gpugo_module_destroy(gpugo_module_t m) {
+  if (!m) return;
+  if (m->mod) (void)hipModuleUnload(m->mod);
+  delete m;
+}
+
+static int check_module_problem(gpugo_module_t m, int32_t d, const double *params, int64_t n_params) {
+  if (!m) return set_err(GPUGO_E_INVALID, "null module");
+  if (d != m->dim) return set_err(GPUGO_E_INVALID, "module is compiled for d=%d, got d=%d", m->dim, d);
+  if (n_params != m->nparams)
+    return set_err(GPUGO_E_INVALID, "module expects %d parameters, got %lld", m->nparams, (long long)n_params);
+  if (n_params > 0 && !params) return set_err(GPUGO_E_INVALID, "null params");
+  for (int64_t i = 0; i < n_params; ++i)
+    if (!std::isfinite(params[i])) return set_err(GPUGO_E_DOMAIN, "params[%lld] is not finite", (long long)i);
+  return GPUGO_OK;
+}
+
+extern "C" int gpugo_eval_boxes(gpugo_module_t m, int32_t d, const double *params, int64_t n_params, int64_t n,
+                                const double *lo, const double *hi, const double *dom_lo, const double *dom_hi,
+                                int32_t n_samples, int32_t rounding, double *flo, double *fhi, double *glo,
+                                double *ghi, int32_t *kkt, double *ub) {
+  int rc = check_module_problem(m, d, params, n_params);
+  if (rc) return rc;
+  if (rounding != GPUGO_ROUND_EXACT && rounding != GPUGO_ROUND_FAST)
+    return set_err(GPUGO_E_INVALID, "unknown rounding %d", rounding);
+  if (!m->eval[rounding]) return set_err(GPUGO_E_INVALID, "module has no %s variant", rounding ? "fast" : "exact");
+  if (n < 0) return set_err(GPUGO_E_INVALID, "negative n");
+  if (n == 0) return GPUGO_OK;
+  if (!lo || !hi) return set_err(GPUGO_E_INVALID, "null box pointer");
+  if ((kkt || glo || ghi) && (!dom_lo || !dom_hi) && kkt) return set_err(GPUGO_E_INVALID, "kkt needs dom_lo/dom_hi");
+  if (ub && n_samples < 1) return set_err(GPUGO_E_INVALID, "n_samples must be >= 1");
+  rc = check_intervals(lo, hi, n * d, "box");
+  if (rc) return rc;
+  const size_t nb = sizeof(double) * (size_t)n * d;
+  DevBuf d_lo, d_hi, d_par, d_dlo, d_dhi, d_flo, d_fhi, d_glo, d_ghi, d_ub, d_kkt;
+  HIPCHECK(d_lo.alloc(nb));
+  HIPCHECK(d_hi.alloc(nb));
+  HIPCHECK(hipMemcpy(d_lo.p, lo, nb, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_hi.p, hi, nb, hipMemcpyHostToDevice));
+  HIPCHECK(d_par.alloc(sizeof(double) * (size_t)(n_params > 0 ? n_params : 1)));
+  if (n_params > 0) HIPCHECK(hipMemcpy(d_par.p, params, sizeof(double) * n_params, hipMemcpyHostToDevice));
+  HIPCHECK(d_dlo.alloc(sizeof(double) * d));
+  HIPCHECK(d_dhi.alloc(sizeof(double) * d));
+  if (dom_lo && dom_hi) {
+    HIPCHECK(hipMemcpy(d_dlo.p, dom_lo, sizeof(double) * d, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(d_dhi.p, dom_hi, sizeof(double) * d, hipMemcpyHostToDevice));
+  }
+  EvalArgs a;
+  memset(&a, 0, sizeof a);
+  a.params = d_par.as<double>();
+  a.lo = d_lo.as<double>();
+  a.hi = d_hi.as<double>();
+  a.dom_lo = d_dlo.as<double>();
+  a.dom_hi = d_dhi.as<double>();
+  const size_t n1 = sizeof(double) * (size_t)n;
+  if (flo || fhi) {
+    HIPCHECK(d_flo.alloc(n1));
+    HIPCHECK(d_fhi.alloc(n1));
+    a.flo = d_flo.as<double>();
+    a.fhi = d_fhi.as<double>();
+  }
+  if (glo || ghi) {
+    HIPCHECK(d_glo.alloc(nb));
+    HIPCHECK(d_ghi.alloc(nb));
+    a.glo = d_glo.as<double>();
+    a.ghi = d_ghi.as<double>();
+  }
+  if (ub) {
+    HIPCHECK(d_ub.alloc(n1));
+    a.ub = d_ub.as<double>();
+  }
+  if (kkt) {
+    HIPCHECK(d_kkt.alloc(sizeof(int) * (size_t)n));
+    a.kkt = d_kkt.as<int>();
+  }
+  a.n = n;
+  a.d = d;
+  a.n_samples = n_samples;
+  void *args[] = {&a};
+  unsigned grid = (unsigned)((n + 63) / 64);
+  HIPCHECK(hipModuleLaunchKernel(m->eval[rounding], grid, 1, 1, 64, 1, 1, 0, nullptr, args, nullptr));
+  HIPCHECK(hipDeviceSynchronize());
+  if (flo) HIPCHECK(hipMemcpy(flo, d_flo.p, n1, hipMemcpyDeviceToHost));
+  if (fhi) HIPCHECK(hipMemcpy(fhi, d_fhi.p, n1, hipMemcpyDeviceToHost));
+  if (glo) HIPCHECK(hipMemcpy(glo, d_glo.p, nb, hipMemcpyDeviceToHost));
+  if (ghi) HIPCHECK(hipMemcpy(ghi, d_ghi.p, nb, hipMemcpyDeviceToHost));
+  if (ub) HIPCHECK(hipMemcpy(ub, d_ub.p, n1, hipMemcpyDeviceToHost));
+  if (kkt) HIPCHECK(hipMemcpy(kkt, d_kkt.p, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost));
+  return GPUGO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// block helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned wave_incl_scan(unsigned v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    unsigned t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// exclusive scan over the block; sh holds >= nwaves+1 entries
+__device__ unsigned block_excl_scan(unsigned v, unsigned *sh, unsigned *total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  unsigned inc = wave_incl_scan(v);
+  if (lane == 63) sh[w] = inc;
+  __syncthreads();
+  if (w == 0) {
+    unsigned x = lane < nw ? sh[lane] : 0u;
+    unsigned xi = wave_incl_scan(x);
+    if (lane < nw) sh[lane] = xi - x;
+    if (lane == nw - 1) sh[nw] = xi;
+  }
+  __syncthreads();
+  unsigned r = sh[w] + inc - v;
+  *total = sh[nw];
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// decode (round-to-nearest, one fma per bound, top child ends at U)
+// 1-Ackley/ackley50.py:565-593 (device) and :820-948 (host)
+// ---------------------------------------------------------------------------
+__device__ void decode_box(const double *plo, const double *phi, const double *pw, int win, int d, int wid, int c,
+                           unsigned pos, double *lo, double *hi) {
+  for (int j = 0; j < d; ++j) {
+    lo[j] = plo[j];
+    hi[j] = phi[j];
+  }
+  unsigned r = pos;
+  for (int k = 0; k < c; ++k) {
+    int j = win + k;
+    if (j >= d) j -= d;
+    unsigned q = r / (unsigned)wid, dg = r - q * (unsigned)wid;
+    r = q;
+    lo[j] = __builtin_fma(pw[j], (double)dg, plo[j]);
+    hi[j] = (dg + 1 == (unsigned)wid) ? phi[j] : __builtin_fma(pw[j], (double)(dg + 1), plo[j]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// select: the K smallest (lb, parent, pos) keys of the live queue, exact,
+// in one workgroup (radix select on 128-bit keys, 12-bit digits), then the
+// stop tests and materialisation of the selected boxes as new parents.
+// ---------------------------------------------------------------------------
+#define SEL_T 1024
+#define SEL_BINS 4096
+#define SEL_FINAL 1024
+#define SEL_KMAX 1024
+#define GPUGO_DMAX_SELECT 4096
+
+typedef unsigned __int128 u128;
+
+struct SelectArgs {
+  DevState *st;
+  double *q_lb;
+  unsigned *q_par, *q_pos;
+  double *plo, *phi, *pw;
+  int *pwin;
+  double *out_lo, *out_hi, *out_lb;
+  long long out_cap, p_cap;
+  int d, wid, c, k_select;
+  double tol_f, tol_x;
+};
+
+__device__ __forceinline__ u128 qkey(const double *lb, const unsigned *par, const unsigned *pos, long long i) {
+  return ((u128)order_key(lb[i]) << 64) | ((u128)par[i] << 32) | (u128)pos[i];
+}
+
+__device__ void bitonic_sort(u128 *k, unsigned *v, int n) {
+  for (int size = 2; size <= n; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      int t = threadIdx.x;
+      if (t < (n >> 1)) {
+        int i = (t / stride) * 2 * stride + (t % stride);
+        int j = i + stride;
+        bool up = (i & size) == 0;
+        u128 ki = k[i], kj = k[j];
+        if ((ki > kj) == up) {
+          k[i] = kj;
+          k[j] = ki;
+          unsigned tv = v[i];
+          v[i] = v[j];
+          v[j] = tv;
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ int pow2_ceil(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+__global__ __launch_bounds__(SEL_T) void k_select(SelectArgs a) {
+  __shared__ unsigned hist[SEL_BINS];
+  __shared__ u128 skey[SEL_KMAX];
+  __shared__ unsigned sidx[SEL_KMAX];
+  __shared__ u128 ckey[SEL_FINAL];
+  __shared__ unsigned cidx[SEL_FINAL];
+  __shared__ unsigned scan_sh[SEL_T / 64 + 1];
+  __shared__ u128 s_pfx;
+  __shared__ int s_pbits;
+  __shared__ unsigned s_krem, s_region, s_b, s_cb, s_ns, s_nc;
+  __shared__ unsigned long long s_outmin;
+
+  DevState *st = a.st;
+  const int tid = threadIdx.x;
+  if (st->status != GPUGO_ST_RUNNING) {
+    if (tid == 0) st->compact_final = 0;
+    return;
+  }
+  if (st->overflow) {
+    if (tid == 0) {
+      st->status = GPUGO_ST_OVERFLOW;
+      st->q_count = st->compact_count;
+      st->compact_count = 0;
+      st->compact_final = 1;
+    }
+    return;
+  }
+  const long long N = st->compact_count;
+  const double ub = key_value(st->ub_key);
+  if (N == 0) {
+    if (tid == 0) {
+      bool have_out = st->n_out > 0;
+      st->status = have_out ? GPUGO_ST_X_CONVERGED : GPUGO_ST_EXHAUSTED;
+      st->lb_global = have_out ? st->out_min_lb : -__builtin_huge_val();
+      st->width = rnd<double, ROUND_EXACT>::add_up(ub, -st->lb_global);
+      st->q_count = 0;
+      st->compact_count = 0;
+      st->compact_final = 1;
+    }
+    return;
+  }
+  const int K = (int)(N < (long long)a.k_select ? N : (long long)a.k_select);
+  if (N <= K) {
+    for (long long i = tid; i < N; i += SEL_T) {
+      skey[i] = qkey(a.q_lb, a.q_par, a.q_pos, i);
+      sidx[i] = (unsigned)i;
+    }
+    __syncthreads();
+  } else {
+    if (tid == 0) {
+      s_pfx = 0;
+      s_pbits = 0;
+      s_krem = (unsigned)K;
+    }
+    __syncthreads();
+    for (;;) {
+      const int pbits = s_pbits;
+      const int db = 128 - pbits < 12 ? 128 - pbits : 12;
+      const u128 pfx = s_pfx;
+      const unsigned krem = s_krem;
+      const int sh = 128 - pbits - db;
+      for (int b = tid; b < SEL_BINS; b += SEL_T) hist[b] = 0;
+      __syncthreads();
+      for (long long i = tid; i < N; i += SEL_T) {
+        u128 k = qkey(a.q_lb, a.q_par, a.q_pos, i);
+        bool m = (pbits == 0) || ((k >> (128 - pbits)) == (pfx >> (128 - pbits)));
+        if (m) atomicAdd(&hist[(unsigned)(k >> sh) & ((1u << db) - 1u)], 1u);
+      }
+      __syncthreads();
+      unsigned h[4], ts = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        h[q] = hist[4 * tid + q];
+        ts += h[q];
+      }
+      unsigned tot;
+      unsigned cum = block_excl_scan(ts, scan_sh, &tot);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (cum < krem && krem <= cum + h[q]) {
+          s_b = 4 * tid + q;
+          s_cb = cum;
+          s_region = h[q];
+        }
+        cum += h[q];
+      }
+      __syncthreads();
+      if (tid == 0) {
+        s_krem = krem - s_cb;
+        s_pfx = pfx | ((u128)s_b << sh);
+        s_pbits = pbits + db;
+      }
+      __syncthreads();
+      if (s_region == s_krem || s_region <= SEL_FINAL || s_pbits >= 128) break;
+    }
+    // gather: keys strictly below the chosen prefix are selected, keys with
+    // the prefix are candidates for the remaining s_krem slots
+    if (tid == 0) {
+      s_ns = 0;
+      s_nc = 0;
+    }
+    __syncthreads();
+    {
+      const int shp = 128 - s_pbits;
+      const u128 ptop = s_pfx >> shp;
+      for (long long i = tid; i < N; i += SEL_T) {
+        u128 k = qkey(a.q_lb, a.q_par, a.q_pos, i);
+        u128 top = k >> shp;
+        if (top < ptop) {
+          unsigned slot = atomicAdd(&s_ns, 1u);
+          if (slot < SEL_KMAX) {
+            skey[slot] = k;
+            sidx[slot] = (unsigned)i;
+          }
+        } else if (top == ptop) {
+          unsigned slot = atomicAdd(&s_nc, 1u);
+          if (slot < SEL_FINAL) {
+            ckey[slot] = k;
+            cidx[slot] = (unsigned)i;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    const unsigned ns = s_ns, nc = s_nc, krem = s_krem;
+    if (nc > krem) {
+      const int n2 = pow2_ceil((int)nc);
+      for (int i = tid; i < n2; i += SEL_T)
+        if (i >= (int)nc) {
+          ckey[i] = ~(u128)0;
+          cidx[i] = 0xffffffffu;
+        }
+      bitonic_sort(ckey, cidx, n2);
+    }
+    for (unsigned i = tid; i < krem; i += SEL_T) {
+      skey[ns + i] = ckey[i];
+      sidx[ns + i] = cidx[i];
+    }
+    __syncthreads();
+  }
+  // deterministic order of the selection
+  {
+    const int n2 = pow2_ceil(K);
+    for (int i = tid; i < n2; i += SEL_T)
+      if (i >= K) {
+        skey[i] = ~(u128)0;
+        sidx[i] = 0xffffffffu;
+      }
+    bitonic_sort(skey, sidx, n2);
+  }
+
+  // stop test: certified f-enclosure
+  const double lb0 = a.q_lb[sidx[0]];
+  const double lbg = lb0 < st->out_min_lb ? lb0 : st->out_min_lb;
+  if (a.tol_f > 0) {
+    double w = rnd<double, ROUND_EXACT>::add_up(ub, -lbg);
+    if (w <= a.tol_f) {
+      __syncthreads();
+      if (tid == 0) {
+        st->lb_global = lbg;
+        st->width = w;
+        st->status = GPUGO_ST_CERTIFIED;
+        st->q_count = N;
+        st->compact_count = 0;
+        st->compact_final = 1;
+      }
+      return;
+    }
+  }
+
+  // materialise: x-width output rule, then new parent records
+  int is_out = 0;
+  unsigned idx = 0, par = 0, pos = 0;
+  double elb = 0;
+  const int d = a.d;
+  if (tid < K) {
+    idx = sidx[tid];
+    par = a.q_par[idx];
+    pos = a.q_pos[idx];
+    elb = a.q_lb[idx];
+    if (a.tol_x > 0) {
+      const double *plo = a.plo + (long long)par * d, *phi = a.phi + (long long)par * d;
+      const double *pw = a.pw + (long long)par * d;
+      int win = a.pwin[par];
+      unsigned r = pos;
+      // window coordinates shrink; recompute the max over the child
+      double mwin = 0, mrest = 0;
+      for (int j = 0; j < d; ++j) {
+        int k = j - win;
+        if (k < 0) k += d;
+        if (k >= a.c) {
+          double w = phi[j] - plo[j];
+          if (w > mrest) mrest = w;
+        }
+      }
+      for (int k = 0; k < a.c; ++k) {
+        int j = win + k;
+        if (j >= d) j -= d;
+        unsigned q = r / (unsigned)a.wid, dg = r - q * (unsigned)a.wid;
+        r = q;
+        double lo = __builtin_fma(pw[j], (double)dg, plo[j]);
+        double hi = (dg + 1 == (unsigned)a.wid) ? phi[j] : __builtin_fma(pw[j], (double)(dg + 1), plo[j]);
+        if (hi - lo > mwin) mwin = hi - lo;
+      }
+      double m = mwin > mrest ? mwin : mrest;
+      is_out = m < a.tol_x;
+    }
+  }
+  unsigned n_split, n_outs;
+  const unsigned split_rank = block_excl_scan((tid < K && !is_out) ? 1u : 0u, scan_sh, &n_split);
+  const unsigned out_rank = block_excl_scan((tid < K && is_out) ? 1u : 0u, scan_sh, &n_outs);
+  const long long pbase = st->parents_used;
+  const long long obase = st->n_out;
+  if (pbase + (long long)n_split > a.p_cap) {
+    if (tid == 0) {
+      st->overflow |= 2;
+      st->status = GPUGO_ST_OVERFLOW;
+      st->q_count = N;
+      st->compact_count = 0;
+      st->compact_final = 1;
+    }
+    return;
+  }
+  if (tid == 0) s_outmin = order_key(__builtin_huge_val());
+  __syncthreads();
+  if (tid < K) {
+    const double *plo = a.plo + (long long)par * d, *phi = a.phi + (long long)par * d;
+    const double *pw = a.pw + (long long)par * d;
+    const int win = a.pwin[par];
+    if (!is_out) {
+      const long long np = pbase + split_rank;
+      double *nlo = a.plo + np * d, *nhi = a.phi + np * d, *npw = a.pw + np * d;
+      decode_box(plo, phi, pw, win, d, a.wid, a.c, pos, nlo, nhi);
+      for (int j = 0; j < d; ++j) npw[j] = (nhi[j] - nlo[j]) / (double)a.wid;
+      int nw = win + a.c;
+      while (nw >= d) nw -= d;
+      a.pwin[np] = nw;
+    } else {
+      const long long o = obase + out_rank;
+      if (o < a.out_cap) {
+        decode_box(plo, phi, pw, win, d, a.wid, a.c, pos, a.out_lo + o * d, a.out_hi + o * d);
+        a.out_lb[o] = elb;
+      } else {
+        atomicOr(&st->overflow, 4);
+      }
+      atomicMin(&s_outmin, order_key(elb));
+    }
+    a.q_lb[idx] = __builtin_nan("");
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double om = key_value(s_outmin);
+    if (om < st->out_min_lb) st->out_min_lb = om;
+    st->overflow &= ~4;  // output-arena overflow keeps the bound (out_min_lb) rigorous
+    st->lb_global = lbg;
+    st->first_parent = pbase;
+    st->parents_used = pbase + n_split;
+    st->n_split = (int)n_split;
+    st->n_out = obase + n_outs;
+    st->ub_snap = ub;
+    st->q_count = N;
+    st->compact_count = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// compact: keep entries with lb <= ub (consumed entries are NaN)
+// ---------------------------------------------------------------------------
+struct CompactArgs {
+  DevState *st;
+  const double *lb_in;
+  const unsigned *par_in, *pos_in;
+  double *lb_out;
+  unsigned *par_out, *pos_out;
+  long long cap;
+};
+
+__global__ __launch_bounds__(256) void k_compact(CompactArgs a) {
+  __shared__ unsigned wcnt[4];
+  __shared__ unsigned long long s_base;
+  DevState *st = a.st;
+  // after a stop, exactly one more pass runs so the live set lands in the
+  // buffer that st->compactions designates
+  if (st->status != GPUGO_ST_RUNNING && !st->compact_final) return;
+  long long n = st->q_count;
+  if (n > a.cap) n = a.cap;
+  const double ub = key_value(st->ub_key);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (long long base = (long long)blockIdx.x * 256; base < n; base += (long long)gridDim.x * 256) {
+    long long i = base + threadIdx.x;
+    double lb = 0;
+    bool keep = false;
+    if (i < n) {
+      lb = a.lb_in[i];
+      keep = lb <= ub;
+    }
+    unsigned long long mask = __ballot(keep);
+    unsigned pre = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+    if (lane == 0) wcnt[w] = __popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned t = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+      s_base = t ? atomicAdd((unsigned long long *)&st->compact_count, (unsigned long long)t) : 0ull;
+    }
+    __syncthreads();
+    unsigned off = pre;
+    for (int q = 0; q < w; ++q) off += wcnt[q];
+    if (keep) {
+      long long o = (long long)s_base + off;
+      a.lb_out[o] = lb;
+      a.par_out[o] = a.par_in[i];
+      a.pos_out[o] = a.pos_in[i];
+    }
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    atomicMax((unsigned long long *)&st->queue_peak, (unsigned long long)n);
+    st->compactions += 1;
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// smallest live lower bound (used when a run stops without a certificate)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_minlb(const DevState *st, const double *lb, long long n,
+                                                unsigned long long *out) {
+  __shared__ unsigned long long s_min;
+  if (threadIdx.x == 0) s_min = order_key(__builtin_huge_val());
+  __syncthreads();
+  unsigned long long m = order_key(__builtin_huge_val());
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+    double v = lb[i];
+    if (v == v) {
+      unsigned long long k = order_key(v);
+      if (k < m) m = k;
+    }
+  }
+  atomicMin(&s_min, m);
+  __syncthreads();
+  if (threadIdx.x == 0) *out = s_min;
+}
+
+// materialise queue entries as boxes (gpugo_boxes_copy)
+__global__ __launch_bounds__(256) void k_materialize(const double *plo, const double *phi, const double *pw,
+                                                     const int *pwin, const unsigned *q_par, const unsigned *q_pos,
+                                                     const long long *order, long long n, int d, int wid, int c,
+                                                     double *lo, double *hi) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  long long e = order[i];
+  unsigned par = q_par[e], pos = q_pos[e];
+  decode_box(plo + (long long)par * d, phi + (long long)par * d, pw + (long long)par * d, pwin[par], d, wid, c, pos,
+             lo + i * d, hi + i * d);
+}
+
+// ---------------------------------------------------------------------------
+// problems
+// ---------------------------------------------------------------------------
+struct Queue {
+  DevBuf lb, par, pos;
+  double *plb() const { return lb.as<double>(); }
+  unsigned *ppar() const { return par.as<unsigned>(); }
+  unsigned *ppos() const { return pos.as<unsigned>(); }
+};
+
+struct gpugo_problem_s {
+  gpugo_module_t m = nullptr;
+  gpugo_problem_desc desc;
+  std::vector<double> lo, hi, params;
+  int c = 0;
+  unsigned W = 0;
+  long long q_cap = 0, p_cap = 0, out_cap = 0;
+  DevBuf d_lo, d_hi, d_params, plo, phi, pw, pwin, out_lo, out_hi, out_lb, st, scratch;
+  Queue q[2];
+  hipStream_t stream = nullptr;
+  bool started = false;
+  long long steps_enqueued = 0;
+  DevState hs;  // last state read back
+  // state polling: pinned copies of DevState and an event per slot
+  DevState *hpoll = nullptr;
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};
+  // optional timing of the split+bound launches
+  bool timing = false;
+  std::vector<hipEvent_t> tev;   // pairs (start, stop) recorded since the last read
+  size_t tev_used = 0;
+  double bound_ms = 0;
+  long long bound_launches = 0;
+};
+
+static int timing_collect(gpugo_problem_t p) {
+  for (size_t i = 0; i + 1 < p->tev_used; i += 2) {
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, p->tev[i], p->tev[i + 1]));
+    p->bound_ms += ms;
+  }
+  p->tev_used = 0;
+  return GPUGO_OK;
+}
+
+static int ipow_checked(int wid, int c, unsigned *out) {
+  unsigned long long w = 1;
+  for (int k = 0; k < c; ++k) {
+    w *= (unsigned long long)wid;
+    if (w > 0xffffffffULL) return 0;
+  }
+  *out = (unsigned)w;
+  return 1;
+}
+
+extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *desc, gpugo_problem_t *out) {
+  if (!desc || !out) return set_err(GPUGO_E_INVALID, "null pointer");
+  *out = nullptr;
+  const int d = desc->d;
+  if (d < 1) return set_err(GPUGO_E_INVALID, "d must be >= 1");
+  if (d > GPUGO_DMAX_SELECT) return set_err(GPUGO_E_INVALID, "d must be <= %d", GPUGO_DMAX_SELECT);
+  int rc = check_module_problem(m, d, desc->params, desc->n_params);
+  if (rc) return rc;
+  if (!desc->lo || !desc->hi) return set_err(GPUGO_E_INVALID, "null box");
+  rc = check_intervals(desc->lo, desc->hi, d, "box");
+  if (rc) return rc;
+  for (int j = 0; j < d; ++j)
+    if (!std::isfinite(desc->lo[j]) || !std::isfinite(desc->hi[j]))
+      return set_err(GPUGO_E_DOMAIN, "box coordinate %d is not finite", j);
+  if (desc->wid < 2 || desc->wid > 64) return set_err(GPUGO_E_INVALID, "wid must be in [2,64]");
+  if (desc->cycle < 1) return set_err(GPUGO_E_INVALID, "cycle must be >= 1");
+  if (desc->k_select < 1 || desc->k_select > SEL_KMAX)
+    return set_err(GPUGO_E_INVALID, "k_select must be in [1,%d]", SEL_KMAX);
+  if (desc->n_samples < 1 || desc->n_samples > 64) return set_err(GPUGO_E_INVALID, "n_samples must be in [1,64]");
+  if (desc->precision != GPUGO_PREC_F64) return set_err(GPUGO_E_INVALID, "only GPUGO_PREC_F64 problems are supported");
+  if (desc->rounding != GPUGO_ROUND_EXACT && desc->rounding != GPUGO_ROUND_FAST)
+    return set_err(GPUGO_E_INVALID, "unknown rounding %d", desc->rounding);
+  if (!m->bound[desc->rounding])
+    return set_err(GPUGO_E_INVALID, "module has no %s variant", desc->rounding ? "fast" : "exact");
+  if (std::isnan(desc->tol_f) || std::isnan(desc->tol_x) || std::isnan(desc->init_ub))
+    return set_err(GPUGO_E_DOMAIN, "NaN tolerance or init_ub");
+  if (!(desc->tol_f > 0) && !(desc->tol_x > 0)) return set_err(GPUGO_E_INVALID, "set tol_f or tol_x");
+  auto *p = new gpugo_problem_s;
+  p->m = m;
+  p->desc = *desc;
+  p->lo.assign(desc->lo, desc->lo + d);
+  p->hi.assign(desc->hi, desc->hi + d);
+  p->params.assign(desc->params, desc->params + desc->n_params);
+  p->desc.lo = p->lo.data();
+  p->desc.hi = p->hi.data();
+  p->desc.params = p->params.data();
+  p->c = desc->cycle < d ? desc->cycle : d;
+  if (!ipow_checked(desc->wid, p->c, &p->W)) {
+    delete p;
+    return set_err(GPUGO_E_INVALID, "wid^cycle must be < 2^32");
+  }
+  long long qc = desc->max_queue > 0 ? desc->max_queue : (1LL << 24);
+  if (qc > 0xfffffff0LL) qc = 0xfffffff0LL;
+  long long pc = desc->max_parents > 0 ? desc->max_parents : (1LL << 20);
+  p->q_cap = qc;
+  p->p_cap = pc;
+  p->out_cap = 1 << 16;
+  if (p->desc.max_iters <= 0) p->desc.max_iters = 1LL << 40;
+  auto fail = [&](int code) {
+    delete p;
+    return code;
+  };
+  hipError_t e;
+#define PALLOC(buf, bytes)                                                                     \
+  if ((e = (buf).alloc(bytes)) != hipSuccess)                                                  \
+    return fail(set_err(e == hipErrorOutOfMemory ? GPUGO_E_NOMEM : GPUGO_E_HIP, "alloc %s: %s", \
+                        #buf, hipGetErrorString(e)))
+  PALLOC(p->d_lo, sizeof(double) * d);
+  PALLOC(p->d_hi, sizeof(double) * d);
+  PALLOC(p->d_params, sizeof(double) * (desc->n_params > 0 ? desc->n_params : 1));
+  PALLOC(p->plo, sizeof(double) * d * pc);
+  PALLOC(p->phi, sizeof(double) * d * pc);
+  PALLOC(p->pw, sizeof(double) * d * pc);
+  PALLOC(p->pwin, sizeof(int) * pc);
+  PALLOC(p->out_lo, sizeof(double) * d * p->out_cap);
+  PALLOC(p->out_hi, sizeof(double) * d * p->out_cap);
+  PALLOC(p->out_lb, sizeof(double) * p->out_cap);
+  PALLOC(p->st, sizeof(DevState));
+  PALLOC(p->scratch, 64);
+  for (int b = 0; b < 2; ++b) {
+    PALLOC(p->q[b].lb, sizeof(double) * qc);
+    PALLOC(p->q[b].par, sizeof(unsigned) * qc);
+    PALLOC(p->q[b].pos, sizeof(unsigned) * qc);
+  }
+#undef PALLOC
+  if ((e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) != hipSuccess)
+    return fail(set_err(GPUGO_E_HIP, "stream: %s", hipGetErrorString(e)));
+  if ((e = hipMemcpy(p->d_lo.p, p->lo.data(), sizeof(double) * d, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(p->d_hi.p, p->hi.data(), sizeof(double) * d, hipMemcpyHostToDevice)) != hipSuccess ||
+      (desc->n_params > 0 &&
+       (e = hipMemcpy(p->d_params.p, p->params.data(), sizeof(double) * desc->n_params, hipMemcpyHostToDevice)) !=
+           hipSuccess))
+    return fail(set_err(GPUGO_E_HIP, "upload: %s", hipGetErrorString(e)));
+  if ((e = hipHostMalloc((void **)&p->hpoll, 2 * sizeof(DevState), hipHostMallocDefault)) != hipSuccess)
+    return fail(set_err(GPUGO_E_NOMEM, "pinned poll buffer: %s", hipGetErrorString(e)));
+  for (int b = 0; b < 2; ++b)
+    if ((e = hipEventCreateWithFlags(&p->poll_ev[b], hipEventDisableTiming)) != hipSuccess)
+      return fail(set_err(GPUGO_E_HIP, "event: %s", hipGetErrorString(e)));
+  memset(&p->hs, 0, sizeof p->hs);
+  *out = p;
+  return GPUGO_OK;
+}
+
+extern "C" void gpugo_problem_destroy(gpugo_problem_t p) {
+  if (!p) return;
+  if (p->stream) {
+    (void)hipStreamSynchronize(p->stream);
+    (void)hipStreamDestroy(p->stream);
+  }
+  for (int b = 0; b < 2; ++b)
+    if (p->poll_ev[b]) (void)hipEventDestroy(p->poll_ev[b]);
+  for (auto ev : p->tev) (void)hipEventDestroy(ev);
+  if (p->hpoll) (void)hipHostFree(p->hpoll);
+  delete p;
+}
+
+extern "C" int gpugo_problem_reset(gpugo_problem_t p) {
+  if (!p) return set_err(GPUGO_E_INVALID, "null problem");
+  HIPCHECK(hipStreamSynchronize(p->stream));
+  p->started = false;
+  p->steps_enqueued = 0;
+  p->tev_used = 0;
+  p->bound_ms = 0;
+  p->bound_launches = 0;
+  memset(&p->hs, 0, sizeof p->hs);
+  return GPUGO_OK;
+}
+
+extern "C" int gpugo_problem_timing(gpugo_problem_t p, int enable) {
+  if (!p) return set_err(GPUGO_E_INVALID, "null problem");
+  HIPCHECK(hipStreamSynchronize(p->stream));
+  int rc = timing_collect(p);
+  if (rc) return rc;
+  p->timing = enable != 0;
+  return GPUGO_OK;
+}
+
+static int root_init(gpugo_problem_t p) {
+  const int d = p->desc.d;
+  std::vector<double> pw(d);
+  for (int j = 0; j < d; ++j) pw[j] = (p->hi[j] - p->lo[j]) / (double)p->desc.wid;
+  DevState s;
+  memset(&s, 0, sizeof s);
+  s.ub_key = order_key(p->desc.init_ub);
+  s.ub_snap = p->desc.init_ub;
+  s.lb_global = -__builtin_huge_val();
+  s.out_min_lb = __builtin_huge_val();
+  s.width = __builtin_huge_val();
+  s.parents_used = 1;
+  s.first_parent = 0;
+  s.n_split = 1;
+  s.status = GPUGO_ST_RUNNING;
+  int win0 = 0;
+  p->hs = s;
+  HIPCHECK(hipMemcpyAsync(p->plo.p, p->lo.data(), sizeof(double) * d, hipMemcpyHostToDevice, p->stream));
+  HIPCHECK(hipMemcpyAsync(p->phi.p, p->hi.data(), sizeof(double) * d, hipMemcpyHostToDevice, p->stream));
+  HIPCHECK(hipMemcpyAsync(p->pw.p, pw.data(), sizeof(double) * d, hipMemcpyHostToDevice, p->stream));
+  HIPCHECK(hipMemcpyAsync(p->pwin.p, &win0, sizeof(int), hipMemcpyHostToDevice, p->stream));
+  HIPCHECK(hipMemcpyAsync(p->st.p, &p->hs, sizeof(DevState), hipMemcpyHostToDevice, p->stream));
+  HIPCHECK(hipStreamSynchronize(p->stream));
+  return GPUGO_OK;
+}
+
+static int enqueue_step(gpugo_problem_t p) {
+  const int d = p->desc.d;
+  const int cur = (int)(p->steps_enqueued & 1);
+  Queue &A = p->q[cur], &B = p->q[cur ^ 1];
+  DevState *st = p->st.as<DevState>();
+  if (p->steps_enqueued > 0) {
+    SelectArgs sa;
+    sa.st = st;
+    sa.q_lb = A.plb();
+    sa.q_par = A.ppar();
+    sa.q_pos = A.ppos();
+    sa.plo = p->plo.as<double>();
+    sa.phi = p->phi.as<double>();
+    sa.pw = p->pw.as<double>();
+    sa.pwin = p->pwin.as<int>();
+    sa.out_lo = p->out_lo.as<double>();
+    sa.out_hi = p->out_hi.as<double>();
+    sa.out_lb = p->out_lb.as<double>();
+    sa.out_cap = p->out_cap;
+    sa.p_cap = p->p_cap;
+    sa.d = d;
+    sa.wid = p->desc.wid;
+    sa.c = p->c;
+    sa.k_select = p->desc.k_select;
+    sa.tol_f = p->desc.tol_f;
+    sa.tol_x = p->desc.tol_x;
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_T), 0, p->stream, sa);
+    HIPCHECK(hipGetLastError());
+  }
+  BoundArgs ba;
+  memset(&ba, 0, sizeof ba);
+  ba.plo = p->plo.as<double>();
+  ba.phi = p->phi.as<double>();
+  ba.pw = p->pw.as<double>();
+  ba.pwin = p->pwin.as<int>();
+  ba.params = p->d_params.as<double>();
+  ba.dom_lo = p->d_lo.as<double>();
+  ba.dom_hi = p->d_hi.as<double>();
+  ba.q_lb = A.plb();
+  ba.q_par = A.ppar();
+  ba.q_pos = A.ppos();
+  ba.st = st;
+  ba.q_cap = p->q_cap;
+  ba.d = d;
+  ba.wid = p->desc.wid;
+  ba.c = p->c;
+  ba.n_samples = p->desc.n_samples;
+  ba.W = p->W;
+  ba.inv_wid = 1.0 / (double)p->desc.wid;
+  ba.half_inv_wid = 0.5 / (double)p->desc.wid;
+  {
+    long long maxk = p->steps_enqueued == 0 ? 1 : p->desc.k_select;
+    long long nthreads = maxk * (long long)p->W;
+    long long blocks = (nthreads + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    void *args[] = {&ba};
+    if (p->timing) {
+      while (p->tev.size() < p->tev_used + 2) {
+        hipEvent_t ev;
+        HIPCHECK(hipEventCreate(&ev));
+        p->tev.push_back(ev);
+      }
+      HIPCHECK(hipEventRecord(p->tev[p->tev_used++], p->stream));
+    }
+    HIPCHECK(hipModuleLaunchKernel(p->m->bound[p->desc.rounding], (unsigned)blocks, 1, 1, 256, 1, 1, 0, p->stream,
+                                   args, nullptr));
+    if (p->timing) HIPCHECK(hipEventRecord(p->tev[p->tev_used++], p->stream));
+  }
+  CompactArgs ca;
+  ca.st = st;
+  ca.lb_in = A.plb();
+  ca.par_in = A.ppar();
+  ca.pos_in = A.ppos();
+  ca.lb_out = B.plb();
+  ca.par_out = B.ppar();
+  ca.pos_out = B.ppos();
+  ca.cap = p->q_cap;
+  hipLaunchKernelGGL(k_compact, dim3(512), dim3(256), 0, p->stream, ca);
+  HIPCHECK(hipGetLastError());
+  ++p->steps_enqueued;
+  return GPUGO_OK;
+}
+
+static int read_state(gpugo_problem_t p) {
+  HIPCHECK(hipMemcpyAsync(&p->hs, p->st.p, sizeof(DevState), hipMemcpyDeviceToHost, p->stream));
+  HIPCHECK(hipStreamSynchronize(p->stream));
+  return GPUGO_OK;
+}
+
+static int live_buffer(gpugo_problem_t p) { return (int)(p->hs.compactions & 1); }
+
+static int fill_result(gpugo_problem_t p, gpugo_result *res, double ms, bool max_iters_hit) {
+  DevState &s = p->hs;
+  memset(res, 0, sizeof *res);
+  double ub = key_value(s.ub_key);
+  double lb = s.lb_global;
+  if (s.status == GPUGO_ST_RUNNING || s.status == GPUGO_ST_OVERFLOW) {
+    // recompute the smallest live lower bound exactly
+    const int lbuf = live_buffer(p);
+    long long n = s.compact_count;
+    unsigned long long *d_key = p->scratch.as<unsigned long long>();
+    hipLaunchKernelGGL(k_minlb, dim3(1), dim3(1024), 0, p->stream, p->st.as<DevState>(), p->q[lbuf].plb(), n, d_key);
+    HIPCHECK(hipGetLastError());
+    unsigned long long k;
+    HIPCHECK(hipMemcpyAsync(&k, d_key, sizeof k, hipMemcpyDeviceToHost, p->stream));
+    HIPCHECK(hipStreamSynchronize(p->stream));
+    lb = key_value(k);
+    if (s.out_min_lb < lb) lb = s.out_min_lb;
+    if (n == 0 && s.n_out == 0) lb = -__builtin_huge_val();
+  }
+  res->lb = lb;
+  res->ub = ub;
+  {
+    // ub - lb rounded up on the host: nextafter of the nearest difference
+    double w = ub - lb;
+    if (std::isfinite(w) && w > -__builtin_huge_val()) w = std::nextafter(w, __builtin_huge_val());
+    res->width = (s.status == GPUGO_ST_CERTIFIED || s.status == GPUGO_ST_X_CONVERGED) ? s.width : w;
+  }
+  res->status = s.status;
+  if (s.status == GPUGO_ST_RUNNING && max_iters_hit) res->status = GPUGO_ST_MAX_ITERS;
+  res->iterations = s.iterations;
+  res->boxes_bounded = s.bounded;
+  res->pruned_value = s.pruned_value;
+  res->pruned_kkt = s.pruned_kkt;
+  res->queue_size = s.compact_count;
+  res->queue_peak = s.queue_peak;
+  res->n_out = s.n_out;
+  res->ms = ms;
+  return GPUGO_OK;
+}
+
+static int run_steps(gpugo_problem_t p, int64_t iters, gpugo_result *res, bool solve) {
+  if (!p || !res) return set_err(GPUGO_E_INVALID, "null pointer");
+  auto t0 = std::chrono::steady_clock::now();
+  if (!p->started) {
+    int rc = root_init(p);
+    if (rc) return rc;
+    p->started = true;
+  }
+  const long long it0 = p->hs.iterations;
+  if (p->hs.status == GPUGO_ST_RUNNING) {
+    // Steps are enqueued in batches; after each batch the device state is
+    // copied into a pinned slot.  The host waits only for the previous
+    // batch's copy, so it stays one batch ahead of the device; steps
+    // enqueued after a stop are no-ops on the device.  Exactly `iters`
+    // steps are enqueued at most.
+    const int64_t B = 4;
+    int64_t done = 0;
+    int slot = 0, pending = 0;
+    DevState *st = p->st.as<DevState>();
+    while (done < iters) {
+      int64_t batch = B < iters - done ? B : iters - done;
+      for (int64_t i = 0; i < batch; ++i) {
+        int rc = enqueue_step(p);
+        if (rc) return rc;
+      }
+      done += batch;
+      HIPCHECK(hipMemcpyAsync(&p->hpoll[slot], st, sizeof(DevState), hipMemcpyDeviceToHost, p->stream));
+      HIPCHECK(hipEventRecord(p->poll_ev[slot], p->stream));
+      ++pending;
+      if (pending == 2) {
+        const int old = slot ^ 1;
+        HIPCHECK(hipEventSynchronize(p->poll_ev[old]));
+        --pending;
+        if (p->hpoll[old].status != GPUGO_ST_RUNNING) break;
+      }
+      slot ^= 1;
+    }
+  }
+  int rc = read_state(p);
+  if (rc) return rc;
+  if (p->timing) {
+    // working launches come first in the recorded order
+    long long nwork = p->hs.iterations - it0;
+    size_t keep = (size_t)(2 * nwork) < p->tev_used ? (size_t)(2 * nwork) : p->tev_used;
+    p->tev_used = keep;
+    rc = timing_collect(p);
+    if (rc) return rc;
+    p->bound_launches += nwork;
+  }
+  p->tev_used = 0;
+  double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  bool hit = solve && p->hs.status == GPUGO_ST_RUNNING;
+  rc = fill_result(p, res, ms, hit);
+  if (rc) return rc;
+  res->bound_ms = p->bound_ms;
+  res->bound_launches = p->bound_launches;
+  if (p->hs.overflow & 1) return set_err(GPUGO_E_CAPACITY, "box queue capacity (%lld) exceeded", p->q_cap);
+  if (p->hs.overflow & 2) return set_err(GPUGO_E_CAPACITY, "parent arena capacity (%lld) exceeded", p->p_cap);
+  return GPUGO_OK;
+}
+
+extern "C" int gpugo_step(gpugo_problem_t p, int64_t iters, gpugo_result *res) {
+  if (iters < 0) return set_err(GPUGO_E_INVALID, "negative iters");
+  return run_steps(p, iters, res, false);
+}
+
+extern "C" int gpugo_solve(gpugo_problem_t p, gpugo_result *res) {
+  if (!p) return set_err(GPUGO_E_INVALID, "null problem");
+  long long left = p->desc.max_iters - p->hs.iterations;
+  if (left < 0) left = 0;
+  return run_steps(p, left, res, true);
+}
+
+extern "C" int gpugo_boxes_copy(gpugo_problem_t p, int64_t max, double *lo, double *hi, double *lb, int64_t *n) {
+  if (!p || !n) return set_err(GPUGO_E_INVALID, "null pointer");
+  if (max < 0) return set_err(GPUGO_E_INVALID, "negative max");
+  *n = 0;
+  if (!p->started) return GPUGO_OK;
+  int rc = read_state(p);
+  if (rc) return rc;
+  const int d = p->desc.d;
+  const long long nlive = p->hs.compact_count;
+  if (nlive == 0) {
+    // output boxes of the x-width rule
+    long long no = p->hs.n_out < p->out_cap ? p->hs.n_out : p->out_cap;
+    *n = no;
+    long long m = no < max ? no : max;
+    if (m > 0) {
+      if (lo) HIPCHECK(hipMemcpy(lo, p->out_lo.p, sizeof(double) * d * m, hipMemcpyDeviceToHost));
+      if (hi) HIPCHECK(hipMemcpy(hi, p->out_hi.p, sizeof(double) * d * m, hipMemcpyDeviceToHost));
+      if (lb) HIPCHECK(hipMemcpy(lb, p->out_lb.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+    }
+    return GPUGO_OK;
+  }
+  *n = nlive;
+  long long m = nlive < max ? nlive : max;
+  if (m == 0) return GPUGO_OK;
+  const Queue &Q = p->q[live_buffer(p)];
+  std::vector<double> hlb(nlive);
+  std::vector<unsigned> hpar(nlive), hpos(nlive);
+  HIPCHECK(hipMemcpy(hlb.data(), Q.lb.p, sizeof(double) * nlive, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(hpar.data(), Q.par.p, sizeof(unsigned) * nlive, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(hpos.data(), Q.pos.p, sizeof(unsigned) * nlive, hipMemcpyDeviceToHost));
+  std::vector<long long> order(nlive);
+  for (long long i = 0; i < nlive; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](long long x, long long y) {
+    if (hlb[x] != hlb[y]) return hlb[x] < hlb[y];
+    if (hpar[x] != hpar[y]) return hpar[x] < hpar[y];
+    return hpos[x] < hpos[y];
+  });
+  order.resize(m);
+  DevBuf d_order, d_lo, d_hi;
+  HIPCHECK(d_order.alloc(sizeof(long long) * m));
+  HIPCHECK(d_lo.alloc(sizeof(double) * d * m));
+  HIPCHECK(d_hi.alloc(sizeof(double) * d * m));
+  HIPCHECK(hipMemcpy(d_order.p, order.data(), sizeof(long long) * m, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_materialize, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, p->stream, p->plo.as<double>(),
+                     p->phi.as<double>(), p->pw.as<double>(), p->pwin.as<int>(), Q.ppar(), Q.ppos(),
+                     d_order.as<long long>(), m, d, p->desc.wid, p->c, d_lo.as<double>(), d_hi.as<double>());
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(p->stream));
+  if (lo) HIPCHECK(hipMemcpy(lo, d_lo.p, sizeof(double) * d * m, hipMemcpyDeviceToHost));
+  if (hi) HIPCHECK(hipMemcpy(hi, d_hi.p, sizeof(double) * d * m, hipMemcpyDeviceToHost));
+  if (lb)
+    for (long long i = 0; i < m; ++i) lb[i] = hlb[order[i]];
+  return GPUGO_OK;
+}

@@ -1,0 +1,349 @@
+"""Expression -> HIP emitter for objective modules.
+
+The emitted text is: ia.hpp + bnb_args.hpp + the objective's code lists
+(obj_value, obj_grad, obj_kkt) + bnb_kernel.hpp.  It is compiled by hipRTC
+(through libgpugo.so) into one code object per objective.  This replaces
+Numba's JIT of each driver's code lists (GPUGO_1.1/1-Ackley/ackley50.py:
+116-479, 602-766).
+
+Lowering rules (the same op sequence a hand-written code list would use):
+  scalar * I, I * scalar   -> scale      I + scalar -> addc     I - scalar -> addc(I, -s)
+  scalar - I               -> rsubc      I / scalar -> divc     scalar / I -> div(point, I)
+  I op I                   -> add/sub/mul/div
+  sqr, pown, sqrt, exp, sin, cos -> the interval functions of ia.hpp
+Repeated subexpressions are evaluated once per code list.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+from typing import Dict, List, Optional, Tuple
+
+from .expr import Expr, Objective, wrap, Const
+
+_CSRC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc")
+
+
+def _lit(v: float) -> str:
+    """Exact C literal of a double."""
+    if v != v:
+        return "__builtin_nan(\"\")"
+    if v == float("inf"):
+        return "__builtin_huge_val()"
+    if v == float("-inf"):
+        return "(-__builtin_huge_val())"
+    return float(v).hex()
+
+
+class _Func:
+    """Straight-line code of one device function with CSE."""
+
+    def __init__(self):
+        self.lines: List[str] = []
+        self.memo: Dict[tuple, Tuple[str, str]] = {}
+        self.n = 0
+
+    def tmp(self, kind: str, code: str) -> str:
+        name = f"t{self.n}"
+        self.n += 1
+        ty = "iv<double>" if kind == "i" else "double"
+        self.lines.append(f"  const {ty} {name} = {code};")
+        return name
+
+    def as_iv(self, kc: Tuple[str, str]) -> str:
+        kind, code = kc
+        return code if kind == "i" else f"point<double>({code})"
+
+    def lower(self, e: Expr) -> Tuple[str, str]:
+        """Returns (kind, code): kind 's' = exact scalar C expression, 'i' = interval."""
+        k = e.key()
+        if k in self.memo:
+            return self.memo[k]
+        r = self._lower(e)
+        self.memo[k] = r
+        return r
+
+    def _lower(self, e: Expr) -> Tuple[str, str]:
+        op = e.op
+        if op == "var":
+            return ("i", f"x[{e.val}]")
+        if op == "const":
+            return ("s", _lit(e.val))
+        if op == "param":
+            return ("s", f"P[{e.val}]")
+        if op == "iconst":
+            return ("i", self.tmp("i", f"mk<double>({_lit(e.val[0])}, {_lit(e.val[1])})"))
+        if e.is_scalar():
+            if op == "neg":
+                return ("s", f"(-{self.lower(e.args[0])[1]})")
+            if op == "mul":
+                a, b = (self.lower(x)[1] for x in e.args)
+                return ("s", f"({a} * {b})")
+        if op == "neg":
+            return ("i", self.tmp("i", f"A::neg({self.as_iv(self.lower(e.args[0]))})"))
+        if op in ("sqr", "sqrt", "exp", "sin", "cos"):
+            a = self.as_iv(self.lower(e.args[0]))
+            return ("i", self.tmp("i", f"A::{op}({a})"))
+        if op == "pown":
+            a = self.as_iv(self.lower(e.args[0]))
+            return ("i", self.tmp("i", f"A::pown({a}, {int(e.val)})"))
+        ea, eb = e.args
+        a, b = self.lower(ea), self.lower(eb)
+        if op == "mul":
+            if a[0] == "s" and b[0] == "s":
+                return ("i", self.tmp("i", f"A::scale(point<double>({a[1]}), {b[1]})"))
+            if a[0] == "s":
+                s_code, s_expr, iv_code = a[1], ea, b[1]
+            elif b[0] == "s":
+                s_code, s_expr, iv_code = b[1], eb, a[1]
+            else:
+                return ("i", self.tmp("i", f"A::mul({a[1]}, {b[1]})"))
+            sign = s_expr.scalar_sign()
+            fn = "scale_pos" if sign > 0 else ("scale_neg" if sign < 0 else "scale")
+            return ("i", self.tmp("i", f"A::{fn}({iv_code}, {s_code})"))
+        if op == "add":
+            if a[0] == "s" and b[0] == "s":
+                return ("i", self.tmp("i", f"A::addc(point<double>({a[1]}), {b[1]})"))
+            if a[0] == "s":
+                return ("i", self.tmp("i", f"A::addc({b[1]}, {a[1]})"))
+            if b[0] == "s":
+                return ("i", self.tmp("i", f"A::addc({a[1]}, {b[1]})"))
+            return ("i", self.tmp("i", f"A::add({a[1]}, {b[1]})"))
+        if op == "sub":
+            if a[0] == "s" and b[0] == "s":
+                return ("i", self.tmp("i", f"A::rsubc({a[1]}, point<double>({b[1]}))"))
+            if a[0] == "s":
+                return ("i", self.tmp("i", f"A::rsubc({a[1]}, {b[1]})"))
+            if b[0] == "s":
+                return ("i", self.tmp("i", f"A::addc({a[1]}, -{b[1]})"))
+            return ("i", self.tmp("i", f"A::sub({a[1]}, {b[1]})"))
+        if op == "div":
+            if b[0] == "s":
+                ia = self.as_iv(a)
+                fn = "divc_pos" if eb.scalar_sign() > 0 else "divc"
+                return ("i", self.tmp("i", f"A::{fn}({ia}, {b[1]})"))
+            return ("i", self.tmp("i", f"A::div({self.as_iv(a)}, {b[1]})"))
+        raise ValueError(f"cannot lower op {op}")
+
+
+# ---------------------------------------------------------------------------
+# symbolic differentiation (used when the objective has no gradient lists)
+# ---------------------------------------------------------------------------
+def diff(e: Expr, j: int, memo: Optional[dict] = None) -> Optional[Expr]:
+    """d e / d X[j]; None means identically zero."""
+    if memo is None:
+        memo = {}
+    k = e.key()
+    if k in memo:
+        return memo[k]
+    r = _diff(e, j, memo)
+    memo[k] = r
+    return r
+
+
+def _mul(a: Optional[Expr], b: Optional[Expr]) -> Optional[Expr]:
+    if a is None or b is None:
+        return None
+    if a.op == "const" and a.val == 1.0:
+        return b
+    if b.op == "const" and b.val == 1.0:
+        return a
+    return a * b
+
+
+def _add(a: Optional[Expr], b: Optional[Expr]) -> Optional[Expr]:
+    if a is None:
+        return b
+    if b is None:
+        return a
+    return a + b
+
+
+def _neg(a: Optional[Expr]) -> Optional[Expr]:
+    return None if a is None else -a
+
+
+def _diff(e: Expr, j: int, memo) -> Optional[Expr]:
+    op = e.op
+    if op == "var":
+        return Const(1.0) if e.val == j else None
+    if op in ("const", "param", "iconst"):
+        return None
+    if not e.depends_on_x():
+        return None
+    if op == "neg":
+        return _neg(diff(e.args[0], j, memo))
+    if op == "add":
+        return _add(diff(e.args[0], j, memo), diff(e.args[1], j, memo))
+    if op == "sub":
+        return _add(diff(e.args[0], j, memo), _neg(diff(e.args[1], j, memo)))
+    if op == "mul":
+        a, b = e.args
+        return _add(_mul(diff(a, j, memo), b), _mul(a, diff(b, j, memo)))
+    if op == "div":
+        a, b = e.args
+        da, db = diff(a, j, memo), diff(b, j, memo)
+        t1 = None if da is None else da / b
+        t2 = None if db is None else (a * db) / (b * b)
+        return _add(t1, _neg(t2))
+    a = e.args[0]
+    da = diff(a, j, memo)
+    if da is None:
+        return None
+    if op == "sqr":
+        return _mul(a * 2.0, da)
+    if op == "pown":
+        n = int(e.val)
+        if n == 0:
+            return None
+        inner = Const(float(n)) if n == 1 else (Expr("pown", (a,), n - 1) * float(n) if n > 2 else a * 2.0)
+        return _mul(inner, da)
+    if op == "sqrt":
+        return _mul(da, 0.5 / e)
+    if op == "exp":
+        return _mul(e, da)
+    if op == "sin":
+        return _mul(Expr("cos", (a,)), da)
+    if op == "cos":
+        return _neg(_mul(Expr("sin", (a,)), da))
+    raise ValueError(f"cannot differentiate op {op}")
+
+
+def gradient(obj: Objective) -> List[Expr]:
+    if obj.grad is not None:
+        return [wrap(g) for g in obj.grad]
+    out = []
+    for j in range(obj.d):
+        g = diff(obj.value, j)
+        out.append(g if g is not None else Const(0.0))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# module text
+# ---------------------------------------------------------------------------
+def code_lists(obj: Objective) -> str:
+    obj.check()
+    grads = gradient(obj)
+    d = obj.d
+    out = [f"#define GPUGO_D {d}", f"#define GPUGO_NPARAMS {obj.n_params}",
+           "namespace gpugo {"]
+
+    # value
+    f = _Func()
+    v = f.lower(obj.value)
+    out.append("template <int R>\n__device__ __forceinline__ iv<double> obj_value("
+               "const iv<double> *__restrict__ x, const double *__restrict__ P) {")
+    out.append("  typedef IA<double, R> A;")
+    out += f.lines
+    out.append(f"  return {f.as_iv(v)};")
+    out.append("}")
+
+    # gradient (parity hook)
+    f = _Func()
+    body = []
+    for j, g in enumerate(grads):
+        r = f.lower(g)
+        body.append((j, f.as_iv(r), len(f.lines)))
+    out.append("template <int R>\n__device__ __forceinline__ void obj_grad("
+               "const iv<double> *__restrict__ x, const double *__restrict__ P, iv<double> *__restrict__ g) {")
+    out.append("  typedef IA<double, R> A;")
+    out += f.lines
+    for j, code, _ in body:
+        out.append(f"  g[{j}] = {code};")
+    out.append("}")
+
+    # monotonicity test with early exit per component
+    # (GPUGO_1.1/1-Ackley/ackley50.py:458-477)
+    f = _Func()
+    out.append("template <int R>\n__device__ __forceinline__ bool obj_kkt("
+               "const iv<double> *__restrict__ x, const double *__restrict__ P, "
+               "const double *__restrict__ dlo, const double *__restrict__ dhi) {")
+    out.append("  typedef IA<double, R> A;")
+    for j, g in enumerate(grads):
+        start = len(f.lines)
+        r = f.lower(g)
+        out += f.lines[start:]
+        gv = f.as_iv(r)
+        out.append(f"  {{ const iv<double> gj = {gv};")
+        out.append(f"    if (gj.lo > 0.0 && x[{j}].lo != dlo[{j}]) return true;")
+        out.append(f"    if (gj.hi < 0.0 && x[{j}].hi != dhi[{j}]) return true; }}")
+    out.append("  return false;")
+    out.append("}")
+    out.append("}  // namespace gpugo")
+    return "\n".join(out) + "\n"
+
+
+# Algorithmic work per interval operation, in FLOP (the accounting used for
+# the roofline): +,-,scalar ops 2; sqr 2; pown(k) 2*ceil(log2 k); interval
+# product 8; interval quotient 10; sqrt 2; exp/sin/cos 2 endpoints x 20.
+_FLOP = {"add": 2, "sub": 2, "neg": 0, "scale": 2, "addc": 2, "rsubc": 2, "divc": 2, "sqr": 2,
+         "mul": 8, "div": 10, "sqrt": 2, "exp": 40, "sin": 40, "cos": 40}
+
+
+def _count(exprs) -> int:
+    seen = set()
+    total = 0
+
+    def walk(e):
+        nonlocal total
+        k = e.key()
+        if k in seen:
+            return
+        seen.add(k)
+        for a in e.args:
+            walk(a)
+        op = e.op
+        if op in ("var", "const", "param", "iconst") or e.is_scalar():
+            return
+        if op == "pown":
+            n = max(int(e.val), 2)
+            total += 2 * (n - 1).bit_length()
+            return
+        if op in ("add", "sub", "mul", "div"):
+            sa, sb = e.args[0].is_scalar(), e.args[1].is_scalar()
+            if op == "mul" and (sa or sb):
+                total += _FLOP["scale"]
+                return
+            if op == "div" and sb:
+                total += _FLOP["divc"]
+                return
+        total += _FLOP[op]
+
+    for e in exprs:
+        walk(wrap(e))
+    return total
+
+
+def flop_counts(obj: Objective) -> dict:
+    """Algorithmic FLOP of the code lists of one child box: value enclosure,
+    full gradient enclosure (the monotonicity test may stop earlier) and one
+    sample evaluation (the value code list on a point)."""
+    v = _count([obj.value])
+    g = _count(gradient(obj))
+    return {"value": v, "grad": g, "sample": v}
+
+
+def _read(name: str) -> str:
+    with open(os.path.join(_CSRC, name)) as fh:
+        return fh.read()
+
+
+def module_source(obj: Objective, roundings=("fast",)) -> str:
+    """Full hipRTC source; `roundings` selects the kernel variants built."""
+    bad = set(roundings) - {"exact", "fast"}
+    if bad or not roundings:
+        raise ValueError(f"roundings must be a non-empty subset of ('exact', 'fast'), got {roundings}")
+    return "\n".join([
+        "// generated by gpugo_amd.emit for objective " + obj.name,
+        f"#define GPUGO_WITH_EXACT {int('exact' in roundings)}",
+        f"#define GPUGO_WITH_FAST {int('fast' in roundings)}",
+        _read("ia.hpp"),
+        _read("bnb_args.hpp"),
+        code_lists(obj),
+        _read("bnb_kernel.hpp"),
+    ])
+
+
+def source_key(src: str, options: str = "") -> str:
+    return hashlib.sha256((options + "\0" + src).encode()).hexdigest()[:32]

@@ -1,0 +1,203 @@
+"""Expression IR for objectives.
+
+An objective is written once as an expression of the coordinates ``X[j]``
+and the coefficient vector ``P[i]``; gpugo_amd.emit lowers it to interval
+code lists (value, gradient, monotonicity test) in HIP.  This replaces the
+per-script hand-written device functions <fn>Value_IA / <fn>Derivative_IA /
+<fn>Sample_IA of the reference (GPUGO_1.1/1-Ackley/ackley50.py:116-479).
+
+Values are intervals except *exact scalars*: literal doubles, parameters,
+their negations and their products with powers of two, which are exact in
+binary floating point and are passed to the interval ops as scalars
+(``scale``, ``addc``, ``rsubc``, ``divc``).  Anything else is evaluated with
+outward rounding.  Irrational constants are written as ``IConst(lo, hi)``
+with bracketing doubles (``PI``, ``E``).
+"""
+from __future__ import annotations
+
+import math
+from typing import Iterable, Sequence
+
+_OPS_UNARY = ("neg", "sqr", "sqrt", "exp", "sin", "cos")
+_OPS_BINARY = ("add", "sub", "mul", "div")
+
+
+class Expr:
+    __slots__ = ("op", "args", "val", "_key", "_hash")
+
+    def __init__(self, op: str, args: tuple = (), val=None):
+        self.op = op
+        self.args = args
+        self.val = val
+        self._key = None
+        self._hash = None
+
+    # structural identity (used for common-subexpression elimination)
+    def key(self):
+        if self._key is None:
+            self._key = (self.op, self.val, tuple(a.key() for a in self.args))
+        return self._key
+
+    def __hash__(self):
+        if self._hash is None:
+            self._hash = hash(self.key())
+        return self._hash
+
+    def __eq__(self, other):  # structural
+        return isinstance(other, Expr) and self.key() == other.key()
+
+    # arithmetic
+    def __add__(self, o): return Expr("add", (self, wrap(o)))
+    def __radd__(self, o): return Expr("add", (wrap(o), self))
+    def __sub__(self, o): return Expr("sub", (self, wrap(o)))
+    def __rsub__(self, o): return Expr("sub", (wrap(o), self))
+    def __mul__(self, o): return Expr("mul", (self, wrap(o)))
+    def __rmul__(self, o): return Expr("mul", (wrap(o), self))
+    def __truediv__(self, o): return Expr("div", (self, wrap(o)))
+    def __rtruediv__(self, o): return Expr("div", (wrap(o), self))
+    def __neg__(self): return Expr("neg", (self,))
+
+    def __pow__(self, n):
+        if not isinstance(n, int) or n < 0:
+            raise ValueError("only non-negative integer powers are supported")
+        if n == 2:
+            return sqr(self)
+        return Expr("pown", (self,), n)
+
+    def __repr__(self):
+        if self.op == "var":
+            return f"X[{self.val}]"
+        if self.op == "param":
+            return f"P[{self.val}]"
+        if self.op == "const":
+            return repr(self.val)
+        if self.op == "iconst":
+            return f"[{self.val[0]!r},{self.val[1]!r}]"
+        if self.op == "pown":
+            return f"pown({self.args[0]!r},{self.val})"
+        return f"{self.op}({', '.join(repr(a) for a in self.args)})"
+
+    # exact scalars -------------------------------------------------------
+    def is_scalar(self) -> bool:
+        """True if the value is a single double known exactly (no rounding)."""
+        if self.op in ("const", "param"):
+            return True
+        if self.op == "neg":
+            return self.args[0].is_scalar()
+        if self.op == "mul":
+            a, b = self.args
+            return (a.is_scalar() and b.is_scalar() and
+                    (_is_pow2_const(a) or _is_pow2_const(b)))
+        return False
+
+    def depends_on_x(self) -> bool:
+        if self.op == "var":
+            return True
+        return any(a.depends_on_x() for a in self.args)
+
+    def scalar_sign(self):
+        """+1 / -1 if the sign of an exact scalar is known statically, else 0."""
+        if self.op == "const":
+            return 1 if self.val > 0 else (-1 if self.val < 0 else 0)
+        if self.op == "neg":
+            return -self.args[0].scalar_sign()
+        if self.op == "mul":
+            return self.args[0].scalar_sign() * self.args[1].scalar_sign()
+        return 0
+
+
+def _is_pow2_const(e: Expr) -> bool:
+    if e.op != "const" or e.val == 0 or not math.isfinite(e.val):
+        return False
+    m, _ = math.frexp(abs(e.val))
+    return m == 0.5
+
+
+def wrap(v) -> Expr:
+    if isinstance(v, Expr):
+        return v
+    if isinstance(v, (int, float)):
+        f = float(v)
+        if isinstance(v, int) and float(int(f)) != v:
+            raise ValueError(f"integer constant {v} is not exact in double")
+        return Expr("const", (), f)
+    raise TypeError(f"cannot use {type(v).__name__} in an objective expression")
+
+
+def X(j: int) -> Expr:
+    return Expr("var", (), int(j))
+
+
+def P(i: int) -> Expr:
+    return Expr("param", (), int(i))
+
+
+def Const(v: float) -> Expr:
+    return Expr("const", (), float(v))
+
+
+def IConst(lo: float, hi: float) -> Expr:
+    if not lo <= hi:
+        raise ValueError("IConst needs lo <= hi")
+    return Expr("iconst", (), (float(lo), float(hi)))
+
+
+PI = IConst(float.fromhex("0x1.921fb54442d18p+1"), float.fromhex("0x1.921fb54442d19p+1"))
+E = IConst(float.fromhex("0x1.5bf0a8b145769p+1"), float.fromhex("0x1.5bf0a8b14576ap+1"))
+
+
+def sqr(e) -> Expr: return Expr("sqr", (wrap(e),))
+def sqrt(e) -> Expr: return Expr("sqrt", (wrap(e),))
+def exp(e) -> Expr: return Expr("exp", (wrap(e),))
+def sin(e) -> Expr: return Expr("sin", (wrap(e),))
+def cos(e) -> Expr: return Expr("cos", (wrap(e),))
+
+
+def pown(e, n: int) -> Expr:
+    if n == 2:
+        return sqr(e)
+    return Expr("pown", (wrap(e),), int(n))
+
+
+def fsum(terms: Iterable) -> Expr:
+    """Left fold in the given order: ((t0 + t1) + t2) + ..."""
+    it = iter(terms)
+    try:
+        acc = wrap(next(it))
+    except StopIteration:
+        return Const(0.0)
+    for t in it:
+        acc = acc + t
+    return acc
+
+
+class Objective:
+    """An objective with its code lists.
+
+    value     Expr of X[0..d-1] and P[0..n_params-1]
+    grad      list of d Exprs (exact derivative enclosures); if None the
+              emitter differentiates ``value`` symbolically
+    name      used for the hipRTC program name and the module cache key
+    """
+
+    def __init__(self, d: int, n_params: int, value: Expr, grad: Sequence[Expr] | None = None,
+                 name: str = "objective"):
+        self.d = int(d)
+        self.n_params = int(n_params)
+        self.value = wrap(value)
+        self.grad = list(grad) if grad is not None else None
+        self.name = name
+        if self.grad is not None and len(self.grad) != self.d:
+            raise ValueError(f"gradient has {len(self.grad)} components, d={self.d}")
+
+    def check(self):
+        def walk(e):
+            if e.op == "var" and not 0 <= e.val < self.d:
+                raise ValueError(f"X[{e.val}] out of range for d={self.d}")
+            if e.op == "param" and not 0 <= e.val < self.n_params:
+                raise ValueError(f"P[{e.val}] out of range for n_params={self.n_params}")
+            for a in e.args:
+                walk(a)
+        walk(self.value)
+        for g in self.grad or ():
+            walk(wrap(g))

@@ -1,0 +1,162 @@
+"""Objective families and the seeded benchmark configurations.
+
+Three families cover the benchmark configurations and, as special cases,
+three of the reference's objectives:
+
+  sepcos        f = sum_i a_i x_i^2 + b_i (1 - cos(c_i x_i))
+                params [a(d), b(d), c(d)]; Rastrigin
+                (GPUGO_1.1/7-Rastrigin/rastrigin50.py:167-188) is a=1, b=10,
+                c=2pi (x^2 - 10 cos(2 pi x) + 10 per coordinate).
+  sepcos_chain  sepcos + sum_{i<d-1} w_i (x_{i+1} - x_i^2)^2 + (1 - x_i)^2
+                params [a(d), b(d), c(d), w(d-1)]; Rosenbrock
+                (GPUGO_1.1/11-Rosenbrock/rosenbrock50.py:170-198) is
+                a=b=0, w=100.
+  ackley_q      f = -A exp(-B sqrt(mean x^2)) - exp(mean cos(C x)) + A + e
+                    + sum_i q_i x_i^4 / 50
+                params [A, B, C, q(d)]; Ackley
+                (GPUGO_1.1/1-Ackley/ackley50.py:171-212) is A=20, B=0.02,
+                C=2pi, q=0.
+
+Gradients are given explicitly (as the reference hand-writes
+<fn>Derivative_IA); sums are left folds in increasing index.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from .expr import E, Objective, P, X, cos, exp, fsum, pown, sin, sqr, sqrt
+
+FAMILIES = ("sepcos", "sepcos_chain", "ackley_q")
+
+
+def n_params(family: str, d: int) -> int:
+    return {"sepcos": 3 * d, "sepcos_chain": 4 * d - 1, "ackley_q": 3 + d}[family]
+
+
+def _sep_terms(d):
+    a = lambda i: P(i)
+    b = lambda i: P(d + i)
+    c = lambda i: P(2 * d + i)
+    return [sqr(X(i)) * a(i) + (1.0 - cos(X(i) * c(i))) * b(i) for i in range(d)]
+
+
+def _sep_grad(d, j):
+    a, b, c = P(j), P(d + j), P(2 * d + j)
+    return X(j) * (2.0 * a) + (sin(X(j) * c) * c) * b
+
+
+def objective(family: str, d: int) -> Objective:
+    if family == "sepcos":
+        value = fsum(_sep_terms(d))
+        grad = [_sep_grad(d, j) for j in range(d)]
+    elif family == "sepcos_chain":
+        w = lambda i: P(3 * d + i)
+        chain = [sqr(X(i + 1) - sqr(X(i))) * w(i) + sqr(1.0 - X(i)) for i in range(d - 1)]
+        value = fsum(_sep_terms(d) + chain)
+        grad = []
+        for j in range(d):
+            g = _sep_grad(d, j)
+            if j + 1 < d:
+                g = g + ((X(j) * (X(j + 1) - sqr(X(j)))) * (-4.0 * w(j)) - (1.0 - X(j)) * 2.0)
+            if j > 0:
+                g = g + (X(j) - sqr(X(j - 1))) * (2.0 * w(j - 1))
+            grad.append(g)
+    elif family == "ackley_q":
+        A, B, C = P(0), P(1), P(2)
+        q = lambda i: P(3 + i)
+        s1 = fsum(sqr(X(i)) for i in range(d))
+        s2 = fsum(cos(X(i) * C) for i in range(d))
+        s3 = fsum(pown(X(i), 4) * q(i) for i in range(d))
+        r = sqrt(s1 / float(d))
+        t1 = exp(r * (-B))
+        v1 = t1 * (-A)
+        t2 = exp(s2 / float(d))
+        value = ((v1 - t2) + (A + E)) + s3 / 50.0
+        k1 = (((t1 / r) * B) * A) / float(d)
+        k2 = (exp(s2 / float(d)) * C) / float(d)
+        grad = [(k1 * X(j) + k2 * sin(X(j) * C)) + (pown(X(j), 3) * (4.0 * q(j))) / 50.0
+                for j in range(d)]
+    else:
+        raise ValueError(f"unknown family {family!r}")
+    return Objective(d, n_params(family, d), value, grad, name=f"{family}_d{d}")
+
+
+# ---------------------------------------------------------------------------
+# seeded problem instances
+# ---------------------------------------------------------------------------
+@dataclass
+class ProblemSpec:
+    family: str
+    d: int
+    params: np.ndarray
+    lo: np.ndarray
+    hi: np.ndarray
+    tol: float
+    f_star: Optional[float] = None      # closed form where known
+    name: str = ""
+    grid: tuple = (2, 10)               # (wid, cycle) default schedule
+    k_select: int = 64
+    extra: dict = field(default_factory=dict)
+
+
+def draw_sepcos(rng, d):
+    a = rng.uniform(0.5, 2.0, d)
+    b = rng.uniform(1.0, 10.0, d)
+    c = rng.uniform(1.0, 2.0 * math.pi, d)
+    return a, b, c
+
+
+def config0(seed: int = 0) -> ProblemSpec:
+    """d=2 separable, box [-5.12,5.12]^2, 64x64 grid, tol 1e-6; f* = 0 at x = 0."""
+    rng = np.random.default_rng(seed)
+    d = 2
+    a, b, c = draw_sepcos(rng, d)
+    return ProblemSpec("sepcos", d, np.concatenate([a, b, c]), np.full(d, -5.12), np.full(d, 5.12),
+                       1e-6, f_star=0.0, name="C0 sepcos d=2", grid=(64, 2), k_select=1)
+
+
+def config1(seed: int = 1) -> ProblemSpec:
+    """d=10 separable + chain, box [-5,10]^10, reference grid 4^10, tol 1e-6."""
+    rng = np.random.default_rng(seed)
+    d = 10
+    a, b, c = draw_sepcos(rng, d)
+    w = rng.uniform(1.0, 100.0, d - 1)
+    return ProblemSpec("sepcos_chain", d, np.concatenate([a, b, c, w]), np.full(d, -5.0),
+                       np.full(d, 10.0), 1e-6, name="C1 sepcos_chain d=10", grid=(4, 10), k_select=1)
+
+
+def config2(seed: int = 2, d: int = 50) -> ProblemSpec:
+    """d=50 Ackley-type + sum q x^4/50, box [-32,32]^50, tol 1e-6; f* = 0 at x = 0."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(10.0, 30.0)
+    B = rng.uniform(0.1, 0.3)
+    C = rng.uniform(math.pi, 3.0 * math.pi)
+    q = rng.uniform(0.0, 0.1, d)
+    return ProblemSpec("ackley_q", d, np.concatenate([[A, B, C], q]), np.full(d, -32.0),
+                       np.full(d, 32.0), 1e-6, f_star=0.0, name=f"C2 ackley_q d={d}", grid=(3, 12),
+                       k_select=1)
+
+
+def rastrigin(d: int) -> ProblemSpec:
+    """Reference Rastrigin on its box [-5.5, 6] (7-Rastrigin/rastrigin50.py:852)."""
+    p = np.concatenate([np.ones(d), np.full(d, 10.0), np.full(d, 2.0 * math.pi)])
+    return ProblemSpec("sepcos", d, p, np.full(d, -5.5), np.full(d, 6.0), 1e-6, f_star=0.0,
+                       name=f"rastrigin d={d}")
+
+
+def rosenbrock(d: int) -> ProblemSpec:
+    """Reference Rosenbrock on [-30, 30] (11-Rosenbrock/rosenbrock50.py:1059)."""
+    p = np.concatenate([np.zeros(d), np.zeros(d), np.ones(d), np.full(d - 1, 100.0)])
+    return ProblemSpec("sepcos_chain", d, p, np.full(d, -30.0), np.full(d, 30.0), 1e-6, f_star=0.0,
+                       name=f"rosenbrock d={d}")
+
+
+def ackley(d: int) -> ProblemSpec:
+    """Reference Ackley on [-35, 40] (1-Ackley/ackley50.py:958)."""
+    p = np.concatenate([[20.0, 0.02, 2.0 * math.pi], np.zeros(d)])
+    return ProblemSpec("ackley_q", d, p, np.full(d, -35.0), np.full(d, 40.0), 1e-6, f_star=0.0,
+                       name=f"ackley d={d}")

@@ -1,0 +1,214 @@
+/*
+ * gpugo.h -- C ABI of libgpugo.so, the MI355X (gfx950) interval
+ * branch-and-bound global optimizer.
+ *
+ * Plain C types only: pointers, sizes, scalars and opaque handles.  Every
+ * function returns an int status (GPUGO_OK == 0, negative on failure) unless
+ * stated otherwise; no exception or abort crosses this boundary.  The text of
+ * the last failure on the calling thread is returned by gpugo_last_error().
+ *
+ * Host pointers passed in are borrowed for the duration of the call only
+ * (their contents are copied).  Device memory belongs to the handle that
+ * allocated it until the matching *_destroy call.  Result structs are
+ * allocated by the caller.
+ *
+ * Reference interfaces replaced (paths relative to the GPUGO repository,
+ * GPUGO_1.1/ omitted):
+ *   - the interval-arithmetic module `ia.*` (Updated Interval Arithmetic
+ *     Operations/IA.py:57-570, and the per-folder IA<Fn>.py modules, e.g.
+ *     1-Ackley/IAAckley.py:147-467)                        -> gpugo_ia_eval
+ *   - the per-objective code lists <fn>Value_IA / <fn>Sample_IA /
+ *     <fn>Derivative_IA that each driver defines (1-Ackley/ackley50.py:116-479)
+ *     and Numba's JIT of them                               -> gpugo_module_*
+ *   - the kernels sampling / ruleOutGPU / constructFlagArr
+ *     (1-Ackley/ackley50.py:602-810) together with the host driver loop
+ *     (1-Ackley/ackley50.py:950-1395)                       -> gpugo_problem_*,
+ *                                                              gpugo_step,
+ *                                                              gpugo_solve
+ *   - completeRegion (1-Ackley/ackley50.py:820-948), the host rebuild of a
+ *     subregion from (parent, pos, cycleIdx)               -> gpugo_boxes_copy
+ */
+#ifndef GPUGO_H
+#define GPUGO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define GPUGO_OK            0
+#define GPUGO_E_INVALID    -1  /* bad argument (null pointer, size, enum)      */
+#define GPUGO_E_DOMAIN     -2  /* NaN or inverted interval in the input        */
+#define GPUGO_E_NOMEM      -3  /* device allocation failed                     */
+#define GPUGO_E_COMPILE    -4  /* hipRTC failure; log in gpugo_last_error()    */
+#define GPUGO_E_HIP        -5  /* HIP runtime error                            */
+#define GPUGO_E_NODEV      -6  /* no usable GPU                                */
+#define GPUGO_E_CAPACITY   -7  /* box queue or parent arena full: the solve
+                                  stopped without a certificate                */
+
+/* ---- interval operations (gpugo_ia_eval) ---------------------------------
+ * Binary ops read (a, b); scalar ops read (a, s); unary ops read a only.    */
+enum gpugo_op {
+  GPUGO_OP_ADD   = 0,   /* a + b                         IA.py:57-85 add     */
+  GPUGO_OP_SUB   = 1,   /* a - b                         IA.py:87-115 minus  */
+  GPUGO_OP_MUL   = 2,   /* a * b                         IA.py:117-152       */
+  GPUGO_OP_DIV   = 3,   /* a / b, extended division      IA.py:154-207       */
+  GPUGO_OP_NEG   = 4,   /* -a                                                */
+  GPUGO_OP_SCALE = 5,   /* a * s, any sign of s          IA.py:306-333 times */
+  GPUGO_OP_ADDC  = 6,   /* a + s                         IA.py:335-361 plus  */
+  GPUGO_OP_RSUBC = 7,   /* s - a                                             */
+  GPUGO_OP_DIVC  = 8,   /* a / s, s != 0                                     */
+  GPUGO_OP_SQR   = 9,   /* a^2                           IA.py:209-304 pow 2 */
+  GPUGO_OP_POWN  = 10,  /* a^n, n = (int)s >= 0          IA.py:209-304       */
+  GPUGO_OP_SQRT  = 11,  /* sqrt(a), domain a >= 0        power(., 1/2)       */
+  GPUGO_OP_EXP   = 12,  /* exp(a)                        IA.py:532-570       */
+  GPUGO_OP_SIN   = 13,  /* sin(a)                        IA.py:363-446       */
+  GPUGO_OP_COS   = 14,  /* cos(a)                        IA.py:448-530       */
+  GPUGO_OP_COUNT = 15
+};
+
+enum gpugo_prec {
+  GPUGO_PREC_F64 = 0,    /* FP64 intervals                                  */
+  GPUGO_PREC_F32 = 1     /* FP32 intervals (inputs rounded outward to FP32) */
+};
+
+enum gpugo_round {
+  GPUGO_ROUND_EXACT = 0, /* +,-,*,/,sqrt give the true round-down/round-up
+                            results (bit-identical to IEEE directed rounding) */
+  GPUGO_ROUND_FAST  = 1  /* round-to-nearest then one-ulp outward step:
+                            rigorous, at most one ulp wider per operation     */
+};
+
+/* ---- library ----------------------------------------------------------- */
+int         gpugo_init(int device);          /* select and initialise a GPU */
+const char *gpugo_last_error(void);          /* thread-local, never NULL    */
+int         gpugo_version(void);             /* major*10000+minor*100+patch */
+int         gpugo_device_count(void);
+
+/* Batched interval op on host arrays of n intervals.  Replaces the device
+ * functions of IA.py (see enum gpugo_op for line references).  blo/bhi may
+ * be NULL for unary and scalar ops. */
+int gpugo_ia_eval(int op, int prec, int rounding, int64_t n,
+                  const double *alo, const double *ahi,
+                  const double *blo, const double *bhi, double s,
+                  double *olo, double *ohi);
+
+/* ---- objective modules (hipRTC) -----------------------------------------
+ * A module is one objective's generated HIP source (value, gradient/KKT and
+ * sample code lists plus the fused kernels), compiled for gfx950.
+ * Replaces Numba's @cuda.jit of the per-driver code lists
+ * (1-Ackley/ackley50.py:116-479). */
+typedef struct gpugo_module_s *gpugo_module_t;
+
+int  gpugo_module_compile(const char *source, const char *name,
+                          const char *extra_options, gpugo_module_t *out);
+/* Compile only (no GPU needed, e.g. to fill an on-disk cache ahead of time).
+ * The code object is kept per thread; gpugo_rtc_code_copy copies it out. */
+int  gpugo_rtc_compile(const char *source, const char *name,
+                       const char *extra_options, size_t *code_size);
+int  gpugo_rtc_code_copy(void *buf, size_t size);
+int  gpugo_module_load(const void *code_object, size_t size,
+                       gpugo_module_t *out);
+/* Copy the compiled code object out (for an on-disk cache).  With buf NULL
+ * only *size is set. */
+int  gpugo_module_code(gpugo_module_t m, void *buf, size_t *size);
+void gpugo_module_destroy(gpugo_module_t m);
+
+/* Evaluate the module's objective on n explicit boxes (host arrays, row-major
+ * n x d).  Any output pointer may be NULL to skip it.
+ *   flo/fhi      value enclosure                 (<fn>Value_IA)
+ *   glo/ghi      gradient enclosure, n x d       (<fn>Derivative_IA)
+ *   kkt          1 if the monotonicity test prunes the box
+ *   ub           min over n_samples diagonal points of the upper bound of f
+ *                                                (<fn>Sample_IA)
+ * dom_lo/dom_hi is the initial box used by the monotonicity test's boundary
+ * rule (1-Ackley/ackley50.py:458-477).  rounding is enum gpugo_round. */
+int gpugo_eval_boxes(gpugo_module_t m, int32_t d,
+                     const double *params, int64_t n_params,
+                     int64_t n, const double *lo, const double *hi,
+                     const double *dom_lo, const double *dom_hi,
+                     int32_t n_samples, int32_t rounding,
+                     double *flo, double *fhi, double *glo, double *ghi,
+                     int32_t *kkt, double *ub);
+
+/* ---- problems -------------------------------------------------------------
+ * One box-constrained minimisation.  Replaces the module-level constants and
+ * the while-loop of each driver (1-Ackley/ackley50.py:958-1015, :1033-1380). */
+typedef struct {
+  int32_t       d;          /* dimension                                     */
+  int32_t       wid;        /* subintervals per split coordinate (wid, :527) */
+  int32_t       cycle;      /* coordinates split per step (cycleSize, :530)  */
+  int32_t       k_select;   /* boxes selected per step (reference: 1)        */
+  int32_t       n_samples;  /* diagonal samples per child (numSamples :1015) */
+  int32_t       precision;  /* enum gpugo_prec for the pruning pass          */
+  int32_t       rounding;   /* enum gpugo_round                              */
+  int32_t       reserved0;
+  const double *lo;         /* initial box, d values each (initialRegion)    */
+  const double *hi;
+  const double *params;     /* objective coefficients                        */
+  int64_t       n_params;
+  double        tol_f;      /* certify ub - lb <= tol_f; <= 0 disables       */
+  double        tol_x;      /* reference x-width output rule; <= 0 disables  */
+  double        init_ub;    /* initial upper bound (lowestUpperVal, :994)    */
+  int64_t       max_queue;  /* box-queue capacity (entries)                  */
+  int64_t       max_parents;/* parent-arena capacity (boxes)                 */
+  int64_t       max_iters;  /* iteration cap for gpugo_solve                 */
+} gpugo_problem_desc;
+
+enum gpugo_status {
+  GPUGO_ST_RUNNING     = 0,
+  GPUGO_ST_CERTIFIED   = 1,  /* ub - lb <= tol_f                             */
+  GPUGO_ST_X_CONVERGED = 2,  /* reference rule: queue drained into outputs   */
+  GPUGO_ST_MAX_ITERS   = 3,
+  GPUGO_ST_EXHAUSTED   = 4,  /* queue empty                                  */
+  GPUGO_ST_OVERFLOW    = 5   /* capacity exceeded, no certificate            */
+};
+
+typedef struct {
+  double  lb;               /* certified lower bound of the global minimum   */
+  double  ub;               /* best upper bound found (incumbent)            */
+  double  width;            /* ub - lb, rounded up                           */
+  int32_t status;           /* enum gpugo_status                             */
+  int32_t reserved0;
+  int64_t iterations;       /* steps taken                                   */
+  int64_t boxes_bounded;    /* children that received a value enclosure      */
+  int64_t pruned_value;     /* children pruned by lb > ub                    */
+  int64_t pruned_kkt;       /* children pruned by the monotonicity test      */
+  int64_t queue_size;       /* live boxes after the last step                */
+  int64_t queue_peak;
+  int64_t n_out;            /* boxes in the output list (x-width rule)       */
+  double  ms;               /* wall time inside gpugo_step/gpugo_solve       */
+  double  bound_ms;         /* summed split+bound kernel time (HIP events),
+                               0 unless gpugo_problem_timing enabled it      */
+  int64_t bound_launches;   /* split+bound launches that did work            */
+} gpugo_result;
+
+typedef struct gpugo_problem_s *gpugo_problem_t;
+
+int  gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *desc,
+                          gpugo_problem_t *out);
+/* Return the problem to its initial state (root box, init_ub) keeping all
+ * device allocations; the next step starts a new solve. */
+int  gpugo_problem_reset(gpugo_problem_t p);
+/* enable != 0: time every split+bound launch with HIP events on the
+ * problem's stream (reported in gpugo_result.bound_ms). */
+int  gpugo_problem_timing(gpugo_problem_t p, int enable);
+/* Run up to `iters` steps (fewer if the problem terminates). */
+int  gpugo_step(gpugo_problem_t p, int64_t iters, gpugo_result *res);
+/* Run to termination or desc.max_iters. */
+int  gpugo_solve(gpugo_problem_t p, gpugo_result *res);
+/* Copy live boxes (or, when the x-width rule finished, the output boxes) to
+ * the host: lo/hi are max x d row-major, lb has max entries.  *n receives the
+ * number of boxes available (which may exceed max). */
+int  gpugo_boxes_copy(gpugo_problem_t p, int64_t max, double *lo, double *hi,
+                      double *lb, int64_t *n);
+void gpugo_problem_destroy(gpugo_problem_t p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUGO_H */

@@ -1,0 +1,128 @@
+"""GPU interval ops (gpugo_ia_eval) against the oracle and exact references.
+
+Bar: EXACT mode equals the oracle's true directed rounding bit for bit for
++ - * / sqrt and the scalar ops (the documented exception, nonzero
+products below 2^-968 whose residual may underflow, may be one ulp wider);
+FAST mode contains the oracle's result and is at most one ulp wider per
+endpoint; exp/sin/cos contain the 256-bit range in both modes and stay
+within 4 ulps of it; FP32 results contain the exact result.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests._exact import mp, random_intervals, trig_range, ulps_between
+
+pytestmark = pytest.mark.gpu
+
+RATIONAL = ["add", "sub", "mul", "div", "neg", "scale", "addc", "rsubc", "divc", "sqr", "pown", "sqrt"]
+OPC = {"add": 0, "sub": 1, "mul": 2, "div": 3, "neg": 4, "scale": 5, "addc": 6, "rsubc": 7,
+       "divc": 8, "sqr": 9, "pown": 10, "sqrt": 11, "exp": 12, "sin": 13, "cos": 14}
+TINY = 2.0 ** -968
+
+
+def inputs(op, rng):
+    alo, ahi = random_intervals(rng, 3000)
+    blo, bhi = random_intervals(rng, 3000)
+    blo, bhi = blo[::-1].copy(), bhi[::-1].copy()
+    if op == "sqrt":
+        alo, ahi = np.sort(np.abs(np.stack([alo, ahi])), axis=0)
+    s = {"scale": -2.7, "addc": 1.3, "rsubc": 1.0, "divc": -3.0, "pown": 5}.get(op, 0.0)
+    return alo, ahi, blo, bhi, s
+
+
+def tiny_case(lo, hi):
+    return ((np.abs(lo) < TINY) & (lo != 0)) | ((np.abs(hi) < TINY) & (hi != 0))
+
+
+@pytest.mark.parametrize("op", RATIONAL)
+def test_exact_mode_bit_identical(gpu, oracle, op):
+    rng = np.random.default_rng(OPC[op])
+    alo, ahi, blo, bhi, s = inputs(op, rng)
+    want = oracle.ia_eval(OPC[op], alo, ahi, blo, bhi, s)
+    got = gpu.ia_eval(op, alo, ahi, blo, bhi, s, rounding="exact")
+    same = (got[0] == want[0]) & (got[1] == want[1])
+    # documented exception: one ulp wider where a product underflows
+    exc = tiny_case(want[0], want[1]) | tiny_case(got[0], got[1])
+    assert np.all(same | exc), np.flatnonzero(~(same | exc))[:10]
+    assert np.all(got[0][exc] <= want[0][exc]) and np.all(got[1][exc] >= want[1][exc])
+    assert same.mean() > 0.97
+
+
+@pytest.mark.parametrize("op", RATIONAL)
+def test_fast_mode_contains_and_one_ulp(gpu, oracle, op):
+    rng = np.random.default_rng(100 + OPC[op])
+    alo, ahi, blo, bhi, s = inputs(op, rng)
+    want = oracle.ia_eval(OPC[op], alo, ahi, blo, bhi, s)
+    got = gpu.ia_eval(op, alo, ahi, blo, bhi, s, rounding="fast")
+    assert np.all(got[0] <= want[0]) and np.all(got[1] >= want[1])
+    fin = np.isfinite(want[0]) & np.isfinite(want[1]) & np.isfinite(got[0]) & np.isfinite(got[1])
+    fin &= ~tiny_case(want[0], want[1])
+    per_op = 2 * 5 if op == "pown" else 2   # pown: one step per multiplication
+    for i in np.flatnonzero(fin):
+        assert ulps_between(got[0][i], want[0][i]) <= per_op
+        assert ulps_between(want[1][i], got[1][i]) <= per_op
+
+
+@pytest.mark.parametrize("fn", ["exp", "sin", "cos"])
+@pytest.mark.parametrize("rounding", ["exact", "fast"])
+def test_transcendentals_contain_truth(gpu, oracle, fn, rounding):
+    rng = np.random.default_rng(7)
+    n = 4000
+    c = rng.uniform(-700, 700, n) if fn == "exp" else rng.uniform(-300, 300, n)
+    w = 10.0 ** rng.uniform(-15, 0.8, n)
+    w[: n // 2] = 0.0                       # points: checks the nearest-value error bound
+    lo_in, hi_in = c - w, c + w
+    k = np.arange(-40, 40)
+    if fn != "exp":
+        lo_in[n // 2: n // 2 + 80] = k * math.pi / 2 - 1e-11
+        hi_in[n // 2: n // 2 + 80] = k * math.pi / 2 + 1e-11
+    got = gpu.ia_eval(fn, lo_in, hi_in, rounding=rounding)
+    orc = oracle.ia_eval(OPC[fn], lo_in, hi_in)
+    for i in range(n):
+        if fn == "exp":
+            tlo, thi = mp.exp(mp.mpf(lo_in[i])), mp.exp(mp.mpf(hi_in[i]))
+        else:
+            tlo, thi = trig_range(lo_in[i], hi_in[i], fn)
+        assert mp.mpf(got[0][i]) <= tlo and mp.mpf(got[1][i]) >= thi, (i, lo_in[i], hi_in[i])
+        if tlo not in (0, -1) and got[0][i] > 0 or (tlo < 0 and tlo != -1):
+            assert ulps_between(got[0][i], float(tlo)) <= 4
+        if thi != 1 and math.isfinite(got[1][i]):
+            assert ulps_between(float(thi), got[1][i]) <= 4
+        if math.isfinite(orc[1][i]) and math.isfinite(got[1][i]) and orc[0][i] > 1e-300:
+            assert abs(ulps_between(orc[0][i], got[0][i])) <= 3
+            assert abs(ulps_between(orc[1][i], got[1][i])) <= 3
+
+
+@pytest.mark.parametrize("op", ["add", "mul", "div", "sqr", "sqrt", "scale", "exp", "cos"])
+def test_fp32_contains(gpu, oracle, op):
+    rng = np.random.default_rng(9)
+    alo, ahi, blo, bhi, s = inputs(op, rng)
+    if op in ("exp", "cos"):
+        c = rng.uniform(-20, 20, 2000)
+        alo, ahi = c - 0.01, c + 0.02
+    ok = (np.abs(alo) < 1e30) & (np.abs(ahi) < 1e30) & (np.abs(alo) > 1e-30) & (np.abs(blo) > 1e-30) \
+        & (np.abs(bhi) < 1e30)
+    alo, ahi, blo, bhi = alo[ok], ahi[ok], blo[ok], bhi[ok]
+    for rounding in ("exact", "fast"):
+        got = gpu.ia_eval(op, alo, ahi, blo, bhi, s, prec="fp32", rounding=rounding)
+        if op in ("exp", "cos"):
+            for i in range(alo.size):
+                tlo, thi = ((mp.exp(mp.mpf(alo[i])), mp.exp(mp.mpf(ahi[i]))) if op == "exp"
+                            else trig_range(alo[i], ahi[i], "cos"))
+                assert mp.mpf(got[0][i]) <= tlo and mp.mpf(got[1][i]) >= thi
+        else:
+            want = oracle.ia_eval(OPC[op], alo, ahi, blo, bhi, s)
+            fin = np.isfinite(want[0]) & np.isfinite(want[1])
+            assert np.all(got[0][fin] <= want[0][fin]) and np.all(got[1][fin] >= want[1][fin])
+
+
+def test_domain_errors(gpu):
+    from gpugo_amd import GpugoError
+    with pytest.raises(GpugoError, match="DOMAIN"):
+        gpu.ia_eval("exp", [1.0], [0.0])
+    with pytest.raises(GpugoError, match="DOMAIN"):
+        gpu.ia_eval("divc", [1.0], [2.0], s=0.0)
+    lo, hi = gpu.ia_eval("add", [], [], [], [])
+    assert lo.size == 0

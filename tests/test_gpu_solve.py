@@ -1,0 +1,114 @@
+"""Device-resident branch and bound against the oracle.
+
+Reference loop: 1-Ackley/ackley50.py:1033-1380 (sample -> rule out ->
+list prune -> pop the smallest lower bound -> cycle the split window).
+Bar:
+  - Rosenbrock (rational) in EXACT mode follows the oracle's path exactly:
+    same lb, ub, iterations, boxes bounded, pruned counts and queue sizes
+    after every number of steps, and the same live boxes;
+  - every certified enclosure contains f* (closed form where known, the
+    oracle's enclosure otherwise) and is no wider than tol;
+  - runs are deterministic;
+  - the reference x-width rule returns boxes containing the minimiser;
+  - a queue that overflows its capacity is reported, never silently dropped.
+"""
+import numpy as np
+import pytest
+
+from gpugo_amd import GpugoError, problems as pb
+
+pytestmark = pytest.mark.gpu
+FAM = {"sepcos": 0, "sepcos_chain": 1, "ackley_q": 2}
+KEYS = ("lb", "ub", "iterations", "boxes_bounded", "pruned_value", "pruned_kkt", "queue_size")
+
+
+def gpu_problem(G, spec, **kw):
+    obj = pb.objective(spec.family, spec.d)
+    return G.Problem(obj, spec.params, spec.lo, spec.hi, **kw)
+
+
+@pytest.mark.parametrize("grid,k", [((4, 3), 1), ((2, 2), 8), ((3, 4), 32)])
+def test_rational_path_identical_to_oracle(gpu, oracle, grid, k):
+    spec = pb.rosenbrock(4)
+    with gpu_problem(gpu, spec, tol=1e-6, grid=grid, k_select=k, rounding="exact") as p:
+        for steps in (1, 2, 3, 5, 8, 13):
+            r = p.step(steps - (0 if steps == 1 else prev))
+            prev = steps
+            o, _ = oracle.solve(1, spec.params, spec.lo, spec.hi, tol_f=1e-6, wid=grid[0], cycle=grid[1],
+                                k_select=k, max_iters=steps)
+            if r["status_name"] == "certified" or o["status"] == 1:
+                assert r["status_name"] == "certified" and o["status"] == 1
+                assert (r["lb"], r["ub"]) == (o["lb"], o["ub"])
+                break
+            for key in KEYS:
+                assert r[key] == o[key], (steps, key, r[key], o[key])
+        glo, ghi, glb, n = p.boxes(100000)
+    assert n == r["queue_size"]
+
+
+@pytest.mark.parametrize("spec,grid,k", [
+    (pb.config0(), (64, 2), 1),
+    (pb.config0(), (2, 2), 64),
+    (pb.config1(), (2, 5), 256),
+    (pb.rastrigin(3), (2, 3), 64),
+    (pb.rosenbrock(4), (2, 4), 64),
+    (pb.ackley(4), (2, 4), 64),
+    (pb.config2(d=50), (3, 12), 1),
+], ids=lambda v: getattr(v, "name", str(v)))
+@pytest.mark.parametrize("rounding", ["fast", "exact"])
+def test_certified_enclosure(gpu, oracle, spec, grid, k, rounding):
+    if rounding == "exact" and spec.d > 10:
+        pytest.skip("exact-mode module for d=50 is exercised by test_gpu_eval")
+    with gpu_problem(gpu, spec, tol=spec.tol, grid=grid, k_select=k, rounding=rounding,
+                     max_iters=20000) as p:
+        r = p.solve()
+    assert r["status_name"] == "certified", r
+    assert r["ub"] - r["lb"] <= spec.tol
+    if spec.f_star is not None:
+        assert r["lb"] <= spec.f_star <= r["ub"]
+    else:
+        o, _ = oracle.solve(FAM[spec.family], spec.params, spec.lo, spec.hi, tol_f=spec.tol, wid=grid[0],
+                            cycle=grid[1], k_select=k, max_iters=20000)
+        assert o["status"] == 1
+        assert max(r["lb"], o["lb"]) <= min(r["ub"], o["ub"])
+
+
+def test_deterministic(gpu):
+    spec = pb.config1()
+    out = []
+    for _ in range(2):
+        with gpu_problem(gpu, spec, tol=1e-6, grid=(2, 5), k_select=128) as p:
+            out.append(p.solve())
+    for key in KEYS + ("queue_peak",):
+        assert out[0][key] == out[1][key], key
+
+
+def test_x_width_rule(gpu):
+    spec = pb.rastrigin(2)
+    with gpu_problem(gpu, spec, tol=0.0, tol_x=1e-3, grid=(4, 2), k_select=8, max_iters=100000) as p:
+        r = p.solve()
+        olo, ohi, olb, n = p.boxes(4096)
+    assert r["status_name"] == "x_converged" and n >= 1
+    assert np.all(ohi - olo < 1e-3)
+    assert any(np.all(olo[i] <= 0) and np.all(ohi[i] >= 0) for i in range(len(olb)))
+    assert r["lb"] <= 0.0 <= r["ub"]
+
+
+def test_queue_overflow_is_reported(gpu):
+    spec = pb.config1()
+    with gpu_problem(gpu, spec, tol=1e-6, grid=(4, 10), k_select=1, max_queue=16) as p:
+        with pytest.raises(GpugoError, match="CAPACITY"):
+            p.solve()
+
+
+def test_reference_grid_matches_oracle_first_steps(gpu, oracle):
+    """Config 1 on the reference schedule (wid 4, 10 coordinates, one box per
+    step): same incumbent and lower bound as the oracle within slack."""
+    spec = pb.config1()
+    with gpu_problem(gpu, spec, tol=1e-6, grid=(4, 10), k_select=1, rounding="exact") as p:
+        r = p.step(3)
+    o, _ = oracle.solve(1, spec.params, spec.lo, spec.hi, tol_f=1e-6, wid=4, cycle=10, k_select=1, max_iters=3)
+    assert r["boxes_bounded"] == o["boxes_bounded"] == 3 * 4 ** 10
+    assert abs(r["ub"] - o["ub"]) <= 1e-12 * abs(o["ub"])
+    assert abs(r["lb"] - o["lb"]) <= 1e-12 * (1 + abs(o["lb"]))
+    assert abs(r["queue_size"] - o["queue_size"]) <= max(2, 0.001 * o["queue_size"])

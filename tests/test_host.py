@@ -1,0 +1,151 @@
+"""Host-side product checks that need no GPU.
+
+- the objective families (gpugo_amd.problems) are the functions their
+  definitions say: a generic interpreter of the expression IR, run in
+  256-bit mpmath, is compared with the definitions written independently in
+  tests/test_oracle_objectives.py;
+- the explicit gradient lists equal mpmath's derivative of the value IR;
+- the emitter's symbolic differentiation equals mpmath's derivative;
+- libgpugo.so loads and exports every function include/gpugo.h declares;
+- argument and domain errors are reported before any GPU work.
+"""
+import ctypes
+import math
+import os
+import re
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+from gpugo_amd import _lib, emit, expr as E, problems as pb
+from tests.test_oracle_objectives import f_true
+
+mp.mp.prec = 256
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def ir_eval(e, x, P, memo=None):
+    """Point value of an IR expression in mpmath (generic interpreter)."""
+    if memo is None:
+        memo = {}
+    k = id(e)
+    if k in memo:
+        return memo[k]
+    op = e.op
+    if op == "var":
+        r = x[e.val]
+    elif op == "param":
+        r = mp.mpf(float(P[e.val]))
+    elif op == "const":
+        r = mp.mpf(e.val)
+    elif op == "iconst":
+        lo, hi = e.val
+        r = mp.pi if abs(lo - math.pi) < 1e-15 else mp.e
+    else:
+        a = [ir_eval(t, x, P, memo) for t in e.args]
+        r = {"add": lambda: a[0] + a[1], "sub": lambda: a[0] - a[1], "mul": lambda: a[0] * a[1],
+             "div": lambda: a[0] / a[1], "neg": lambda: -a[0], "sqr": lambda: a[0] ** 2,
+             "pown": lambda: a[0] ** e.val, "sqrt": lambda: mp.sqrt(a[0]), "exp": lambda: mp.exp(a[0]),
+             "sin": lambda: mp.sin(a[0]), "cos": lambda: mp.cos(a[0])}[op]()
+    memo[k] = r
+    return r
+
+
+SPECS = [pb.config0(), pb.config1(), pb.config2(d=7), pb.rastrigin(3), pb.rosenbrock(4), pb.ackley(4)]
+
+
+@pytest.mark.parametrize("spec", SPECS, ids=lambda s: s.name)
+def test_family_ir_matches_definition(spec):
+    obj = pb.objective(spec.family, spec.d)
+    obj.check()
+    rng = np.random.default_rng(0)
+    for _ in range(10):
+        xf = rng.uniform(spec.lo, spec.hi)
+        x = [mp.mpf(float(v)) for v in xf]
+        v = ir_eval(obj.value, x, spec.params)
+        t = f_true(spec.family, spec.params, xf)
+        assert abs(v - t) <= mp.mpf("1e-60") * (1 + abs(t))
+
+
+@pytest.mark.parametrize("spec", SPECS, ids=lambda s: s.name)
+def test_gradient_lists_match_derivative(spec):
+    obj = pb.objective(spec.family, spec.d)
+    rng = np.random.default_rng(1)
+    xf = rng.uniform(spec.lo, spec.hi)
+    x = [mp.mpf(float(v)) for v in xf]
+    for j in range(spec.d):
+        def fj(s, j=j):
+            y = list(x)
+            y[j] = s
+            return ir_eval(obj.value, y, spec.params)
+        want = mp.diff(fj, x[j])
+        got = ir_eval(obj.grad[j], x, spec.params)
+        assert abs(got - want) <= mp.mpf("1e-40") * (1 + abs(want)), j
+
+
+def test_symbolic_diff():
+    x0, x1, x2 = E.X(0), E.X(1), E.X(2)
+    f = E.exp(E.sin(x0 * 1.5) * x1) + E.sqrt(E.sqr(x1) + 2.0) / (x2 + 3.0) - E.pown(x0 - x2, 3) * E.P(0)
+    obj = E.Objective(3, 1, f, None, name="t")
+    grads = emit.gradient(obj)
+    P = [0.7]
+    x = [mp.mpf("0.3"), mp.mpf("-1.2"), mp.mpf("0.9")]
+    for j in range(3):
+        def fj(s, j=j):
+            y = list(x)
+            y[j] = s
+            return ir_eval(f, y, P)
+        assert abs(ir_eval(grads[j], x, P) - mp.diff(fj, x[j])) < mp.mpf("1e-50")
+
+
+def test_emitter_lowering_rules():
+    spec = pb.config2(d=3)
+    src = emit.code_lists(pb.objective(spec.family, spec.d))
+    # A + e enters as an interval constant plus a scalar; 1/d as a positive divisor
+    assert "mk<double>(0x1.5bf0a8b145769p+1, 0x1.5bf0a8b14576ap+1)" in src
+    assert "A::divc_pos(" in src and "A::scale(" in src
+    assert src.count("bool obj_kkt") == 1 and "return true;" in src
+    full = emit.module_source(pb.objective("sepcos", 2), ("exact", "fast"))
+    assert "#define GPUGO_WITH_EXACT 1" in full and "gpugo_bound_fast" in full
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "gpugo.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(gpugo_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_header():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = header_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(_lib.EXPORTS) <= set(names)
+
+
+def test_errors_before_gpu():
+    lib = _lib.load()
+    d = lambda *v: np.array(v, dtype=np.float64)
+    a, b, o1, o2 = d(1.0), d(2.0), d(0.0), d(0.0)
+    P = _lib.ptr
+    assert lib.gpugo_ia_eval(99, 0, 0, 1, P(a), P(b), None, None, 0.0, P(o1), P(o2)) == _lib.E_INVALID
+    assert b"unknown op" in lib.gpugo_last_error()
+    assert lib.gpugo_ia_eval(0, 0, 0, 1, P(a), P(b), None, None, 0.0, P(o1), P(o2)) == _lib.E_INVALID
+    nan = d(float("nan"))
+    assert lib.gpugo_ia_eval(12, 0, 0, 1, P(nan), P(b), None, None, 0.0, P(o1), P(o2)) == _lib.E_DOMAIN
+    assert lib.gpugo_ia_eval(12, 0, 0, 1, P(b), P(a), None, None, 0.0, P(o1), P(o2)) == _lib.E_DOMAIN
+    assert b"inverted" in lib.gpugo_last_error()
+    assert lib.gpugo_ia_eval(8, 0, 0, 1, P(a), P(b), None, None, 0.0, P(o1), P(o2)) == _lib.E_DOMAIN
+    if lib.gpugo_device_count() == 0:
+        assert lib.gpugo_init(0) == _lib.E_NODEV
+
+
+def test_rtc_compile_without_gpu(tmp_path, monkeypatch):
+    """hipRTC builds a module for gfx950 with no GPU present."""
+    import gpugo_amd.solver as S
+    monkeypatch.setattr(S, "_CACHE_DIR", str(tmp_path))
+    path = S.rtc_compile_to_cache(pb.objective("sepcos", 2), roundings=("fast",))
+    data = open(path, "rb").read()
+    assert len(data) > 1000 and data[:4] == b"\x7fELF"
