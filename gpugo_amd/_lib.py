@@ -23,6 +23,7 @@ OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "neg": 4, "scale": 5, "addc": 6, 
        "divc": 8, "sqr": 9, "pown": 10, "sqrt": 11, "exp": 12, "sin": 13, "cos": 14}
 PREC = {"fp64": 0, "fp32": 1}
 ROUND = {"exact": 0, "fast": 1}
+FLAG_MEAN_VALUE = 1
 STATUS = {0: "running", 1: "certified", 2: "x_converged", 3: "max_iters", 4: "exhausted", 5: "overflow"}
 
 EXPORTS = [
@@ -44,7 +45,7 @@ class ProblemDesc(ctypes.Structure):
     _fields_ = [
         ("d", ctypes.c_int32), ("wid", ctypes.c_int32), ("cycle", ctypes.c_int32),
         ("k_select", ctypes.c_int32), ("n_samples", ctypes.c_int32), ("precision", ctypes.c_int32),
-        ("rounding", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("rounding", ctypes.c_int32), ("flags", ctypes.c_int32),
         ("lo", ctypes.POINTER(ctypes.c_double)), ("hi", ctypes.POINTER(ctypes.c_double)),
         ("params", ctypes.POINTER(ctypes.c_double)), ("n_params", ctypes.c_int64),
         ("tol_f", ctypes.c_double), ("tol_x", ctypes.c_double), ("init_ub", ctypes.c_double),

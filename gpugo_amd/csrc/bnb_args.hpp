@@ -46,7 +46,7 @@ struct BoundArgs {
   long long q_cap;
   int d, wid, c, n_samples;
   unsigned int W;                // wid^c children per parent
-  int pad0;
+  int flags;                     // GPUGO_FLAG_* of the problem
   double inv_wid, half_inv_wid;  // exact digit extraction: q = floor(r*inv + half)
 };
 

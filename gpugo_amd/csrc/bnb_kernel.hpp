@@ -33,6 +33,17 @@ __device__ __forceinline__ double sample_ub(const iv<double> *x, const double *_
   return best;
 }
 
+// Monotonicity test on a computed gradient enclosure
+// (1-Ackley/ackley50.py:458-477).
+__device__ __forceinline__ bool kkt_from_grad(const iv<double> *x, const iv<double> *g,
+                                              const double *__restrict__ dlo, const double *__restrict__ dhi) {
+  bool prune = false;
+#pragma unroll
+  for (int j = 0; j < GPUGO_D; ++j)
+    prune |= (g[j].lo > 0.0 && x[j].lo != dlo[j]) || (g[j].hi < 0.0 && x[j].hi != dhi[j]);
+  return prune;
+}
+
 // Child `pos` of parent record `p`: window coordinates (win + k) mod d for
 // k < c take base-wid digit k of pos (least significant first); child
 // bounds are lo + w*digit and lo + w*(digit+1) with one fma each, w the
@@ -76,7 +87,7 @@ __device__ __forceinline__ void decode_child(const BoundArgs &a, long long p, un
   }
 }
 
-template <int R>
+template <int R, bool MV>
 __device__ __forceinline__ void bound_impl(const BoundArgs &a) {
   __shared__ unsigned s_wcnt[4];
   __shared__ unsigned long long s_base;
@@ -114,9 +125,39 @@ __device__ __forceinline__ void bound_impl(const BoundArgs &a) {
       if (lb <= ub0) {
         // a child pruned by value cannot improve the incumbent: its sample
         // bound is >= its lower bound > ub0; so only survivors are sampled
-        su = sample_ub<R>(x, a.params, a.n_samples);
-        keep = !obj_kkt<R>(x, a.params, a.dom_lo, a.dom_hi);
-        pk = !keep;
+        if (MV) {
+          // mean-value form f(c) + sum_j G_j (X_j - c_j) from the gradient
+          // the monotonicity test needs anyway; c is the S = 1 sample point
+          typedef IA<double, R> A;
+          iv<double> g[GPUGO_D], xc[GPUGO_D];
+          obj_grad<R>(x, a.params, g);
+#pragma unroll
+          for (int j = 0; j < GPUGO_D; ++j) {
+            double c = __builtin_fma((x[j].hi - x[j].lo) * 0.5, 1.0, x[j].lo);
+            xc[j] = mk<double>(c, c);
+          }
+          iv<double> s = obj_value<R>(xc, a.params);
+          su = a.n_samples == 1 ? s.hi : sample_ub<R>(x, a.params, a.n_samples);
+#pragma unroll
+          for (int j = 0; j < GPUGO_D; ++j) {
+            iv<double> dx = mk<double>(A::RR::sub_dn(x[j].lo, xc[j].lo), A::RR::sub_up(x[j].hi, xc[j].lo));
+            s = A::add(s, A::mul(g[j], dx));
+          }
+          if (s.lo > lb) lb = s.lo;
+          if (lb <= ub0) {
+            keep = !kkt_from_grad(x, g, a.dom_lo, a.dom_hi);
+            pk = !keep;
+          } else {
+            pv = true;
+          }
+        } else {
+          // monotonicity test before the sample: the gradient reuses the
+          // value code list's temporaries, which then die before the sample
+          // evaluation (fewer live registers)
+          keep = !obj_kkt<R>(x, a.params, a.dom_lo, a.dom_hi);
+          pk = !keep;
+          su = sample_ub<R>(x, a.params, a.n_samples);
+        }
       } else {
         pv = true;
       }
@@ -204,7 +245,10 @@ __device__ __forceinline__ void eval_impl(const EvalArgs &a) {
 // name and reports a missing variant as an error.
 #if GPUGO_WITH_EXACT
 extern "C" __global__ __launch_bounds__(256) void gpugo_bound_exact(gpugo::BoundArgs a) {
-  gpugo::bound_impl<gpugo::ROUND_EXACT>(a);
+  gpugo::bound_impl<gpugo::ROUND_EXACT, false>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void gpugo_bound_exact_mv(gpugo::BoundArgs a) {
+  gpugo::bound_impl<gpugo::ROUND_EXACT, true>(a);
 }
 extern "C" __global__ __launch_bounds__(64) void gpugo_eval_exact(gpugo::EvalArgs a) {
   gpugo::eval_impl<gpugo::ROUND_EXACT>(a);
@@ -212,7 +256,10 @@ extern "C" __global__ __launch_bounds__(64) void gpugo_eval_exact(gpugo::EvalArg
 #endif
 #if GPUGO_WITH_FAST
 extern "C" __global__ __launch_bounds__(256) void gpugo_bound_fast(gpugo::BoundArgs a) {
-  gpugo::bound_impl<gpugo::ROUND_FAST>(a);
+  gpugo::bound_impl<gpugo::ROUND_FAST, false>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void gpugo_bound_fast_mv(gpugo::BoundArgs a) {
+  gpugo::bound_impl<gpugo::ROUND_FAST, true>(a);
 }
 extern "C" __global__ __launch_bounds__(64) void gpugo_eval_fast(gpugo::EvalArgs a) {
   gpugo::eval_impl<gpugo::ROUND_FAST>(a);

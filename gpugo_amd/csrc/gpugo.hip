@@ -208,6 +208,7 @@ extern "C" int gpugo_ia_eval(int op, int prec, int rounding, int64_t n, const do
 struct gpugo_module_s {
   hipModule_t mod = nullptr;
   hipFunction_t bound[2] = {nullptr, nullptr};  // [rounding]
+  hipFunction_t bound_mv[2] = {nullptr, nullptr};  // [rounding], mean-value variant
   hipFunction_t eval[2] = {nullptr, nullptr};
   int dim = 0, nparams = 0, abi = 0;
   std::vector<char> code;
@@ -232,10 +233,12 @@ static int module_finish(gpugo_module_t m) {
     return set_err(GPUGO_E_INVALID, "module ABI %d, library expects %d", m->abi, GPUGO_MODULE_ABI);
   if (h[3] & 1) {
     HIPCHECK(hipModuleGetFunction(&m->bound[0], m->mod, "gpugo_bound_exact"));
+    HIPCHECK(hipModuleGetFunction(&m->bound_mv[0], m->mod, "gpugo_bound_exact_mv"));
     HIPCHECK(hipModuleGetFunction(&m->eval[0], m->mod, "gpugo_eval_exact"));
   }
   if (h[3] & 2) {
     HIPCHECK(hipModuleGetFunction(&m->bound[1], m->mod, "gpugo_bound_fast"));
+    HIPCHECK(hipModuleGetFunction(&m->bound_mv[1], m->mod, "gpugo_bound_fast_mv"));
     HIPCHECK(hipModuleGetFunction(&m->eval[1], m->mod, "gpugo_eval_fast"));
   }
   return GPUGO_OK;
@@ -971,6 +974,7 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
     return set_err(GPUGO_E_INVALID, "k_select must be in [1,%d]", SEL_KMAX);
   if (desc->n_samples < 1 || desc->n_samples > 64) return set_err(GPUGO_E_INVALID, "n_samples must be in [1,64]");
   if (desc->precision != GPUGO_PREC_F64) return set_err(GPUGO_E_INVALID, "only GPUGO_PREC_F64 problems are supported");
+  if (desc->flags & ~GPUGO_FLAG_MEAN_VALUE) return set_err(GPUGO_E_INVALID, "unknown flags 0x%x", desc->flags);
   if (desc->rounding != GPUGO_ROUND_EXACT && desc->rounding != GPUGO_ROUND_FAST)
     return set_err(GPUGO_E_INVALID, "unknown rounding %d", desc->rounding);
   if (!m->bound[desc->rounding])
@@ -1154,6 +1158,7 @@ static int enqueue_step(gpugo_problem_t p) {
   ba.W = p->W;
   ba.inv_wid = 1.0 / (double)p->desc.wid;
   ba.half_inv_wid = 0.5 / (double)p->desc.wid;
+  ba.flags = p->desc.flags;
   {
     long long maxk = p->steps_enqueued == 0 ? 1 : p->desc.k_select;
     long long nthreads = maxk * (long long)p->W;
@@ -1168,8 +1173,9 @@ static int enqueue_step(gpugo_problem_t p) {
       }
       HIPCHECK(hipEventRecord(p->tev[p->tev_used++], p->stream));
     }
-    HIPCHECK(hipModuleLaunchKernel(p->m->bound[p->desc.rounding], (unsigned)blocks, 1, 1, 256, 1, 1, 0, p->stream,
-                                   args, nullptr));
+    hipFunction_t fn = (p->desc.flags & GPUGO_FLAG_MEAN_VALUE) ? p->m->bound_mv[p->desc.rounding]
+                                                               : p->m->bound[p->desc.rounding];
+    HIPCHECK(hipModuleLaunchKernel(fn, (unsigned)blocks, 1, 1, 256, 1, 1, 0, p->stream, args, nullptr));
     if (p->timing) HIPCHECK(hipEventRecord(p->tev[p->tev_used++], p->stream));
   }
   CompactArgs ca;

@@ -130,14 +130,15 @@ class Problem:
     def __init__(self, obj: Objective, params, lo, hi, *, tol: float = 1e-6, tol_x: float = 0.0,
                  grid: Sequence[int] = (4, 10), k_select: int = 1, n_samples: int = 1,
                  init_ub: float = np.inf, rounding: str = "fast", max_queue: int = 0,
-                 max_parents: int = 0, max_iters: int = 0, module: Optional[Module] = None):
+                 max_parents: int = 0, max_iters: int = 0, module: Optional[Module] = None,
+                 mean_value: bool = False):
         self.module = module or get_module(obj, roundings=(rounding,))
         self.obj = obj
         self._lo, self._hi, self._params = f64(lo), f64(hi), f64(params)
         wid, cycle = grid
         self.desc = _lib.ProblemDesc(
             d=obj.d, wid=int(wid), cycle=int(cycle), k_select=int(k_select), n_samples=int(n_samples),
-            precision=0, rounding=_lib.ROUND[rounding], reserved0=0, lo=ptr(self._lo), hi=ptr(self._hi),
+            precision=0, rounding=_lib.ROUND[rounding], flags=_lib.FLAG_MEAN_VALUE if mean_value else 0, lo=ptr(self._lo), hi=ptr(self._hi),
             params=ptr(self._params), n_params=self._params.size, tol_f=float(tol), tol_x=float(tol_x),
             init_ub=float(init_ub), max_queue=int(max_queue), max_parents=int(max_parents),
             max_iters=int(max_iters))

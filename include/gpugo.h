@@ -134,6 +134,12 @@ int gpugo_eval_boxes(gpugo_module_t m, int32_t d,
                      double *flo, double *fhi, double *glo, double *ghi,
                      int32_t *kkt, double *ub);
 
+/* gpugo_problem_desc.flags */
+#define GPUGO_FLAG_MEAN_VALUE 1  /* also bound each child by the mean-value form
+                                    f(c) + sum_j G_j (X_j - c_j), c the midpoint,
+                                    G the gradient enclosure (not in the
+                                    reference; tightens lower bounds)          */
+
 /* ---- problems -------------------------------------------------------------
  * One box-constrained minimisation.  Replaces the module-level constants and
  * the while-loop of each driver (1-Ackley/ackley50.py:958-1015, :1033-1380). */
@@ -145,7 +151,7 @@ typedef struct {
   int32_t       n_samples;  /* diagonal samples per child (numSamples :1015) */
   int32_t       precision;  /* enum gpugo_prec for the pruning pass          */
   int32_t       rounding;   /* enum gpugo_round                              */
-  int32_t       reserved0;
+  int32_t       flags;      /* GPUGO_FLAG_* bits                             */
   const double *lo;         /* initial box, d values each (initialRegion)    */
   const double *hi;
   const double *params;     /* objective coefficients                        */

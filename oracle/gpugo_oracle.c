@@ -376,15 +376,39 @@ static void family_grad(int fam, int d, const double *P, const iv *x, iv *g)
 /* Monotonicity test (1-Ackley/ackley50.py:458-477): prune when a gradient
  * component has constant sign and the box does not touch the initial box on
  * the side where the minimum over the box would lie. */
+static int kkt_test(int d, const iv *x, const double *dlo, const double *dhi, const iv *g)
+{
+  for (int j = 0; j < d; ++j) {
+    if (g[j].lo > 0 && x[j].lo != dlo[j]) return 1;
+    if (g[j].hi < 0 && x[j].hi != dhi[j]) return 1;
+  }
+  return 0;
+}
+
 static int family_kkt(int fam, int d, const double *P, const iv *x,
                       const double *dlo, const double *dhi, iv *gbuf)
 {
   family_grad(fam, d, P, x, gbuf);
+  return kkt_test(d, x, dlo, dhi, gbuf);
+}
+
+/* Mean-value enclosure f(X) in f(c) + sum_j G_j (X_j - c_j), c the box
+ * midpoint (computed in round-to-nearest), G an enclosure of the gradient
+ * over X.  Returns its lower bound (FE_UPWARD held by the caller). */
+static double family_mv_lower(int fam, int d, const double *P, const iv *x, const iv *g, iv *xs)
+{
+  fesetround(FE_TONEAREST);
   for (int j = 0; j < d; ++j) {
-    if (gbuf[j].lo > 0 && x[j].lo != dlo[j]) return 1;
-    if (gbuf[j].hi < 0 && x[j].hi != dhi[j]) return 1;
+    double c = fma((x[j].hi - x[j].lo) / 2.0, 1.0, x[j].lo);
+    xs[j] = mk(c, c);
   }
-  return 0;
+  fesetround(FE_UPWARD);
+  iv s = family_value(fam, d, P, xs);
+  for (int j = 0; j < d; ++j) {
+    iv dx = mk(rd_sub(x[j].lo, xs[j].lo), ru_sub(x[j].hi, xs[j].lo));
+    s = iv_add(s, iv_mul(g[j], dx));
+  }
+  return s.lo;
 }
 
 /* Round-to-nearest section: sample points along the diagonal
@@ -665,7 +689,7 @@ int oracle_solve(const oracle_problem *pr, gpugo_result *res,
 #ifdef _OPENMP
       tid = omp_get_thread_num();
 #endif
-      iv *x = malloc(sizeof(iv) * d), *g = malloc(sizeof(iv) * d);
+      iv *x = malloc(sizeof(iv) * d), *g = malloc(sizeof(iv) * d), *xs = malloc(sizeof(iv) * d);
       double *bl = malloc(sizeof(double) * d), *bh = malloc(sizeof(double) * d);
       int oldt = fegetround();
 #pragma omp for schedule(static)
@@ -680,7 +704,15 @@ int oracle_solve(const oracle_problem *pr, gpugo_result *res,
         double lb = f.lo;
         if (lb != lb) lb = -INFINITY;
         if (lb > ub) { ++pv; continue; }
-        if (family_kkt(fam, d, pr->params, x, pr->lo, pr->hi, g)) { ++pk; continue; }
+        if (pr->flags & ORACLE_MEAN_VALUE) {
+          /* second-order lower bound from the gradient already needed for
+           * the monotonicity test */
+          family_grad(fam, d, pr->params, x, g);
+          double lmv = family_mv_lower(fam, d, pr->params, x, g, xs);
+          if (lmv > lb) lb = lmv;
+          if (lb > ub) { ++pv; continue; }
+          if (kkt_test(d, x, pr->lo, pr->hi, g)) { ++pk; continue; }
+        } else if (family_kkt(fam, d, pr->params, x, pr->lo, pr->hi, g)) { ++pk; continue; }
         if (tcnt[tid] == tcap[tid]) {
           int64_t ncap = tcap[tid] ? tcap[tid] * 2 : 4096;
           qent *nb = realloc(tbuf[tid], sizeof(qent) * ncap);
@@ -691,7 +723,7 @@ int oracle_solve(const oracle_problem *pr, gpugo_result *res,
         tbuf[tid][tcnt[tid]++] = e;
       }
       fesetround(oldt);
-      free(x); free(g); free(bl); free(bh);
+      free(x); free(g); free(xs); free(bl); free(bh);
     }
     if (alloc_fail) { rc = -2; goto done; }
     pruned_v += pv; pruned_k += pk;

@@ -4,8 +4,11 @@
 #include <stdint.h>
 #include "../include/gpugo.h"
 
+/* oracle_problem.flags */
+#define ORACLE_MEAN_VALUE 1   /* also bound by the mean-value form */
+
 typedef struct {
-  int32_t d, family, wid, cycle, k_select, n_samples, threads, reserved;
+  int32_t d, family, wid, cycle, k_select, n_samples, threads, flags;
   const double *lo, *hi, *params;
   double tol_f, tol_x, init_ub;
   int64_t max_iters;

@@ -40,7 +40,7 @@ class OracleProblem(ctypes.Structure):
     _fields_ = [
         ("d", ctypes.c_int32), ("family", ctypes.c_int32), ("wid", ctypes.c_int32),
         ("cycle", ctypes.c_int32), ("k_select", ctypes.c_int32), ("n_samples", ctypes.c_int32),
-        ("threads", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("threads", ctypes.c_int32), ("flags", ctypes.c_int32),
         ("lo", ctypes.POINTER(ctypes.c_double)), ("hi", ctypes.POINTER(ctypes.c_double)),
         ("params", ctypes.POINTER(ctypes.c_double)),
         ("tol_f", ctypes.c_double), ("tol_x", ctypes.c_double), ("init_ub", ctypes.c_double),
@@ -49,7 +49,7 @@ class OracleProblem(ctypes.Structure):
 
 
 def build(force: bool = False) -> str:
-    """Compile the oracle with its Makefile (gcc)."""
+    """Compile the oracle with its Makefile (host clang, strict FP)."""
     if force or not os.path.exists(_LIB_PATH):
         subprocess.run(["make", "-s", "-C", _HERE], check=True)
     return _LIB_PATH
@@ -151,12 +151,13 @@ def split_bound(family, params, plo, phi, pwin, wid, cycle, n_samples, dom_lo, d
 
 
 def solve(family: int, params, lo, hi, *, tol_f=1e-6, tol_x=0.0, wid=4, cycle=10, k_select=1,
-          n_samples=1, init_ub=float("inf"), max_iters=1 << 40, threads=0, out_max=4096):
+          n_samples=1, init_ub=float("inf"), max_iters=1 << 40, threads=0, out_max=4096,
+          mean_value=False):
     """Run the oracle branch-and-bound; returns (result dict, output boxes)."""
     lo, hi, params = _f64(lo), _f64(hi), _f64(params)
     d = lo.size
     pr = OracleProblem(d=d, family=family, wid=wid, cycle=cycle, k_select=k_select,
-                       n_samples=n_samples, threads=threads, reserved=0,
+                       n_samples=n_samples, threads=threads, flags=1 if mean_value else 0,
                        lo=_p(lo), hi=_p(hi), params=_p(params), tol_f=tol_f, tol_x=tol_x,
                        init_ub=init_ub, max_iters=max_iters)
     res = GpugoResult()
