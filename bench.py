@@ -35,9 +35,9 @@ sys.path.insert(0, ROOT)
 METRIC = "time-to-certified enclosure (tol 1e-6) + boxes bounded/s; served p50/p99 latency"
 FP64_PEAK_TFLOPS = 78.6
 SCHEDULES = {
-    # (wid, cycle) subregion grid and boxes selected per step
-    "tuned": {"grid": (2, 5), "k_select": 256},
-    "reference": {"grid": (4, 10), "k_select": 1},
+    # (wid, cycle) subregion grid, boxes selected per step, mean-value bound
+    "tuned": {"grid": (4, 5), "k_select": 256, "mean_value": True},
+    "reference": {"grid": (4, 10), "k_select": 1, "mean_value": False},
 }
 
 
@@ -50,7 +50,20 @@ def parse():
     ap.add_argument("--rounding", choices=["fast", "exact"], default="fast")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES (scales HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_probe(nbytes):
+    """A plain copy kernel of known size, for scaling FETCH_SIZE/WRITE_SIZE."""
+    import torch
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}))
 
 
 def cpu_baseline(spec, sched, seconds):
@@ -66,7 +79,7 @@ def cpu_baseline(spec, sched, seconds):
         t0 = time.perf_counter()
         res, _ = O.solve(fam, spec.params, spec.lo, spec.hi, tol_f=spec.tol, wid=sched["grid"][0],
                          cycle=sched["grid"][1], k_select=sched["k_select"], n_samples=1,
-                         max_iters=100000, threads=threads)
+                         max_iters=100000, threads=threads, mean_value=sched["mean_value"])
         times.append(time.perf_counter() - t0)
         boxes.append(res["boxes_bounded"])
         if times[-1] > seconds:
@@ -92,6 +105,9 @@ def cpu_reference_schedule(spec, steps=3):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(problems shard by independent boxes/problems, see DESIGN.md)", file=sys.stderr)
@@ -104,7 +120,8 @@ def main():
     obj = pb.objective(spec.family, spec.d)
     G.init(0)
     prob = G.Problem(obj, spec.params, spec.lo, spec.hi, tol=spec.tol, grid=sched["grid"],
-                     k_select=sched["k_select"], n_samples=1, rounding=args.rounding, max_iters=200000)
+                     k_select=sched["k_select"], n_samples=1, rounding=args.rounding, max_iters=200000,
+                     mean_value=sched["mean_value"])
     for _ in range(args.warmup):
         prob.reset()
         prob.solve()
@@ -127,7 +144,11 @@ def main():
     launches = sum(x["bound_launches"] for x in results)
 
     fl = emit.flop_counts(obj)
-    flops = boxes * fl["value"] + (boxes - pv) * (fl["sample"] + fl["grad"])
+    # survivors of the value test also get the sample and the gradient; with
+    # the mean-value bound they also get the midpoint value and d interval
+    # products and additions (10 FLOP each)
+    extra = fl["sample"] + fl["grad"] + (10 * spec.d if sched["mean_value"] else 0)
+    flops = boxes * fl["value"] + (boxes - pv) * extra
     achieved = flops / (bound_ms * 1e-3) / 1e12 if bound_ms > 0 else 0.0
 
     out = {
@@ -145,15 +166,16 @@ def main():
         "data": "synthetic (seeded coefficients)",
         "config": {"workload": "config1: sepcos_chain d=10, box [-5,10]^10, tol 1e-6, seed 1",
                    "schedule": args.schedule, "grid": list(sched["grid"]),
-                   "k_select": sched["k_select"], "n_samples": 1, "rounding": args.rounding},
+                   "k_select": sched["k_select"], "n_samples": 1, "rounding": args.rounding,
+                   "mean_value": sched["mean_value"]},
         "time_to_certified_s": per_solve,
         "boxes_bounded_per_s": boxes / (t1 - t0),
         "boxes_bounded_per_solve": boxes // args.steps,
         "iterations_per_solve": r["iterations"],
         "certified": certified,
         "lb": r["lb"], "ub": r["ub"], "width": r["width"],
-        "roofline": {"bound": "valu", "kernel": "gpugo_bound_fast" if args.rounding == "fast"
-                     else "gpugo_bound_exact",
+        "roofline": {"bound": "valu",
+                     "kernel": f"gpugo_bound_{args.rounding}" + ("_mv" if sched["mean_value"] else ""),
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "flop_per_launch": flops / max(launches, 1),

@@ -27,6 +27,8 @@ struct DevState {
   long long n_out;               // output boxes (x-width rule)
   long long queue_peak;
   long long compactions;         // compaction passes run (selects the live buffer)
+  unsigned long long lbmin_key;  // smallest / largest live lower bound after the
+  unsigned long long lbmax_key;  // last compaction (order keys)
   int n_split;                   // boxes split in this step
   int status;                    // gpugo_status
   int overflow;                  // 1: queue append, 2: parent arena, 4: output arena

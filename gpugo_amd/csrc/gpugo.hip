@@ -515,33 +515,94 @@ __device__ __forceinline__ u128 qkey(const double *lb, const unsigned *par, cons
   return ((u128)order_key(lb[i]) << 64) | ((u128)par[i] << 32) | (u128)pos[i];
 }
 
-__device__ void bitonic_sort(u128 *k, unsigned *v, int n) {
-  for (int size = 2; size <= n; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      __syncthreads();
-      int t = threadIdx.x;
-      if (t < (n >> 1)) {
-        int i = (t / stride) * 2 * stride + (t % stride);
-        int j = i + stride;
-        bool up = (i & size) == 0;
-        u128 ki = k[i], kj = k[j];
-        if ((ki > kj) == up) {
-          k[i] = kj;
-          k[j] = ki;
-          unsigned tv = v[i];
-          v[i] = v[j];
-          v[j] = tv;
-        }
+// bin of a lower bound in [lmin, lmax]: monotone non-decreasing in lb
+__device__ __forceinline__ unsigned value_bin(double lb, double lmin, double scale) {
+  double t = (lb - lmin) * scale;
+  int b = (int)t;
+  return (unsigned)(b < 0 ? 0 : (b >= SEL_BINS ? SEL_BINS - 1 : b));
+}
+
+// Radix refinement on the 128-bit keys (12-bit digits) until the bin that
+// holds the K-th smallest key has at most SEL_FINAL entries or fills the
+// remaining slots exactly; then keys below that bin go to skey/sidx and the
+// bin's keys to ckey/cidx.  Used when the value histogram cannot separate
+// the keys (many equal lower bounds).
+__device__ void radix_gather(const SelectArgs &a, long long N, int K, unsigned *hist, u128 *skey, unsigned *sidx,
+                             u128 *ckey, unsigned *cidx, unsigned *scan_sh, u128 &s_pfx, int &s_pbits,
+                             unsigned &s_krem, unsigned &s_region, unsigned &s_b, unsigned &s_cb, unsigned &s_ns,
+                             unsigned &s_nc) {
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    s_pfx = 0;
+    s_pbits = 0;
+    s_krem = (unsigned)K;
+  }
+  __syncthreads();
+  for (;;) {
+    const int pbits = s_pbits;
+    const int db = 128 - pbits < 12 ? 128 - pbits : 12;
+    const u128 pfx = s_pfx;
+    const unsigned krem = s_krem;
+    const int sh = 128 - pbits - db;
+    for (int b = tid; b < SEL_BINS; b += SEL_T) hist[b] = 0;
+    __syncthreads();
+    for (long long i = tid; i < N; i += SEL_T) {
+      u128 k = qkey(a.q_lb, a.q_par, a.q_pos, i);
+      bool m = (pbits == 0) || ((k >> (128 - pbits)) == (pfx >> (128 - pbits)));
+      if (m) atomicAdd(&hist[(unsigned)(k >> sh) & ((1u << db) - 1u)], 1u);
+    }
+    __syncthreads();
+    unsigned h[4], ts = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      h[q] = hist[4 * tid + q];
+      ts += h[q];
+    }
+    unsigned tot;
+    unsigned cum = block_excl_scan(ts, scan_sh, &tot);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (cum < krem && krem <= cum + h[q]) {
+        s_b = 4 * tid + q;
+        s_cb = cum;
+        s_region = h[q];
+      }
+      cum += h[q];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      s_krem = krem - s_cb;
+      s_pfx = pfx | ((u128)s_b << sh);
+      s_pbits = pbits + db;
+    }
+    __syncthreads();
+    if (s_region == s_krem || s_region <= SEL_FINAL || s_pbits >= 128) break;
+  }
+  if (tid == 0) {
+    s_ns = 0;
+    s_nc = 0;
+  }
+  __syncthreads();
+  const int shp = 128 - s_pbits;
+  const u128 ptop = s_pfx >> shp;
+  for (long long i = tid; i < N; i += SEL_T) {
+    u128 k = qkey(a.q_lb, a.q_par, a.q_pos, i);
+    u128 top = k >> shp;
+    if (top < ptop) {
+      unsigned slot = atomicAdd(&s_ns, 1u);
+      if (slot < SEL_KMAX) {
+        skey[slot] = k;
+        sidx[slot] = (unsigned)i;
+      }
+    } else if (top == ptop) {
+      unsigned slot = atomicAdd(&s_nc, 1u);
+      if (slot < SEL_FINAL) {
+        ckey[slot] = k;
+        cidx[slot] = (unsigned)i;
       }
     }
   }
   __syncthreads();
-}
-
-__device__ __forceinline__ int pow2_ceil(int n) {
-  int p = 1;
-  while (p < n) p <<= 1;
-  return p;
 }
 
 __global__ __launch_bounds__(SEL_T) void k_select(SelectArgs a) {
@@ -593,25 +654,18 @@ __global__ __launch_bounds__(SEL_T) void k_select(SelectArgs a) {
     }
     __syncthreads();
   } else {
-    if (tid == 0) {
-      s_pfx = 0;
-      s_pbits = 0;
-      s_krem = (unsigned)K;
-    }
-    __syncthreads();
-    for (;;) {
-      const int pbits = s_pbits;
-      const int db = 128 - pbits < 12 ? 128 - pbits : 12;
-      const u128 pfx = s_pfx;
-      const unsigned krem = s_krem;
-      const int sh = 128 - pbits - db;
+    // Fast path: one histogram pass over the live range [lbmin, lbmax]
+    // (tracked by the compaction).  The bin map is monotone in lb, so every
+    // key in a bin below the boundary bin is smaller than every key in it.
+    bool fast_done = false;
+    const double lmin = key_value(st->lbmin_key), lmax = key_value(st->lbmax_key);
+    const double span = lmax - lmin;
+    if (lmin > -__builtin_huge_val() && lmax < __builtin_huge_val() && span > 0) {
+      const double scale = (double)SEL_BINS / span;
       for (int b = tid; b < SEL_BINS; b += SEL_T) hist[b] = 0;
       __syncthreads();
-      for (long long i = tid; i < N; i += SEL_T) {
-        u128 k = qkey(a.q_lb, a.q_par, a.q_pos, i);
-        bool m = (pbits == 0) || ((k >> (128 - pbits)) == (pfx >> (128 - pbits)));
-        if (m) atomicAdd(&hist[(unsigned)(k >> sh) & ((1u << db) - 1u)], 1u);
-      }
+      for (long long i = tid; i < N; i += SEL_T)
+        atomicAdd(&hist[value_bin(a.q_lb[i], lmin, scale)], 1u);
       __syncthreads();
       unsigned h[4], ts = 0;
 #pragma unroll
@@ -623,7 +677,7 @@ __global__ __launch_bounds__(SEL_T) void k_select(SelectArgs a) {
       unsigned cum = block_excl_scan(ts, scan_sh, &tot);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        if (cum < krem && krem <= cum + h[q]) {
+        if (cum < (unsigned)K && (unsigned)K <= cum + h[q]) {
           s_b = 4 * tid + q;
           s_cb = cum;
           s_region = h[q];
@@ -631,70 +685,71 @@ __global__ __launch_bounds__(SEL_T) void k_select(SelectArgs a) {
         cum += h[q];
       }
       __syncthreads();
-      if (tid == 0) {
-        s_krem = krem - s_cb;
-        s_pfx = pfx | ((u128)s_b << sh);
-        s_pbits = pbits + db;
-      }
-      __syncthreads();
-      if (s_region == s_krem || s_region <= SEL_FINAL || s_pbits >= 128) break;
-    }
-    // gather: keys strictly below the chosen prefix are selected, keys with
-    // the prefix are candidates for the remaining s_krem slots
-    if (tid == 0) {
-      s_ns = 0;
-      s_nc = 0;
-    }
-    __syncthreads();
-    {
-      const int shp = 128 - s_pbits;
-      const u128 ptop = s_pfx >> shp;
-      for (long long i = tid; i < N; i += SEL_T) {
-        u128 k = qkey(a.q_lb, a.q_par, a.q_pos, i);
-        u128 top = k >> shp;
-        if (top < ptop) {
-          unsigned slot = atomicAdd(&s_ns, 1u);
-          if (slot < SEL_KMAX) {
-            skey[slot] = k;
-            sidx[slot] = (unsigned)i;
-          }
-        } else if (top == ptop) {
-          unsigned slot = atomicAdd(&s_nc, 1u);
-          if (slot < SEL_FINAL) {
-            ckey[slot] = k;
-            cidx[slot] = (unsigned)i;
+      if (s_region <= SEL_FINAL || s_region == (unsigned)K - s_cb) {
+        if (tid == 0) {
+          s_ns = 0;
+          s_nc = 0;
+          s_krem = (unsigned)K - s_cb;
+        }
+        __syncthreads();
+        const unsigned bb = s_b;
+        for (long long i = tid; i < N; i += SEL_T) {
+          const unsigned bin = value_bin(a.q_lb[i], lmin, scale);
+          if (bin < bb) {
+            unsigned slot = atomicAdd(&s_ns, 1u);
+            if (slot < SEL_KMAX) {
+              skey[slot] = qkey(a.q_lb, a.q_par, a.q_pos, i);
+              sidx[slot] = (unsigned)i;
+            }
+          } else if (bin == bb) {
+            unsigned slot = atomicAdd(&s_nc, 1u);
+            if (slot < SEL_FINAL) {
+              ckey[slot] = qkey(a.q_lb, a.q_par, a.q_pos, i);
+              cidx[slot] = (unsigned)i;
+            }
           }
         }
+        __syncthreads();
+        fast_done = true;
       }
     }
-    __syncthreads();
+    if (!fast_done) radix_gather(a, N, K, hist, skey, sidx, ckey, cidx, scan_sh, s_pfx, s_pbits, s_krem, s_region,
+                                 s_b, s_cb, s_ns, s_nc);
+    // the s_krem smallest candidates fill the remaining slots (rank by
+    // counting: keys are unique)
     const unsigned ns = s_ns, nc = s_nc, krem = s_krem;
-    if (nc > krem) {
-      const int n2 = pow2_ceil((int)nc);
-      for (int i = tid; i < n2; i += SEL_T)
-        if (i >= (int)nc) {
-          ckey[i] = ~(u128)0;
-          cidx[i] = 0xffffffffu;
-        }
-      bitonic_sort(ckey, cidx, n2);
-    }
-    for (unsigned i = tid; i < krem; i += SEL_T) {
-      skey[ns + i] = ckey[i];
-      sidx[ns + i] = cidx[i];
+    if (tid < (int)nc) {
+      const u128 mine = ckey[tid];
+      unsigned r = 0;
+      for (unsigned j = 0; j < nc; ++j) r += ckey[j] < mine;
+      if (r < krem) {
+        skey[ns + r] = mine;
+        sidx[ns + r] = cidx[tid];
+      }
     }
     __syncthreads();
   }
-  // deterministic order of the selection
+  // deterministic order of the selection: rank by counting into ckey/cidx
   {
-    const int n2 = pow2_ceil(K);
-    for (int i = tid; i < n2; i += SEL_T)
-      if (i >= K) {
-        skey[i] = ~(u128)0;
-        sidx[i] = 0xffffffffu;
-      }
-    bitonic_sort(skey, sidx, n2);
+    u128 mine = 0;
+    unsigned myidx = 0, r = 0;
+    if (tid < K) {
+      mine = skey[tid];
+      myidx = sidx[tid];
+      for (int j = 0; j < K; ++j) r += skey[j] < mine;
+    }
+    __syncthreads();
+    if (tid < K) {
+      ckey[r] = mine;
+      cidx[r] = myidx;
+    }
+    __syncthreads();
+    if (tid < K) {
+      skey[tid] = ckey[tid];
+      sidx[tid] = cidx[tid];
+    }
+    __syncthreads();
   }
-
   // stop test: certified f-enclosure
   const double lb0 = a.q_lb[sidx[0]];
   const double lbg = lb0 < st->out_min_lb ? lb0 : st->out_min_lb;
@@ -806,6 +861,8 @@ __global__ __launch_bounds__(SEL_T) void k_select(SelectArgs a) {
     st->ub_snap = ub;
     st->q_count = N;
     st->compact_count = 0;
+    st->lbmin_key = ~0ull;
+    st->lbmax_key = 0ull;
   }
 }
 
@@ -828,10 +885,17 @@ __global__ __launch_bounds__(256) void k_compact(CompactArgs a) {
   // after a stop, exactly one more pass runs so the live set lands in the
   // buffer that st->compactions designates
   if (st->status != GPUGO_ST_RUNNING && !st->compact_final) return;
+  __shared__ unsigned long long s_kmin, s_kmax;
   long long n = st->q_count;
   if (n > a.cap) n = a.cap;
   const double ub = key_value(st->ub_key);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // range of the kept lower bounds (order keys; feeds the next select)
+  unsigned long long kmin = ~0ull, kmax = 0ull;
+  if (threadIdx.x == 0) {
+    s_kmin = ~0ull;
+    s_kmax = 0ull;
+  }
   for (long long base = (long long)blockIdx.x * 256; base < n; base += (long long)gridDim.x * 256) {
     long long i = base + threadIdx.x;
     double lb = 0;
@@ -839,6 +903,11 @@ __global__ __launch_bounds__(256) void k_compact(CompactArgs a) {
     if (i < n) {
       lb = a.lb_in[i];
       keep = lb <= ub;
+      if (keep) {
+        const unsigned long long k = order_key(lb);
+        kmin = k < kmin ? k : kmin;
+        kmax = k > kmax ? k : kmax;
+      }
     }
     unsigned long long mask = __ballot(keep);
     unsigned pre = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
@@ -858,6 +927,21 @@ __global__ __launch_bounds__(256) void k_compact(CompactArgs a) {
       a.pos_out[o] = a.pos_in[i];
     }
     __syncthreads();
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long t0 = __shfl_xor(kmin, o, 64), t1 = __shfl_xor(kmax, o, 64);
+    kmin = t0 < kmin ? t0 : kmin;
+    kmax = t1 > kmax ? t1 : kmax;
+  }
+  if (lane == 0 && kmin <= kmax) {
+    atomicMin(&s_kmin, kmin);
+    atomicMax(&s_kmax, kmax);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_kmin <= s_kmax) {
+    atomicMin(&st->lbmin_key, s_kmin);
+    atomicMax(&st->lbmax_key, s_kmax);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     atomicMax((unsigned long long *)&st->queue_peak, (unsigned long long)n);
@@ -1097,6 +1181,8 @@ static int root_init(gpugo_problem_t p) {
   s.first_parent = 0;
   s.n_split = 1;
   s.status = GPUGO_ST_RUNNING;
+  s.lbmin_key = ~0ull;
+  s.lbmax_key = 0ull;
   int win0 = 0;
   p->hs = s;
   HIPCHECK(hipMemcpyAsync(p->plo.p, p->lo.data(), sizeof(double) * d, hipMemcpyHostToDevice, p->stream));

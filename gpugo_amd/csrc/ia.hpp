@@ -53,6 +53,7 @@ template <> struct fp<double> {
   static constexpr double INV_PI_HI = 0x1.45f306dc9c883p-2;
   static constexpr double TWO_PI_LO = 0x1.921fb54442d18p+2;
   static constexpr double TRIG_BIG = 0x1p40;
+  static constexpr double TRIG_NARROW = 0x1p-40;
   static constexpr int W_TRANS = 2;
   __device__ static __forceinline__ I bits(double x) { return __double_as_longlong(x); }
   __device__ static __forceinline__ double from(I b) { return __longlong_as_double(b); }
@@ -77,6 +78,7 @@ template <> struct fp<float> {
   static constexpr float INV_PI_HI = 0x1.45f308p-2f;
   static constexpr float TWO_PI_LO = 0x1.921fb4p+2f;
   static constexpr float TRIG_BIG = 0x1p20f;
+  static constexpr float TRIG_NARROW = 0x1p-16f;
   static constexpr int W_TRANS = 3;
   __device__ static __forceinline__ I bits(float x) { return __float_as_int(x); }
   __device__ static __forceinline__ float from(I b) { return __int_as_float(b); }
@@ -324,7 +326,16 @@ template <typename T, int R> struct IA {
     const T one = T(1);
     if (!(fp<T>::abs_(a.lo) <= fp<T>::TRIG_BIG && fp<T>::abs_(a.hi) <= fp<T>::TRIG_BIG))
       return mk<T>(-one, one);
-    if (RR::sub_up(a.hi, a.lo) >= fp<T>::TWO_PI_LO) return mk<T>(-one, one);
+    const T w = RR::sub_up(a.hi, a.lo);
+    if (w <= fp<T>::TRIG_NARROW) {
+      // narrow argument (sample points): |f(x) - f(a)| <= |x - a| <= w since
+      // |sin'|, |cos'| <= 1, so one evaluation and a widening by w suffice
+      T v = SHIFT ? fp<T>::sin_(a.lo) : fp<T>::cos_(a.lo);
+      T l = max_<T>(RR::sub_dn(widen_dn(v), w), -one);
+      T h = min_<T>(RR::add_up(widen_up(v), w), one);
+      return mk<T>(l, h);
+    }
+    if (w >= fp<T>::TWO_PI_LO) return mk<T>(-one, one);
     T va = SHIFT ? fp<T>::sin_(a.lo) : fp<T>::cos_(a.lo);
     T vb = SHIFT ? fp<T>::sin_(a.hi) : fp<T>::cos_(a.hi);
     T l = max_<T>(widen_dn(min_<T>(va, vb)), -one);

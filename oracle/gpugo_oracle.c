@@ -70,6 +70,8 @@ static const double E_LO      = 0x1.5bf0a8b145769p+1;
 static const double E_HI      = 0x1.5bf0a8b14576ap+1;
 /* above this magnitude the extremum search is skipped and [-1,1] returned */
 static const double TRIG_BIG  = 0x1p40;
+/* at or below this argument width one evaluation plus the Lipschitz bound */
+static const double TRIG_NARROW = 0x1p-40;
 
 /* ---------------------------------------------------------------------------
  * Directed rounding.  Every function in this section assumes FE_UPWARD.
@@ -215,7 +217,16 @@ static iv iv_trig(iv a, int shift)
 {
   if (!(a.lo <= a.hi)) return mk(NAN, NAN);
   if (fabs(a.lo) > TRIG_BIG || fabs(a.hi) > TRIG_BIG) return mk(-1.0, 1.0);
-  if (ru_sub(a.hi, a.lo) >= TWO_PI_LO) return mk(-1.0, 1.0);
+  double w = ru_sub(a.hi, a.lo);
+  if (w <= TRIG_NARROW) {
+    /* narrow argument: sin and cos are 1-Lipschitz, so one evaluation
+     * widened by the argument width encloses the range */
+    double v = shift ? sin_rn(a.lo) : cos_rn(a.lo);
+    double lo = rd_sub(down_ulps(v, ORACLE_W_TRANS), w);
+    double hi = ru_add(up_ulps(v, ORACLE_W_TRANS), w);
+    return mk(lo < -1.0 ? -1.0 : lo, hi > 1.0 ? 1.0 : hi);
+  }
+  if (w >= TWO_PI_LO) return mk(-1.0, 1.0);
   double va = shift ? sin_rn(a.lo) : cos_rn(a.lo);
   double vb = shift ? sin_rn(a.hi) : cos_rn(a.hi);
   double lo = down_ulps(va < vb ? va : vb, ORACLE_W_TRANS);

@@ -86,11 +86,17 @@ def test_transcendentals_contain_truth(gpu, oracle, fn, rounding):
         else:
             tlo, thi = trig_range(lo_in[i], hi_in[i], fn)
         assert mp.mpf(got[0][i]) <= tlo and mp.mpf(got[1][i]) >= thi, (i, lo_in[i], hi_in[i])
+        # narrow sin/cos arguments (<= 2^-40) may add their width (Lipschitz path)
+        w = hi_in[i] - lo_in[i]
+        extra = 2 * w if (fn != "exp" and w <= 2.0 ** -40) else 0.0
         if tlo not in (0, -1) and got[0][i] > 0 or (tlo < 0 and tlo != -1):
-            assert ulps_between(got[0][i], float(tlo)) <= 4
+            assert ulps_between(got[0][i], float(tlo)) <= 4 or float(tlo) - got[0][i] <= extra
         if thi != 1 and math.isfinite(got[1][i]):
-            assert ulps_between(float(thi), got[1][i]) <= 4
-        if math.isfinite(orc[1][i]) and math.isfinite(got[1][i]) and orc[0][i] > 1e-300:
+            assert ulps_between(float(thi), got[1][i]) <= 4 or got[1][i] - float(thi) <= extra
+        # FAST rounds the argument width up, so at the narrow-path threshold the
+        # two sides may take different (both rigorous) paths
+        at_threshold = fn != "exp" and 2.0 ** -42 <= w <= 2.0 ** -38
+        if math.isfinite(orc[1][i]) and math.isfinite(got[1][i]) and orc[0][i] > 1e-300 and not at_threshold:
             assert abs(ulps_between(orc[0][i], got[0][i])) <= 3
             assert abs(ulps_between(orc[1][i], got[1][i])) <= 3
 

@@ -157,11 +157,15 @@ def test_transcendental_contains_true_range(oracle, fn):
             tlo, thi = trig_range(lo_in[i], hi_in[i], fn)
         assert mp.mpf(lo[i]) <= tlo and mp.mpf(hi[i]) >= thi, (fn, i, lo_in[i], hi_in[i])
         # width bound: at most 4 ulps beyond the true range (2-ulp widening
-        # of a <= 1-ulp nearest value, plus the conservative extremum test)
+        # of a <= 1-ulp nearest value, plus the conservative extremum test);
+        # sin/cos of arguments narrower than 2^-40 use one evaluation widened
+        # by the argument width (Lipschitz bound), which may add that width
+        w = hi_in[i] - lo_in[i]
+        extra = w if (fn != "exp" and w <= 2.0 ** -40) else 0.0
         if tlo != -1 and tlo != 0:
-            assert ulps_between(lo[i], float(tlo)) <= 4, (fn, i)
+            assert ulps_between(lo[i], float(tlo)) <= 4 or float(tlo) - lo[i] <= 2 * extra, (fn, i)
         if thi != 1 and math.isfinite(hi[i]):
-            assert ulps_between(float(thi), hi[i]) <= 4, (fn, i)
+            assert ulps_between(float(thi), hi[i]) <= 4 or hi[i] - float(thi) <= 2 * extra, (fn, i)
 
 
 def test_checker_catches_nearest_rounding():
