@@ -33,6 +33,8 @@ struct DevState {
   int status;                    // gpugo_status
   int overflow;                  // 1: queue append, 2: parent arena, 4: output arena
   int compact_final;             // 1: the stop step's compaction still has to run
+  int fresh;                     // batch slots: 1 until the root step ran
+  int pad1;
 };
 
 // Arguments of the module kernel gpugo_bound_{exact,fast}.
@@ -61,6 +63,33 @@ struct EvalArgs {
   int *kkt;
   long long n;
   int d, n_samples;
+};
+
+// ---- batches: many independent problems, one slot each -------------------
+// Per-slot problem description (written by the host when a problem is
+// admitted).  Pools are indexed by slot with fixed strides.
+struct SlotInfo {
+  int active;                    // 1 while the slot holds a problem
+  int family;                    // objective family index in the batch module
+  int d, c, wid, k_select, n_samples, flags;
+  unsigned W;                    // wid^c
+  int fresh;                     // 1 until the slot's first (root) step ran
+  long long max_iters;
+  double tol_f, inv_wid, half_inv_wid;
+};
+
+// Arguments of the batch module kernel gpugo_batch_bound_{exact,fast}.
+struct BatchBoundArgs {
+  const SlotInfo *info;
+  DevState *st;                  // [slots]
+  const double *plo, *phi, *pw;  // [slots][pcap][dmax]
+  const int *pwin;               // [slots][pcap]
+  const double *params;          // [slots][pmax]
+  const double *dom_lo, *dom_hi; // [slots][dmax]
+  double *q_lb;                  // [slots][2][qcap]
+  unsigned *q_par, *q_pos;
+  long long pcap, qcap;
+  int slots, vblocks, pmax, pad0;
 };
 
 __host__ __device__ inline unsigned long long order_key(double x) {

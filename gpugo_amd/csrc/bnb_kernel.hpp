@@ -265,6 +265,12 @@ extern "C" __global__ __launch_bounds__(64) void gpugo_eval_fast(gpugo::EvalArgs
   gpugo::eval_impl<gpugo::ROUND_FAST>(a);
 }
 #endif
+// declared coefficient signs (+1: >= 0, -1: <= 0, 0: any); the library
+// rejects problems that violate them because the code lists assume them
+extern "C" __global__ void gpugo_param_signs(signed char *out) {
+  const signed char s[] = {GPUGO_PSIGN};
+  for (int i = threadIdx.x; i < GPUGO_NPARAMS; i += blockDim.x) out[i] = s[i];
+}
 extern "C" __global__ void gpugo_info(int *out) {
   out[0] = GPUGO_D;
   out[1] = GPUGO_NPARAMS;

@@ -211,6 +211,7 @@ struct gpugo_module_s {
   hipFunction_t bound_mv[2] = {nullptr, nullptr};  // [rounding], mean-value variant
   hipFunction_t eval[2] = {nullptr, nullptr};
   int dim = 0, nparams = 0, abi = 0;
+  std::vector<signed char> psign;  // declared coefficient signs
   std::vector<char> code;
 };
 
@@ -240,6 +241,17 @@ static int module_finish(gpugo_module_t m) {
     HIPCHECK(hipModuleGetFunction(&m->bound[1], m->mod, "gpugo_bound_fast"));
     HIPCHECK(hipModuleGetFunction(&m->bound_mv[1], m->mod, "gpugo_bound_fast_mv"));
     HIPCHECK(hipModuleGetFunction(&m->eval[1], m->mod, "gpugo_eval_fast"));
+  }
+  // declared coefficient signs
+  m->psign.assign(m->nparams, 0);
+  if (m->nparams > 0) {
+    hipFunction_t fs;
+    HIPCHECK(hipModuleGetFunction(&fs, m->mod, "gpugo_param_signs"));
+    DevBuf ds;
+    HIPCHECK(ds.alloc(m->nparams));
+    void *sargs[] = {&ds.p};
+    HIPCHECK(hipModuleLaunchKernel(fs, 1, 1, 1, 256, 1, 1, 0, nullptr, sargs, nullptr));
+    HIPCHECK(hipMemcpy(m->psign.data(), ds.p, m->nparams, hipMemcpyDeviceToHost));
   }
   return GPUGO_OK;
 }
@@ -354,8 +366,13 @@ static int check_module_problem(gpugo_module_t m, int32_t d, const double *param
   if (n_params != m->nparams)
     return set_err(GPUGO_E_INVALID, "module expects %d parameters, got %lld", m->nparams, (long long)n_params);
   if (n_params > 0 && !params) return set_err(GPUGO_E_INVALID, "null params");
-  for (int64_t i = 0; i < n_params; ++i)
+  for (int64_t i = 0; i < n_params; ++i) {
     if (!std::isfinite(params[i])) return set_err(GPUGO_E_DOMAIN, "params[%lld] is not finite", (long long)i);
+    const int s = i < (int64_t)m->psign.size() ? m->psign[i] : 0;
+    if ((s > 0 && params[i] < 0) || (s < 0 && params[i] > 0))
+      return set_err(GPUGO_E_DOMAIN, "params[%lld] = %g violates the sign the module was compiled for (%s)",
+                     (long long)i, params[i], s > 0 ? ">= 0" : "<= 0");
+  }
   return GPUGO_OK;
 }
 
@@ -509,6 +526,9 @@ struct SelectArgs {
   long long out_cap, p_cap;
   int d, wid, c, k_select;
   double tol_f, tol_x;
+  long long max_iters;   // > 0: stop with GPUGO_ST_MAX_ITERS on the device (batch slots)
+  int pstride;           // doubles per parent record (d, or the batch's dmax)
+  int pad0;
 };
 
 __device__ __forceinline__ u128 qkey(const double *lb, const unsigned *par, const unsigned *pos, long long i) {
@@ -605,7 +625,7 @@ __device__ void radix_gather(const SelectArgs &a, long long N, int K, unsigned *
   __syncthreads();
 }
 
-__global__ __launch_bounds__(SEL_T) void k_select(SelectArgs a) {
+__device__ void select_body(const SelectArgs &a) {
   __shared__ unsigned hist[SEL_BINS];
   __shared__ u128 skey[SEL_KMAX];
   __shared__ unsigned sidx[SEL_KMAX];
@@ -623,9 +643,16 @@ __global__ __launch_bounds__(SEL_T) void k_select(SelectArgs a) {
     if (tid == 0) st->compact_final = 0;
     return;
   }
-  if (st->overflow) {
+  if (st->fresh) {
+    // a batch slot's first step splits the root record the host wrote
+    __syncthreads();
+    if (tid == 0) st->fresh = 0;
+    return;
+  }
+  if (st->overflow || (a.max_iters > 0 && st->iterations >= a.max_iters)) {
+    __syncthreads();
     if (tid == 0) {
-      st->status = GPUGO_ST_OVERFLOW;
+      st->status = st->overflow ? GPUGO_ST_OVERFLOW : GPUGO_ST_MAX_ITERS;
       st->q_count = st->compact_count;
       st->compact_count = 0;
       st->compact_final = 1;
@@ -780,8 +807,8 @@ __global__ __launch_bounds__(SEL_T) void k_select(SelectArgs a) {
     pos = a.q_pos[idx];
     elb = a.q_lb[idx];
     if (a.tol_x > 0) {
-      const double *plo = a.plo + (long long)par * d, *phi = a.phi + (long long)par * d;
-      const double *pw = a.pw + (long long)par * d;
+      const double *plo = a.plo + (long long)par * a.pstride, *phi = a.phi + (long long)par * a.pstride;
+      const double *pw = a.pw + (long long)par * a.pstride;
       int win = a.pwin[par];
       unsigned r = pos;
       // window coordinates shrink; recompute the max over the child
@@ -825,12 +852,12 @@ __global__ __launch_bounds__(SEL_T) void k_select(SelectArgs a) {
   if (tid == 0) s_outmin = order_key(__builtin_huge_val());
   __syncthreads();
   if (tid < K) {
-    const double *plo = a.plo + (long long)par * d, *phi = a.phi + (long long)par * d;
-    const double *pw = a.pw + (long long)par * d;
+    const double *plo = a.plo + (long long)par * a.pstride, *phi = a.phi + (long long)par * a.pstride;
+    const double *pw = a.pw + (long long)par * a.pstride;
     const int win = a.pwin[par];
     if (!is_out) {
       const long long np = pbase + split_rank;
-      double *nlo = a.plo + np * d, *nhi = a.phi + np * d, *npw = a.pw + np * d;
+      double *nlo = a.plo + np * a.pstride, *nhi = a.phi + np * a.pstride, *npw = a.pw + np * a.pstride;
       decode_box(plo, phi, pw, win, d, a.wid, a.c, pos, nlo, nhi);
       for (int j = 0; j < d; ++j) npw[j] = (nhi[j] - nlo[j]) / (double)a.wid;
       int nw = win + a.c;
@@ -863,6 +890,128 @@ __global__ __launch_bounds__(SEL_T) void k_select(SelectArgs a) {
     st->compact_count = 0;
     st->lbmin_key = ~0ull;
     st->lbmax_key = 0ull;
+  }
+}
+
+__global__ __launch_bounds__(SEL_T) void k_select(SelectArgs a) { select_body(a); }
+
+// ---------------------------------------------------------------------------
+// batches: one workgroup per slot for select and compaction
+// ---------------------------------------------------------------------------
+struct BatchArgs {
+  const SlotInfo *info;
+  DevState *st;
+  double *q_lb;
+  unsigned *q_par, *q_pos;
+  double *plo, *phi, *pw;
+  int *pwin;
+  long long pcap, qcap;
+  int dmax, pad0;
+};
+
+__global__ __launch_bounds__(SEL_T) void kb_select(BatchArgs b) {
+  const int slot = blockIdx.x;
+  const SlotInfo I = b.info[slot];
+  if (!I.active) return;
+  DevState *st = b.st + slot;
+  const long long qoff = ((long long)slot * 2 + (st->compactions & 1)) * b.qcap;
+  const long long poff = (long long)slot * b.pcap;
+  SelectArgs sa;
+  sa.st = st;
+  sa.q_lb = b.q_lb + qoff;
+  sa.q_par = b.q_par + qoff;
+  sa.q_pos = b.q_pos + qoff;
+  sa.plo = b.plo + poff * b.dmax;
+  sa.phi = b.phi + poff * b.dmax;
+  sa.pw = b.pw + poff * b.dmax;
+  sa.pwin = b.pwin + poff;
+  sa.out_lo = sa.out_hi = sa.out_lb = nullptr;
+  sa.out_cap = 0;
+  sa.p_cap = b.pcap;
+  sa.d = I.d;
+  sa.wid = I.wid;
+  sa.c = I.c;
+  sa.k_select = I.k_select;
+  sa.tol_f = I.tol_f;
+  sa.tol_x = 0.0;
+  sa.max_iters = I.max_iters;
+  sa.pstride = b.dmax;
+  sa.pad0 = 0;
+  select_body(sa);
+}
+
+// The slot's queue is compacted by one workgroup, keeping the order.
+__global__ __launch_bounds__(256) void kb_compact(BatchArgs b) {
+  __shared__ unsigned wcnt[4];
+  __shared__ unsigned long long s_kmin, s_kmax;
+  const int slot = blockIdx.x;
+  const SlotInfo I = b.info[slot];
+  if (!I.active) return;
+  DevState *st = b.st + slot;
+  if (st->status != GPUGO_ST_RUNNING && !st->compact_final) return;
+  const int cur = (int)(st->compactions & 1);
+  const long long in = ((long long)slot * 2 + cur) * b.qcap, out = ((long long)slot * 2 + (cur ^ 1)) * b.qcap;
+  long long n = st->q_count;
+  if (n > b.qcap) n = b.qcap;
+  const double ub = key_value(st->ub_key);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned long long kmin = ~0ull, kmax = 0ull;
+  if (threadIdx.x == 0) {
+    s_kmin = ~0ull;
+    s_kmax = 0ull;
+  }
+  long long outn = 0;
+  for (long long base = 0; base < n; base += 256) {
+    const long long i = base + threadIdx.x;
+    double lb = 0;
+    bool keep = false;
+    if (i < n) {
+      lb = b.q_lb[in + i];
+      keep = lb <= ub;
+      if (keep) {
+        const unsigned long long k = order_key(lb);
+        kmin = k < kmin ? k : kmin;
+        kmax = k > kmax ? k : kmax;
+      }
+    }
+    const unsigned long long mask = __ballot(keep);
+    const unsigned pre =
+        __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+    if (lane == 0) wcnt[w] = (unsigned)__popcll(mask);
+    __syncthreads();
+    unsigned off = pre, tot = 0;
+    for (int q = 0; q < 4; ++q) {
+      if (q < w) off += wcnt[q];
+      tot += wcnt[q];
+    }
+    if (keep) {
+      const long long o = out + outn + off;
+      b.q_lb[o] = lb;
+      b.q_par[o] = b.q_par[in + i];
+      b.q_pos[o] = b.q_pos[in + i];
+    }
+    outn += tot;
+    __syncthreads();
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long t0 = __shfl_xor(kmin, o, 64), t1 = __shfl_xor(kmax, o, 64);
+    kmin = t0 < kmin ? t0 : kmin;
+    kmax = t1 > kmax ? t1 : kmax;
+  }
+  if (lane == 0 && kmin <= kmax) {
+    atomicMin(&s_kmin, kmin);
+    atomicMax(&s_kmax, kmax);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    st->compact_count = outn;
+    if (s_kmin <= s_kmax) {
+      st->lbmin_key = s_kmin < st->lbmin_key ? s_kmin : st->lbmin_key;
+      st->lbmax_key = s_kmax > st->lbmax_key ? s_kmax : st->lbmax_key;
+    }
+    if ((long long)n > st->queue_peak) st->queue_peak = n;
+    st->compactions += 1;
   }
 }
 
@@ -1220,6 +1369,9 @@ static int enqueue_step(gpugo_problem_t p) {
     sa.k_select = p->desc.k_select;
     sa.tol_f = p->desc.tol_f;
     sa.tol_x = p->desc.tol_x;
+    sa.max_iters = 0;
+    sa.pstride = d;
+    sa.pad0 = 0;
     hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_T), 0, p->stream, sa);
     HIPCHECK(hipGetLastError());
   }

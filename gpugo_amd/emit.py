@@ -71,6 +71,33 @@ class _Func:
             return ("s", _lit(e.val))
         if op == "param":
             return ("s", f"P[{e.val}]")
+        if op == "dim":
+            return ("s", "((double)d)")
+        if op == "guard":
+            raise ValueError("Guarded terms are only allowed inside gsum")
+        if op == "gsum":
+            # left fold with terms that exist only for j < d: a mutable
+            # accumulator, each guarded term in its own scope
+            acc = f"t{self.n}"
+            self.n += 1
+            terms = list(e.args)
+            if terms[0].op == "guard":
+                init = "mk<double>(0.0, 0.0)"
+            else:
+                init = self.as_iv(self.lower(terms.pop(0)))
+            self.lines.append(f"  iv<double> {acc} = {init};")
+            for t in terms:
+                if t.op == "guard":
+                    self.lines.append(f"  if ({t.val} < d) {{")
+                    saved = dict(self.memo)
+                    code = self.as_iv(self.lower(t.args[0]))
+                    self.lines.append(f"  {acc} = A::add({acc}, {code});")
+                    self.memo = saved
+                    self.lines.append("  }")
+                else:
+                    code = self.as_iv(self.lower(t))
+                    self.lines.append(f"  {acc} = A::add({acc}, {code});")
+            return ("i", acc)
         if op == "iconst":
             return ("i", self.tmp("i", f"mk<double>({_lit(e.val[0])}, {_lit(e.val[1])})"))
         if e.is_scalar():
@@ -222,52 +249,92 @@ def gradient(obj: Objective) -> List[Expr]:
 # ---------------------------------------------------------------------------
 # module text
 # ---------------------------------------------------------------------------
-def code_lists(obj: Objective) -> str:
+def param_signs(obj: Objective) -> List[int]:
+    """Declared sign of each coefficient (0 = any), collected from the IR."""
+    signs = [0] * obj.n_params
+    seen = set()
+
+    def walk(e):
+        k = id(e)
+        if k in seen:
+            return
+        seen.add(k)
+        if e.op == "param" and e.val2:
+            if signs[e.val] not in (0, e.val2):
+                raise ValueError(f"P[{e.val}] declared with both signs")
+            signs[e.val] = e.val2
+        for a in e.args:
+            walk(a)
+
+    walk(obj.value)
+    for g in obj.grad or ():
+        walk(wrap(g))
+    return signs
+
+
+def code_lists(obj: Objective, suffix: str = "", dynamic: bool = False, header: bool = True) -> str:
+    """obj_value / obj_grad / obj_kkt for one objective.  With dynamic=True
+    the functions take the runtime dimension d <= obj.d (coordinates j >= d
+    are absent) -- the variable-dimension form used by batches."""
     obj.check()
     grads = gradient(obj)
     d = obj.d
-    out = [f"#define GPUGO_D {d}", f"#define GPUGO_NPARAMS {obj.n_params}",
-           "namespace gpugo {"]
+    dp = ", int d" if dynamic else ""
+    out = []
+    if header:
+        signs = param_signs(obj)
+        out += [f"#define GPUGO_D {d}", f"#define GPUGO_NPARAMS {obj.n_params}",
+                "#define GPUGO_PSIGN " + (", ".join(str(v) for v in signs) if signs else "0")]
+    out.append("namespace gpugo {")
 
     # value
     f = _Func()
     v = f.lower(obj.value)
-    out.append("template <int R>\n__device__ __forceinline__ iv<double> obj_value("
-               "const iv<double> *__restrict__ x, const double *__restrict__ P) {")
+    out.append(f"template <int R>\n__device__ __forceinline__ iv<double> obj_value{suffix}("
+               f"const iv<double> *__restrict__ x, const double *__restrict__ P{dp}) {{")
     out.append("  typedef IA<double, R> A;")
     out += f.lines
     out.append(f"  return {f.as_iv(v)};")
     out.append("}")
 
-    # gradient (parity hook)
+    def component(f, j, g, body_fn):
+        """Lines for gradient component j; absent coordinates skip it."""
+        guarded = dynamic and j > 0
+        if guarded:
+            f.lines.append(f"  if ({j} < d) {{")
+            saved = dict(f.memo)
+        code = f.as_iv(f.lower(g))
+        f.lines += body_fn(code)
+        if guarded:
+            f.memo = saved
+            f.lines.append("  }")
+
+    # gradient (parity hook, mean-value bound)
     f = _Func()
-    body = []
     for j, g in enumerate(grads):
-        r = f.lower(g)
-        body.append((j, f.as_iv(r), len(f.lines)))
-    out.append("template <int R>\n__device__ __forceinline__ void obj_grad("
-               "const iv<double> *__restrict__ x, const double *__restrict__ P, iv<double> *__restrict__ g) {")
+        if dynamic and j > 0:
+            f.lines.append(f"  g[{j}] = mk<double>(0.0, 0.0);")
+        component(f, j, g, lambda code, j=j: [f"  g[{j}] = {code};"])
+    out.append(f"template <int R>\n__device__ __forceinline__ void obj_grad{suffix}("
+               f"const iv<double> *__restrict__ x, const double *__restrict__ P, "
+               f"iv<double> *__restrict__ g{dp}) {{")
     out.append("  typedef IA<double, R> A;")
     out += f.lines
-    for j, code, _ in body:
-        out.append(f"  g[{j}] = {code};")
     out.append("}")
 
     # monotonicity test with early exit per component
     # (GPUGO_1.1/1-Ackley/ackley50.py:458-477)
     f = _Func()
-    out.append("template <int R>\n__device__ __forceinline__ bool obj_kkt("
-               "const iv<double> *__restrict__ x, const double *__restrict__ P, "
-               "const double *__restrict__ dlo, const double *__restrict__ dhi) {")
-    out.append("  typedef IA<double, R> A;")
     for j, g in enumerate(grads):
-        start = len(f.lines)
-        r = f.lower(g)
-        out += f.lines[start:]
-        gv = f.as_iv(r)
-        out.append(f"  {{ const iv<double> gj = {gv};")
-        out.append(f"    if (gj.lo > 0.0 && x[{j}].lo != dlo[{j}]) return true;")
-        out.append(f"    if (gj.hi < 0.0 && x[{j}].hi != dhi[{j}]) return true; }}")
+        component(f, j, g, lambda code, j=j: [
+            f"  {{ const iv<double> gj = {code};",
+            f"    if (gj.lo > 0.0 && x[{j}].lo != dlo[{j}]) return true;",
+            f"    if (gj.hi < 0.0 && x[{j}].hi != dhi[{j}]) return true; }}"])
+    out.append(f"template <int R>\n__device__ __forceinline__ bool obj_kkt{suffix}("
+               f"const iv<double> *__restrict__ x, const double *__restrict__ P, "
+               f"const double *__restrict__ dlo, const double *__restrict__ dhi{dp}) {{")
+    out.append("  typedef IA<double, R> A;")
+    out += f.lines
     out.append("  return false;")
     out.append("}")
     out.append("}  // namespace gpugo")

@@ -23,19 +23,20 @@ _OPS_BINARY = ("add", "sub", "mul", "div")
 
 
 class Expr:
-    __slots__ = ("op", "args", "val", "_key", "_hash")
+    __slots__ = ("op", "args", "val", "val2", "_key", "_hash")
 
-    def __init__(self, op: str, args: tuple = (), val=None):
+    def __init__(self, op: str, args: tuple = (), val=None, val2=None):
         self.op = op
         self.args = args
         self.val = val
+        self.val2 = val2      # param: declared sign (+1 / -1) or None
         self._key = None
         self._hash = None
 
     # structural identity (used for common-subexpression elimination)
     def key(self):
         if self._key is None:
-            self._key = (self.op, self.val, tuple(a.key() for a in self.args))
+            self._key = (self.op, self.val, self.val2, tuple(a.key() for a in self.args))
         return self._key
 
     def __hash__(self):
@@ -80,7 +81,7 @@ class Expr:
     # exact scalars -------------------------------------------------------
     def is_scalar(self) -> bool:
         """True if the value is a single double known exactly (no rounding)."""
-        if self.op in ("const", "param"):
+        if self.op in ("const", "param", "dim"):
             return True
         if self.op == "neg":
             return self.args[0].is_scalar()
@@ -99,6 +100,10 @@ class Expr:
         """+1 / -1 if the sign of an exact scalar is known statically, else 0."""
         if self.op == "const":
             return 1 if self.val > 0 else (-1 if self.val < 0 else 0)
+        if self.op == "dim":
+            return 1
+        if self.op == "param":
+            return 0 if self.val2 is None else self.val2
         if self.op == "neg":
             return -self.args[0].scalar_sign()
         if self.op == "mul":
@@ -128,8 +133,35 @@ def X(j: int) -> Expr:
     return Expr("var", (), int(j))
 
 
-def P(i: int) -> Expr:
-    return Expr("param", (), int(i))
+def P(i: int, sign: int | None = None) -> Expr:
+    """Coefficient P[i].  `sign` = +1 (or -1) declares P[i] >= 0 (<= 0); the
+    emitter then uses the one-sided scalar product, and problems whose
+    coefficients violate a declared sign are rejected."""
+    if sign not in (None, 1, -1):
+        raise ValueError("sign must be None, +1 or -1")
+    return Expr("param", (), int(i), sign)
+
+
+def Dim() -> Expr:
+    """The problem dimension d as an exact scalar (runtime value in the
+    variable-dimension code lists of a batch)."""
+    return Expr("dim")
+
+
+def Guarded(j: int, e) -> Expr:
+    """A term that exists only when coordinate j exists (j < d); inside a
+    sum it is skipped otherwise.  Used by variable-dimension code lists."""
+    return Expr("guard", (wrap(e),), int(j))
+
+
+def gsum(terms) -> Expr:
+    """Left fold like fsum, allowing Guarded terms (skipped when absent)."""
+    terms = [wrap(t) for t in terms]
+    if not terms:
+        return Const(0.0)
+    if not any(t.op == "guard" for t in terms):
+        return fsum(terms)
+    return Expr("gsum", tuple(terms))
 
 
 def Const(v: float) -> Expr:

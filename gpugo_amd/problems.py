@@ -28,7 +28,7 @@ from typing import List, Optional
 
 import numpy as np
 
-from .expr import E, Objective, P, X, cos, exp, fsum, pown, sin, sqr, sqrt
+from .expr import E, Dim, Guarded, Objective, P, X, cos, exp, fsum, gsum, pown, sin, sqr, sqrt
 
 FAMILIES = ("sepcos", "sepcos_chain", "ackley_q")
 
@@ -37,52 +37,80 @@ def n_params(family: str, d: int) -> int:
     return {"sepcos": 3 * d, "sepcos_chain": 4 * d - 1, "ackley_q": 3 + d}[family]
 
 
-def _sep_terms(d):
-    a = lambda i: P(i)
-    b = lambda i: P(d + i)
-    c = lambda i: P(2 * d + i)
-    return [sqr(X(i)) * a(i) + (1.0 - cos(X(i) * c(i))) * b(i) for i in range(d)]
-
-
-def _sep_grad(d, j):
-    a, b, c = P(j), P(d + j), P(2 * d + j)
-    return X(j) * (2.0 * a) + (sin(X(j) * c) * c) * b
-
-
-def objective(family: str, d: int) -> Objective:
-    if family == "sepcos":
-        value = fsum(_sep_terms(d))
-        grad = [_sep_grad(d, j) for j in range(d)]
-    elif family == "sepcos_chain":
-        w = lambda i: P(3 * d + i)
-        chain = [sqr(X(i + 1) - sqr(X(i))) * w(i) + sqr(1.0 - X(i)) for i in range(d - 1)]
-        value = fsum(_sep_terms(d) + chain)
-        grad = []
-        for j in range(d):
-            g = _sep_grad(d, j)
-            if j + 1 < d:
-                g = g + ((X(j) * (X(j + 1) - sqr(X(j)))) * (-4.0 * w(j)) - (1.0 - X(j)) * 2.0)
-            if j > 0:
-                g = g + (X(j) - sqr(X(j - 1))) * (2.0 * w(j - 1))
-            grad.append(g)
+def _build(family: str, d: int, dynamic: bool) -> Objective:
+    """Code lists of a family.  dynamic=False: fixed dimension d.  dynamic=True:
+    d is the maximum dimension, the runtime dimension is Dim(), coordinates
+    j >= Dim() are absent (Guarded) and coefficients use the layout of d.
+    Both forms perform the same operations in the same order for the
+    coordinates that exist.  All coefficients are declared >= 0."""
+    fold = gsum if dynamic else fsum
+    G = (lambda j, e: Guarded(j, e) if (dynamic and j > 0) else e)
+    nd = Dim() if dynamic else float(d)
+    p = lambda i: P(i, +1)
+    if family in ("sepcos", "sepcos_chain"):
+        a, b, c = (lambda i: p(i)), (lambda i: p(d + i)), (lambda i: p(2 * d + i))
+        sep = [G(i, sqr(X(i)) * a(i) + (1.0 - cos(X(i) * c(i))) * b(i)) for i in range(d)]
+        sep_g = lambda j: X(j) * (2.0 * a(j)) + (sin(X(j) * c(j)) * c(j)) * b(j)
+        if family == "sepcos":
+            value = fold(sep)
+            grad = [sep_g(j) for j in range(d)]
+        else:
+            w = lambda i: p(3 * d + i)
+            chain = [G(i + 1, sqr(X(i + 1) - sqr(X(i))) * w(i) + sqr(1.0 - X(i))) for i in range(d - 1)]
+            value = fold(sep + chain)
+            grad = []
+            for j in range(d):
+                terms = [sep_g(j)]
+                if j + 1 < d:
+                    terms.append(G(j + 1, (X(j) * (X(j + 1) - sqr(X(j)))) * (-4.0 * w(j))
+                                   - (1.0 - X(j)) * 2.0))
+                if j > 0:
+                    terms.append((X(j) - sqr(X(j - 1))) * (2.0 * w(j - 1)))
+                grad.append(fold(terms))
     elif family == "ackley_q":
-        A, B, C = P(0), P(1), P(2)
-        q = lambda i: P(3 + i)
-        s1 = fsum(sqr(X(i)) for i in range(d))
-        s2 = fsum(cos(X(i) * C) for i in range(d))
-        s3 = fsum(pown(X(i), 4) * q(i) for i in range(d))
-        r = sqrt(s1 / float(d))
+        A, B, C = p(0), p(1), p(2)
+        q = lambda i: p(3 + i)
+        s1 = fold(G(i, sqr(X(i))) for i in range(d))
+        s2 = fold(G(i, cos(X(i) * C)) for i in range(d))
+        s3 = fold(G(i, pown(X(i), 4) * q(i)) for i in range(d))
+        r = sqrt(s1 / nd)
         t1 = exp(r * (-B))
         v1 = t1 * (-A)
-        t2 = exp(s2 / float(d))
+        t2 = exp(s2 / nd)
         value = ((v1 - t2) + (A + E)) + s3 / 50.0
-        k1 = (((t1 / r) * B) * A) / float(d)
-        k2 = (exp(s2 / float(d)) * C) / float(d)
+        k1 = (((t1 / r) * B) * A) / nd
+        k2 = (exp(s2 / nd) * C) / nd
         grad = [(k1 * X(j) + k2 * sin(X(j) * C)) + (pown(X(j), 3) * (4.0 * q(j))) / 50.0
                 for j in range(d)]
     else:
         raise ValueError(f"unknown family {family!r}")
-    return Objective(d, n_params(family, d), value, grad, name=f"{family}_d{d}")
+    tag = "dyn" if dynamic else "d"
+    return Objective(d, n_params(family, d), value, grad, name=f"{family}_{tag}{d}")
+
+
+def objective(family: str, d: int) -> Objective:
+    return _build(family, d, False)
+
+
+def objective_dynamic(family: str, dmax: int) -> Objective:
+    """Variable-dimension form (runtime d <= dmax) used by batches."""
+    return _build(family, dmax, True)
+
+
+def pad_params(family: str, d: int, params, dmax: int) -> np.ndarray:
+    """Coefficients of a d-dimensional problem in the dmax layout of
+    objective_dynamic (absent coordinates get 0)."""
+    params = np.asarray(params, dtype=np.float64)
+    out = np.zeros(n_params(family, dmax))
+    if family == "ackley_q":
+        out[:3] = params[:3]
+        out[3:3 + d] = params[3:3 + d]
+        return out
+    blocks = 4 if family == "sepcos_chain" else 3
+    for k in range(blocks):
+        n = d - 1 if k == 3 else d
+        out[k * dmax:k * dmax + n] = params[k * d:k * d + n]
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -104,7 +104,9 @@ def test_emitter_lowering_rules():
     src = emit.code_lists(pb.objective(spec.family, spec.d))
     # A + e enters as an interval constant plus a scalar; 1/d as a positive divisor
     assert "mk<double>(0x1.5bf0a8b145769p+1, 0x1.5bf0a8b14576ap+1)" in src
-    assert "A::divc_pos(" in src and "A::scale(" in src
+    # coefficients are declared >= 0: -A and -B scale one-sidedly
+    assert "A::divc_pos(" in src and "A::scale_neg(t" in src and "A::scale_pos(" in src
+    assert "#define GPUGO_PSIGN 1, 1, 1, 1, 1, 1" in src
     assert src.count("bool obj_kkt") == 1 and "return true;" in src
     full = emit.module_source(pb.objective("sepcos", 2), ("exact", "fast"))
     assert "#define GPUGO_WITH_EXACT 1" in full and "gpugo_bound_fast" in full
