@@ -31,7 +31,9 @@ EXPORTS = [
     "gpugo_module_compile", "gpugo_rtc_compile", "gpugo_rtc_code_copy", "gpugo_module_load",
     "gpugo_module_code", "gpugo_module_destroy", "gpugo_eval_boxes", "gpugo_problem_create",
     "gpugo_step", "gpugo_solve", "gpugo_boxes_copy", "gpugo_problem_destroy", "gpugo_problem_reset",
-    "gpugo_problem_timing",
+    "gpugo_problem_timing", "gpugo_batch_bytes", "gpugo_mem_info", "gpugo_batch_create",
+    "gpugo_batch_admit", "gpugo_batch_run", "gpugo_batch_sync", "gpugo_batch_results",
+    "gpugo_batch_counts", "gpugo_batch_destroy",
 ]
 
 
@@ -50,6 +52,14 @@ class ProblemDesc(ctypes.Structure):
         ("params", ctypes.POINTER(ctypes.c_double)), ("n_params", ctypes.c_int64),
         ("tol_f", ctypes.c_double), ("tol_x", ctypes.c_double), ("init_ub", ctypes.c_double),
         ("max_queue", ctypes.c_int64), ("max_parents", ctypes.c_int64), ("max_iters", ctypes.c_int64),
+    ]
+
+
+class BatchDesc(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_int32), ("steps_per_launch", ctypes.c_int32), ("rounding", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32), ("max_queue", ctypes.c_int64), ("max_parents", ctypes.c_int64),
+        ("bound_blocks", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -106,6 +116,17 @@ def load():
     lib.gpugo_problem_timing.argtypes = [vp, ctypes.c_int]
     lib.gpugo_problem_destroy.argtypes = [vp]
     lib.gpugo_problem_destroy.restype = None
+    I64 = ctypes.POINTER(ctypes.c_int64)
+    lib.gpugo_batch_bytes.argtypes = [vp, ctypes.POINTER(BatchDesc), I64]
+    lib.gpugo_mem_info.argtypes = [I64, I64]
+    lib.gpugo_batch_create.argtypes = [vp, ctypes.POINTER(BatchDesc), ctypes.POINTER(vp)]
+    lib.gpugo_batch_admit.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ProblemDesc), ctypes.c_int64, I32]
+    lib.gpugo_batch_run.argtypes = [vp, ctypes.c_int64]
+    lib.gpugo_batch_sync.argtypes = [vp]
+    lib.gpugo_batch_results.argtypes = [vp, ctypes.c_int64, I64, ctypes.POINTER(Result), I64]
+    lib.gpugo_batch_counts.argtypes = [vp, I64, I64, I64]
+    lib.gpugo_batch_destroy.argtypes = [vp]
+    lib.gpugo_batch_destroy.restype = None
     _lib = lib
     return lib
 

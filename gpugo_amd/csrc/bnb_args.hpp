@@ -73,12 +73,16 @@ struct SlotInfo {
   int family;                    // objective family index in the batch module
   int d, c, wid, k_select, n_samples, flags;
   unsigned W;                    // wid^c
-  int fresh;                     // 1 until the slot's first (root) step ran
+  int pad0;
   long long max_iters;
   double tol_f, inv_wid, half_inv_wid;
 };
 
 // Arguments of the batch module kernel gpugo_batch_bound_{exact,fast}.
+// The step's children of all slots form one list of wavefront-sized work
+// items: slot s owns waves [wave_off[s], wave_off[s+1]) (exclusive scan of
+// ceil(n_children / 64), written by the plan kernel), so a wavefront always
+// works on one slot and the family switch never diverges.
 struct BatchBoundArgs {
   const SlotInfo *info;
   DevState *st;                  // [slots]
@@ -88,8 +92,17 @@ struct BatchBoundArgs {
   const double *dom_lo, *dom_hi; // [slots][dmax]
   double *q_lb;                  // [slots][2][qcap]
   unsigned *q_par, *q_pos;
+  const long long *wave_off;     // [slots + 1]
   long long pcap, qcap;
-  int slots, vblocks, pmax, pad0;
+  int slots, pmax;
+};
+
+// Admission record of a batch slot, followed in memory by params[pmax],
+// lo[dmax], hi[dmax] and pw[dmax] (the root parent record).
+struct AdmitHead {
+  int slot, pad;
+  SlotInfo info;
+  DevState st;
 };
 
 __host__ __device__ inline unsigned long long order_key(double x) {

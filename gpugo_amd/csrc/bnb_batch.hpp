@@ -4,8 +4,8 @@
 // bnb_args.hpp and the variable-dimension code lists of each family with the
 // dispatchers fam_value / fam_kkt / fam_grad.
 //
-// Block b works on slot b / vblocks: the slot's family and dimension are
-// uniform within a block, so the family switch does not diverge.  Per slot
+// A wavefront works on one slot at a time, so the slot's family and
+// dimension are uniform and the family switch does not diverge.  Per slot
 // the work is the same as the single-problem kernel (bnb_kernel.hpp) and the
 // reference kernels sampling + ruleOutGPU (GPUGO_1.1/1-Ackley/ackley50.py:
 // 602-766), on the slot's own queue, parent records and incumbent.
@@ -85,54 +85,59 @@ __device__ __forceinline__ void batch_decode(const BatchBoundArgs &a, int slot, 
   }
 }
 
+__device__ __forceinline__ long long wave_uniform(long long v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// Persistent split+bound over the work list of all slots.  Each wavefront
+// takes wave items gw, gw + (waves in the grid), ...; item gw belongs to the
+// slot s with wave_off[s] <= gw < wave_off[s+1] (binary search, starting
+// from the previous item's slot since gw increases).  Survivors are appended
+// to the slot's queue with one atomic per wavefront.
 template <int R>
 __device__ __forceinline__ void batch_bound_impl(const BatchBoundArgs &a) {
-  __shared__ unsigned s_wcnt[4];
-  __shared__ unsigned long long s_base;
-  __shared__ unsigned long long s_ubmin;
-  __shared__ unsigned s_pv, s_pk;
-  const int slot = blockIdx.x / a.vblocks, vb = blockIdx.x % a.vblocks;
-  if (slot >= a.slots) return;
-  const SlotInfo I = a.info[slot];
-  DevState *st = a.st + slot;
-  if (!I.active || st->status != 0 || st->fresh) return;
-  const long long n_child = (long long)st->n_split * (long long)I.W;
-  if (n_child == 0) return;
-  const long long first = st->first_parent;
-  const double ub0 = st->ub_snap;
-  const int cur = (int)(st->compactions & 1);
-  const long long qoff = ((long long)slot * 2 + cur) * a.qcap;
-  double *__restrict__ q_lb = a.q_lb + qoff;
-  unsigned *__restrict__ q_par = a.q_par + qoff;
-  unsigned *__restrict__ q_pos = a.q_pos + qoff;
-  const double *__restrict__ P = a.params + (long long)slot * a.pmax;
-  const double *__restrict__ dlo = a.dom_lo + (long long)slot * GPUGO_DMAX;
-  const double *__restrict__ dhi = a.dom_hi + (long long)slot * GPUGO_DMAX;
-  const int d = I.d, f = I.family;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const double invW = 1.0 / (double)I.W, halfInvW = 0.5 / (double)I.W;
-  if (threadIdx.x == 0) {
-    s_ubmin = order_key(__builtin_huge_val());
-    s_pv = 0;
-    s_pk = 0;
-  }
-  __syncthreads();
-  for (long long base = (long long)vb * 256; base < n_child; base += (long long)a.vblocks * 256) {
-    const long long t = base + threadIdx.x;
-    const bool active = t < n_child;
+  const int lane = threadIdx.x & 63;
+  const long long grid_waves = (long long)gridDim.x * (blockDim.x >> 6);
+  const long long total = a.wave_off[a.slots];
+  int slot = 0;
+  for (long long gw = wave_uniform((long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); gw < total;
+       gw += grid_waves) {
+    int lo = slot, hi = a.slots;  // wave_off[lo] <= gw < wave_off[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (a.wave_off[mid] <= gw) lo = mid;
+      else hi = mid;
+    }
+    slot = lo;
+    const SlotInfo &I = a.info[slot];
+    DevState *st = a.st + slot;
+    const long long n_child = (long long)st->n_split * (long long)I.W;
+    const long long first = st->first_parent;
+    const double ub0 = st->ub_snap;
+    const int cur = (int)(st->compactions & 1);
+    const long long qoff = ((long long)slot * 2 + cur) * a.qcap;
+    const double *__restrict__ P = a.params + (long long)slot * a.pmax;
+    const double *__restrict__ dlo = a.dom_lo + (long long)slot * GPUGO_DMAX;
+    const double *__restrict__ dhi = a.dom_hi + (long long)slot * GPUGO_DMAX;
+    const int d = I.d, f = I.family;
+    const long long t = (gw - a.wave_off[slot]) * 64 + lane;
     long long pl = 0;
     unsigned pos = 0;
     double lb = __builtin_huge_val(), su = __builtin_huge_val();
     bool keep = false, pv = false, pk = false;
-    if (active) {
-      pl = (long long)__builtin_fma((double)t, invW, halfInvW);
+    if (t < n_child) {
+      pl = (long long)__builtin_fma((double)t, 1.0 / (double)I.W, 0.5 / (double)I.W);
       pos = (unsigned)(t - pl * (long long)I.W);
       iv<double> x[GPUGO_DMAX];
       batch_decode(a, slot, I, first + pl, pos, x);
-      iv<double> fv = fam_value<R>(f, x, P, d);
+      const iv<double> fv = fam_value<R>(f, x, P, d);
       lb = fv.lo;
       if (!(lb == lb)) lb = -__builtin_huge_val();
       if (lb <= ub0) {
+        // as in the single-problem kernel: monotonicity test, then the
+        // sample bound of the survivor of the value test
         keep = !fam_kkt<R>(f, x, P, dlo, dhi, d);
         pk = !keep;
         su = batch_sample_ub<R>(f, x, P, d, I.n_samples);
@@ -143,49 +148,35 @@ __device__ __forceinline__ void batch_bound_impl(const BatchBoundArgs &a) {
     const unsigned long long mask = __ballot(keep);
     const unsigned pre =
         __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-    if (lane == 0) s_wcnt[w] = (unsigned)__popcll(mask);
     double m = su;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-      double t2 = __shfl_xor(m, o, 64);
+      const double t2 = __shfl_xor(m, o, 64);
       m = t2 < m ? t2 : m;
     }
     const unsigned long long pvm = __ballot(pv), pkm = __ballot(pk);
+    unsigned long long base = 0;
     if (lane == 0) {
-      if (m < __builtin_huge_val()) atomicMin(&s_ubmin, order_key(m));
-      if (pvm) atomicAdd(&s_pv, (unsigned)__popcll(pvm));
-      if (pkm) atomicAdd(&s_pk, (unsigned)__popcll(pkm));
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const unsigned tot = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
-      unsigned long long b = 0;
+      const unsigned tot = (unsigned)__popcll(mask);
       if (tot) {
-        b = atomicAdd((unsigned long long *)&st->q_count, (unsigned long long)tot);
-        if (b + tot > (unsigned long long)a.qcap) atomicOr(&st->overflow, 1);
+        base = atomicAdd((unsigned long long *)&st->q_count, (unsigned long long)tot);
+        if (base + tot > (unsigned long long)a.qcap) atomicOr(&st->overflow, 1);
       }
-      s_base = b;
+      if (m < __builtin_huge_val()) {
+        const unsigned long long k = order_key(m);
+        if (k < *(volatile unsigned long long *)&st->ub_key) atomicMin(&st->ub_key, k);
+      }
+      if (pvm) atomicAdd((unsigned long long *)&st->pruned_value, (unsigned long long)__popcll(pvm));
+      if (pkm) atomicAdd((unsigned long long *)&st->pruned_kkt, (unsigned long long)__popcll(pkm));
     }
-    __syncthreads();
+    base = (unsigned long long)__shfl((long long)base, 0, 64);
     if (keep) {
-      unsigned off = pre;
-      for (int q = 0; q < w; ++q) off += s_wcnt[q];
-      const unsigned long long o = s_base + off;
+      const unsigned long long o = base + pre;
       if (o < (unsigned long long)a.qcap) {
-        q_lb[o] = lb;
-        q_par[o] = (unsigned)(first + pl);
-        q_pos[o] = pos;
+        a.q_lb[qoff + o] = lb;
+        a.q_par[qoff + o] = (unsigned)(first + pl);
+        a.q_pos[qoff + o] = pos;
       }
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    if (s_ubmin < *(volatile unsigned long long *)&st->ub_key) atomicMin(&st->ub_key, s_ubmin);
-    if (s_pv) atomicAdd((unsigned long long *)&st->pruned_value, (unsigned long long)s_pv);
-    if (s_pk) atomicAdd((unsigned long long *)&st->pruned_kkt, (unsigned long long)s_pk);
-    if (vb == 0) {
-      st->iterations += 1;
-      st->bounded += n_child;
     }
   }
 }

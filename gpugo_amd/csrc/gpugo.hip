@@ -213,14 +213,58 @@ struct gpugo_module_s {
   int dim = 0, nparams = 0, abi = 0;
   std::vector<signed char> psign;  // declared coefficient signs
   std::vector<char> code;
+  // batch modules (several families, runtime d <= dim)
+  bool batch = false;
+  hipFunction_t batch_bound[2] = {nullptr, nullptr};
+  int nfam = 0, pmax = 0;
+  std::vector<int> fam_nparams;
+  std::vector<signed char> fam_psign;  // [nfam][pmax]
 };
 
 #define GPUGO_MODULE_ABI 1
 
+static int batch_module_finish(gpugo_module_t m, hipFunction_t info) {
+  m->batch = true;
+  DevBuf d;
+  HIPCHECK(d.alloc(64 * sizeof(int)));
+  void *args[] = {&d.p};
+  HIPCHECK(hipModuleLaunchKernel(info, 1, 1, 1, 1, 1, 1, 0, nullptr, args, nullptr));
+  int h[64];
+  HIPCHECK(hipMemcpy(h, d.p, sizeof h, hipMemcpyDeviceToHost));
+  m->dim = h[0];
+  m->nfam = h[1];
+  m->abi = h[2];
+  m->pmax = h[4];
+  if (m->abi != GPUGO_MODULE_ABI)
+    return set_err(GPUGO_E_INVALID, "module ABI %d, library expects %d", m->abi, GPUGO_MODULE_ABI);
+  if (m->nfam < 1 || m->nfam > 59 || m->dim < 1 || m->pmax < 1)
+    return set_err(GPUGO_E_INVALID, "malformed batch module (dmax %d, families %d)", m->dim, m->nfam);
+  m->fam_nparams.assign(h + 5, h + 5 + m->nfam);
+  if (h[3] & 1) HIPCHECK(hipModuleGetFunction(&m->batch_bound[0], m->mod, "gpugo_batch_bound_exact"));
+  if (h[3] & 2) HIPCHECK(hipModuleGetFunction(&m->batch_bound[1], m->mod, "gpugo_batch_bound_fast"));
+  hipFunction_t fs;
+  HIPCHECK(hipModuleGetFunction(&fs, m->mod, "gpugo_batch_param_signs"));
+  DevBuf ds;
+  const size_t ns = (size_t)m->nfam * m->pmax;
+  HIPCHECK(ds.alloc(ns));
+  void *sargs[] = {&ds.p};
+  HIPCHECK(hipModuleLaunchKernel(fs, 1, 1, 1, 256, 1, 1, 0, nullptr, sargs, nullptr));
+  m->fam_psign.assign(ns, 0);
+  HIPCHECK(hipMemcpy(m->fam_psign.data(), ds.p, ns, hipMemcpyDeviceToHost));
+  return GPUGO_OK;
+}
+
 static int module_finish(gpugo_module_t m) {
   HIPCHECK(hipModuleLoadData(&m->mod, m->code.data()));
   hipFunction_t info;
-  HIPCHECK(hipModuleGetFunction(&info, m->mod, "gpugo_info"));
+  if (hipModuleGetFunction(&info, m->mod, "gpugo_info") != hipSuccess) {
+    (void)hipGetLastError();  // the lookup's error must not surface at a later launch check
+    if (hipModuleGetFunction(&info, m->mod, "gpugo_batch_info") != hipSuccess) {
+      (void)hipGetLastError();
+      return set_err(GPUGO_E_INVALID, "code object is neither an objective module nor a batch module");
+    }
+    return batch_module_finish(m, info);
+  }
   DevBuf d;
   HIPCHECK(d.alloc(4 * sizeof(int)));
   void *args[] = {&d.p};
@@ -362,6 +406,7 @@ extern "C" void gpugo_module_destroy(gpugo_module_t m) {
 
 static int check_module_problem(gpugo_module_t m, int32_t d, const double *params, int64_t n_params) {
   if (!m) return set_err(GPUGO_E_INVALID, "null module");
+  if (m->batch) return set_err(GPUGO_E_INVALID, "batch module: use gpugo_batch_*");
   if (d != m->dim) return set_err(GPUGO_E_INVALID, "module is compiled for d=%d, got d=%d", m->dim, d);
   if (n_params != m->nparams)
     return set_err(GPUGO_E_INVALID, "module expects %d parameters, got %lld", m->nparams, (long long)n_params);
@@ -650,10 +695,27 @@ __device__ void select_body(const SelectArgs &a) {
     return;
   }
   if (st->overflow || (a.max_iters > 0 && st->iterations >= a.max_iters)) {
+    // stop; the enclosure is [smallest live lower bound, ub] (the queue is
+    // compacted, so it holds no consumed entries), none after an overflow
+    const long long N = st->compact_count;
+    if (tid == 0) s_outmin = order_key(__builtin_huge_val());
+    __syncthreads();
+    unsigned long long mk_ = order_key(__builtin_huge_val());
+    for (long long i = tid; i < N; i += SEL_T) {
+      const unsigned long long k = order_key(a.q_lb[i]);
+      mk_ = k < mk_ ? k : mk_;
+    }
+    atomicMin(&s_outmin, mk_);
     __syncthreads();
     if (tid == 0) {
-      st->status = st->overflow ? GPUGO_ST_OVERFLOW : GPUGO_ST_MAX_ITERS;
-      st->q_count = st->compact_count;
+      const bool ovf = st->overflow != 0;
+      double lbq = key_value(s_outmin);
+      if (st->out_min_lb < lbq) lbq = st->out_min_lb;
+      if (ovf || (N == 0 && st->n_out == 0)) lbq = -__builtin_huge_val();
+      st->lb_global = lbq;
+      st->width = rnd<double, ROUND_EXACT>::add_up(key_value(st->ub_key), -lbq);
+      st->status = ovf ? GPUGO_ST_OVERFLOW : GPUGO_ST_MAX_ITERS;
+      st->q_count = N;
       st->compact_count = 0;
       st->compact_final = 1;
     }
@@ -899,14 +961,16 @@ __global__ __launch_bounds__(SEL_T) void k_select(SelectArgs a) { select_body(a)
 // batches: one workgroup per slot for select and compaction
 // ---------------------------------------------------------------------------
 struct BatchArgs {
-  const SlotInfo *info;
+  SlotInfo *info;
   DevState *st;
   double *q_lb;
   unsigned *q_par, *q_pos;
   double *plo, *phi, *pw;
   int *pwin;
+  double *params, *dom_lo, *dom_hi;
+  long long *wave_off;
   long long pcap, qcap;
-  int dmax, pad0;
+  int dmax, pmax, slots, pad0;
 };
 
 __global__ __launch_bounds__(SEL_T) void kb_select(BatchArgs b) {
@@ -1012,6 +1076,83 @@ __global__ __launch_bounds__(256) void kb_compact(BatchArgs b) {
     }
     if ((long long)n > st->queue_peak) st->queue_peak = n;
     st->compactions += 1;
+  }
+}
+
+// Work list of the step: per running slot, iterations/bounded as the
+// single-problem kernel counts them, and the exclusive scan of its
+// wavefront count ceil(n_children / 64) into wave_off[0..slots].
+__device__ __forceinline__ long long slot_waves(const BatchArgs &b, int s) {
+  const SlotInfo &I = b.info[s];
+  if (!I.active) return 0;
+  const DevState *st = b.st + s;
+  if (st->status != GPUGO_ST_RUNNING || st->n_split <= 0) return 0;
+  return ((long long)st->n_split * (long long)I.W + 63) >> 6;
+}
+
+__global__ __launch_bounds__(1024) void kb_plan(BatchArgs b) {
+  __shared__ long long sh[17];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int per = (b.slots + 1023) / 1024;
+  const int s0 = tid * per, s1 = (s0 + per < b.slots) ? s0 + per : b.slots;
+  long long local = 0;
+  for (int s = s0; s < s1; ++s) {
+    const long long nw = slot_waves(b, s);
+    if (nw) {
+      DevState *st = b.st + s;
+      st->iterations += 1;
+      st->bounded += (long long)st->n_split * (long long)b.info[s].W;
+    }
+    local += nw;
+  }
+  long long inc = local;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) sh[w] = inc;
+  __syncthreads();
+  if (w == 0) {
+    const long long x = lane < 16 ? sh[lane] : 0;
+    long long xi = x;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+      const long long t = __shfl_up(xi, o, 64);
+      if (lane >= o) xi += t;
+    }
+    if (lane < 16) sh[lane] = xi - x;
+    if (lane == 15) sh[16] = xi;
+  }
+  __syncthreads();
+  long long run = sh[w] + inc - local;
+  for (int s = s0; s < s1; ++s) {
+    b.wave_off[s] = run;
+    run += slot_waves(b, s);
+  }
+  if (tid == 0) b.wave_off[b.slots] = sh[16];
+}
+
+// Admission: one workgroup per record writes the slot's description,
+// coefficients, initial box, root parent record and fresh state.
+__global__ __launch_bounds__(64) void kb_admit(BatchArgs b, const char *recs, long long rec_bytes) {
+  const AdmitHead *h = (const AdmitHead *)(recs + (long long)blockIdx.x * rec_bytes);
+  const int s = h->slot;
+  const double *pr = (const double *)(h + 1);
+  const double *lo = pr + b.pmax, *hi = lo + b.dmax, *pw = hi + b.dmax;
+  const long long root = (long long)s * b.pcap;
+  for (int i = threadIdx.x; i < b.pmax; i += 64) b.params[(long long)s * b.pmax + i] = pr[i];
+  for (int j = threadIdx.x; j < b.dmax; j += 64) {
+    b.dom_lo[(long long)s * b.dmax + j] = lo[j];
+    b.dom_hi[(long long)s * b.dmax + j] = hi[j];
+    b.plo[root * b.dmax + j] = lo[j];
+    b.phi[root * b.dmax + j] = hi[j];
+    b.pw[root * b.dmax + j] = pw[j];
+  }
+  if (threadIdx.x == 0) {
+    b.pwin[root] = 0;
+    b.st[s] = h->st;
+    b.info[s] = h->info;
   }
 }
 
@@ -1604,5 +1745,470 @@ extern "C" int gpugo_boxes_copy(gpugo_problem_t p, int64_t max, double *lo, doub
   if (hi) HIPCHECK(hipMemcpy(hi, d_hi.p, sizeof(double) * d * m, hipMemcpyDeviceToHost));
   if (lb)
     for (long long i = 0; i < m; ++i) lb[i] = hlb[order[i]];
+  return GPUGO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+static_assert(sizeof(AdmitHead) % 8 == 0, "admission records must keep doubles aligned");
+
+struct BatchHostSlot {
+  int running = 0;
+  int64_t tag = 0;
+  long long admit_poll = 0;  // polls with a smaller sequence number predate the admission
+  std::chrono::steady_clock::time_point t0;
+};
+
+struct gpugo_batch_s {
+  gpugo_module_t m = nullptr;
+  gpugo_batch_desc desc;
+  int dmax = 0, pmax = 0, slots = 0, bound_grid = 0;
+  long long qcap = 0, pcap = 0, rec_bytes = 0;
+  DevBuf info, st, plo, phi, pw, pwin, params, dom_lo, dom_hi, q_lb, q_par, q_pos, wave_off, recs;
+  hipStream_t stream = nullptr;
+  BatchArgs ba;
+  BatchBoundArgs bb;
+  hipFunction_t bound = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  // admission staging: two pinned buffers, each reusable once its copy ran
+  char *stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  bool stage_busy[2] = {false, false};
+  int cur_stage = 0, n_rec = 0;
+  std::vector<int> free_list;
+  std::vector<BatchHostSlot> hs;
+  long long running = 0;
+  // state polling: pinned copies of all slot states, at most two in flight
+  DevState *hpoll[2] = {nullptr, nullptr};
+  hipEvent_t pev[2] = {nullptr, nullptr};
+  long long poll_seq[2] = {0, 0};
+  int pend_first = 0, pend_n = 0;
+  long long polls = 0;
+  std::vector<std::pair<int64_t, gpugo_result>> done;
+  size_t done_head = 0;
+};
+
+static int batch_sizes(gpugo_module_t m, const gpugo_batch_desc *desc, long long *qcap, long long *pcap,
+                       long long *rec_bytes, int64_t *bytes) {
+  if (!m || !desc) return set_err(GPUGO_E_INVALID, "null pointer");
+  if (!m->batch) return set_err(GPUGO_E_INVALID, "not a batch module");
+  if (desc->slots < 1 || desc->slots > (1 << 20)) return set_err(GPUGO_E_INVALID, "slots must be in [1, 2^20]");
+  if (desc->steps_per_launch < 1 || desc->steps_per_launch > 1024)
+    return set_err(GPUGO_E_INVALID, "steps_per_launch must be in [1, 1024]");
+  if (desc->rounding != GPUGO_ROUND_EXACT && desc->rounding != GPUGO_ROUND_FAST)
+    return set_err(GPUGO_E_INVALID, "unknown rounding %d", desc->rounding);
+  if (!m->batch_bound[desc->rounding])
+    return set_err(GPUGO_E_INVALID, "module has no %s variant", desc->rounding ? "fast" : "exact");
+  if (desc->max_queue < 0 || desc->max_parents < 0 || desc->bound_blocks < 0)
+    return set_err(GPUGO_E_INVALID, "negative capacity");
+  const long long q = desc->max_queue ? desc->max_queue : 65536, p = desc->max_parents ? desc->max_parents : 16384;
+  if (q > 0xfffffff0LL || p > 0xfffffff0LL) return set_err(GPUGO_E_INVALID, "per-slot capacity too large");
+  const long long S = desc->slots, D = m->dim, PM = m->pmax;
+  const long long rb = (long long)sizeof(AdmitHead) + 8 * (PM + 3 * D);
+  long long total = S * (long long)(sizeof(SlotInfo) + sizeof(DevState)) + S * p * (3 * D * 8 + 4) + S * PM * 8 +
+                    S * 2 * D * 8 + S * 2 * q * 16 + (S + 1) * 8 + S * rb;
+  *qcap = q;
+  *pcap = p;
+  *rec_bytes = rb;
+  *bytes = total;
+  return GPUGO_OK;
+}
+
+extern "C" int gpugo_batch_bytes(gpugo_module_t m, const gpugo_batch_desc *desc, int64_t *bytes) {
+  if (!bytes) return set_err(GPUGO_E_INVALID, "null pointer");
+  long long q, p, rb;
+  return batch_sizes(m, desc, &q, &p, &rb, bytes);
+}
+
+extern "C" int gpugo_mem_info(int64_t *free_bytes, int64_t *total_bytes) {
+  if (!free_bytes || !total_bytes) return set_err(GPUGO_E_INVALID, "null pointer");
+  size_t f = 0, t = 0;
+  HIPCHECK(hipMemGetInfo(&f, &t));
+  *free_bytes = (int64_t)f;
+  *total_bytes = (int64_t)t;
+  return GPUGO_OK;
+}
+
+static int batch_enqueue_step(gpugo_batch_t b) {
+  hipLaunchKernelGGL(kb_select, dim3((unsigned)b->slots), dim3(SEL_T), 0, b->stream, b->ba);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(kb_plan, dim3(1), dim3(1024), 0, b->stream, b->ba);
+  HIPCHECK(hipGetLastError());
+  void *args[] = {&b->bb};
+  HIPCHECK(hipModuleLaunchKernel(b->bound, (unsigned)b->bound_grid, 1, 1, 256, 1, 1, 0, b->stream, args, nullptr));
+  hipLaunchKernelGGL(kb_compact, dim3((unsigned)b->slots), dim3(256), 0, b->stream, b->ba);
+  HIPCHECK(hipGetLastError());
+  return GPUGO_OK;
+}
+
+extern "C" void gpugo_batch_destroy(gpugo_batch_t b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (int k = 0; k < 2; ++k) {
+    if (b->stage[k]) (void)hipHostFree(b->stage[k]);
+    if (b->hpoll[k]) (void)hipHostFree(b->hpoll[k]);
+    if (b->stage_ev[k]) (void)hipEventDestroy(b->stage_ev[k]);
+    if (b->pev[k]) (void)hipEventDestroy(b->pev[k]);
+  }
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+extern "C" int gpugo_batch_create(gpugo_module_t m, const gpugo_batch_desc *desc, gpugo_batch_t *out) {
+  if (!out) return set_err(GPUGO_E_INVALID, "null pointer");
+  *out = nullptr;
+  long long qcap, pcap, rb;
+  int64_t total;
+  int rc = batch_sizes(m, desc, &qcap, &pcap, &rb, &total);
+  if (rc) return rc;
+  auto *b = new gpugo_batch_s;
+  b->m = m;
+  b->desc = *desc;
+  b->dmax = m->dim;
+  b->pmax = m->pmax;
+  b->slots = desc->slots;
+  b->qcap = qcap;
+  b->pcap = pcap;
+  b->rec_bytes = rb;
+  b->bound = m->batch_bound[desc->rounding];
+  auto fail = [&](int code) {
+    gpugo_batch_destroy(b);
+    return code;
+  };
+  const long long S = b->slots, D = b->dmax;
+  hipError_t e;
+#define BALLOC(buf, bytes)                                                                     \
+  if ((e = (buf).alloc(bytes)) != hipSuccess)                                                  \
+    return fail(set_err(e == hipErrorOutOfMemory ? GPUGO_E_NOMEM : GPUGO_E_HIP, "alloc %s: %s", \
+                        #buf, hipGetErrorString(e)))
+  BALLOC(b->info, sizeof(SlotInfo) * S);
+  BALLOC(b->st, sizeof(DevState) * S);
+  BALLOC(b->plo, sizeof(double) * S * pcap * D);
+  BALLOC(b->phi, sizeof(double) * S * pcap * D);
+  BALLOC(b->pw, sizeof(double) * S * pcap * D);
+  BALLOC(b->pwin, sizeof(int) * S * pcap);
+  BALLOC(b->params, sizeof(double) * S * b->pmax);
+  BALLOC(b->dom_lo, sizeof(double) * S * D);
+  BALLOC(b->dom_hi, sizeof(double) * S * D);
+  BALLOC(b->q_lb, sizeof(double) * S * 2 * qcap);
+  BALLOC(b->q_par, sizeof(unsigned) * S * 2 * qcap);
+  BALLOC(b->q_pos, sizeof(unsigned) * S * 2 * qcap);
+  BALLOC(b->wave_off, sizeof(long long) * (S + 1));
+  BALLOC(b->recs, (size_t)(rb * S));
+#undef BALLOC
+  if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess)
+    return fail(set_err(GPUGO_E_HIP, "stream: %s", hipGetErrorString(e)));
+  for (int k = 0; k < 2; ++k) {
+    if ((e = hipHostMalloc((void **)&b->stage[k], (size_t)(rb * S), hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostMalloc((void **)&b->hpoll[k], sizeof(DevState) * S, hipHostMallocDefault)) != hipSuccess)
+      return fail(set_err(GPUGO_E_NOMEM, "pinned buffers: %s", hipGetErrorString(e)));
+    if ((e = hipEventCreateWithFlags(&b->stage_ev[k], hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&b->pev[k], hipEventDisableTiming)) != hipSuccess)
+      return fail(set_err(GPUGO_E_HIP, "event: %s", hipGetErrorString(e)));
+  }
+  // all slots inactive
+  {
+    std::vector<SlotInfo> zi(S);
+    memset(zi.data(), 0, sizeof(SlotInfo) * S);
+    if ((e = hipMemcpy(b->info.p, zi.data(), sizeof(SlotInfo) * S, hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(set_err(GPUGO_E_HIP, "upload: %s", hipGetErrorString(e)));
+    std::vector<DevState> zs(S);
+    memset(zs.data(), 0, sizeof(DevState) * S);
+    for (auto &z : zs) z.status = GPUGO_ST_EXHAUSTED;
+    if ((e = hipMemcpy(b->st.p, zs.data(), sizeof(DevState) * S, hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(set_err(GPUGO_E_HIP, "upload: %s", hipGetErrorString(e)));
+  }
+  BatchArgs &a = b->ba;
+  a.info = b->info.as<SlotInfo>();
+  a.st = b->st.as<DevState>();
+  a.q_lb = b->q_lb.as<double>();
+  a.q_par = b->q_par.as<unsigned>();
+  a.q_pos = b->q_pos.as<unsigned>();
+  a.plo = b->plo.as<double>();
+  a.phi = b->phi.as<double>();
+  a.pw = b->pw.as<double>();
+  a.pwin = b->pwin.as<int>();
+  a.params = b->params.as<double>();
+  a.dom_lo = b->dom_lo.as<double>();
+  a.dom_hi = b->dom_hi.as<double>();
+  a.wave_off = b->wave_off.as<long long>();
+  a.pcap = pcap;
+  a.qcap = qcap;
+  a.dmax = b->dmax;
+  a.pmax = b->pmax;
+  a.slots = b->slots;
+  a.pad0 = 0;
+  BatchBoundArgs &g = b->bb;
+  memset(&g, 0, sizeof g);
+  g.info = a.info;
+  g.st = a.st;
+  g.plo = a.plo;
+  g.phi = a.phi;
+  g.pw = a.pw;
+  g.pwin = a.pwin;
+  g.params = a.params;
+  g.dom_lo = a.dom_lo;
+  g.dom_hi = a.dom_hi;
+  g.q_lb = a.q_lb;
+  g.q_par = a.q_par;
+  g.q_pos = a.q_pos;
+  g.wave_off = a.wave_off;
+  g.pcap = pcap;
+  g.qcap = qcap;
+  g.slots = b->slots;
+  g.pmax = b->pmax;
+  if (desc->bound_blocks > 0) {
+    b->bound_grid = desc->bound_blocks;
+  } else {
+    int dev = 0, cus = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    b->bound_grid = 8 * (cus > 0 ? cus : 256);
+  }
+  b->hs.resize(S);
+  b->free_list.reserve(S);
+  for (long long s = S - 1; s >= 0; --s) b->free_list.push_back((int)s);
+  if (desc->use_graph) {
+    if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return fail(set_err(GPUGO_E_HIP, "graph capture: %s", hipGetErrorString(e)));
+    for (int k = 0; k < desc->steps_per_launch; ++k) {
+      rc = batch_enqueue_step(b);
+      if (rc) {
+        hipGraph_t gtmp = nullptr;
+        (void)hipStreamEndCapture(b->stream, &gtmp);
+        if (gtmp) (void)hipGraphDestroy(gtmp);
+        return fail(rc);
+      }
+    }
+    if ((e = hipStreamEndCapture(b->stream, &b->graph)) != hipSuccess)
+      return fail(set_err(GPUGO_E_HIP, "graph capture: %s", hipGetErrorString(e)));
+    if ((e = hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0)) != hipSuccess)
+      return fail(set_err(GPUGO_E_HIP, "graph instantiate: %s", hipGetErrorString(e)));
+  }
+  *out = b;
+  return GPUGO_OK;
+}
+
+extern "C" int gpugo_batch_admit(gpugo_batch_t b, int32_t family, const gpugo_problem_desc *desc, int64_t tag,
+                                 int32_t *slot_out) {
+  if (!b || !desc) return set_err(GPUGO_E_INVALID, "null pointer");
+  gpugo_module_t m = b->m;
+  if (family < 0 || family >= m->nfam) return set_err(GPUGO_E_INVALID, "family %d out of range [0,%d)", family, m->nfam);
+  const int d = desc->d;
+  if (d < 1 || d > b->dmax) return set_err(GPUGO_E_INVALID, "d must be in [1,%d] for this batch module", b->dmax);
+  const int np = m->fam_nparams[family];
+  if (desc->n_params != np)
+    return set_err(GPUGO_E_INVALID, "family %d expects %d parameters (dmax layout), got %lld", family, np,
+                   (long long)desc->n_params);
+  if (np > 0 && !desc->params) return set_err(GPUGO_E_INVALID, "null params");
+  for (int i = 0; i < np; ++i) {
+    const double v = desc->params[i];
+    if (!std::isfinite(v)) return set_err(GPUGO_E_DOMAIN, "params[%d] is not finite", i);
+    const int s = m->fam_psign[(size_t)family * m->pmax + i];
+    if ((s > 0 && v < 0) || (s < 0 && v > 0))
+      return set_err(GPUGO_E_DOMAIN, "params[%d] = %g violates the sign the module was compiled for", i, v);
+  }
+  if (!desc->lo || !desc->hi) return set_err(GPUGO_E_INVALID, "null box");
+  int rc = check_intervals(desc->lo, desc->hi, d, "box");
+  if (rc) return rc;
+  for (int j = 0; j < d; ++j)
+    if (!std::isfinite(desc->lo[j]) || !std::isfinite(desc->hi[j]))
+      return set_err(GPUGO_E_DOMAIN, "box coordinate %d is not finite", j);
+  if (desc->wid < 2 || desc->wid > 64) return set_err(GPUGO_E_INVALID, "wid must be in [2,64]");
+  if (desc->cycle < 1) return set_err(GPUGO_E_INVALID, "cycle must be >= 1");
+  if (desc->k_select < 1 || desc->k_select > SEL_KMAX)
+    return set_err(GPUGO_E_INVALID, "k_select must be in [1,%d]", SEL_KMAX);
+  if (desc->n_samples < 1 || desc->n_samples > 64) return set_err(GPUGO_E_INVALID, "n_samples must be in [1,64]");
+  if (std::isnan(desc->tol_f) || std::isnan(desc->init_ub)) return set_err(GPUGO_E_DOMAIN, "NaN tolerance or init_ub");
+  if (!(desc->tol_f > 0)) return set_err(GPUGO_E_INVALID, "batch problems need tol_f > 0");
+  const int c = desc->cycle < d ? desc->cycle : d;
+  unsigned W = 0;
+  if (!ipow_checked(desc->wid, c, &W) || W > (1u << 24))
+    return set_err(GPUGO_E_INVALID, "wid^cycle must be <= 2^24 in a batch");
+  if ((long long)desc->k_select * W > b->qcap)
+    return set_err(GPUGO_E_INVALID, "k_select * wid^cycle = %lld exceeds the per-slot queue (%lld)",
+                   (long long)desc->k_select * W, b->qcap);
+  if (b->free_list.empty()) return set_err(GPUGO_E_CAPACITY, "no free slot");
+  // staging buffer: wait until its previous copy has run
+  const int k = b->cur_stage;
+  if (b->n_rec == 0 && b->stage_busy[k]) {
+    HIPCHECK(hipEventSynchronize(b->stage_ev[k]));
+    b->stage_busy[k] = false;
+  }
+  const int s = b->free_list.back();
+  char *rec = b->stage[k] + (long long)b->n_rec * b->rec_bytes;
+  memset(rec, 0, (size_t)b->rec_bytes);
+  AdmitHead *h = (AdmitHead *)rec;
+  h->slot = s;
+  SlotInfo &I = h->info;
+  I.active = 1;
+  I.family = family;
+  I.d = d;
+  I.c = c;
+  I.wid = desc->wid;
+  I.k_select = desc->k_select;
+  I.n_samples = desc->n_samples;
+  I.flags = 0;
+  I.W = W;
+  I.max_iters = desc->max_iters > 0 ? desc->max_iters : (1LL << 40);
+  I.tol_f = desc->tol_f;
+  I.inv_wid = 1.0 / (double)desc->wid;
+  I.half_inv_wid = 0.5 / (double)desc->wid;
+  DevState &z = h->st;
+  z.ub_key = order_key(desc->init_ub);
+  z.ub_snap = desc->init_ub;
+  z.lb_global = -__builtin_huge_val();
+  z.out_min_lb = __builtin_huge_val();
+  z.width = __builtin_huge_val();
+  z.parents_used = 1;
+  z.first_parent = 0;
+  z.n_split = 1;
+  z.status = GPUGO_ST_RUNNING;
+  z.lbmin_key = ~0ull;
+  z.lbmax_key = 0ull;
+  z.fresh = 1;
+  double *pr = (double *)(h + 1);
+  double *lo = pr + b->pmax, *hi = lo + b->dmax, *pw = hi + b->dmax;
+  for (int i = 0; i < np; ++i) pr[i] = desc->params[i];
+  for (int j = 0; j < d; ++j) {
+    lo[j] = desc->lo[j];
+    hi[j] = desc->hi[j];
+    pw[j] = (desc->hi[j] - desc->lo[j]) / (double)desc->wid;
+  }
+  ++b->n_rec;
+  b->free_list.pop_back();
+  BatchHostSlot &H = b->hs[s];
+  H.running = 1;
+  H.tag = tag;
+  H.admit_poll = b->polls;
+  H.t0 = std::chrono::steady_clock::now();
+  ++b->running;
+  if (slot_out) *slot_out = s;
+  return GPUGO_OK;
+}
+
+static int batch_flush(gpugo_batch_t b) {
+  if (b->n_rec == 0) return GPUGO_OK;
+  const int k = b->cur_stage;
+  HIPCHECK(hipMemcpyAsync(b->recs.p, b->stage[k], (size_t)(b->n_rec * b->rec_bytes), hipMemcpyHostToDevice,
+                          b->stream));
+  HIPCHECK(hipEventRecord(b->stage_ev[k], b->stream));
+  b->stage_busy[k] = true;
+  hipLaunchKernelGGL(kb_admit, dim3((unsigned)b->n_rec), dim3(64), 0, b->stream, b->ba, (const char *)b->recs.p,
+                     b->rec_bytes);
+  HIPCHECK(hipGetLastError());
+  b->n_rec = 0;
+  b->cur_stage ^= 1;
+  return GPUGO_OK;
+}
+
+// result of a finished slot from its final state
+static void batch_result(const DevState &s, gpugo_result *r, double ms) {
+  memset(r, 0, sizeof *r);
+  r->lb = s.lb_global;
+  r->ub = key_value(s.ub_key);
+  r->width = s.width;
+  r->status = s.status;
+  r->iterations = s.iterations;
+  r->boxes_bounded = s.bounded;
+  r->pruned_value = s.pruned_value;
+  r->pruned_kkt = s.pruned_kkt;
+  r->queue_size = s.compact_count;
+  r->queue_peak = s.queue_peak;
+  r->n_out = s.n_out;
+  r->ms = ms;
+}
+
+// wait for the oldest poll in flight and collect the slots it shows finished
+static int batch_collect_one(gpugo_batch_t b) {
+  if (b->pend_n == 0) return GPUGO_OK;
+  const int k = b->pend_first;
+  HIPCHECK(hipEventSynchronize(b->pev[k]));
+  const DevState *ps = b->hpoll[k];
+  const long long seq = b->poll_seq[k];
+  const auto now = std::chrono::steady_clock::now();
+  for (int s = 0; s < b->slots; ++s) {
+    BatchHostSlot &H = b->hs[s];
+    if (!H.running || H.admit_poll > seq || ps[s].status == GPUGO_ST_RUNNING) continue;
+    gpugo_result r;
+    batch_result(ps[s], &r, std::chrono::duration<double, std::milli>(now - H.t0).count());
+    b->done.emplace_back(H.tag, r);
+    H.running = 0;
+    b->free_list.push_back(s);
+    --b->running;
+  }
+  b->pend_first ^= 1;
+  --b->pend_n;
+  return GPUGO_OK;
+}
+
+extern "C" int gpugo_batch_run(gpugo_batch_t b, int64_t launches) {
+  if (!b) return set_err(GPUGO_E_INVALID, "null batch");
+  if (launches < 0) return set_err(GPUGO_E_INVALID, "negative launches");
+  int rc = batch_flush(b);
+  if (rc) return rc;
+  for (int64_t i = 0; i < launches && b->running > 0; ++i) {
+    if (b->gexec) {
+      HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
+    } else {
+      for (int k = 0; k < b->desc.steps_per_launch; ++k) {
+        rc = batch_enqueue_step(b);
+        if (rc) return rc;
+      }
+    }
+    if (b->pend_n == 2) {
+      rc = batch_collect_one(b);
+      if (rc) return rc;
+    }
+    const int k = (b->pend_first + b->pend_n) & 1;
+    HIPCHECK(hipMemcpyAsync(b->hpoll[k], b->st.p, sizeof(DevState) * b->slots, hipMemcpyDeviceToHost, b->stream));
+    HIPCHECK(hipEventRecord(b->pev[k], b->stream));
+    b->poll_seq[k] = b->polls++;
+    ++b->pend_n;
+    if (b->pend_n == 2) {
+      rc = batch_collect_one(b);
+      if (rc) return rc;
+    }
+  }
+  return GPUGO_OK;
+}
+
+extern "C" int gpugo_batch_sync(gpugo_batch_t b) {
+  if (!b) return set_err(GPUGO_E_INVALID, "null batch");
+  while (b->pend_n > 0) {
+    int rc = batch_collect_one(b);
+    if (rc) return rc;
+  }
+  HIPCHECK(hipStreamSynchronize(b->stream));
+  return GPUGO_OK;
+}
+
+extern "C" int gpugo_batch_results(gpugo_batch_t b, int64_t max, int64_t *tags, gpugo_result *res, int64_t *n) {
+  if (!b || !n) return set_err(GPUGO_E_INVALID, "null pointer");
+  if (max < 0) return set_err(GPUGO_E_INVALID, "negative max");
+  int64_t k = 0;
+  while (k < max && b->done_head < b->done.size()) {
+    if (tags) tags[k] = b->done[b->done_head].first;
+    if (res) res[k] = b->done[b->done_head].second;
+    ++k;
+    ++b->done_head;
+  }
+  if (b->done_head == b->done.size()) {
+    b->done.clear();
+    b->done_head = 0;
+  }
+  *n = k;
+  return GPUGO_OK;
+}
+
+extern "C" int gpugo_batch_counts(gpugo_batch_t b, int64_t *running, int64_t *free_slots, int64_t *pending) {
+  if (!b) return set_err(GPUGO_E_INVALID, "null batch");
+  if (running) *running = b->running;
+  if (free_slots) *free_slots = (int64_t)b->free_list.size();
+  if (pending) *pending = (int64_t)(b->done.size() - b->done_head);
   return GPUGO_OK;
 }

@@ -412,5 +412,61 @@ def module_source(obj: Objective, roundings=("fast",)) -> str:
     ])
 
 
+def batch_module_source(objs, roundings=("fast",)) -> str:
+    """hipRTC source of a batch module: the families `objs` (variable-
+    dimension objectives from problems.objective_dynamic, all with the same
+    maximum dimension) behind the dispatchers fam_value / fam_kkt, plus the
+    persistent batch split+bound kernel."""
+    bad = set(roundings) - {"exact", "fast"}
+    if bad or not roundings:
+        raise ValueError(f"roundings must be a non-empty subset of ('exact', 'fast'), got {roundings}")
+    objs = list(objs)
+    if not objs:
+        raise ValueError("a batch module needs at least one family")
+    dmax = objs[0].d
+    if any(o.d != dmax for o in objs):
+        raise ValueError("all families of a batch module share one maximum dimension")
+    pmax = max(o.n_params for o in objs)
+    psign = []
+    for o in objs:
+        s = param_signs(o)
+        psign += s + [0] * (pmax - len(s))
+    lines = [
+        "// generated by gpugo_amd.emit: batch module " + ", ".join(o.name for o in objs),
+        f"#define GPUGO_WITH_EXACT {int('exact' in roundings)}",
+        f"#define GPUGO_WITH_FAST {int('fast' in roundings)}",
+        f"#define GPUGO_DMAX {dmax}",
+        f"#define GPUGO_NFAM {len(objs)}",
+        f"#define GPUGO_PMAX {pmax}",
+        "#define GPUGO_FAM_NPARAMS " + ", ".join(str(o.n_params) for o in objs),
+        "#define GPUGO_FAM_PSIGN " + ", ".join(str(v) for v in psign),
+        _read("ia.hpp"),
+        _read("bnb_args.hpp"),
+    ]
+    for i, o in enumerate(objs):
+        lines.append(code_lists(o, suffix=f"_f{i}", dynamic=True, header=False))
+    val = "\n".join(f"    case {i}: return obj_value_f{i}<R>(x, P, d);" for i in range(len(objs)))
+    kkt = "\n".join(f"    case {i}: return obj_kkt_f{i}<R>(x, P, dlo, dhi, d);" for i in range(len(objs)))
+    lines.append(f"""namespace gpugo {{
+template <int R>
+__device__ __forceinline__ iv<double> fam_value(int f, const iv<double> *x, const double *P, int d) {{
+  switch (f) {{
+{val}
+    default: return mk<double>(-__builtin_huge_val(), __builtin_huge_val());
+  }}
+}}
+template <int R>
+__device__ __forceinline__ bool fam_kkt(int f, const iv<double> *x, const double *P, const double *dlo,
+                                        const double *dhi, int d) {{
+  switch (f) {{
+{kkt}
+    default: return false;
+  }}
+}}
+}}  // namespace gpugo""")
+    lines.append(_read("bnb_batch.hpp"))
+    return "\n".join(lines)
+
+
 def source_key(src: str, options: str = "") -> str:
     return hashlib.sha256((options + "\0" + src).encode()).hexdigest()[:32]

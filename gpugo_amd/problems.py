@@ -169,6 +169,54 @@ def config2(seed: int = 2, d: int = 50) -> ProblemSpec:
                        k_select=1)
 
 
+def _family_params(rng, family: str, d: int) -> np.ndarray:
+    """Coefficients of a family drawn as in the C0-C2 workloads."""
+    if family == "sepcos":
+        return np.concatenate(draw_sepcos(rng, d))
+    if family == "sepcos_chain":
+        a, b, c = draw_sepcos(rng, d)
+        return np.concatenate([a, b, c, rng.uniform(1.0, 100.0, d - 1)])
+    A = rng.uniform(10.0, 30.0)
+    B = rng.uniform(0.1, 0.3)
+    C = rng.uniform(math.pi, 3.0 * math.pi)
+    return np.concatenate([[A, B, C], rng.uniform(0.0, 0.1, d)])
+
+
+def batch_problem(k: int) -> ProblemSpec:
+    """Problem k of the 4096-problem batch: family uniform over the three
+    families, d = round(exp(U[ln 2, ln 32])), box [-h, h]^d with h ~ U[2, 32],
+    tol in {1e-4, 1e-6}; drawn from default_rng([3, k])."""
+    rng = np.random.default_rng([3, k])
+    family = FAMILIES[int(rng.integers(0, 3))]
+    d = int(round(math.exp(rng.uniform(math.log(2.0), math.log(32.0)))))
+    h = rng.uniform(2.0, 32.0)
+    tol = (1e-4, 1e-6)[int(rng.integers(0, 2))]
+    params = _family_params(rng, family, d)
+    f_star = None if family == "sepcos_chain" else 0.0
+    return ProblemSpec(family, d, params, np.full(d, -h), np.full(d, h), tol, f_star=f_star,
+                       name=f"batch#{k} {family} d={d}")
+
+
+def served_problem(k: int) -> ProblemSpec:
+    """Problem k of the served workload: family uniform, d log-uniform in
+    [2, 100], tol log-uniform in [1e-8, 1e-4], box [-h, h]^d with
+    h ~ U[2, 32]; drawn from default_rng([4, k])."""
+    rng = np.random.default_rng([4, k])
+    family = FAMILIES[int(rng.integers(0, 3))]
+    d = int(round(math.exp(rng.uniform(math.log(2.0), math.log(100.0)))))
+    tol = float(math.exp(rng.uniform(math.log(1e-8), math.log(1e-4))))
+    h = rng.uniform(2.0, 32.0)
+    params = _family_params(rng, family, d)
+    f_star = None if family == "sepcos_chain" else 0.0
+    return ProblemSpec(family, d, params, np.full(d, -h), np.full(d, h), tol, f_star=f_star,
+                       name=f"served#{k} {family} d={d}")
+
+
+def arrivals(lam: float, n: int, seed: int = 40) -> np.ndarray:
+    """Poisson arrival times (s) of n requests at rate lam."""
+    return np.cumsum(np.random.default_rng(seed).exponential(1.0 / lam, n))
+
+
 def rastrigin(d: int) -> ProblemSpec:
     """Reference Rastrigin on its box [-5.5, 6] (7-Rastrigin/rastrigin50.py:852)."""
     p = np.concatenate([np.ones(d), np.full(d, 10.0), np.full(d, 2.0 * math.pi)])

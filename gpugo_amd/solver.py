@@ -33,14 +33,18 @@ _mod_lock = threading.Lock()
 
 def rtc_compile_to_cache(obj: Objective, options: str = "", roundings=("fast",)) -> str:
     """Compile an objective module with hipRTC (no GPU needed) into the cache."""
+    return compile_source_to_cache(module_source(obj, tuple(roundings)), obj.name, options)
+
+
+def compile_source_to_cache(src: str, name: str, options: str = "") -> str:
+    """hipRTC-compile generated module text into the on-disk code-object cache."""
     lib = _lib.load()
-    src = module_source(obj, tuple(roundings))
     key = source_key(src, options)
     path = os.path.join(_CACHE_DIR, key + ".co")
     if os.path.exists(path):
         return path
     size = ctypes.c_size_t(0)
-    check(lib.gpugo_rtc_compile(src.encode(), (obj.name + ".hip").encode(), options.encode(),
+    check(lib.gpugo_rtc_compile(src.encode(), (name + ".hip").encode(), options.encode(),
                                 ctypes.byref(size)))
     buf = ctypes.create_string_buffer(size.value)
     check(lib.gpugo_rtc_code_copy(buf, size.value))

@@ -213,6 +213,61 @@ int  gpugo_boxes_copy(gpugo_problem_t p, int64_t max, double *lo, double *hi,
                       double *lb, int64_t *n);
 void gpugo_problem_destroy(gpugo_problem_t p);
 
+/* ---- batches: many independent problems resident on one GPU -------------
+ * No counterpart in the reference, which solves one problem per script run
+ * (1-Ackley/ackley50.py:950-1395); each slot runs exactly the per-problem
+ * loop above (same select, split+bound and compaction semantics), so a
+ * problem's result in a batch equals its gpugo_solve result with the same
+ * schedule and rounding (the mean-value flag is not available here).
+ *
+ * A batch module (gpugo_amd.emit.batch_module_source) holds several
+ * objective families in variable-dimension form (runtime d <= dmax).  The
+ * batch owns `slots` device-resident problem slots: queues, parent arenas,
+ * incumbents and states stay in HBM across the whole run.  Problems are
+ * admitted into free slots at any time (continuous batching); every launch
+ * advances all running slots by `steps_per_launch` steps (one hipGraph when
+ * use_graph is set); finished slots are reported and freed.
+ * Threading: a batch handle is used by one host thread at a time. */
+typedef struct {
+  int32_t slots;             /* concurrent problems                          */
+  int32_t steps_per_launch;  /* B&B steps per launch (>= 1)                  */
+  int32_t rounding;          /* enum gpugo_round (module must hold it)       */
+  int32_t use_graph;         /* 1: capture the steps of a launch in a graph  */
+  int64_t max_queue;         /* live boxes per slot (0: 65536)               */
+  int64_t max_parents;       /* parent records per slot (0: 16384)           */
+  int32_t bound_blocks;      /* persistent split+bound grid (0: 4 per CU)    */
+  int32_t reserved;
+} gpugo_batch_desc;
+
+typedef struct gpugo_batch_s *gpugo_batch_t;
+
+/* Device bytes a batch with this description allocates (admission by memory:
+ * compare with gpugo_mem_info before creating). */
+int  gpugo_batch_bytes(gpugo_module_t m, const gpugo_batch_desc *desc, int64_t *bytes);
+int  gpugo_mem_info(int64_t *free_bytes, int64_t *total_bytes);
+int  gpugo_batch_create(gpugo_module_t m, const gpugo_batch_desc *desc, gpugo_batch_t *out);
+/* Admit one problem of objective family `family` into a free slot.  desc
+ * uses the per-problem fields (d <= dmax, lo, hi, params in the family's
+ * dmax layout, wid, cycle, k_select, n_samples, tol_f, init_ub, max_iters);
+ * tol_x, flags and the capacities are not used (per-slot capacities come
+ * from the batch).  Returns GPUGO_E_CAPACITY when no slot is free.  The
+ * problem starts with the next launch. */
+int  gpugo_batch_admit(gpugo_batch_t b, int32_t family, const gpugo_problem_desc *desc, int64_t tag,
+                       int32_t *slot);
+/* Enqueue `launches` launches.  The host keeps one launch in flight and
+ * collects the slots that finished (see gpugo_batch_results).  Returns early
+ * when no slot is running. */
+int  gpugo_batch_run(gpugo_batch_t b, int64_t launches);
+/* Wait for the launch in flight and collect what finished. */
+int  gpugo_batch_sync(gpugo_batch_t b);
+/* Move up to `max` finished results (tags[i], res[i]) out of the batch;
+ * res.ms is the time from admission to the host seeing the result. */
+int  gpugo_batch_results(gpugo_batch_t b, int64_t max, int64_t *tags, gpugo_result *res, int64_t *n);
+/* running: slots admitted and not yet reported; free_slots: slots that can
+ * take a problem now; pending: finished results not yet taken. */
+int  gpugo_batch_counts(gpugo_batch_t b, int64_t *running, int64_t *free_slots, int64_t *pending);
+void gpugo_batch_destroy(gpugo_batch_t b);
+
 #ifdef __cplusplus
 }
 #endif
