@@ -18,71 +18,22 @@
 namespace gpugo {
 
 template <int R>
-__device__ __forceinline__ double batch_sample_ub(int f, const iv<double> *x, const double *__restrict__ P, int d,
+__device__ __forceinline__ double batch_sample_ub(int f, const ChildBox &x, const double *__restrict__ P, int d,
                                                   int S) {
   double best = __builtin_huge_val();
   for (int t = 0; t < S; ++t) {
-    iv<double> xs[GPUGO_DMAX];
-#pragma unroll
-    for (int j = 0; j < GPUGO_DMAX; ++j) {
-      if (j < d) {
-        double w = x[j].hi - x[j].lo;
-        double h = (S == 1) ? w * 0.5 : w / (double)(S + 1);
-        double p = __builtin_fma(h, (double)(t + 1), x[j].lo);
-        xs[j] = mk<double>(p, p);
-      } else {
-        xs[j] = mk<double>(0.0, 0.0);
-      }
-    }
-    iv<double> fv = fam_value<R>(f, xs, P, d);
+    const DiagPoint xs{x, t, S};
+    const iv<double> fv = fam_value<R>(f, xs, P, d);
     if (fv.hi < best) best = fv.hi;
   }
   return best;
 }
 
-// decode with a runtime dimension d <= DMAX (see decode_child)
-__device__ __forceinline__ void batch_decode(const BatchBoundArgs &a, int slot, const SlotInfo &I, long long p,
-                                             unsigned pos, iv<double> *x) {
+__device__ __forceinline__ ChildBox batch_child(const BatchBoundArgs &a, int slot, const SlotInfo &I, long long p,
+                                                unsigned pos) {
   const long long rec = ((long long)slot * a.pcap + p);
-  const double *__restrict__ plo = a.plo + rec * GPUGO_DMAX;
-  const double *__restrict__ phi = a.phi + rec * GPUGO_DMAX;
-  const double *__restrict__ pw = a.pw + rec * GPUGO_DMAX;
-  const int win = a.pwin[rec];
-  const int d = I.d;
-  const unsigned wid = (unsigned)I.wid;
-  const int bpd = 32 - __builtin_clz(wid - 1u);
-  unsigned long long packed = 0;
-  if ((wid & (wid - 1u)) == 0u) {
-    packed = pos;
-  } else {
-    unsigned r = pos;
-    for (int k = 0; k < I.c; ++k) {
-      unsigned q = (unsigned)__builtin_fma((double)r, I.inv_wid, I.half_inv_wid);
-      unsigned dg = r - q * wid;
-      packed |= (unsigned long long)dg << (k * bpd);
-      r = q;
-    }
-  }
-  const unsigned long long mask = (1ull << bpd) - 1ull;
-#pragma unroll
-  for (int j = 0; j < GPUGO_DMAX; ++j) {
-    if (j < d) {
-      const double L = plo[j], U = phi[j];
-      int k = j - win;
-      if (k < 0) k += d;
-      if (k < I.c) {
-        const unsigned dg = (unsigned)((packed >> (k * bpd)) & mask);
-        const double w = pw[j];
-        const double lo = __builtin_fma(w, (double)dg, L);
-        const double hi = (dg + 1u == wid) ? U : __builtin_fma(w, (double)(dg + 1u), L);
-        x[j] = mk<double>(lo, hi);
-      } else {
-        x[j] = mk<double>(L, U);
-      }
-    } else {
-      x[j] = mk<double>(0.0, 0.0);
-    }
-  }
+  return child_box(a.plo + rec * GPUGO_DMAX, a.phi + rec * GPUGO_DMAX, a.pw + rec * GPUGO_DMAX, a.pwin[rec], I.d, I.c,
+                   (unsigned)I.wid, I.inv_wid, I.half_inv_wid, pos);
 }
 
 __device__ __forceinline__ long long wave_uniform(long long v) {
@@ -130,8 +81,7 @@ __device__ __forceinline__ void batch_bound_impl(const BatchBoundArgs &a) {
     if (t < n_child) {
       pl = (long long)__builtin_fma((double)t, 1.0 / (double)I.W, 0.5 / (double)I.W);
       pos = (unsigned)(t - pl * (long long)I.W);
-      iv<double> x[GPUGO_DMAX];
-      batch_decode(a, slot, I, first + pl, pos, x);
+      const ChildBox x = batch_child(a, slot, I, first + pl, pos);
       const iv<double> fv = fam_value<R>(f, x, P, d);
       lb = fv.lo;
       if (!(lb == lb)) lb = -__builtin_huge_val();

@@ -1,0 +1,85 @@
+// bnb_box.hpp -- children of a parent record as coordinate accessors, for
+// the code lists of objective and batch modules (x[j] is an interval).
+// Included (as text, by gpugo_amd.emit) after ia.hpp and bnb_args.hpp.
+#pragma once
+
+namespace gpugo {
+
+// Child `pos` of a parent record as a coordinate accessor: x[j]
+// decodes coordinate j when it is used (same arithmetic as decode_child in
+// bnb_kernel.hpp: window coordinates (win + k) mod d for k < c take base-wid
+// digit k of pos, bounds L + w*digit by one fma, the top child ends at U),
+// so no array of d intervals stays live across the code lists.
+struct ChildBox {
+  const double *__restrict__ plo;
+  const double *__restrict__ phi;
+  const double *__restrict__ pw;
+  unsigned long long packed, mask;
+  int win, d, c, bpd;
+  unsigned wid;
+  __device__ __forceinline__ iv<double> operator[](int j) const {
+    // The empty asm hides packed/win from the optimiser, so the digit
+    // shifts and window tests are recomputed at each use instead of being
+    // kept live for all d coordinates, and the parent loads stay here.
+    unsigned long long packed = this->packed;
+    int win = this->win;
+    asm volatile("" : "+v"(packed), "+v"(win)::"memory");
+    const double L = plo[j], U = phi[j];
+    int k = j - win;
+    if (k < 0) k += d;
+    if (k < c) {
+      const unsigned dg = (unsigned)((packed >> (k * bpd)) & mask);
+      const double w = pw[j];
+      const double lo = __builtin_fma(w, (double)dg, L);
+      const double hi = (dg + 1u == wid) ? U : __builtin_fma(w, (double)(dg + 1u), L);
+      return mk<double>(lo, hi);
+    }
+    return mk<double>(L, U);
+  }
+};
+
+// Sample point t of S on the diagonal of a box,
+// x_j = lo_j + (hi_j - lo_j)/(S+1) * (t+1) (1-Ackley/ackley50.py:288-290).
+struct DiagPoint {
+  ChildBox b;
+  int t, S;
+  __device__ __forceinline__ iv<double> operator[](int j) const {
+    const iv<double> x = b[j];
+    const double w = x.hi - x.lo;
+    const double h = (S == 1) ? w * 0.5 : w / (double)(S + 1);
+    const double p = __builtin_fma(h, (double)(t + 1), x.lo);
+    return mk<double>(p, p);
+  }
+};
+
+__device__ __forceinline__ ChildBox child_box(const double *plo, const double *phi, const double *pw, int win, int d,
+                                              int c, unsigned wid, double inv_wid, double half_inv_wid, unsigned pos) {
+  ChildBox x;
+  x.plo = plo;
+  x.phi = phi;
+  x.pw = pw;
+  x.win = win;
+  x.d = d;
+  x.c = c;
+  x.wid = wid;
+  // digits packed at bpd bits each (wid^c < 2^32 keeps c*bpd <= 64);
+  // q = floor(r / wid) exactly as fma(r, 1/wid, 1/(2 wid)) for r < 2^32
+  x.bpd = 32 - __builtin_clz(wid - 1u);
+  x.mask = (1ull << x.bpd) - 1ull;
+  unsigned long long packed = 0;
+  if ((wid & (wid - 1u)) == 0u) {
+    packed = pos;
+  } else {
+    unsigned r = pos;
+    for (int k = 0; k < c; ++k) {
+      const unsigned q = (unsigned)__builtin_fma((double)r, inv_wid, half_inv_wid);
+      const unsigned dg = r - q * wid;
+      packed |= (unsigned long long)dg << (k * x.bpd);
+      r = q;
+    }
+  }
+  x.packed = packed;
+  return x;
+}
+
+}  // namespace gpugo

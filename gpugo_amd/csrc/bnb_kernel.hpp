@@ -1,14 +1,22 @@
 // bnb_kernel.hpp -- kernels of an objective module.  Included (as text, by
 // gpugo_amd.emit) after ia.hpp, bnb_args.hpp and the generated code lists,
 // which define GPUGO_D, GPUGO_NPARAMS and
-//   obj_value<R>(x, P)            interval value          (<fn>Value_IA)
-//   obj_grad<R>(x, P, g)          interval gradient       (<fn>Derivative_IA)
-//   obj_kkt<R>(x, P, dlo, dhi)    monotonicity prune test (<fn>Derivative_IA)
+//   obj_value<R>(x, P, d)          interval value          (<fn>Value_IA)
+//   obj_grad<R>(x, P, g, d)        interval gradient       (<fn>Derivative_IA)
+//   obj_kkt<R>(x, P, dlo, dhi, d)  monotonicity prune test (<fn>Derivative_IA)
+// where x[j] is the j-th coordinate interval (an array or an accessor) and
+// d the runtime dimension (= GPUGO_D; objectives built with per-coordinate
+// guards branch on it, which keeps the code lists' live ranges short)
 // Reference kernels replaced: sampling and ruleOutGPU
 // (GPUGO_1.1/1-Ackley/ackley50.py:602-766).
 #pragma once
 
 #define GPUGO_MODULE_ABI_V 1
+// dimensions above this decode child coordinates on use instead of into an
+// array (the array would keep 4 d registers live through the code lists)
+#ifndef GPUGO_LAZY_D
+#define GPUGO_LAZY_D 16
+#endif
 
 namespace gpugo {
 
@@ -16,7 +24,7 @@ namespace gpugo {
 // x_j = lo_j + (hi_j - lo_j)/(S+1) * (t+1) (1-Ackley/ackley50.py:288-290);
 // S = 1 is the midpoint.
 template <int R>
-__device__ __forceinline__ double sample_ub(const iv<double> *x, const double *__restrict__ P, int S) {
+__device__ __forceinline__ double sample_ub(const iv<double> *x, const double *__restrict__ P, int S, int d) {
   double best = __builtin_huge_val();
   for (int t = 0; t < S; ++t) {
     iv<double> xs[GPUGO_D];
@@ -27,10 +35,42 @@ __device__ __forceinline__ double sample_ub(const iv<double> *x, const double *_
       double p = __builtin_fma(h, (double)(t + 1), x[j].lo);
       xs[j] = mk<double>(p, p);
     }
-    iv<double> f = obj_value<R>(xs, P);
+    iv<double> f = obj_value<R>(xs, P, d);
     if (f.hi < best) best = f.hi;
   }
   return best;
+}
+
+// The same bound on a lazily decoded child (coordinate accessor).
+template <int R>
+__device__ __forceinline__ double sample_ub(const ChildBox &x, const double *__restrict__ P, int S, int d) {
+  double best = __builtin_huge_val();
+  for (int t = 0; t < S; ++t) {
+    const DiagPoint xs{x, t, S};
+    const iv<double> f = obj_value<R>(xs, P, d);
+    if (f.hi < best) best = f.hi;
+  }
+  return best;
+}
+
+// value test, then for survivors the monotonicity test and the sample bound
+// (the gradient reuses the value code list's temporaries, which then die
+// before the sample evaluation: fewer live registers)
+template <int R, class XS>
+__device__ __forceinline__ void bound_plain(const BoundArgs &a, const XS &x, double ub0, double &lb, double &su,
+                                            bool &keep, bool &pv, bool &pk) {
+  const iv<double> f = obj_value<R>(x, a.params, a.d);
+  lb = f.lo;
+  if (!(lb == lb)) lb = -__builtin_huge_val();
+  if (lb <= ub0) {
+    // a child pruned by value cannot improve the incumbent: its sample
+    // bound is >= its lower bound > ub0; so only survivors are sampled
+    keep = !obj_kkt<R>(x, a.params, a.dom_lo, a.dom_hi, a.d);
+    pk = !keep;
+    su = sample_ub<R>(x, a.params, a.n_samples, a.d);
+  } else {
+    pv = true;
+  }
 }
 
 // Monotonicity test on a computed gradient enclosure
@@ -117,27 +157,35 @@ __device__ __forceinline__ void bound_impl(const BoundArgs &a) {
     if (active) {
       pl = (long long)__builtin_fma((double)t, invW, halfInvW);
       pos = (unsigned)(t - pl * (long long)a.W);
-      iv<double> x[GPUGO_D];
-      decode_child(a, first + pl, pos, x);
-      iv<double> f = obj_value<R>(x, a.params);
-      lb = f.lo;
-      if (!(lb == lb)) lb = -__builtin_huge_val();
-      if (lb <= ub0) {
-        // a child pruned by value cannot improve the incumbent: its sample
-        // bound is >= its lower bound > ub0; so only survivors are sampled
-        if (MV) {
+      if (!MV && GPUGO_D > GPUGO_LAZY_D) {
+        // large d: coordinates decoded where the code lists use them
+        const long long p = first + pl;
+        const ChildBox x = child_box(a.plo + p * GPUGO_D, a.phi + p * GPUGO_D, a.pw + p * GPUGO_D, a.pwin[p],
+                                     GPUGO_D, a.c, (unsigned)a.wid, a.inv_wid, a.half_inv_wid, pos);
+        bound_plain<R>(a, x, ub0, lb, su, keep, pv, pk);
+      } else if (!MV) {
+        iv<double> x[GPUGO_D];
+        decode_child(a, first + pl, pos, x);
+        bound_plain<R>(a, (const iv<double> *)x, ub0, lb, su, keep, pv, pk);
+      } else {
+        iv<double> x[GPUGO_D];
+        decode_child(a, first + pl, pos, x);
+        iv<double> f = obj_value<R>((const iv<double> *)x, a.params, a.d);
+        lb = f.lo;
+        if (!(lb == lb)) lb = -__builtin_huge_val();
+        if (lb <= ub0) {
           // mean-value form f(c) + sum_j G_j (X_j - c_j) from the gradient
           // the monotonicity test needs anyway; c is the S = 1 sample point
           typedef IA<double, R> A;
           iv<double> g[GPUGO_D], xc[GPUGO_D];
-          obj_grad<R>(x, a.params, g);
+          obj_grad<R>((const iv<double> *)x, a.params, g, a.d);
 #pragma unroll
           for (int j = 0; j < GPUGO_D; ++j) {
             double c = __builtin_fma((x[j].hi - x[j].lo) * 0.5, 1.0, x[j].lo);
             xc[j] = mk<double>(c, c);
           }
-          iv<double> s = obj_value<R>(xc, a.params);
-          su = a.n_samples == 1 ? s.hi : sample_ub<R>(x, a.params, a.n_samples);
+          iv<double> s = obj_value<R>((const iv<double> *)xc, a.params, a.d);
+          su = a.n_samples == 1 ? s.hi : sample_ub<R>((const iv<double> *)x, a.params, a.n_samples, a.d);
 #pragma unroll
           for (int j = 0; j < GPUGO_D; ++j) {
             iv<double> dx = mk<double>(A::RR::sub_dn(x[j].lo, xc[j].lo), A::RR::sub_up(x[j].hi, xc[j].lo));
@@ -151,15 +199,8 @@ __device__ __forceinline__ void bound_impl(const BoundArgs &a) {
             pv = true;
           }
         } else {
-          // monotonicity test before the sample: the gradient reuses the
-          // value code list's temporaries, which then die before the sample
-          // evaluation (fewer live registers)
-          keep = !obj_kkt<R>(x, a.params, a.dom_lo, a.dom_hi);
-          pk = !keep;
-          su = sample_ub<R>(x, a.params, a.n_samples);
+          pv = true;
         }
-      } else {
-        pv = true;
       }
     }
     // order-free append of survivors: one global atomic per block
@@ -221,21 +262,21 @@ __device__ __forceinline__ void eval_impl(const EvalArgs &a) {
 #pragma unroll
   for (int j = 0; j < GPUGO_D; ++j) x[j] = mk<double>(a.lo[i * GPUGO_D + j], a.hi[i * GPUGO_D + j]);
   if (a.flo) {
-    iv<double> f = obj_value<R>(x, a.params);
+    iv<double> f = obj_value<R>(x, a.params, a.d);
     a.flo[i] = f.lo;
     a.fhi[i] = f.hi;
   }
   if (a.glo) {
     iv<double> g[GPUGO_D];
-    obj_grad<R>(x, a.params, g);
+    obj_grad<R>(x, a.params, g, a.d);
 #pragma unroll
     for (int j = 0; j < GPUGO_D; ++j) {
       a.glo[i * GPUGO_D + j] = g[j].lo;
       a.ghi[i * GPUGO_D + j] = g[j].hi;
     }
   }
-  if (a.kkt) a.kkt[i] = obj_kkt<R>(x, a.params, a.dom_lo, a.dom_hi) ? 1 : 0;
-  if (a.ub) a.ub[i] = sample_ub<R>(x, a.params, a.n_samples);
+  if (a.kkt) a.kkt[i] = obj_kkt<R>(x, a.params, a.dom_lo, a.dom_hi, a.d) ? 1 : 0;
+  if (a.ub) a.ub[i] = sample_ub<R>(x, a.params, a.n_samples, a.d);
 }
 
 }  // namespace gpugo

@@ -1,7 +1,7 @@
 """Expression -> HIP emitter for objective modules.
 
-The emitted text is: ia.hpp + bnb_args.hpp + the objective's code lists
-(obj_value, obj_grad, obj_kkt) + bnb_kernel.hpp.  It is compiled by hipRTC
+The emitted text is: ia.hpp + bnb_args.hpp + bnb_box.hpp + the objective's
+code lists (obj_value, obj_grad, obj_kkt) + bnb_kernel.hpp.  It is compiled by hipRTC
 (through libgpugo.so) into one code object per objective.  This replaces
 Numba's JIT of each driver's code lists (GPUGO_1.1/1-Ackley/ackley50.py:
 116-479, 602-766).
@@ -279,7 +279,7 @@ def code_lists(obj: Objective, suffix: str = "", dynamic: bool = False, header: 
     obj.check()
     grads = gradient(obj)
     d = obj.d
-    dp = ", int d" if dynamic else ""
+    dp = ", int d"
     out = []
     if header:
         signs = param_signs(obj)
@@ -290,8 +290,10 @@ def code_lists(obj: Objective, suffix: str = "", dynamic: bool = False, header: 
     # value
     f = _Func()
     v = f.lower(obj.value)
-    out.append(f"template <int R>\n__device__ __forceinline__ iv<double> obj_value{suffix}("
-               f"const iv<double> *__restrict__ x, const double *__restrict__ P{dp}) {{")
+    # x is an array of intervals or an accessor whose operator[] decodes
+    # coordinate j on use (fewer live registers at large d)
+    out.append(f"template <int R, class XS>\n__device__ __forceinline__ iv<double> obj_value{suffix}("
+               f"const XS &x, const double *__restrict__ P{dp}) {{")
     out.append("  typedef IA<double, R> A;")
     out += f.lines
     out.append(f"  return {f.as_iv(v)};")
@@ -315,8 +317,8 @@ def code_lists(obj: Objective, suffix: str = "", dynamic: bool = False, header: 
         if dynamic and j > 0:
             f.lines.append(f"  g[{j}] = mk<double>(0.0, 0.0);")
         component(f, j, g, lambda code, j=j: [f"  g[{j}] = {code};"])
-    out.append(f"template <int R>\n__device__ __forceinline__ void obj_grad{suffix}("
-               f"const iv<double> *__restrict__ x, const double *__restrict__ P, "
+    out.append(f"template <int R, class XS>\n__device__ __forceinline__ void obj_grad{suffix}("
+               f"const XS &x, const double *__restrict__ P, "
                f"iv<double> *__restrict__ g{dp}) {{")
     out.append("  typedef IA<double, R> A;")
     out += f.lines
@@ -330,8 +332,8 @@ def code_lists(obj: Objective, suffix: str = "", dynamic: bool = False, header: 
             f"  {{ const iv<double> gj = {code};",
             f"    if (gj.lo > 0.0 && x[{j}].lo != dlo[{j}]) return true;",
             f"    if (gj.hi < 0.0 && x[{j}].hi != dhi[{j}]) return true; }}"])
-    out.append(f"template <int R>\n__device__ __forceinline__ bool obj_kkt{suffix}("
-               f"const iv<double> *__restrict__ x, const double *__restrict__ P, "
+    out.append(f"template <int R, class XS>\n__device__ __forceinline__ bool obj_kkt{suffix}("
+               f"const XS &x, const double *__restrict__ P, "
                f"const double *__restrict__ dlo, const double *__restrict__ dhi{dp}) {{")
     out.append("  typedef IA<double, R> A;")
     out += f.lines
@@ -407,6 +409,7 @@ def module_source(obj: Objective, roundings=("fast",)) -> str:
         f"#define GPUGO_WITH_FAST {int('fast' in roundings)}",
         _read("ia.hpp"),
         _read("bnb_args.hpp"),
+        _read("bnb_box.hpp"),
         code_lists(obj),
         _read("bnb_kernel.hpp"),
     ])
@@ -442,21 +445,22 @@ def batch_module_source(objs, roundings=("fast",)) -> str:
         "#define GPUGO_FAM_PSIGN " + ", ".join(str(v) for v in psign),
         _read("ia.hpp"),
         _read("bnb_args.hpp"),
+        _read("bnb_box.hpp"),
     ]
     for i, o in enumerate(objs):
         lines.append(code_lists(o, suffix=f"_f{i}", dynamic=True, header=False))
     val = "\n".join(f"    case {i}: return obj_value_f{i}<R>(x, P, d);" for i in range(len(objs)))
     kkt = "\n".join(f"    case {i}: return obj_kkt_f{i}<R>(x, P, dlo, dhi, d);" for i in range(len(objs)))
     lines.append(f"""namespace gpugo {{
-template <int R>
-__device__ __forceinline__ iv<double> fam_value(int f, const iv<double> *x, const double *P, int d) {{
+template <int R, class XS>
+__device__ __forceinline__ iv<double> fam_value(int f, const XS &x, const double *P, int d) {{
   switch (f) {{
 {val}
     default: return mk<double>(-__builtin_huge_val(), __builtin_huge_val());
   }}
 }}
-template <int R>
-__device__ __forceinline__ bool fam_kkt(int f, const iv<double> *x, const double *P, const double *dlo,
+template <int R, class XS>
+__device__ __forceinline__ bool fam_kkt(int f, const XS &x, const double *P, const double *dlo,
                                         const double *dhi, int d) {{
   switch (f) {{
 {kkt}

@@ -88,8 +88,14 @@ def _build(family: str, d: int, dynamic: bool) -> Objective:
     return Objective(d, n_params(family, d), value, grad, name=f"{family}_{tag}{d}")
 
 
+# Above this dimension the fixed-d code lists use the guarded form too: the
+# per-coordinate branches on the runtime d (always taken) split the code
+# into short blocks, so coordinates decoded on use do not stay live.
+GUARDED_D = 16
+
+
 def objective(family: str, d: int) -> Objective:
-    return _build(family, d, False)
+    return _build(family, d, d > GUARDED_D)
 
 
 def objective_dynamic(family: str, dmax: int) -> Objective:
