@@ -50,6 +50,11 @@ def parse():
     ap.add_argument("--rounding", choices=["fast", "exact"], default="fast")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--legs", default="c1,c2,batch,served",
+                    help="comma list: c1 (headline), c2 (d=50), batch (4096 problems), served (Poisson)")
+    ap.add_argument("--batch-n", type=int, default=4096)
+    ap.add_argument("--seq-sample", type=int, default=256,
+                    help="problems of the batch solved one at a time for the sequential baseline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (scales HBM byte counters)")
     return ap.parse_args()
@@ -103,6 +108,168 @@ def cpu_reference_schedule(spec, steps=3):
     return {"boxes_bounded_per_s": res["boxes_bounded"] / dt, "steps": steps, "cores": threads}
 
 
+def bound_roofline(obj, results, mean_value=False, kernel="gpugo_bound_fast"):
+    """Algorithmic FP64 FLOP of the split+bound launches over their
+    HIP-event time: every child gets the value code list, survivors of the
+    value test also the gradient and the sample (with the mean-value bound
+    also the midpoint value and d interval products and additions)."""
+    from gpugo_amd import emit
+    fl = emit.flop_counts(obj)
+    boxes = sum(x["boxes_bounded"] for x in results)
+    pv = sum(x["pruned_value"] for x in results)
+    bound_ms = sum(x["bound_ms"] for x in results)
+    launches = sum(x["bound_launches"] for x in results)
+    extra = fl["sample"] + fl["grad"] + (10 * obj.d if mean_value else 0)
+    flops = boxes * fl["value"] + (boxes - pv) * extra
+    achieved = flops / (bound_ms * 1e-3) / 1e12 if bound_ms > 0 else 0.0
+    return {"bound": "valu", "kernel": kernel + ("_mv" if mean_value else ""),
+            "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "flop_per_launch": flops / max(launches, 1), "flop_per_child": flops / max(boxes, 1),
+            "avg_launch_ms": bound_ms / max(launches, 1), "launches": launches}
+
+
+def leg_c2(G, pb, steps=5):
+    """configs[2]-shaped workload (d = 50): GPU time to certified enclosure
+    against the oracle on all host cores with the same schedule."""
+    from oracle import oracle as O
+    spec = pb.config2()
+    obj = pb.objective(spec.family, spec.d)
+    grid, k = (3, 12), 1
+    p = G.Problem(obj, spec.params, spec.lo, spec.hi, tol=spec.tol, grid=grid, k_select=k, max_iters=10000)
+    p.solve()
+    times, res = [], []
+    p.timing(True)
+    for _ in range(steps):
+        p.reset()
+        t0 = time.perf_counter()
+        res.append(p.solve())
+        times.append(time.perf_counter() - t0)
+    p.timing(False)
+    p.close()
+    r = res[-1]
+    O.build()
+    threads = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    o, _ = O.solve(2, spec.params, spec.lo, spec.hi, tol_f=spec.tol, wid=grid[0], cycle=grid[1], k_select=k,
+                   max_iters=10000, threads=threads)
+    tc = time.perf_counter() - t0
+    gpu = statistics.median(times)
+    return {"workload": "config2: ackley_q d=50, box [-32,32]^50, tol 1e-6, seed 2",
+            "grid": list(grid), "k_select": k, "dtype": "f64",
+            "time_to_certified_s": gpu, "certified": all(x["status_name"] == "certified" for x in res),
+            "lb": r["lb"], "ub": r["ub"], "width": r["width"], "iterations": r["iterations"],
+            "boxes_bounded": r["boxes_bounded"], "boxes_bounded_per_s": r["boxes_bounded"] / gpu,
+            "roofline": bound_roofline(obj, res),
+            "cpu_baseline": {"value": tc, "unit": "s", "cores": threads, "kind": "port",
+                             "sample": "one complete solve, same workload and schedule",
+                             "certified": o["status"] == 1, "lb": o["lb"], "ub": o["ub"]},
+            "speedup_vs_cpu": tc / gpu}
+
+
+def run_batch(Batch, probs, **kw):
+    b = Batch(slots=len(probs), **kw)
+    t0 = time.perf_counter()
+    for p in probs:
+        b.admit(p)
+    res = {}
+    while True:
+        b.run(1)
+        for tag, r in b.results():
+            res[tag] = r
+        if b.counts()[0] == 0:
+            break
+    b.sync()
+    for tag, r in b.results():
+        res[tag] = r
+    el = time.perf_counter() - t0
+    b.close()
+    return res, el
+
+
+def leg_batch(G, pb, n, seq_sample):
+    """n independent problems in one batch (schedules from the
+    tuned table) against the same problems solved one at a time with the
+    single-problem solver (same schedules and capacities)."""
+    from gpugo_amd.batch import Batch, BatchProblem, get_batch_module
+    from gpugo_amd.schedule import schedule_for
+    qcap, pcap, maxit = 1 << 18, 8192, 2000
+    specs = [pb.batch_problem(k) for k in range(n)]
+    probs = []
+    for k, s in enumerate(specs):
+        grid, ks = schedule_for(s.family, s.d)
+        probs.append(BatchProblem.from_spec(s, grid=grid, k_select=ks, max_iters=maxit, tag=k))
+    m = get_batch_module(pb.FAMILIES, 32, ("fast",))
+    kw = dict(module=m, max_queue=qcap, max_parents=pcap, steps_per_launch=8)
+    run_batch(Batch, probs[:64], **kw)  # warm-up
+    res, el = run_batch(Batch, probs, **kw)
+    cert = sum(r["status_name"] == "certified" for r in res.values())
+    # sequential: the first seq_sample problems, one gpugo_solve each
+    sample = probs[:seq_sample]
+    mods = {}
+    for p in sample:
+        key = (p.family, p.d)
+        if key not in mods:
+            mods[key] = G.get_module(pb.objective(p.family, p.d), roundings=("fast",))
+    seq = {}
+    t0 = time.perf_counter()
+    for p in sample:
+        obj = pb.objective(p.family, p.d)
+        with G.Problem(obj, p.params, p.lo, p.hi, tol=p.tol, grid=p.grid, k_select=p.k_select,
+                       max_iters=maxit, max_queue=qcap, max_parents=pcap, module=mods[(p.family, p.d)]) as q:
+            try:
+                seq[p.tag] = q.solve()
+            except G.GpugoError:  # capacity exceeded: no certificate
+                seq[p.tag] = {"status_name": "overflow"}
+    t_seq = time.perf_counter() - t0
+
+    def same_result(a, b):
+        if a["status_name"] == "overflow" or b["status_name"] == "overflow":
+            return a["status_name"] == b["status_name"]
+        return all(a[key] == b[key] for key in ("lb", "ub", "width", "iterations", "boxes_bounded"))
+
+    same = sum(same_result(seq[t], res[t]) for t in seq)
+    seq_cert = sum(r["status_name"] == "certified" for r in seq.values())
+    widths = [r["width"] for r in res.values() if r["status_name"] == "certified"]
+    return {"workload": f"configs[3]: {n} problems (families C0-C2, d log-uniform [2,32], box [-h,h]^d, "
+                        "h~U[2,32], tol in {1e-4,1e-6}), seeds [3,k]",
+            "slots": n, "steps_per_launch": 8, "graph": True, "max_queue": qcap, "max_parents": pcap,
+            "max_iters": maxit, "time_s": el, "problems_per_s": n / el, "certified": cert,
+            "certified_per_s": cert / el,
+            "status_counts": {k: sum(r["status_name"] == k for r in res.values())
+                              for k in sorted({r["status_name"] for r in res.values()})},
+            "width_max_certified": max(widths) if widths else None,
+            "boxes_bounded": sum(r["boxes_bounded"] for r in res.values()),
+            "sequential": {"problems": len(sample), "time_s": t_seq, "problems_per_s": len(sample) / t_seq,
+                           "certified": seq_cert, "identical_to_batch": same},
+            "speedup_problems_per_s": (n / el) / (len(sample) / t_seq)}
+
+
+def leg_served(pb, lams=(100.0, 1000.0), seconds=3.0):
+    """Poisson arrivals, continuous batching, p50/p99 latency."""
+    from gpugo_amd.serve import Server, latency_summary
+    out = {}
+    with Server(mem_fraction=0.5) as srv:
+        slots = dict(srv.slots)
+        for lam in lams:
+            n = int(lam * seconds)
+            arr = pb.arrivals(lam, n, seed=40)
+            specs = [pb.served_problem(k) for k in range(n)]
+            t0 = time.perf_counter()
+            done = srv.run_trace(specs, arr, time_limit=seconds * 10 + 30)
+            el = time.perf_counter() - t0
+            summ = latency_summary(done)
+            summ.update({"lambda_per_s": lam, "requests": n, "wall_s": el,
+                         "throughput_per_s": len(done) / el,
+                         "status_counts": {k: sum(r.result["status_name"] == k for r in done)
+                                           for k in sorted({r.result["status_name"] for r in done})}})
+            out[f"lambda_{int(lam)}"] = summ
+    out["slots_per_dmax_class"] = {str(k): v for k, v in slots.items()}
+    out["workload"] = ("configs[4]: families C0-C2, d log-uniform [2,100], tol log-uniform [1e-8,1e-4], "
+                       "box [-h,h]^d h~U[2,32], seeds [4,k]; arrivals exponential(1/lambda), seed 40")
+    return out
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -115,6 +282,7 @@ def main():
     import gpugo_amd as G
     from gpugo_amd import emit, problems as pb
 
+    legs = set(args.legs.split(","))
     spec = pb.config1()
     sched = SCHEDULES[args.schedule]
     obj = pb.objective(spec.family, spec.d)
@@ -184,6 +352,15 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(spec, sched, args.cpu_seconds)
         out["cpu_reference_schedule"] = cpu_reference_schedule(spec)
+    if "c2" in legs:
+        print("# leg c2", file=sys.stderr, flush=True)
+        out["c2"] = leg_c2(G, pb)
+    if "batch" in legs:
+        print("# leg batch", file=sys.stderr, flush=True)
+        out["batch"] = leg_batch(G, pb, args.batch_n, args.seq_sample)
+    if "served" in legs:
+        print("# leg served", file=sys.stderr, flush=True)
+        out["served"] = leg_served(pb)
     print(json.dumps(out), flush=True)
 
 

@@ -363,7 +363,10 @@ def _count(exprs) -> int:
         for a in e.args:
             walk(a)
         op = e.op
-        if op in ("var", "const", "param", "iconst") or e.is_scalar():
+        if op in ("var", "const", "param", "iconst", "guard") or e.is_scalar():
+            return
+        if op == "gsum":
+            total += _FLOP["add"] * (len(e.args) - 1)
             return
         if op == "pown":
             n = max(int(e.val), 2)
