@@ -129,6 +129,31 @@ def bound_roofline(obj, results, mean_value=False, kernel="gpugo_bound_fast"):
             "avg_launch_ms": bound_ms / max(launches, 1), "launches": launches}
 
 
+def leg_fp32(G, spec, obj, sched, args, fp64_per_solve, r64):
+    """The same workload in the FP32-prune / FP64-certify precision form."""
+    prob = G.Problem(obj, spec.params, spec.lo, spec.hi, tol=spec.tol, grid=sched["grid"],
+                     k_select=sched["k_select"], n_samples=1, rounding=args.rounding, max_iters=200000,
+                     mean_value=sched["mean_value"], precision="fp32")
+    for _ in range(args.warmup):
+        prob.reset()
+        prob.solve()
+    res = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        prob.reset()
+        res.append(prob.solve())
+    t = (time.perf_counter() - t0) / args.steps
+    prob.close()
+    r = res[-1]
+    return {"precision": "fp32 prune, fp64 certify", "time_to_certified_s": t,
+            "speedup_vs_fp64": fp64_per_solve / t,
+            "certified": all(x["status_name"] == "certified" for x in res),
+            "lb": r["lb"], "ub": r["ub"], "width": r["width"], "iterations": r["iterations"],
+            "boxes_bounded": r["boxes_bounded"], "pruned_value": r["pruned_value"],
+            "fp64_form": {"lb": r64["lb"], "ub": r64["ub"], "width": r64["width"]},
+            "enclosures_intersect": max(r["lb"], r64["lb"]) <= min(r["ub"], r64["ub"])}
+
+
 def leg_c2(G, pb, steps=5):
     """configs[2]-shaped workload (d = 50): GPU time to certified enclosure
     against the oracle on all host cores with the same schedule."""
@@ -155,7 +180,21 @@ def leg_c2(G, pb, steps=5):
                    max_iters=10000, threads=threads)
     tc = time.perf_counter() - t0
     gpu = statistics.median(times)
+    # precision form: FP32 prune, FP64 certify
+    p = G.Problem(obj, spec.params, spec.lo, spec.hi, tol=spec.tol, grid=grid, k_select=k, max_iters=10000,
+                  precision="fp32")
+    p.solve()
+    t32 = []
+    for _ in range(steps):
+        p.reset()
+        t0 = time.perf_counter()
+        r32 = p.solve()
+        t32.append(time.perf_counter() - t0)
+    p.close()
     return {"workload": "config2: ackley_q d=50, box [-32,32]^50, tol 1e-6, seed 2",
+            "fp32_prune": {"time_to_certified_s": statistics.median(t32), "lb": r32["lb"], "ub": r32["ub"],
+                           "width": r32["width"], "certified": r32["status_name"] == "certified",
+                           "note": "the first step has no incumbent, so nothing is ruled out in FP32 there"},
             "grid": list(grid), "k_select": k, "dtype": "f64",
             "time_to_certified_s": gpu, "certified": all(x["status_name"] == "certified" for x in res),
             "lb": r["lb"], "ub": r["ub"], "width": r["width"], "iterations": r["iterations"],
@@ -349,6 +388,8 @@ def main():
                      "flop_per_launch": flops / max(launches, 1),
                      "avg_launch_ms": bound_ms / max(launches, 1), "launches": launches},
     }
+    if "c1" in legs:
+        out["fp32_prune"] = leg_fp32(G, spec, obj, sched, args, per_solve, r)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(spec, sched, args.cpu_seconds)
         out["cpu_reference_schedule"] = cpu_reference_schedule(spec)

@@ -33,7 +33,7 @@ EXPORTS = [
     "gpugo_step", "gpugo_solve", "gpugo_boxes_copy", "gpugo_problem_destroy", "gpugo_problem_reset",
     "gpugo_problem_timing", "gpugo_batch_bytes", "gpugo_mem_info", "gpugo_batch_create",
     "gpugo_batch_admit", "gpugo_batch_run", "gpugo_batch_sync", "gpugo_batch_results",
-    "gpugo_batch_counts", "gpugo_batch_destroy",
+    "gpugo_batch_counts", "gpugo_batch_destroy", "gpugo_eval_boxes_f32",
 ]
 
 
@@ -117,6 +117,8 @@ def load():
     lib.gpugo_problem_destroy.argtypes = [vp]
     lib.gpugo_problem_destroy.restype = None
     I64 = ctypes.POINTER(ctypes.c_int64)
+    lib.gpugo_eval_boxes_f32.argtypes = [vp, ctypes.c_int32, D, ctypes.c_int64, ctypes.c_int64, D, D,
+                                         ctypes.c_int32, D, D]
     lib.gpugo_batch_bytes.argtypes = [vp, ctypes.POINTER(BatchDesc), I64]
     lib.gpugo_mem_info.argtypes = [I64, I64]
     lib.gpugo_batch_create.argtypes = [vp, ctypes.POINTER(BatchDesc), ctypes.POINTER(vp)]

@@ -34,13 +34,14 @@ class BatchModule:
     """hipRTC batch module: `families` at maximum dimension `dmax`."""
 
     def __init__(self, families: Sequence[str] = pb.FAMILIES, dmax: int = 32, roundings=("fast",),
-                 options: str = ""):
+                 options: str = "", mean_value: bool = False):
         _lib.init()
         self.families = tuple(families)
         self.dmax = int(dmax)
         self.roundings = tuple(roundings)
+        self.mean_value = bool(mean_value)
         objs = [pb.objective_dynamic(f, self.dmax) for f in self.families]
-        src = batch_module_source(objs, self.roundings)
+        src = batch_module_source(objs, self.roundings, self.mean_value)
         path = compile_source_to_cache(src, f"batch_d{self.dmax}", options)
         with open(path, "rb") as fh:
             code = fh.read()
@@ -59,18 +60,19 @@ class BatchModule:
             self.handle = None
 
 
-def batch_source(families: Sequence[str] = pb.FAMILIES, dmax: int = 32, roundings=("fast",)) -> str:
-    return batch_module_source([pb.objective_dynamic(f, dmax) for f in families], tuple(roundings))
+def batch_source(families: Sequence[str] = pb.FAMILIES, dmax: int = 32, roundings=("fast",),
+                 mean_value: bool = False) -> str:
+    return batch_module_source([pb.objective_dynamic(f, dmax) for f in families], tuple(roundings), mean_value)
 
 
 def get_batch_module(families: Sequence[str] = pb.FAMILIES, dmax: int = 32, roundings=("fast",),
-                     options: str = "") -> BatchModule:
+                     options: str = "", mean_value: bool = False) -> BatchModule:
     roundings = tuple(sorted(set(roundings)))
-    key = (tuple(families), int(dmax), roundings, options)
+    key = (tuple(families), int(dmax), roundings, options, bool(mean_value))
     with _bmod_lock:
         m = _bmods.get(key)
         if m is None:
-            m = BatchModule(families, dmax, roundings, options)
+            m = BatchModule(families, dmax, roundings, options, mean_value)
             _bmods[key] = m
         return m
 
@@ -90,6 +92,7 @@ class BatchProblem:
     init_ub: float = np.inf
     max_iters: int = 0
     tag: int = 0
+    mean_value: bool = False
     extra: dict = field(default_factory=dict)
 
     @staticmethod
@@ -136,7 +139,8 @@ class Batch:
         lo, hi = f64(prob.lo), f64(prob.hi)
         wid, cycle = prob.grid
         desc = _lib.ProblemDesc(d=int(prob.d), wid=int(wid), cycle=int(cycle), k_select=int(prob.k_select),
-                                n_samples=int(prob.n_samples), precision=0, rounding=self.desc.rounding, flags=0,
+                                n_samples=int(prob.n_samples), precision=0, rounding=self.desc.rounding,
+                                flags=_lib.FLAG_MEAN_VALUE if prob.mean_value else 0,
                                 lo=ptr(lo), hi=ptr(hi), params=ptr(params), n_params=params.size,
                                 tol_f=float(prob.tol), tol_x=0.0, init_ub=float(prob.init_ub), max_queue=0,
                                 max_parents=0, max_iters=int(prob.max_iters))

@@ -50,9 +50,13 @@ struct BoundArgs {
   long long q_cap;
   int d, wid, c, n_samples;
   unsigned int W;                // wid^c children per parent
-  int flags;                     // GPUGO_FLAG_* of the problem
+  int flags;                     // GPUGO_FLAG_* of the problem (+ FLAG_F32_PRUNE)
   double inv_wid, half_inv_wid;  // exact digit extraction: q = floor(r*inv + half)
+  const void *params32;          // coefficients as outward FP32 intervals (iv<float>)
 };
+
+// BoundArgs.flags bit of the FP32 pruning pass (problem precision F32)
+#define GPUGO_FLAG_F32_PRUNE 2
 
 // Arguments of the module kernel gpugo_eval_{exact,fast} (parity hook).
 struct EvalArgs {

@@ -2,7 +2,9 @@
 // problems (slots) of several objective families and dimensions d <= DMAX
 // in one launch.  Included (as text, by gpugo_amd.emit) after ia.hpp,
 // bnb_args.hpp and the variable-dimension code lists of each family with the
-// dispatchers fam_value / fam_kkt / fam_grad.
+// dispatchers fam_value / fam_kkt / fam_grad.  In modules built with
+// GPUGO_BATCH_MV, slots whose flags have GPUGO_FLAG_MEAN_VALUE use the
+// mean-value bound (it multiplies the hipRTC time several times).
 //
 // A wavefront works on one slot at a time, so the slot's family and
 // dimension are uniform and the family switch does not diverge.  Per slot
@@ -12,7 +14,7 @@
 #pragma once
 
 #ifndef GPUGO_MODULE_ABI_V
-#define GPUGO_MODULE_ABI_V 1
+#define GPUGO_MODULE_ABI_V 2
 #endif
 
 namespace gpugo {
@@ -22,7 +24,7 @@ __device__ __forceinline__ double batch_sample_ub(int f, const ChildBox &x, cons
                                                   int S) {
   double best = __builtin_huge_val();
   for (int t = 0; t < S; ++t) {
-    const DiagPoint xs{x, t, S};
+    const DiagPoint<ChildBox> xs{x, t, S};
     const iv<double> fv = fam_value<R>(f, xs, P, d);
     if (fv.hi < best) best = fv.hi;
   }
@@ -86,11 +88,30 @@ __device__ __forceinline__ void batch_bound_impl(const BatchBoundArgs &a) {
       lb = fv.lo;
       if (!(lb == lb)) lb = -__builtin_huge_val();
       if (lb <= ub0) {
-        // as in the single-problem kernel: monotonicity test, then the
-        // sample bound of the survivor of the value test
-        keep = !fam_kkt<R>(f, x, P, dlo, dhi, d);
-        pk = !keep;
-        su = batch_sample_ub<R>(f, x, P, d, I.n_samples);
+#if GPUGO_BATCH_MV
+        if (I.flags & 1) {
+          // mean-value form, as bound_mv in the single-problem kernel
+          const DiagPoint<ChildBox> xc{x, 0, 1};
+          const iv<double> s0 = fam_value<R>(f, xc, P, d);
+          su = I.n_samples == 1 ? s0.hi : batch_sample_ub<R>(f, x, P, d, I.n_samples);
+          MVSink<R, ChildBox> g{x, dlo, dhi, d, s0, false};
+          fam_grad<R>(f, x, P, g, d);
+          if (g.acc.lo > lb) lb = g.acc.lo;
+          if (lb <= ub0) {
+            keep = !g.prune;
+            pk = !keep;
+          } else {
+            pv = true;
+          }
+        } else
+#endif
+        {
+          // as in the single-problem kernel: monotonicity test, then the
+          // sample bound of the survivor of the value test
+          keep = !fam_kkt<R>(f, x, P, dlo, dhi, d);
+          pk = !keep;
+          su = batch_sample_ub<R>(f, x, P, d, I.n_samples);
+        }
       } else {
         pv = true;
       }
@@ -148,7 +169,7 @@ extern "C" __global__ void gpugo_batch_info(int *out) {
   out[0] = GPUGO_DMAX;
   out[1] = GPUGO_NFAM;
   out[2] = GPUGO_MODULE_ABI_V;
-  out[3] = (GPUGO_WITH_EXACT ? 1 : 0) | (GPUGO_WITH_FAST ? 2 : 0);
+  out[3] = (GPUGO_WITH_EXACT ? 1 : 0) | (GPUGO_WITH_FAST ? 2 : 0) | (GPUGO_BATCH_MV ? 4 : 0);
   out[4] = GPUGO_PMAX;
   const int np[] = {GPUGO_FAM_NPARAMS};
   for (int i = 0; i < GPUGO_NFAM; ++i) out[5 + i] = np[i];

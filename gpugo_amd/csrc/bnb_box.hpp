@@ -38,10 +38,12 @@ struct ChildBox {
   }
 };
 
-// Sample point t of S on the diagonal of a box,
-// x_j = lo_j + (hi_j - lo_j)/(S+1) * (t+1) (1-Ackley/ackley50.py:288-290).
+// Sample point t of S on the diagonal of a box (an array or accessor),
+// x_j = lo_j + (hi_j - lo_j)/(S+1) * (t+1) (1-Ackley/ackley50.py:288-290);
+// t = 0, S = 1 is the midpoint.
+template <class XS>
 struct DiagPoint {
-  ChildBox b;
+  const XS &b;
   int t, S;
   __device__ __forceinline__ iv<double> operator[](int j) const {
     const iv<double> x = b[j];
@@ -49,6 +51,46 @@ struct DiagPoint {
     const double h = (S == 1) ? w * 0.5 : w / (double)(S + 1);
     const double p = __builtin_fma(h, (double)(t + 1), x.lo);
     return mk<double>(p, p);
+  }
+};
+
+// A box (array or accessor of FP64 intervals) seen as FP32 intervals,
+// each coordinate rounded outward.
+template <class XS>
+struct F32Box {
+  const XS &x;
+  __device__ __forceinline__ iv<float> operator[](int j) const {
+    const iv<double> v = x[j];
+    return mk<float>(to_dn<float>(v.lo), to_up<float>(v.hi));
+  }
+};
+
+// Gradient sink for the mean-value bound: obj_grad's g[j] = G_j folds
+// s += G_j * (X_j - c_j) (c the midpoint, j in increasing order) and the
+// monotonicity test on the same G_j (1-Ackley/ackley50.py:458-477), so no
+// array of d gradient intervals is kept.  Components j >= d are ignored.
+template <int R, class XS>
+struct MVSink {
+  typedef IA<double, R> A;
+  const XS &x;
+  const double *__restrict__ dlo;
+  const double *__restrict__ dhi;
+  int d;
+  iv<double> acc;
+  bool prune;
+  struct Ref {
+    MVSink *s;
+    int j;
+    __device__ __forceinline__ void operator=(const iv<double> &g) const { s->put(j, g); }
+  };
+  __device__ __forceinline__ Ref operator[](int j) { return Ref{this, j}; }
+  __device__ __forceinline__ void put(int j, const iv<double> &g) {
+    if (j >= d) return;
+    const iv<double> xj = x[j];
+    const double c = __builtin_fma((xj.hi - xj.lo) * 0.5, 1.0, xj.lo);
+    const iv<double> dx = mk<double>(A::RR::sub_dn(xj.lo, c), A::RR::sub_up(xj.hi, c));
+    acc = A::add(acc, A::mul(g, dx));
+    prune |= (g.lo > 0.0 && xj.lo != dlo[j]) || (g.hi < 0.0 && xj.hi != dhi[j]);
   }
 };
 

@@ -11,7 +11,10 @@
 // (GPUGO_1.1/1-Ackley/ackley50.py:602-766).
 #pragma once
 
-#define GPUGO_MODULE_ABI_V 1
+#define GPUGO_MODULE_ABI_V 2
+#ifndef GPUGO_WITH_F32
+#define GPUGO_WITH_F32 0
+#endif
 // dimensions above this decode child coordinates on use instead of into an
 // array (the array would keep 4 d registers live through the code lists)
 #ifndef GPUGO_LAZY_D
@@ -20,37 +23,33 @@
 
 namespace gpugo {
 
-// Rigorous upper bound of f at n_samples points of the box diagonal,
-// x_j = lo_j + (hi_j - lo_j)/(S+1) * (t+1) (1-Ackley/ackley50.py:288-290);
-// S = 1 is the midpoint.
-template <int R>
-__device__ __forceinline__ double sample_ub(const iv<double> *x, const double *__restrict__ P, int S, int d) {
+// Rigorous upper bound of f at n_samples points of the box diagonal
+// (DiagPoint); S = 1 is the midpoint.
+template <int R, class XS>
+__device__ __forceinline__ double sample_ub(const XS &x, const double *__restrict__ P, int S, int d) {
   double best = __builtin_huge_val();
   for (int t = 0; t < S; ++t) {
-    iv<double> xs[GPUGO_D];
-#pragma unroll
-    for (int j = 0; j < GPUGO_D; ++j) {
-      double w = x[j].hi - x[j].lo;
-      double h = (S == 1) ? w * 0.5 : w / (double)(S + 1);
-      double p = __builtin_fma(h, (double)(t + 1), x[j].lo);
-      xs[j] = mk<double>(p, p);
-    }
-    iv<double> f = obj_value<R>(xs, P, d);
+    const DiagPoint<XS> xs{x, t, S};
+    const iv<double> f = obj_value<R>(xs, P, d);
     if (f.hi < best) best = f.hi;
   }
   return best;
 }
 
-// The same bound on a lazily decoded child (coordinate accessor).
-template <int R>
-__device__ __forceinline__ double sample_ub(const ChildBox &x, const double *__restrict__ P, int S, int d) {
-  double best = __builtin_huge_val();
-  for (int t = 0; t < S; ++t) {
-    const DiagPoint xs{x, t, S};
-    const iv<double> f = obj_value<R>(xs, P, d);
-    if (f.hi < best) best = f.hi;
-  }
-  return best;
+// FP32 pruning pass: the value code list in FP32 intervals; a child whose
+// FP32 lower bound exceeds the incumbent is ruled out without any FP64
+// work (rigorous: the FP32 enclosure contains f over the child).  Nothing
+// can be ruled out before there is an incumbent.
+template <int R, class XS>
+__device__ __forceinline__ bool f32_prunes(const BoundArgs &a, const XS &x, double ub0) {
+#if GPUGO_WITH_F32
+  if (!(a.flags & GPUGO_FLAG_F32_PRUNE) || !(ub0 < __builtin_huge_val())) return false;
+  const F32Box<XS> xf{x};
+  const iv<float> f = obj_value32<R>(xf, (const iv<float> *)a.params32, a.d);
+  return (double)f.lo > ub0;
+#else
+  return false;
+#endif
 }
 
 // value test, then for survivors the monotonicity test and the sample bound
@@ -73,15 +72,31 @@ __device__ __forceinline__ void bound_plain(const BoundArgs &a, const XS &x, dou
   }
 }
 
-// Monotonicity test on a computed gradient enclosure
-// (1-Ackley/ackley50.py:458-477).
-__device__ __forceinline__ bool kkt_from_grad(const iv<double> *x, const iv<double> *g,
-                                              const double *__restrict__ dlo, const double *__restrict__ dhi) {
-  bool prune = false;
-#pragma unroll
-  for (int j = 0; j < GPUGO_D; ++j)
-    prune |= (g[j].lo > 0.0 && x[j].lo != dlo[j]) || (g[j].hi < 0.0 && x[j].hi != dhi[j]);
-  return prune;
+// value test, then for survivors the mean-value form
+// f(c) + sum_j G_j (X_j - c_j) (c the midpoint, G the gradient enclosure the
+// monotonicity test needs anyway), the monotonicity test and the sample bound.
+template <int R, class XS>
+__device__ __forceinline__ void bound_mv(const BoundArgs &a, const XS &x, double ub0, double &lb, double &su,
+                                         bool &keep, bool &pv, bool &pk) {
+  const iv<double> f = obj_value<R>(x, a.params, a.d);
+  lb = f.lo;
+  if (!(lb == lb)) lb = -__builtin_huge_val();
+  if (lb <= ub0) {
+    const DiagPoint<XS> xc{x, 0, 1};
+    const iv<double> s0 = obj_value<R>(xc, a.params, a.d);
+    su = a.n_samples == 1 ? s0.hi : sample_ub<R>(x, a.params, a.n_samples, a.d);
+    MVSink<R, XS> g{x, a.dom_lo, a.dom_hi, a.d, s0, false};
+    obj_grad<R>(x, a.params, g, a.d);
+    if (g.acc.lo > lb) lb = g.acc.lo;
+    if (lb <= ub0) {
+      keep = !g.prune;
+      pk = !keep;
+    } else {
+      pv = true;
+    }
+  } else {
+    pv = true;
+  }
 }
 
 // Child `pos` of parent record `p`: window coordinates (win + k) mod d for
@@ -157,50 +172,19 @@ __device__ __forceinline__ void bound_impl(const BoundArgs &a) {
     if (active) {
       pl = (long long)__builtin_fma((double)t, invW, halfInvW);
       pos = (unsigned)(t - pl * (long long)a.W);
-      if (!MV && GPUGO_D > GPUGO_LAZY_D) {
-        // large d: coordinates decoded where the code lists use them
+      if (MV || GPUGO_D > GPUGO_LAZY_D) {
+        // coordinates decoded where the code lists use them
         const long long p = first + pl;
         const ChildBox x = child_box(a.plo + p * GPUGO_D, a.phi + p * GPUGO_D, a.pw + p * GPUGO_D, a.pwin[p],
                                      GPUGO_D, a.c, (unsigned)a.wid, a.inv_wid, a.half_inv_wid, pos);
-        bound_plain<R>(a, x, ub0, lb, su, keep, pv, pk);
-      } else if (!MV) {
-        iv<double> x[GPUGO_D];
-        decode_child(a, first + pl, pos, x);
-        bound_plain<R>(a, (const iv<double> *)x, ub0, lb, su, keep, pv, pk);
+        if (f32_prunes<R>(a, x, ub0)) pv = true;
+        else if (MV) bound_mv<R>(a, x, ub0, lb, su, keep, pv, pk);
+        else bound_plain<R>(a, x, ub0, lb, su, keep, pv, pk);
       } else {
         iv<double> x[GPUGO_D];
         decode_child(a, first + pl, pos, x);
-        iv<double> f = obj_value<R>((const iv<double> *)x, a.params, a.d);
-        lb = f.lo;
-        if (!(lb == lb)) lb = -__builtin_huge_val();
-        if (lb <= ub0) {
-          // mean-value form f(c) + sum_j G_j (X_j - c_j) from the gradient
-          // the monotonicity test needs anyway; c is the S = 1 sample point
-          typedef IA<double, R> A;
-          iv<double> g[GPUGO_D], xc[GPUGO_D];
-          obj_grad<R>((const iv<double> *)x, a.params, g, a.d);
-#pragma unroll
-          for (int j = 0; j < GPUGO_D; ++j) {
-            double c = __builtin_fma((x[j].hi - x[j].lo) * 0.5, 1.0, x[j].lo);
-            xc[j] = mk<double>(c, c);
-          }
-          iv<double> s = obj_value<R>((const iv<double> *)xc, a.params, a.d);
-          su = a.n_samples == 1 ? s.hi : sample_ub<R>((const iv<double> *)x, a.params, a.n_samples, a.d);
-#pragma unroll
-          for (int j = 0; j < GPUGO_D; ++j) {
-            iv<double> dx = mk<double>(A::RR::sub_dn(x[j].lo, xc[j].lo), A::RR::sub_up(x[j].hi, xc[j].lo));
-            s = A::add(s, A::mul(g[j], dx));
-          }
-          if (s.lo > lb) lb = s.lo;
-          if (lb <= ub0) {
-            keep = !kkt_from_grad(x, g, a.dom_lo, a.dom_hi);
-            pk = !keep;
-          } else {
-            pv = true;
-          }
-        } else {
-          pv = true;
-        }
+        if (f32_prunes<R>(a, (const iv<double> *)x, ub0)) pv = true;
+        else bound_plain<R>(a, (const iv<double> *)x, ub0, lb, su, keep, pv, pk);
       }
     }
     // order-free append of survivors: one global atomic per block
@@ -279,6 +263,23 @@ __device__ __forceinline__ void eval_impl(const EvalArgs &a) {
   if (a.ub) a.ub[i] = sample_ub<R>(x, a.params, a.n_samples, a.d);
 }
 
+#if GPUGO_WITH_F32
+// FP32 value enclosure of explicit boxes (parity hook of the pruning pass);
+// a.ub carries the coefficient intervals (iv<float>[n_params])
+template <int R>
+__device__ __forceinline__ void eval32_impl(const EvalArgs &a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  iv<double> x[GPUGO_D];
+#pragma unroll
+  for (int j = 0; j < GPUGO_D; ++j) x[j] = mk<double>(a.lo[i * GPUGO_D + j], a.hi[i * GPUGO_D + j]);
+  const F32Box<iv<double>[GPUGO_D]> xf{x};
+  const iv<float> f = obj_value32<R>(xf, (const iv<float> *)a.ub, a.d);
+  a.flo[i] = (double)f.lo;
+  a.fhi[i] = (double)f.hi;
+}
+#endif
+
 }  // namespace gpugo
 
 // Rounding variants instantiated in this module (the generator defines
@@ -293,6 +294,16 @@ extern "C" __global__ __launch_bounds__(256) void gpugo_bound_exact_mv(gpugo::Bo
 }
 extern "C" __global__ __launch_bounds__(64) void gpugo_eval_exact(gpugo::EvalArgs a) {
   gpugo::eval_impl<gpugo::ROUND_EXACT>(a);
+}
+#endif
+#if GPUGO_WITH_F32 && GPUGO_WITH_EXACT
+extern "C" __global__ __launch_bounds__(64) void gpugo_eval32_exact(gpugo::EvalArgs a) {
+  gpugo::eval32_impl<gpugo::ROUND_EXACT>(a);
+}
+#endif
+#if GPUGO_WITH_F32 && GPUGO_WITH_FAST
+extern "C" __global__ __launch_bounds__(64) void gpugo_eval32_fast(gpugo::EvalArgs a) {
+  gpugo::eval32_impl<gpugo::ROUND_FAST>(a);
 }
 #endif
 #if GPUGO_WITH_FAST
@@ -317,4 +328,5 @@ extern "C" __global__ void gpugo_info(int *out) {
   out[1] = GPUGO_NPARAMS;
   out[2] = GPUGO_MODULE_ABI_V;
   out[3] = (GPUGO_WITH_EXACT ? 1 : 0) | (GPUGO_WITH_FAST ? 2 : 0);
+  out[4] = GPUGO_WITH_F32;
 }

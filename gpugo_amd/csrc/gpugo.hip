@@ -77,19 +77,6 @@ extern "C" int gpugo_init(int device) {
 // ---------------------------------------------------------------------------
 // batched interval ops
 // ---------------------------------------------------------------------------
-template <typename T> __device__ __forceinline__ T to_dn(double x);
-template <typename T> __device__ __forceinline__ T to_up(double x);
-template <> __device__ __forceinline__ double to_dn<double>(double x) { return x; }
-template <> __device__ __forceinline__ double to_up<double>(double x) { return x; }
-template <> __device__ __forceinline__ float to_dn<float>(double x) {
-  float f = (float)x;
-  return ((double)f > x) ? pred<float>(f) : f;
-}
-template <> __device__ __forceinline__ float to_up<float>(double x) {
-  float f = (float)x;
-  return ((double)f < x) ? succ<float>(f) : f;
-}
-
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_ia_eval(int op, long long n, const double *__restrict__ alo,
                                                  const double *__restrict__ ahi, const double *__restrict__ blo,
@@ -210,18 +197,20 @@ struct gpugo_module_s {
   hipFunction_t bound[2] = {nullptr, nullptr};  // [rounding]
   hipFunction_t bound_mv[2] = {nullptr, nullptr};  // [rounding], mean-value variant
   hipFunction_t eval[2] = {nullptr, nullptr};
+  hipFunction_t eval32[2] = {nullptr, nullptr};  // FP32 value (precision F32 modules)
   int dim = 0, nparams = 0, abi = 0;
+  bool f32 = false;                // has the FP32 value code list (precision F32)
   std::vector<signed char> psign;  // declared coefficient signs
   std::vector<char> code;
   // batch modules (several families, runtime d <= dim)
-  bool batch = false;
+  bool batch = false, batch_mv = false;
   hipFunction_t batch_bound[2] = {nullptr, nullptr};
   int nfam = 0, pmax = 0;
   std::vector<int> fam_nparams;
   std::vector<signed char> fam_psign;  // [nfam][pmax]
 };
 
-#define GPUGO_MODULE_ABI 1
+#define GPUGO_MODULE_ABI 2
 
 static int batch_module_finish(gpugo_module_t m, hipFunction_t info) {
   m->batch = true;
@@ -240,6 +229,7 @@ static int batch_module_finish(gpugo_module_t m, hipFunction_t info) {
   if (m->nfam < 1 || m->nfam > 59 || m->dim < 1 || m->pmax < 1)
     return set_err(GPUGO_E_INVALID, "malformed batch module (dmax %d, families %d)", m->dim, m->nfam);
   m->fam_nparams.assign(h + 5, h + 5 + m->nfam);
+  m->batch_mv = (h[3] & 4) != 0;
   if (h[3] & 1) HIPCHECK(hipModuleGetFunction(&m->batch_bound[0], m->mod, "gpugo_batch_bound_exact"));
   if (h[3] & 2) HIPCHECK(hipModuleGetFunction(&m->batch_bound[1], m->mod, "gpugo_batch_bound_fast"));
   hipFunction_t fs;
@@ -266,14 +256,18 @@ static int module_finish(gpugo_module_t m) {
     return batch_module_finish(m, info);
   }
   DevBuf d;
-  HIPCHECK(d.alloc(4 * sizeof(int)));
+  HIPCHECK(d.alloc(8 * sizeof(int)));
+  HIPCHECK(hipMemset(d.p, 0, 8 * sizeof(int)));
   void *args[] = {&d.p};
   HIPCHECK(hipModuleLaunchKernel(info, 1, 1, 1, 1, 1, 1, 0, nullptr, args, nullptr));
-  int h[4] = {0, 0, 0, 0};
+  int h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   HIPCHECK(hipMemcpy(h, d.p, sizeof h, hipMemcpyDeviceToHost));
   m->dim = h[0];
   m->nparams = h[1];
   m->abi = h[2];
+  m->f32 = h[4] != 0;
+  if (m->f32 && (h[3] & 1)) HIPCHECK(hipModuleGetFunction(&m->eval32[0], m->mod, "gpugo_eval32_exact"));
+  if (m->f32 && (h[3] & 2)) HIPCHECK(hipModuleGetFunction(&m->eval32[1], m->mod, "gpugo_eval32_fast"));
   if (m->abi != GPUGO_MODULE_ABI)
     return set_err(GPUGO_E_INVALID, "module ABI %d, library expects %d", m->abi, GPUGO_MODULE_ABI);
   if (h[3] & 1) {
@@ -492,6 +486,53 @@ extern "C" int gpugo_eval_boxes(gpugo_module_t m, int32_t d, const double *param
   if (ghi) HIPCHECK(hipMemcpy(ghi, d_ghi.p, nb, hipMemcpyDeviceToHost));
   if (ub) HIPCHECK(hipMemcpy(ub, d_ub.p, n1, hipMemcpyDeviceToHost));
   if (kkt) HIPCHECK(hipMemcpy(kkt, d_kkt.p, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost));
+  return GPUGO_OK;
+}
+
+extern "C" int gpugo_eval_boxes_f32(gpugo_module_t m, int32_t d, const double *params, int64_t n_params, int64_t n,
+                                    const double *lo, const double *hi, int32_t rounding, double *flo, double *fhi) {
+  int rc = check_module_problem(m, d, params, n_params);
+  if (rc) return rc;
+  if (rounding != GPUGO_ROUND_EXACT && rounding != GPUGO_ROUND_FAST)
+    return set_err(GPUGO_E_INVALID, "unknown rounding %d", rounding);
+  if (!m->eval32[rounding]) return set_err(GPUGO_E_INVALID, "module has no FP32 code list for this rounding");
+  if (n < 0) return set_err(GPUGO_E_INVALID, "negative n");
+  if (n == 0) return GPUGO_OK;
+  if (!lo || !hi || !flo || !fhi) return set_err(GPUGO_E_INVALID, "null pointer");
+  rc = check_intervals(lo, hi, n * d, "box");
+  if (rc) return rc;
+  const size_t nb = sizeof(double) * (size_t)n * d, n1 = sizeof(double) * (size_t)n;
+  std::vector<float> p32(2 * (n_params > 0 ? n_params : 1), 0.0f);
+  for (int64_t i = 0; i < n_params; ++i) {
+    const double v = params[i];
+    float f = (float)v;
+    p32[2 * i] = (double)f > v ? std::nextafter(f, -__builtin_huge_valf()) : f;
+    p32[2 * i + 1] = (double)f < v ? std::nextafter(f, __builtin_huge_valf()) : f;
+  }
+  DevBuf d_lo, d_hi, d_p32, d_flo, d_fhi;
+  HIPCHECK(d_lo.alloc(nb));
+  HIPCHECK(d_hi.alloc(nb));
+  HIPCHECK(d_p32.alloc(sizeof(float) * p32.size()));
+  HIPCHECK(d_flo.alloc(n1));
+  HIPCHECK(d_fhi.alloc(n1));
+  HIPCHECK(hipMemcpy(d_lo.p, lo, nb, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_hi.p, hi, nb, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_p32.p, p32.data(), sizeof(float) * p32.size(), hipMemcpyHostToDevice));
+  EvalArgs a;
+  memset(&a, 0, sizeof a);
+  a.lo = d_lo.as<double>();
+  a.hi = d_hi.as<double>();
+  a.ub = (double *)d_p32.p;  // coefficient intervals (iv<float>)
+  a.flo = d_flo.as<double>();
+  a.fhi = d_fhi.as<double>();
+  a.n = n;
+  a.d = d;
+  void *args[] = {&a};
+  HIPCHECK(hipModuleLaunchKernel(m->eval32[rounding], (unsigned)((n + 63) / 64), 1, 1, 64, 1, 1, 0, nullptr, args,
+                                 nullptr));
+  HIPCHECK(hipDeviceSynchronize());
+  HIPCHECK(hipMemcpy(flo, d_flo.p, n1, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(fhi, d_fhi.p, n1, hipMemcpyDeviceToHost));
   return GPUGO_OK;
 }
 
@@ -1291,7 +1332,7 @@ struct gpugo_problem_s {
   int c = 0;
   unsigned W = 0;
   long long q_cap = 0, p_cap = 0, out_cap = 0;
-  DevBuf d_lo, d_hi, d_params, plo, phi, pw, pwin, out_lo, out_hi, out_lb, st, scratch;
+  DevBuf d_lo, d_hi, d_params, d_params32, plo, phi, pw, pwin, out_lo, out_hi, out_lb, st, scratch;
   Queue q[2];
   hipStream_t stream = nullptr;
   bool started = false;
@@ -1347,7 +1388,10 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
   if (desc->k_select < 1 || desc->k_select > SEL_KMAX)
     return set_err(GPUGO_E_INVALID, "k_select must be in [1,%d]", SEL_KMAX);
   if (desc->n_samples < 1 || desc->n_samples > 64) return set_err(GPUGO_E_INVALID, "n_samples must be in [1,64]");
-  if (desc->precision != GPUGO_PREC_F64) return set_err(GPUGO_E_INVALID, "only GPUGO_PREC_F64 problems are supported");
+  if (desc->precision != GPUGO_PREC_F64 && desc->precision != GPUGO_PREC_F32)
+    return set_err(GPUGO_E_INVALID, "unknown precision %d", desc->precision);
+  if (desc->precision == GPUGO_PREC_F32 && !m->f32)
+    return set_err(GPUGO_E_INVALID, "module has no FP32 code list (emit it with f32=True)");
   if (desc->flags & ~GPUGO_FLAG_MEAN_VALUE) return set_err(GPUGO_E_INVALID, "unknown flags 0x%x", desc->flags);
   if (desc->rounding != GPUGO_ROUND_EXACT && desc->rounding != GPUGO_ROUND_FAST)
     return set_err(GPUGO_E_INVALID, "unknown rounding %d", desc->rounding);
@@ -1389,6 +1433,7 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
   PALLOC(p->d_lo, sizeof(double) * d);
   PALLOC(p->d_hi, sizeof(double) * d);
   PALLOC(p->d_params, sizeof(double) * (desc->n_params > 0 ? desc->n_params : 1));
+  PALLOC(p->d_params32, 2 * sizeof(float) * (desc->n_params > 0 ? desc->n_params : 1));
   PALLOC(p->plo, sizeof(double) * d * pc);
   PALLOC(p->phi, sizeof(double) * d * pc);
   PALLOC(p->pw, sizeof(double) * d * pc);
@@ -1412,6 +1457,19 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
        (e = hipMemcpy(p->d_params.p, p->params.data(), sizeof(double) * desc->n_params, hipMemcpyHostToDevice)) !=
            hipSuccess))
     return fail(set_err(GPUGO_E_HIP, "upload: %s", hipGetErrorString(e)));
+  {
+    // coefficients as FP32 intervals rounded outward (FP32 pruning pass)
+    std::vector<float> p32(2 * (desc->n_params > 0 ? desc->n_params : 1), 0.0f);
+    for (int64_t i = 0; i < desc->n_params; ++i) {
+      const double v = desc->params[i];
+      float f = (float)v;
+      p32[2 * i] = (double)f > v ? std::nextafter(f, -__builtin_huge_valf()) : f;
+      p32[2 * i + 1] = (double)f < v ? std::nextafter(f, __builtin_huge_valf()) : f;
+    }
+    if ((e = hipMemcpy(p->d_params32.p, p32.data(), sizeof(float) * p32.size(), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+      return fail(set_err(GPUGO_E_HIP, "upload: %s", hipGetErrorString(e)));
+  }
   if ((e = hipHostMalloc((void **)&p->hpoll, 2 * sizeof(DevState), hipHostMallocDefault)) != hipSuccess)
     return fail(set_err(GPUGO_E_NOMEM, "pinned poll buffer: %s", hipGetErrorString(e)));
   for (int b = 0; b < 2; ++b)
@@ -1537,7 +1595,8 @@ static int enqueue_step(gpugo_problem_t p) {
   ba.W = p->W;
   ba.inv_wid = 1.0 / (double)p->desc.wid;
   ba.half_inv_wid = 0.5 / (double)p->desc.wid;
-  ba.flags = p->desc.flags;
+  ba.flags = p->desc.flags | (p->desc.precision == GPUGO_PREC_F32 ? GPUGO_FLAG_F32_PRUNE : 0);
+  ba.params32 = p->d_params32.p;
   {
     long long maxk = p->steps_enqueued == 0 ? 1 : p->desc.k_select;
     long long nthreads = maxk * (long long)p->W;
@@ -2025,6 +2084,9 @@ extern "C" int gpugo_batch_admit(gpugo_batch_t b, int32_t family, const gpugo_pr
   if (desc->n_samples < 1 || desc->n_samples > 64) return set_err(GPUGO_E_INVALID, "n_samples must be in [1,64]");
   if (std::isnan(desc->tol_f) || std::isnan(desc->init_ub)) return set_err(GPUGO_E_DOMAIN, "NaN tolerance or init_ub");
   if (!(desc->tol_f > 0)) return set_err(GPUGO_E_INVALID, "batch problems need tol_f > 0");
+  if (desc->flags & ~GPUGO_FLAG_MEAN_VALUE) return set_err(GPUGO_E_INVALID, "unknown flags 0x%x", desc->flags);
+  if ((desc->flags & GPUGO_FLAG_MEAN_VALUE) && !m->batch_mv)
+    return set_err(GPUGO_E_INVALID, "batch module built without the mean-value bound");
   const int c = desc->cycle < d ? desc->cycle : d;
   unsigned W = 0;
   if (!ipow_checked(desc->wid, c, &W) || W > (1u << 24))
@@ -2052,7 +2114,7 @@ extern "C" int gpugo_batch_admit(gpugo_batch_t b, int32_t family, const gpugo_pr
   I.wid = desc->wid;
   I.k_select = desc->k_select;
   I.n_samples = desc->n_samples;
-  I.flags = 0;
+  I.flags = desc->flags & GPUGO_FLAG_MEAN_VALUE;
   I.W = W;
   I.max_iters = desc->max_iters > 0 ? desc->max_iters : (1LL << 40);
   I.tol_f = desc->tol_f;

@@ -204,6 +204,20 @@ template <typename T> __device__ __forceinline__ iv<T> entire() {
   return mk<T>(-__builtin_huge_val(), __builtin_huge_val());
 }
 
+// outward conversion of a double to T (exact for double)
+template <typename T> __device__ __forceinline__ T to_dn(double x);
+template <typename T> __device__ __forceinline__ T to_up(double x);
+template <> __device__ __forceinline__ double to_dn<double>(double x) { return x; }
+template <> __device__ __forceinline__ double to_up<double>(double x) { return x; }
+template <> __device__ __forceinline__ float to_dn<float>(double x) {
+  float f = (float)x;
+  return ((double)f > x) ? pred<float>(f) : f;
+}
+template <> __device__ __forceinline__ float to_up<float>(double x) {
+  float f = (float)x;
+  return ((double)f < x) ? succ<float>(f) : f;
+}
+
 template <typename T, int R> struct IA {
   typedef rnd<T, R> RR;
   typedef iv<T> V;
@@ -227,6 +241,12 @@ template <typename T, int R> struct IA {
 
   __device__ static __forceinline__ T zmul_dn(T a, T b) { return (a == T(0) || b == T(0)) ? T(0) : RR::mul_dn(a, b); }
   __device__ static __forceinline__ T zmul_up(T a, T b) { return (a == T(0) || b == T(0)) ? T(0) : RR::mul_up(a, b); }
+
+  // product with an interval known to be >= 0 (a coefficient declared
+  // non-negative and rounded outward to T); 0 * inf counts as 0
+  __device__ static __forceinline__ V mulp(V a, V p) {
+    return mk<T>(zmul_dn(a.lo, a.lo >= T(0) ? p.lo : p.hi), zmul_up(a.hi, a.hi >= T(0) ? p.hi : p.lo));
+  }
 
   __device__ static __forceinline__ V mul(V a, V b) {
     T l = min_<T>(min_<T>(zmul_dn(a.lo, b.lo), zmul_dn(a.lo, b.hi)), min_<T>(zmul_dn(a.hi, b.lo), zmul_dn(a.hi, b.hi)));

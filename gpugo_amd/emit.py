@@ -16,7 +16,10 @@ Repeated subexpressions are evaluated once per code list.
 from __future__ import annotations
 
 import hashlib
+import math
 import os
+
+import numpy as np
 from typing import Dict, List, Optional, Tuple
 
 from .expr import Expr, Objective, wrap, Const
@@ -35,24 +38,54 @@ def _lit(v: float) -> str:
     return float(v).hex()
 
 
-class _Func:
-    """Straight-line code of one device function with CSE."""
+def _f32_bounds(v: float) -> Tuple[float, float]:
+    """Outward FP32 enclosure of a double."""
+    f = np.float32(v)
+    lo = f if float(f) <= v else np.nextafter(f, np.float32(-np.inf))
+    hi = f if float(f) >= v else np.nextafter(f, np.float32(np.inf))
+    return float(lo), float(hi)
 
-    def __init__(self):
+
+def _lit32(v: float) -> str:
+    """C literal of a double that is exactly a float."""
+    if v == float("inf"):
+        return "__builtin_huge_valf()"
+    if v == float("-inf"):
+        return "(-__builtin_huge_valf())"
+    return float(v).hex() + "f"
+
+
+def _pow2(e: Expr) -> bool:
+    """A constant whose products are exact (+-2^k)."""
+    if e.op != "const" or e.val == 0 or e.val != e.val or abs(e.val) == float("inf"):
+        return False
+    m, _ = math.frexp(abs(e.val))
+    return m == 0.5
+
+
+class _Func:
+    """Straight-line code of one device function with CSE.  ty = "double":
+    scalars (constants, coefficients, the dimension) are exact doubles;
+    ty = "float": the FP32 pruning code list, where coefficients are FP32
+    intervals (P[i] is an iv<float> rounded outward from the double) and
+    constants not representable in FP32 become outward intervals."""
+
+    def __init__(self, ty: str = "double"):
         self.lines: List[str] = []
         self.memo: Dict[tuple, Tuple[str, str]] = {}
         self.n = 0
+        self.ty = ty
 
     def tmp(self, kind: str, code: str) -> str:
         name = f"t{self.n}"
         self.n += 1
-        ty = "iv<double>" if kind == "i" else "double"
+        ty = f"iv<{self.ty}>" if kind == "i" else self.ty
         self.lines.append(f"  const {ty} {name} = {code};")
         return name
 
     def as_iv(self, kc: Tuple[str, str]) -> str:
         kind, code = kc
-        return code if kind == "i" else f"point<double>({code})"
+        return code if kind == "i" else f"point<{self.ty}>({code})"
 
     def lower(self, e: Expr) -> Tuple[str, str]:
         """Returns (kind, code): kind 's' = exact scalar C expression, 'i' = interval."""
@@ -65,14 +98,21 @@ class _Func:
 
     def _lower(self, e: Expr) -> Tuple[str, str]:
         op = e.op
+        f32 = self.ty == "float"
+        T = self.ty
         if op == "var":
             return ("i", f"x[{e.val}]")
         if op == "const":
-            return ("s", _lit(e.val))
+            if not f32:
+                return ("s", _lit(e.val))
+            lo, hi = _f32_bounds(e.val)
+            if lo == hi:
+                return ("s", _lit32(lo))
+            return ("i", self.tmp("i", f"mk<float>({_lit32(lo)}, {_lit32(hi)})"))
         if op == "param":
-            return ("s", f"P[{e.val}]")
+            return ("i" if f32 else "s", f"P[{e.val}]")
         if op == "dim":
-            return ("s", "((double)d)")
+            return ("s", f"(({T})d)")
         if op == "guard":
             raise ValueError("Guarded terms are only allowed inside gsum")
         if op == "gsum":
@@ -82,10 +122,10 @@ class _Func:
             self.n += 1
             terms = list(e.args)
             if terms[0].op == "guard":
-                init = "mk<double>(0.0, 0.0)"
+                init = f"mk<{T}>(0, 0)"
             else:
                 init = self.as_iv(self.lower(terms.pop(0)))
-            self.lines.append(f"  iv<double> {acc} = {init};")
+            self.lines.append(f"  iv<{T}> {acc} = {init};")
             for t in terms:
                 if t.op == "guard":
                     self.lines.append(f"  if ({t.val} < d) {{")
@@ -99,13 +139,22 @@ class _Func:
                     self.lines.append(f"  {acc} = A::add({acc}, {code});")
             return ("i", acc)
         if op == "iconst":
+            if f32:
+                lo, hi = _f32_bounds(e.val[0])[0], _f32_bounds(e.val[1])[1]
+                return ("i", self.tmp("i", f"mk<float>({_lit32(lo)}, {_lit32(hi)})"))
             return ("i", self.tmp("i", f"mk<double>({_lit(e.val[0])}, {_lit(e.val[1])})"))
         if e.is_scalar():
             if op == "neg":
-                return ("s", f"(-{self.lower(e.args[0])[1]})")
+                a = self.lower(e.args[0])
+                if a[0] == "s":
+                    return ("s", f"(-{a[1]})")
+                return ("i", self.tmp("i", f"A::neg({a[1]})"))
             if op == "mul":
-                a, b = (self.lower(x)[1] for x in e.args)
-                return ("s", f"({a} * {b})")
+                (ka, a), (kb, b) = (self.lower(x) for x in e.args)
+                # exact only when one factor is +-2^k; otherwise an interval
+                if ka == "s" and kb == "s" and (_pow2(e.args[0]) or _pow2(e.args[1])):
+                    return ("s", f"({a} * {b})")
+                return ("i", self.tmp("i", f"A::mul({self.as_iv((ka, a))}, {self.as_iv((kb, b))})"))
         if op == "neg":
             return ("i", self.tmp("i", f"A::neg({self.as_iv(self.lower(e.args[0]))})"))
         if op in ("sqr", "sqrt", "exp", "sin", "cos"):
@@ -118,19 +167,23 @@ class _Func:
         a, b = self.lower(ea), self.lower(eb)
         if op == "mul":
             if a[0] == "s" and b[0] == "s":
-                return ("i", self.tmp("i", f"A::scale(point<double>({a[1]}), {b[1]})"))
+                return ("i", self.tmp("i", f"A::scale(point<{T}>({a[1]}), {b[1]})"))
             if a[0] == "s":
                 s_code, s_expr, iv_code = a[1], ea, b[1]
             elif b[0] == "s":
                 s_code, s_expr, iv_code = b[1], eb, a[1]
             else:
+                # an interval coefficient of known sign (FP32 code list)
+                for se, sc, ic in ((ea, a[1], b[1]), (eb, b[1], a[1])):
+                    if se.is_scalar() and se.scalar_sign() > 0:
+                        return ("i", self.tmp("i", f"A::mulp({ic}, {sc})"))
                 return ("i", self.tmp("i", f"A::mul({a[1]}, {b[1]})"))
             sign = s_expr.scalar_sign()
             fn = "scale_pos" if sign > 0 else ("scale_neg" if sign < 0 else "scale")
             return ("i", self.tmp("i", f"A::{fn}({iv_code}, {s_code})"))
         if op == "add":
             if a[0] == "s" and b[0] == "s":
-                return ("i", self.tmp("i", f"A::addc(point<double>({a[1]}), {b[1]})"))
+                return ("i", self.tmp("i", f"A::addc(point<{T}>({a[1]}), {b[1]})"))
             if a[0] == "s":
                 return ("i", self.tmp("i", f"A::addc({b[1]}, {a[1]})"))
             if b[0] == "s":
@@ -138,7 +191,7 @@ class _Func:
             return ("i", self.tmp("i", f"A::add({a[1]}, {b[1]})"))
         if op == "sub":
             if a[0] == "s" and b[0] == "s":
-                return ("i", self.tmp("i", f"A::rsubc({a[1]}, point<double>({b[1]}))"))
+                return ("i", self.tmp("i", f"A::rsubc({a[1]}, point<{T}>({b[1]}))"))
             if a[0] == "s":
                 return ("i", self.tmp("i", f"A::rsubc({a[1]}, {b[1]})"))
             if b[0] == "s":
@@ -299,8 +352,9 @@ def code_lists(obj: Objective, suffix: str = "", dynamic: bool = False, header: 
     out.append(f"  return {f.as_iv(v)};")
     out.append("}")
 
-    def component(f, j, g, body_fn):
-        """Lines for gradient component j; absent coordinates skip it."""
+    def component(f, j, g, body_fn, absent=None):
+        """Lines for gradient component j; absent coordinates skip it (or
+        run the `absent` lines instead)."""
         guarded = dynamic and j > 0
         if guarded:
             f.lines.append(f"  if ({j} < d) {{")
@@ -309,17 +363,20 @@ def code_lists(obj: Objective, suffix: str = "", dynamic: bool = False, header: 
         f.lines += body_fn(code)
         if guarded:
             f.memo = saved
-            f.lines.append("  }")
+            f.lines.append("  }" + (f" else {{ {absent} }}" if absent else ""))
 
-    # gradient (parity hook, mean-value bound)
+    # gradient (parity hook, mean-value bound); absent components are zero
+    # and are only written when absent, so a sink sees each present
+    # component exactly once
     f = _Func()
     for j, g in enumerate(grads):
-        if dynamic and j > 0:
-            f.lines.append(f"  g[{j}] = mk<double>(0.0, 0.0);")
-        component(f, j, g, lambda code, j=j: [f"  g[{j}] = {code};"])
-    out.append(f"template <int R, class XS>\n__device__ __forceinline__ void obj_grad{suffix}("
+        component(f, j, g, lambda code, j=j: [f"  g[{j}] = {code};"],
+                  absent=f"g[{j}] = mk<double>(0.0, 0.0);")
+    # g is an array, or a sink whose g[j] = G_j folds each component into a
+    # mean-value sum as it is produced
+    out.append(f"template <int R, class XS, class GS>\n__device__ __forceinline__ void obj_grad{suffix}("
                f"const XS &x, const double *__restrict__ P, "
-               f"iv<double> *__restrict__ g{dp}) {{")
+               f"GS &&g{dp}) {{")
     out.append("  typedef IA<double, R> A;")
     out += f.lines
     out.append("}")
@@ -340,6 +397,21 @@ def code_lists(obj: Objective, suffix: str = "", dynamic: bool = False, header: 
     out.append("  return false;")
     out.append("}")
     out.append("}  // namespace gpugo")
+    return "\n".join(out) + "\n"
+
+
+def value_list_f32(obj: Objective, suffix: str = "") -> str:
+    """obj_value32: the value code list in FP32 intervals (outward rounded,
+    one ulp per operation in FAST mode) for the pruning pass.  P is the
+    coefficient array rounded outward to FP32 intervals."""
+    f = _Func("float")
+    v = f.lower(obj.value)
+    out = ["namespace gpugo {",
+           f"template <int R, class XS>\n__device__ __forceinline__ iv<float> obj_value32{suffix}("
+           f"const XS &x, const iv<float> *__restrict__ P, int d) {{",
+           "  typedef IA<float, R> A;"]
+    out += f.lines
+    out += [f"  return {f.as_iv(v)};", "}", "}  // namespace gpugo"]
     return "\n".join(out) + "\n"
 
 
@@ -401,28 +473,33 @@ def _read(name: str) -> str:
         return fh.read()
 
 
-def module_source(obj: Objective, roundings=("fast",)) -> str:
-    """Full hipRTC source; `roundings` selects the kernel variants built."""
+def module_source(obj: Objective, roundings=("fast",), f32: bool = False) -> str:
+    """Full hipRTC source; `roundings` selects the kernel variants built,
+    `f32` adds the FP32 value code list of the FP32-prune precision form."""
     bad = set(roundings) - {"exact", "fast"}
     if bad or not roundings:
         raise ValueError(f"roundings must be a non-empty subset of ('exact', 'fast'), got {roundings}")
-    return "\n".join([
+    parts = [
         "// generated by gpugo_amd.emit for objective " + obj.name,
         f"#define GPUGO_WITH_EXACT {int('exact' in roundings)}",
         f"#define GPUGO_WITH_FAST {int('fast' in roundings)}",
+        f"#define GPUGO_WITH_F32 {int(bool(f32))}",
         _read("ia.hpp"),
         _read("bnb_args.hpp"),
         _read("bnb_box.hpp"),
         code_lists(obj),
-        _read("bnb_kernel.hpp"),
-    ])
+    ]
+    if f32:
+        parts.append(value_list_f32(obj))
+    parts.append(_read("bnb_kernel.hpp"))
+    return "\n".join(parts)
 
 
-def batch_module_source(objs, roundings=("fast",)) -> str:
+def batch_module_source(objs, roundings=("fast",), mean_value: bool = False) -> str:
     """hipRTC source of a batch module: the families `objs` (variable-
     dimension objectives from problems.objective_dynamic, all with the same
-    maximum dimension) behind the dispatchers fam_value / fam_kkt, plus the
-    persistent batch split+bound kernel."""
+    maximum dimension) behind the dispatchers fam_value / fam_kkt (/ fam_grad
+    with mean_value), plus the persistent batch split+bound kernel."""
     bad = set(roundings) - {"exact", "fast"}
     if bad or not roundings:
         raise ValueError(f"roundings must be a non-empty subset of ('exact', 'fast'), got {roundings}")
@@ -441,6 +518,7 @@ def batch_module_source(objs, roundings=("fast",)) -> str:
         "// generated by gpugo_amd.emit: batch module " + ", ".join(o.name for o in objs),
         f"#define GPUGO_WITH_EXACT {int('exact' in roundings)}",
         f"#define GPUGO_WITH_FAST {int('fast' in roundings)}",
+        f"#define GPUGO_BATCH_MV {int(bool(mean_value))}",
         f"#define GPUGO_DMAX {dmax}",
         f"#define GPUGO_NFAM {len(objs)}",
         f"#define GPUGO_PMAX {pmax}",
@@ -454,6 +532,7 @@ def batch_module_source(objs, roundings=("fast",)) -> str:
         lines.append(code_lists(o, suffix=f"_f{i}", dynamic=True, header=False))
     val = "\n".join(f"    case {i}: return obj_value_f{i}<R>(x, P, d);" for i in range(len(objs)))
     kkt = "\n".join(f"    case {i}: return obj_kkt_f{i}<R>(x, P, dlo, dhi, d);" for i in range(len(objs)))
+    grd = "\n".join(f"    case {i}: obj_grad_f{i}<R>(x, P, g, d); return;" for i in range(len(objs)))
     lines.append(f"""namespace gpugo {{
 template <int R, class XS>
 __device__ __forceinline__ iv<double> fam_value(int f, const XS &x, const double *P, int d) {{
@@ -468,6 +547,13 @@ __device__ __forceinline__ bool fam_kkt(int f, const XS &x, const double *P, con
   switch (f) {{
 {kkt}
     default: return false;
+  }}
+}}
+template <int R, class XS, class GS>
+__device__ __forceinline__ void fam_grad(int f, const XS &x, const double *P, GS &&g, int d) {{
+  switch (f) {{
+{grd}
+    default: return;
   }}
 }}
 }}  // namespace gpugo""")

@@ -91,7 +91,7 @@ def _build(family: str, d: int, dynamic: bool) -> Objective:
 # Above this dimension the fixed-d code lists use the guarded form too: the
 # per-coordinate branches on the runtime d (always taken) split the code
 # into short blocks, so coordinates decoded on use do not stay live.
-GUARDED_D = 16
+GUARDED_D = 8
 
 
 def objective(family: str, d: int) -> Objective:

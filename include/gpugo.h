@@ -134,6 +134,13 @@ int gpugo_eval_boxes(gpugo_module_t m, int32_t d,
                      double *flo, double *fhi, double *glo, double *ghi,
                      int32_t *kkt, double *ub);
 
+/* FP32 value enclosure of n explicit boxes, for modules emitted with the
+ * FP32 code list (precision-F32 problems): the parity hook of the pruning
+ * pass.  Coefficients are rounded outward to FP32 intervals. */
+int gpugo_eval_boxes_f32(gpugo_module_t m, int32_t d, const double *params, int64_t n_params,
+                         int64_t n, const double *lo, const double *hi, int32_t rounding,
+                         double *flo, double *fhi);
+
 /* gpugo_problem_desc.flags */
 #define GPUGO_FLAG_MEAN_VALUE 1  /* also bound each child by the mean-value form
                                     f(c) + sum_j G_j (X_j - c_j), c the midpoint,
@@ -218,7 +225,7 @@ void gpugo_problem_destroy(gpugo_problem_t p);
  * (1-Ackley/ackley50.py:950-1395); each slot runs exactly the per-problem
  * loop above (same select, split+bound and compaction semantics), so a
  * problem's result in a batch equals its gpugo_solve result with the same
- * schedule and rounding (the mean-value flag is not available here).
+ * schedule, rounding and flags.
  *
  * A batch module (gpugo_amd.emit.batch_module_source) holds several
  * objective families in variable-dimension form (runtime d <= dmax).  The
@@ -248,8 +255,8 @@ int  gpugo_mem_info(int64_t *free_bytes, int64_t *total_bytes);
 int  gpugo_batch_create(gpugo_module_t m, const gpugo_batch_desc *desc, gpugo_batch_t *out);
 /* Admit one problem of objective family `family` into a free slot.  desc
  * uses the per-problem fields (d <= dmax, lo, hi, params in the family's
- * dmax layout, wid, cycle, k_select, n_samples, tol_f, init_ub, max_iters);
- * tol_x, flags and the capacities are not used (per-slot capacities come
+ * dmax layout, wid, cycle, k_select, n_samples, tol_f, init_ub, max_iters,
+ * flags); tol_x and the capacities are not used (per-slot capacities come
  * from the batch).  Returns GPUGO_E_CAPACITY when no slot is free.  The
  * problem starts with the next launch. */
 int  gpugo_batch_admit(gpugo_batch_t b, int32_t family, const gpugo_problem_desc *desc, int64_t tag,

@@ -110,3 +110,19 @@ def test_max_iters_and_capacity(gpu, bmod):
         with pytest.raises(GpugoError):
             b.admit(BatchProblem("sepcos", DMAX + 1, np.ones(3 * (DMAX + 1)), -np.ones(DMAX + 1),
                                  np.ones(DMAX + 1), 1e-6))
+
+
+def test_mean_value_batch_equals_single(gpu):
+    m = get_batch_module(pb.FAMILIES, DMAX, ("fast",), mean_value=True)
+    probs = mixed_problems(9, seed=3)
+    for p in probs:
+        p.mean_value = True
+    with Batch(slots=9, module=m, rounding="fast") as b:
+        res = b.solve_all(probs)
+    for p in probs:
+        obj = pb.objective(p.family, p.d)
+        with gpu.Problem(obj, p.params, p.lo, p.hi, tol=p.tol, grid=p.grid, k_select=p.k_select,
+                         rounding="fast", max_iters=p.max_iters, mean_value=True) as q:
+            s = q.solve()
+        for key in KEYS:
+            assert res[p.tag][key] == s[key], (p.tag, key)

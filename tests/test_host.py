@@ -26,7 +26,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def ir_eval(e, x, P, memo=None):
-    """Point value of an IR expression in mpmath (generic interpreter)."""
+    """Point value of an IR expression in mpmath (generic interpreter); the
+    dimension of guarded sums is len(x)."""
     if memo is None:
         memo = {}
     k = id(e)
@@ -39,9 +40,17 @@ def ir_eval(e, x, P, memo=None):
         r = mp.mpf(float(P[e.val]))
     elif op == "const":
         r = mp.mpf(e.val)
+    elif op == "dim":
+        r = mp.mpf(len(x))
     elif op == "iconst":
         lo, hi = e.val
         r = mp.pi if abs(lo - math.pi) < 1e-15 else mp.e
+    elif op == "guard":
+        r = ir_eval(e.args[0], x, P, memo) if e.val < len(x) else mp.mpf(0)
+    elif op == "gsum":
+        r = mp.mpf(0)
+        for t in e.args:
+            r += ir_eval(t, x, P, memo)
     else:
         a = [ir_eval(t, x, P, memo) for t in e.args]
         r = {"add": lambda: a[0] + a[1], "sub": lambda: a[0] - a[1], "mul": lambda: a[0] * a[1],
