@@ -41,6 +41,20 @@ SCHEDULES = {
 }
 
 
+TRAFFIC_FILE = "profiles/traffic_config1.json"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed counter profile
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by a device copy of
+    known size); None if that profile has no row for it."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as fh:
+            return json.load(fh)["kernels"][kernel]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,8 +64,9 @@ def parse():
     ap.add_argument("--rounding", choices=["fast", "exact"], default="fast")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--legs", default="c1,c2,batch,served",
-                    help="comma list: c1 (headline), c2 (d=50), batch (4096 problems), served (Poisson)")
+    ap.add_argument("--legs", default="fp32,c2,batch,served",
+                    help="comma list besides the headline: fp32 (headline workload in the FP32-prune form), "
+                         "c2 (d=50), batch (4096 problems), served (Poisson); 'none' for the headline only")
     ap.add_argument("--batch-n", type=int, default=4096)
     ap.add_argument("--seq-sample", type=int, default=256,
                     help="problems of the batch solved one at a time for the sequential baseline")
@@ -384,11 +399,13 @@ def main():
         "roofline": {"bound": "valu",
                      "kernel": f"gpugo_bound_{args.rounding}" + ("_mv" if sched["mean_value"] else ""),
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": measured_traffic(f"gpugo_bound_{args.rounding}" + ("_mv" if sched["mean_value"] else "")),
+                     "traffic_source": TRAFFIC_FILE,
                      "flop_per_launch": flops / max(launches, 1),
                      "avg_launch_ms": bound_ms / max(launches, 1), "launches": launches},
     }
-    if "c1" in legs:
+    if "fp32" in legs:
         out["fp32_prune"] = leg_fp32(G, spec, obj, sched, args, per_solve, r)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(spec, sched, args.cpu_seconds)

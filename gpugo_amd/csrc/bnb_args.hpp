@@ -34,7 +34,7 @@ struct DevState {
   int overflow;                  // 1: queue append, 2: parent arena, 4: output arena
   int compact_final;             // 1: the stop step's compaction still has to run
   int fresh;                     // batch slots: 1 until the root step ran
-  int pad1;
+  int cblocks;                   // batch slots: compaction workgroups finished this step
 };
 
 // Arguments of the module kernel gpugo_bound_{exact,fast}.

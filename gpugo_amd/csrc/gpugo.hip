@@ -1046,10 +1046,14 @@ __global__ __launch_bounds__(SEL_T) void kb_select(BatchArgs b) {
 }
 
 // The slot's queue is compacted by one workgroup, keeping the order.
+// The slot's queue is compacted by KB_CB workgroups (grid-stride over the
+// queue, one atomic per workgroup and chunk for the output position); the
+// last workgroup of the slot to finish switches the live buffer.
+#define KB_CB 8
 __global__ __launch_bounds__(256) void kb_compact(BatchArgs b) {
   __shared__ unsigned wcnt[4];
-  __shared__ unsigned long long s_kmin, s_kmax;
-  const int slot = blockIdx.x;
+  __shared__ unsigned long long s_kmin, s_kmax, s_base;
+  const int slot = blockIdx.x / KB_CB, sub = blockIdx.x % KB_CB;
   const SlotInfo I = b.info[slot];
   if (!I.active) return;
   DevState *st = b.st + slot;
@@ -1065,8 +1069,7 @@ __global__ __launch_bounds__(256) void kb_compact(BatchArgs b) {
     s_kmin = ~0ull;
     s_kmax = 0ull;
   }
-  long long outn = 0;
-  for (long long base = 0; base < n; base += 256) {
+  for (long long base = (long long)sub * 256; base < n; base += (long long)KB_CB * 256) {
     const long long i = base + threadIdx.x;
     double lb = 0;
     bool keep = false;
@@ -1084,18 +1087,19 @@ __global__ __launch_bounds__(256) void kb_compact(BatchArgs b) {
         __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
     if (lane == 0) wcnt[w] = (unsigned)__popcll(mask);
     __syncthreads();
-    unsigned off = pre, tot = 0;
-    for (int q = 0; q < 4; ++q) {
-      if (q < w) off += wcnt[q];
-      tot += wcnt[q];
+    if (threadIdx.x == 0) {
+      const unsigned t = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+      s_base = t ? atomicAdd((unsigned long long *)&st->compact_count, (unsigned long long)t) : 0ull;
     }
+    __syncthreads();
+    unsigned off = pre;
+    for (int q = 0; q < w; ++q) off += wcnt[q];
     if (keep) {
-      const long long o = out + outn + off;
+      const long long o = out + (long long)s_base + off;
       b.q_lb[o] = lb;
       b.q_par[o] = b.q_par[in + i];
       b.q_pos[o] = b.q_pos[in + i];
     }
-    outn += tot;
     __syncthreads();
   }
 #pragma unroll
@@ -1110,13 +1114,18 @@ __global__ __launch_bounds__(256) void kb_compact(BatchArgs b) {
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    st->compact_count = outn;
     if (s_kmin <= s_kmax) {
-      st->lbmin_key = s_kmin < st->lbmin_key ? s_kmin : st->lbmin_key;
-      st->lbmax_key = s_kmax > st->lbmax_key ? s_kmax : st->lbmax_key;
+      atomicMin(&st->lbmin_key, s_kmin);
+      atomicMax(&st->lbmax_key, s_kmax);
     }
-    if ((long long)n > st->queue_peak) st->queue_peak = n;
-    st->compactions += 1;
+    __threadfence();
+    if (atomicAdd(&st->cblocks, 1) == KB_CB - 1) {
+      // every workgroup of this slot has read `cur` and written its part
+      __threadfence();
+      if (n > st->queue_peak) st->queue_peak = n;
+      st->compactions += 1;
+      st->cblocks = 0;
+    }
   }
 }
 
@@ -1897,7 +1906,7 @@ static int batch_enqueue_step(gpugo_batch_t b) {
   HIPCHECK(hipGetLastError());
   void *args[] = {&b->bb};
   HIPCHECK(hipModuleLaunchKernel(b->bound, (unsigned)b->bound_grid, 1, 1, 256, 1, 1, 0, b->stream, args, nullptr));
-  hipLaunchKernelGGL(kb_compact, dim3((unsigned)b->slots), dim3(256), 0, b->stream, b->ba);
+  hipLaunchKernelGGL(kb_compact, dim3((unsigned)b->slots * KB_CB), dim3(256), 0, b->stream, b->ba);
   HIPCHECK(hipGetLastError());
   return GPUGO_OK;
 }
