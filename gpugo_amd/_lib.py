@@ -24,6 +24,7 @@ OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "neg": 4, "scale": 5, "addc": 6, 
 PREC = {"fp64": 0, "fp32": 1}
 ROUND = {"exact": 0, "fast": 1}
 FLAG_MEAN_VALUE = 1
+FLAG_SPLIT = 4
 STATUS = {0: "running", 1: "certified", 2: "x_converged", 3: "max_iters", 4: "exhausted", 5: "overflow"}
 
 EXPORTS = [

@@ -57,6 +57,9 @@ struct BoundArgs {
 
 // BoundArgs.flags bit of the FP32 pruning pass (problem precision F32)
 #define GPUGO_FLAG_F32_PRUNE 2
+#ifndef GPUGO_FLAG_SPLIT
+#define GPUGO_FLAG_SPLIT 4
+#endif
 
 // Arguments of the module kernel gpugo_eval_{exact,fast} (parity hook).
 struct EvalArgs {

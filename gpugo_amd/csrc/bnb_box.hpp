@@ -94,6 +94,28 @@ struct MVSink {
   }
 };
 
+// Reductions over groups of G consecutive lanes (G a power of two <= 64)
+// that share one box in the term-parallel kernels.  The butterfly leaves
+// the same value on every lane of the group (+ is commutative).
+template <int R, int G>
+__device__ __forceinline__ iv<double> group_sum(iv<double> v) {
+  typedef IA<double, R> A;
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) {
+    const double lo = __shfl_xor(v.lo, o, 64), hi = __shfl_xor(v.hi, o, 64);
+    v = A::add(v, mk<double>(lo, hi));
+  }
+  return v;
+}
+
+template <int G>
+__device__ __forceinline__ bool group_any(bool p) {
+  const unsigned long long m = __ballot(p);
+  const int base = (threadIdx.x & 63) & ~(G - 1);
+  const unsigned long long gm = (G >= 64) ? ~0ull : ((1ull << G) - 1ull);
+  return ((m >> base) & gm) != 0ull;
+}
+
 __device__ __forceinline__ ChildBox child_box(const double *plo, const double *phi, const double *pw, int win, int d,
                                               int c, unsigned wid, double inv_wid, double half_inv_wid, unsigned pos) {
   ChildBox x;

@@ -15,6 +15,10 @@
 #ifndef GPUGO_WITH_F32
 #define GPUGO_WITH_F32 0
 #endif
+#ifndef GPUGO_SPLIT_G
+#define GPUGO_SPLIT_G 0
+#endif
+
 // dimensions above this decode child coordinates on use instead of into an
 // array (the array would keep 4 d registers live through the code lists)
 #ifndef GPUGO_LAZY_D
@@ -238,6 +242,139 @@ __device__ __forceinline__ void bound_impl(const BoundArgs &a) {
   }
 }
 
+#if GPUGO_SPLIT_G > 1
+// Term-parallel split+bound: GPUGO_SPLIT_G consecutive lanes share one
+// child; the code lists' template loops run over j = lane, lane + G, ...
+// and reduce over the group, so a step's children occupy G times more
+// lanes and each lane runs about 1/G of the terms.  The same tests as
+// bound_impl (value, then for survivors the monotonicity test and the
+// sample bound, or the mean-value form); lane 0 of a group reports the box.
+template <int R, int G>
+__device__ __forceinline__ double sample_ub_g(const ChildBox &x, const double *__restrict__ P, int S, int d, int lg) {
+  double best = __builtin_huge_val();
+  for (int t = 0; t < S; ++t) {
+    const DiagPoint<ChildBox> xs{x, t, S};
+    const iv<double> f = obj_value_g<R, G>(xs, P, d, lg);
+    if (f.hi < best) best = f.hi;
+  }
+  return best;
+}
+
+template <int R, bool MV>
+__device__ __forceinline__ void bound_impl_g(const BoundArgs &a) {
+  constexpr int G = GPUGO_SPLIT_G;
+  typedef IA<double, R> A;
+  __shared__ unsigned s_wcnt[4];
+  __shared__ unsigned long long s_base;
+  __shared__ unsigned long long s_ubmin;
+  __shared__ unsigned s_pv, s_pk;
+  DevState *st = a.st;
+  if (st->status != 0) return;
+  const long long n_child = (long long)st->n_split * (long long)a.W;
+  if (n_child == 0) return;
+  const long long first = st->first_parent;
+  const double ub0 = st->ub_snap;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lg = threadIdx.x & (G - 1);
+  const double invW = 1.0 / (double)a.W, halfInvW = 0.5 / (double)a.W;
+  if (threadIdx.x == 0) {
+    s_ubmin = order_key(__builtin_huge_val());
+    s_pv = 0;
+    s_pk = 0;
+  }
+  __syncthreads();
+  for (long long base = (long long)blockIdx.x * 256; base < n_child * G; base += (long long)gridDim.x * 256) {
+    const long long ci = (base + threadIdx.x) / G;
+    const bool active = ci < n_child;
+    long long pl = 0;
+    unsigned pos = 0;
+    double lb = __builtin_huge_val(), su = __builtin_huge_val();
+    bool keep = false, pv = false, pk = false;
+    if (active) {
+      pl = (long long)__builtin_fma((double)ci, invW, halfInvW);
+      pos = (unsigned)(ci - pl * (long long)a.W);
+      const long long p = first + pl;
+      const ChildBox x = child_box(a.plo + p * GPUGO_D, a.phi + p * GPUGO_D, a.pw + p * GPUGO_D, a.pwin[p],
+                                   GPUGO_D, a.c, (unsigned)a.wid, a.inv_wid, a.half_inv_wid, pos);
+      const iv<double> f = obj_value_g<R, G>(x, a.params, a.d, lg);
+      lb = f.lo;
+      if (!(lb == lb)) lb = -__builtin_huge_val();
+      if (lb <= ub0) {
+        if (MV) {
+          const DiagPoint<ChildBox> xc{x, 0, 1};
+          const iv<double> s0 = obj_value_g<R, G>(xc, a.params, a.d, lg);
+          su = a.n_samples == 1 ? s0.hi : sample_ub_g<R, G>(x, a.params, a.n_samples, a.d, lg);
+          MVSink<R, ChildBox> g{x, a.dom_lo, a.dom_hi, a.d, mk<double>(0.0, 0.0), false};
+          obj_grad_g<R, G>(x, a.params, g, a.d, lg);
+          const iv<double> s = A::add(s0, group_sum<R, G>(g.acc));
+          const bool prune = group_any<G>(g.prune);
+          if (s.lo > lb) lb = s.lo;
+          if (lb <= ub0) {
+            keep = !prune;
+            pk = !keep;
+          } else {
+            pv = true;
+          }
+        } else {
+          keep = !obj_kkt_g<R, G>(x, a.params, a.dom_lo, a.dom_hi, a.d, lg);
+          pk = !keep;
+          su = sample_ub_g<R, G>(x, a.params, a.n_samples, a.d, lg);
+        }
+      } else {
+        pv = true;
+      }
+      if (lg != 0) keep = pv = pk = false;
+    }
+    const unsigned long long mask = __ballot(keep);
+    const unsigned pre =
+        __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+    if (lane == 0) s_wcnt[w] = (unsigned)__popcll(mask);
+    double m = su;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      double t2 = __shfl_xor(m, o, 64);
+      m = t2 < m ? t2 : m;
+    }
+    const unsigned long long pvm = __ballot(pv), pkm = __ballot(pk);
+    if (lane == 0) {
+      if (m < __builtin_huge_val()) atomicMin(&s_ubmin, order_key(m));
+      if (pvm) atomicAdd(&s_pv, (unsigned)__popcll(pvm));
+      if (pkm) atomicAdd(&s_pk, (unsigned)__popcll(pkm));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const unsigned tot = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
+      unsigned long long b = 0;
+      if (tot) {
+        b = atomicAdd((unsigned long long *)&st->q_count, (unsigned long long)tot);
+        if (b + tot > (unsigned long long)a.q_cap) atomicOr(&st->overflow, 1);
+      }
+      s_base = b;
+    }
+    __syncthreads();
+    if (keep) {
+      unsigned off = pre;
+      for (int q = 0; q < w; ++q) off += s_wcnt[q];
+      const unsigned long long o = s_base + off;
+      if (o < (unsigned long long)a.q_cap) {
+        a.q_lb[o] = lb;
+        a.q_par[o] = (unsigned)(first + pl);
+        a.q_pos[o] = pos;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (s_ubmin < *(volatile unsigned long long *)&st->ub_key) atomicMin(&st->ub_key, s_ubmin);
+    if (s_pv) atomicAdd((unsigned long long *)&st->pruned_value, (unsigned long long)s_pv);
+    if (s_pk) atomicAdd((unsigned long long *)&st->pruned_kkt, (unsigned long long)s_pk);
+    if (blockIdx.x == 0) {
+      st->iterations += 1;
+      st->bounded += n_child;
+    }
+  }
+}
+#endif
+
 template <int R>
 __device__ __forceinline__ void eval_impl(const EvalArgs &a) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -296,6 +433,22 @@ extern "C" __global__ __launch_bounds__(64) void gpugo_eval_exact(gpugo::EvalArg
   gpugo::eval_impl<gpugo::ROUND_EXACT>(a);
 }
 #endif
+#if GPUGO_SPLIT_G > 1 && GPUGO_WITH_EXACT
+extern "C" __global__ __launch_bounds__(256) void gpugo_bound_exact_g(gpugo::BoundArgs a) {
+  gpugo::bound_impl_g<gpugo::ROUND_EXACT, false>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void gpugo_bound_exact_mv_g(gpugo::BoundArgs a) {
+  gpugo::bound_impl_g<gpugo::ROUND_EXACT, true>(a);
+}
+#endif
+#if GPUGO_SPLIT_G > 1 && GPUGO_WITH_FAST
+extern "C" __global__ __launch_bounds__(256) void gpugo_bound_fast_g(gpugo::BoundArgs a) {
+  gpugo::bound_impl_g<gpugo::ROUND_FAST, false>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void gpugo_bound_fast_mv_g(gpugo::BoundArgs a) {
+  gpugo::bound_impl_g<gpugo::ROUND_FAST, true>(a);
+}
+#endif
 #if GPUGO_WITH_F32 && GPUGO_WITH_EXACT
 extern "C" __global__ __launch_bounds__(64) void gpugo_eval32_exact(gpugo::EvalArgs a) {
   gpugo::eval32_impl<gpugo::ROUND_EXACT>(a);
@@ -329,4 +482,5 @@ extern "C" __global__ void gpugo_info(int *out) {
   out[2] = GPUGO_MODULE_ABI_V;
   out[3] = (GPUGO_WITH_EXACT ? 1 : 0) | (GPUGO_WITH_FAST ? 2 : 0);
   out[4] = GPUGO_WITH_F32;
+  out[5] = GPUGO_SPLIT_G;
 }

@@ -198,6 +198,9 @@ struct gpugo_module_s {
   hipFunction_t bound_mv[2] = {nullptr, nullptr};  // [rounding], mean-value variant
   hipFunction_t eval[2] = {nullptr, nullptr};
   hipFunction_t eval32[2] = {nullptr, nullptr};  // FP32 value (precision F32 modules)
+  hipFunction_t bound_g[2] = {nullptr, nullptr};     // term-parallel variants (split modules)
+  hipFunction_t bound_mv_g[2] = {nullptr, nullptr};
+  int split_g = 0;                                    // lanes per box of those variants
   int dim = 0, nparams = 0, abi = 0;
   bool f32 = false;                // has the FP32 value code list (precision F32)
   std::vector<signed char> psign;  // declared coefficient signs
@@ -266,6 +269,17 @@ static int module_finish(gpugo_module_t m) {
   m->nparams = h[1];
   m->abi = h[2];
   m->f32 = h[4] != 0;
+  m->split_g = h[5];
+  if (m->split_g > 1) {
+    if (h[3] & 1) {
+      HIPCHECK(hipModuleGetFunction(&m->bound_g[0], m->mod, "gpugo_bound_exact_g"));
+      HIPCHECK(hipModuleGetFunction(&m->bound_mv_g[0], m->mod, "gpugo_bound_exact_mv_g"));
+    }
+    if (h[3] & 2) {
+      HIPCHECK(hipModuleGetFunction(&m->bound_g[1], m->mod, "gpugo_bound_fast_g"));
+      HIPCHECK(hipModuleGetFunction(&m->bound_mv_g[1], m->mod, "gpugo_bound_fast_mv_g"));
+    }
+  }
   if (m->f32 && (h[3] & 1)) HIPCHECK(hipModuleGetFunction(&m->eval32[0], m->mod, "gpugo_eval32_exact"));
   if (m->f32 && (h[3] & 2)) HIPCHECK(hipModuleGetFunction(&m->eval32[1], m->mod, "gpugo_eval32_fast"));
   if (m->abi != GPUGO_MODULE_ABI)
@@ -1401,7 +1415,12 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
     return set_err(GPUGO_E_INVALID, "unknown precision %d", desc->precision);
   if (desc->precision == GPUGO_PREC_F32 && !m->f32)
     return set_err(GPUGO_E_INVALID, "module has no FP32 code list (emit it with f32=True)");
-  if (desc->flags & ~GPUGO_FLAG_MEAN_VALUE) return set_err(GPUGO_E_INVALID, "unknown flags 0x%x", desc->flags);
+  if (desc->flags & ~(GPUGO_FLAG_MEAN_VALUE | GPUGO_FLAG_SPLIT))
+    return set_err(GPUGO_E_INVALID, "unknown flags 0x%x", desc->flags);
+  if ((desc->flags & GPUGO_FLAG_SPLIT) && m->split_g < 2)
+    return set_err(GPUGO_E_INVALID, "module has no term-parallel kernels (emit it with split=G)");
+  if ((desc->flags & GPUGO_FLAG_SPLIT) && desc->precision == GPUGO_PREC_F32)
+    return set_err(GPUGO_E_INVALID, "the term-parallel kernels have no FP32 pruning pass");
   if (desc->rounding != GPUGO_ROUND_EXACT && desc->rounding != GPUGO_ROUND_FAST)
     return set_err(GPUGO_E_INVALID, "unknown rounding %d", desc->rounding);
   if (!m->bound[desc->rounding])
@@ -1607,8 +1626,9 @@ static int enqueue_step(gpugo_problem_t p) {
   ba.flags = p->desc.flags | (p->desc.precision == GPUGO_PREC_F32 ? GPUGO_FLAG_F32_PRUNE : 0);
   ba.params32 = p->d_params32.p;
   {
+    const bool split = (p->desc.flags & GPUGO_FLAG_SPLIT) != 0;
     long long maxk = p->steps_enqueued == 0 ? 1 : p->desc.k_select;
-    long long nthreads = maxk * (long long)p->W;
+    long long nthreads = maxk * (long long)p->W * (split ? p->m->split_g : 1);
     long long blocks = (nthreads + 255) / 256;
     if (blocks > 16384) blocks = 16384;
     void *args[] = {&ba};
@@ -1620,8 +1640,9 @@ static int enqueue_step(gpugo_problem_t p) {
       }
       HIPCHECK(hipEventRecord(p->tev[p->tev_used++], p->stream));
     }
-    hipFunction_t fn = (p->desc.flags & GPUGO_FLAG_MEAN_VALUE) ? p->m->bound_mv[p->desc.rounding]
-                                                               : p->m->bound[p->desc.rounding];
+    const bool mv = (p->desc.flags & GPUGO_FLAG_MEAN_VALUE) != 0;
+    hipFunction_t fn = split ? (mv ? p->m->bound_mv_g[p->desc.rounding] : p->m->bound_g[p->desc.rounding])
+                             : (mv ? p->m->bound_mv[p->desc.rounding] : p->m->bound[p->desc.rounding]);
     HIPCHECK(hipModuleLaunchKernel(fn, (unsigned)blocks, 1, 1, 256, 1, 1, 0, p->stream, args, nullptr));
     if (p->timing) HIPCHECK(hipEventRecord(p->tev[p->tev_used++], p->stream));
   }

@@ -22,7 +22,7 @@ import os
 import numpy as np
 from typing import Dict, List, Optional, Tuple
 
-from .expr import Expr, Objective, wrap, Const
+from .expr import Expr, Objective, wrap, Const, has_loop_index
 
 _CSRC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc")
 
@@ -204,6 +204,118 @@ class _Func:
                 return ("i", self.tmp("i", f"A::{fn}({ia}, {b[1]})"))
             return ("i", self.tmp("i", f"A::div({self.as_iv(a)}, {b[1]})"))
         raise ValueError(f"cannot lower op {op}")
+
+
+class _FuncG(_Func):
+    """Term-parallel code lists (the loop templates of an objective): the
+    G lanes of a group share one box; a TSum runs over j = lg, lg + G, ...
+    and ends with a butterfly reduction over the group, so every lane of the
+    group holds the sum; J-indexed leaves read index a*j + b.  Loop-invariant
+    subexpressions are evaluated before the loop."""
+
+    def __init__(self, ty: str = "double"):
+        super().__init__(ty)
+        self.in_loop = False
+
+    def _lower(self, e: Expr) -> Tuple[str, str]:
+        op = e.op
+        if op in ("var", "param") and isinstance(e.val, tuple):
+            if not self.in_loop:
+                raise ValueError("loop index used outside a template loop")
+            _, a, b = e.val
+            idx = f"({a} * j + {b})" if b >= 0 else f"({a} * j - {-b})"
+            return ("i", f"x[{idx}]") if op == "var" else ("s", f"P[{idx}]")
+        if op == "tsum":
+            return self.loop_sum(e)
+        if op == "twhen":
+            cond = {"j+1<d": "j + 1 < d", "j>0": "j > 0"}[e.val]
+            name = f"t{self.n}"
+            self.n += 1
+            self.lines.append(f"  iv<{self.ty}> {name} = mk<{self.ty}>(0, 0);")
+            self.lines.append(f"  if ({cond}) {{")
+            saved = dict(self.memo)
+            code = self.as_iv(self.lower(e.args[0]))
+            self.lines.append(f"  {name} = {code};")
+            self.memo = saved
+            self.lines.append("  }")
+            return ("i", name)
+        return super()._lower(e)
+
+    def hoist(self, body: Expr):
+        """Evaluate the maximal loop-invariant subexpressions of `body`."""
+        def walk(e):
+            if not has_loop_index(e):
+                if e.op not in ("var", "param", "const", "dim", "iconst") and not e.is_scalar():
+                    self.lower(e)
+                return
+            for a in e.args:
+                walk(a)
+        walk(body)
+
+    def loop(self, body: Expr, lim: str, emit_fn):
+        """for (j = lg; j < lim; j += G) { emit_fn(code of body) }"""
+        if self.in_loop:
+            raise ValueError("nested template loops are not supported")
+        self.hoist(body)
+        self.lines.append(f"  for (int j = lg; j < {lim}; j += G) {{")
+        saved = dict(self.memo)
+        self.in_loop = True
+        code = self.as_iv(self.lower(body))
+        self.lines += emit_fn(code)
+        self.in_loop = False
+        self.memo = saved
+        self.lines.append("  }")
+
+    def loop_sum(self, e: Expr) -> Tuple[str, str]:
+        acc = f"t{self.n}"
+        self.n += 1
+        off = e.val
+        lim = "d" if off == 0 else (f"d + {off}" if off > 0 else f"d - {-off}")
+        self.lines.append(f"  iv<{self.ty}> {acc} = mk<{self.ty}>(0, 0);")
+        self.loop(e.args[0], lim, lambda code: [f"    {acc} = A::add({acc}, {code});"])
+        self.lines.append(f"  {acc} = group_sum<R, G>({acc});")
+        return ("i", acc)
+
+
+def split_lists(obj: Objective) -> str:
+    """obj_value_g / obj_grad_g / obj_kkt_g from the objective's loop
+    templates: the term-parallel code lists (G lanes per box)."""
+    t = obj.templates
+    if not t or "value" not in t or "grad" not in t:
+        raise ValueError(f"objective {obj.name} has no loop templates")
+    out = ["namespace gpugo {"]
+    f = _FuncG()
+    v = f.lower(wrap(t["value"]))
+    out.append("template <int R, int G, class XS>\n__device__ __forceinline__ iv<double> obj_value_g("
+               "const XS &x, const double *__restrict__ P, int d, int lg) {")
+    out.append("  typedef IA<double, R> A;")
+    out += f.lines
+    out.append(f"  return {f.as_iv(v)};")
+    out.append("}")
+    grad = wrap(t["grad"])
+    # gradient components j = lg, lg + G, ... into a sink (one lane each)
+    f = _FuncG()
+    f.loop(grad, "d", lambda code: [f"    g[j] = {code};"])
+    out.append("template <int R, int G, class XS, class GS>\n__device__ __forceinline__ void obj_grad_g("
+               "const XS &x, const double *__restrict__ P, GS &&g, int d, int lg) {")
+    out.append("  typedef IA<double, R> A;")
+    out += f.lines
+    out.append("}")
+    # monotonicity test: each lane tests its components, the group ORs
+    f = _FuncG()
+    f.lines.append("  bool prune = false;")
+    f.loop(grad, "d", lambda code: [
+        f"    {{ const iv<double> gj = {code};",
+        "      prune |= (gj.lo > 0.0 && x[j].lo != dlo[j]) || (gj.hi < 0.0 && x[j].hi != dhi[j]); }"])
+    out.append("template <int R, int G, class XS>\n__device__ __forceinline__ bool obj_kkt_g("
+               "const XS &x, const double *__restrict__ P, const double *__restrict__ dlo, "
+               "const double *__restrict__ dhi, int d, int lg) {")
+    out.append("  typedef IA<double, R> A;")
+    out += f.lines
+    out.append("  return group_any<G>(prune);")
+    out.append("}")
+    out.append("}  // namespace gpugo")
+    return "\n".join(out) + "\n"
 
 
 # ---------------------------------------------------------------------------
@@ -473,9 +585,13 @@ def _read(name: str) -> str:
         return fh.read()
 
 
-def module_source(obj: Objective, roundings=("fast",), f32: bool = False) -> str:
+def module_source(obj: Objective, roundings=("fast",), f32: bool = False, split: int = 0) -> str:
     """Full hipRTC source; `roundings` selects the kernel variants built,
-    `f32` adds the FP32 value code list of the FP32-prune precision form."""
+    `f32` adds the FP32 value code list of the FP32-prune precision form,
+    `split` = G > 1 adds the term-parallel kernels (G lanes per box; needs
+    the objective's loop templates)."""
+    if split and (split < 2 or split > 64 or split & (split - 1)):
+        raise ValueError("split must be a power of two in [2, 64]")
     bad = set(roundings) - {"exact", "fast"}
     if bad or not roundings:
         raise ValueError(f"roundings must be a non-empty subset of ('exact', 'fast'), got {roundings}")
@@ -484,6 +600,7 @@ def module_source(obj: Objective, roundings=("fast",), f32: bool = False) -> str
         f"#define GPUGO_WITH_EXACT {int('exact' in roundings)}",
         f"#define GPUGO_WITH_FAST {int('fast' in roundings)}",
         f"#define GPUGO_WITH_F32 {int(bool(f32))}",
+        f"#define GPUGO_SPLIT_G {int(split)}",
         _read("ia.hpp"),
         _read("bnb_args.hpp"),
         _read("bnb_box.hpp"),
@@ -491,6 +608,8 @@ def module_source(obj: Objective, roundings=("fast",), f32: bool = False) -> str
     ]
     if f32:
         parts.append(value_list_f32(obj))
+    if split:
+        parts.append(split_lists(obj))
     parts.append(_read("bnb_kernel.hpp"))
     return "\n".join(parts)
 

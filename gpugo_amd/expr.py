@@ -129,17 +129,67 @@ def wrap(v) -> Expr:
     raise TypeError(f"cannot use {type(v).__name__} in an objective expression")
 
 
-def X(j: int) -> Expr:
+class Idx:
+    """Affine index a*J + b of the loop variable J of a template (TSum and
+    gradient-component templates); X(J + 1), P(J + d) etc."""
+    __slots__ = ("a", "b")
+
+    def __init__(self, a: int = 1, b: int = 0):
+        self.a, self.b = int(a), int(b)
+
+    def __add__(self, k: int): return Idx(self.a, self.b + int(k))
+    __radd__ = __add__
+    def __sub__(self, k: int): return Idx(self.a, self.b - int(k))
+    def __mul__(self, k: int): return Idx(self.a * int(k), self.b * int(k))
+    __rmul__ = __mul__
+
+    def at(self, j: int) -> int:
+        return self.a * j + self.b
+
+
+J = Idx()
+
+
+def X(j) -> Expr:
+    if isinstance(j, Idx):
+        return Expr("var", (), ("J", j.a, j.b))
     return Expr("var", (), int(j))
 
 
-def P(i: int, sign: int | None = None) -> Expr:
+def P(i, sign: int | None = None) -> Expr:
     """Coefficient P[i].  `sign` = +1 (or -1) declares P[i] >= 0 (<= 0); the
     emitter then uses the one-sided scalar product, and problems whose
     coefficients violate a declared sign are rejected."""
     if sign not in (None, 1, -1):
         raise ValueError("sign must be None, +1 or -1")
+    if isinstance(i, Idx):
+        return Expr("param", (), ("J", i.a, i.b), sign)
     return Expr("param", (), int(i), sign)
+
+
+def TSum(body, off: int = 0) -> Expr:
+    """Template sum: sum over j in [0, d + off) of `body` with J = j (left
+    fold in increasing j).  Used by the term-parallel code lists, where the
+    lanes of a group take j = lane, lane + G, ... and reduce."""
+    return Expr("tsum", (wrap(body),), int(off))
+
+
+def TWhen(cond: str, body) -> Expr:
+    """Inside a template: `body` where the condition on j holds, else 0.
+    cond is "j+1<d" or "j>0"."""
+    if cond not in ("j+1<d", "j>0"):
+        raise ValueError(f"unknown template condition {cond!r}")
+    return Expr("twhen", (wrap(body),), cond)
+
+
+def has_loop_index(e: Expr) -> bool:
+    """True if e depends on the loop variable J of an enclosing template
+    (a TSum binds J inside itself)."""
+    if e.op in ("var", "param") and isinstance(e.val, tuple):
+        return True
+    if e.op == "tsum":
+        return False
+    return any(has_loop_index(a) for a in e.args)
 
 
 def Dim() -> Expr:
@@ -213,12 +263,16 @@ class Objective:
     """
 
     def __init__(self, d: int, n_params: int, value: Expr, grad: Sequence[Expr] | None = None,
-                 name: str = "objective"):
+                 name: str = "objective", templates: dict | None = None):
         self.d = int(d)
         self.n_params = int(n_params)
         self.value = wrap(value)
         self.grad = list(grad) if grad is not None else None
         self.name = name
+        # optional loop form of the same function for the term-parallel code
+        # lists: {"value": Expr with TSum nodes, "grad": component template
+        # in J (j in [0, d))}
+        self.templates = templates
         if self.grad is not None and len(self.grad) != self.d:
             raise ValueError(f"gradient has {len(self.grad)} components, d={self.d}")
 

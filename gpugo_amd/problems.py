@@ -28,7 +28,7 @@ from typing import List, Optional
 
 import numpy as np
 
-from .expr import E, Dim, Guarded, Objective, P, X, cos, exp, fsum, gsum, pown, sin, sqr, sqrt
+from .expr import E, J, Dim, Guarded, Objective, P, TSum, TWhen, X, cos, exp, fsum, gsum, pown, sin, sqr, sqrt
 
 FAMILIES = ("sepcos", "sepcos_chain", "ackley_q")
 
@@ -85,7 +85,40 @@ def _build(family: str, d: int, dynamic: bool) -> Objective:
     else:
         raise ValueError(f"unknown family {family!r}")
     tag = "dyn" if dynamic else "d"
-    return Objective(d, n_params(family, d), value, grad, name=f"{family}_{tag}{d}")
+    return Objective(d, n_params(family, d), value, grad, name=f"{family}_{tag}{d}",
+                     templates=_templates(family, d))
+
+
+def _templates(family: str, d: int) -> dict:
+    """The same family as loops over j (for the term-parallel code lists):
+    the value with TSum nodes and one gradient component in J.  d is the
+    coefficient layout (the maximum dimension); loops run to the runtime d."""
+    p = lambda i: P(i, +1)
+    if family in ("sepcos", "sepcos_chain"):
+        a, b, c = (lambda i: p(i)), (lambda i: p(i + d)), (lambda i: p(i + 2 * d))
+        sep = sqr(X(J)) * a(J) + (1.0 - cos(X(J) * c(J))) * b(J)
+        sep_g = X(J) * (2.0 * a(J)) + (sin(X(J) * c(J)) * c(J)) * b(J)
+        if family == "sepcos":
+            return {"value": TSum(sep), "grad": sep_g}
+        w = lambda i: p(i + 3 * d)
+        chain = sqr(X(J + 1) - sqr(X(J))) * w(J) + sqr(1.0 - X(J))
+        fwd = (X(J) * (X(J + 1) - sqr(X(J)))) * (-4.0 * w(J)) - (1.0 - X(J)) * 2.0
+        bwd = (X(J) - sqr(X(J - 1))) * (2.0 * w(J - 1))
+        return {"value": TSum(sep) + TSum(chain, off=-1),
+                "grad": (sep_g + TWhen("j+1<d", fwd)) + TWhen("j>0", bwd)}
+    A, B, C = p(0), p(1), p(2)
+    q = lambda i: p(i + 3)
+    nd = Dim()
+    s1 = TSum(sqr(X(J)))
+    s2 = TSum(cos(X(J) * C))
+    s3 = TSum(pown(X(J), 4) * q(J))
+    r = sqrt(s1 / nd)
+    t1 = exp(r * (-B))
+    value = ((t1 * (-A) - exp(s2 / nd)) + (A + E)) + s3 / 50.0
+    k1 = (((t1 / r) * B) * A) / nd
+    k2 = (exp(s2 / nd) * C) / nd
+    grad = (k1 * X(J) + k2 * sin(X(J) * C)) + (pown(X(J), 3) * (4.0 * q(J))) / 50.0
+    return {"value": value, "grad": grad}
 
 
 # Above this dimension the fixed-d code lists use the guarded form too: the

@@ -31,9 +31,10 @@ _modules = {}
 _mod_lock = threading.Lock()
 
 
-def rtc_compile_to_cache(obj: Objective, options: str = "", roundings=("fast",), f32: bool = False) -> str:
+def rtc_compile_to_cache(obj: Objective, options: str = "", roundings=("fast",), f32: bool = False,
+                         split: int = 0) -> str:
     """Compile an objective module with hipRTC (no GPU needed) into the cache."""
-    return compile_source_to_cache(module_source(obj, tuple(roundings), f32), obj.name, options)
+    return compile_source_to_cache(module_source(obj, tuple(roundings), f32, split), obj.name, options)
 
 
 def compile_source_to_cache(src: str, name: str, options: str = "") -> str:
@@ -59,13 +60,15 @@ def compile_source_to_cache(src: str, name: str, options: str = "") -> str:
 class Module:
     """A compiled objective (one hipRTC code object loaded on the GPU)."""
 
-    def __init__(self, obj: Objective, options: str = "", roundings=("fast",), f32: bool = False):
+    def __init__(self, obj: Objective, options: str = "", roundings=("fast",), f32: bool = False,
+                 split: int = 0):
         _lib.init()
         lib = _lib.load()
         self.obj = obj
         self.roundings = tuple(roundings)
         self.f32 = bool(f32)
-        path = rtc_compile_to_cache(obj, options, self.roundings, self.f32)
+        self.split = int(split)
+        path = rtc_compile_to_cache(obj, options, self.roundings, self.f32, self.split)
         with open(path, "rb") as fh:
             code = fh.read()
         h = ctypes.c_void_p()
@@ -115,13 +118,14 @@ def eval_boxes_f32(module: "Module", params, lo, hi, rounding: str = "fast"):
     return flo, fhi
 
 
-def get_module(obj: Objective, options: str = "", roundings=("fast",), f32: bool = False) -> Module:
+def get_module(obj: Objective, options: str = "", roundings=("fast",), f32: bool = False,
+               split: int = 0) -> Module:
     roundings = tuple(sorted(set(roundings)))
-    key = source_key(module_source(obj, roundings, f32), options)
+    key = source_key(module_source(obj, roundings, f32, split), options)
     with _mod_lock:
         m = _modules.get(key)
         if m is None:
-            m = Module(obj, options, roundings, f32)
+            m = Module(obj, options, roundings, f32, split)
             _modules[key] = m
         return m
 
@@ -147,19 +151,24 @@ class Problem:
                  grid: Sequence[int] = (4, 10), k_select: int = 1, n_samples: int = 1,
                  init_ub: float = np.inf, rounding: str = "fast", max_queue: int = 0,
                  max_parents: int = 0, max_iters: int = 0, module: Optional[Module] = None,
-                 mean_value: bool = False, precision: str = "fp64"):
+                 mean_value: bool = False, precision: str = "fp64", split: int = 0):
         """precision "fp32": children are first bounded in FP32 intervals
         (outward, one ulp per operation) and only those that survive get the
-        FP64 value, gradient and sample; the enclosure is certified in FP64."""
+        FP64 value, gradient and sample; the enclosure is certified in FP64.
+        split = G > 1: term-parallel kernels, G lanes share each child (for
+        steps too small to fill the GPU; needs the objective's loop
+        templates)."""
         if precision not in _lib.PREC:
             raise ValueError(f"precision must be one of {sorted(_lib.PREC)}")
-        self.module = module or get_module(obj, roundings=(rounding,), f32=precision == "fp32")
+        self.module = module or get_module(obj, roundings=(rounding,), f32=precision == "fp32", split=split)
         self.obj = obj
         self._lo, self._hi, self._params = f64(lo), f64(hi), f64(params)
         wid, cycle = grid
         self.desc = _lib.ProblemDesc(
             d=obj.d, wid=int(wid), cycle=int(cycle), k_select=int(k_select), n_samples=int(n_samples),
-            precision=_lib.PREC[precision], rounding=_lib.ROUND[rounding], flags=_lib.FLAG_MEAN_VALUE if mean_value else 0, lo=ptr(self._lo), hi=ptr(self._hi),
+            precision=_lib.PREC[precision], rounding=_lib.ROUND[rounding],
+            flags=(_lib.FLAG_MEAN_VALUE if mean_value else 0) | (_lib.FLAG_SPLIT if split else 0),
+            lo=ptr(self._lo), hi=ptr(self._hi),
             params=ptr(self._params), n_params=self._params.size, tol_f=float(tol), tol_x=float(tol_x),
             init_ub=float(init_ub), max_queue=int(max_queue), max_parents=int(max_parents),
             max_iters=int(max_iters))

@@ -146,6 +146,14 @@ int gpugo_eval_boxes_f32(gpugo_module_t m, int32_t d, const double *params, int6
                                     f(c) + sum_j G_j (X_j - c_j), c the midpoint,
                                     G the gradient enclosure (not in the
                                     reference; tightens lower bounds)          */
+#define GPUGO_FLAG_SPLIT      4  /* term-parallel split+bound: the lanes of a group
+                                    (the module's G, gpugo_amd.emit split=G) share
+                                    one child and split its sums; for small steps
+                                    that leave the GPU partly idle.  Needs a module
+                                    built from loop templates; not with
+                                    GPUGO_PREC_F32.  Certified enclosures are as
+                                    rigorous; bits differ from the unsplit form
+                                    (sums associate differently).               */
 
 /* ---- problems -------------------------------------------------------------
  * One box-constrained minimisation.  Replaces the module-level constants and

@@ -25,19 +25,28 @@ mp.mp.prec = 256
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def ir_eval(e, x, P, memo=None):
+def ir_eval(e, x, P, memo=None, j=None):
     """Point value of an IR expression in mpmath (generic interpreter); the
-    dimension of guarded sums is len(x)."""
+    dimension of guarded sums and loop templates is len(x); j is the loop
+    variable inside a template sum."""
     if memo is None:
         memo = {}
     k = id(e)
-    if k in memo:
+    if k in memo and j is None:
         return memo[k]
     op = e.op
+    idx = (lambda v: v[1] * j + v[2]) if j is not None else None
     if op == "var":
-        r = x[e.val]
+        r = x[idx(e.val)] if isinstance(e.val, tuple) else x[e.val]
     elif op == "param":
-        r = mp.mpf(float(P[e.val]))
+        r = mp.mpf(float(P[idx(e.val) if isinstance(e.val, tuple) else e.val]))
+    elif op == "tsum":
+        r = mp.mpf(0)
+        for jj in range(len(x) + e.val):
+            r += ir_eval(e.args[0], x, P, {}, jj)
+    elif op == "twhen":
+        ok = (j + 1 < len(x)) if e.val == "j+1<d" else (j > 0)
+        r = ir_eval(e.args[0], x, P, {}, j) if ok else mp.mpf(0)
     elif op == "const":
         r = mp.mpf(e.val)
     elif op == "dim":
@@ -46,18 +55,19 @@ def ir_eval(e, x, P, memo=None):
         lo, hi = e.val
         r = mp.pi if abs(lo - math.pi) < 1e-15 else mp.e
     elif op == "guard":
-        r = ir_eval(e.args[0], x, P, memo) if e.val < len(x) else mp.mpf(0)
+        r = ir_eval(e.args[0], x, P, memo, j) if e.val < len(x) else mp.mpf(0)
     elif op == "gsum":
         r = mp.mpf(0)
         for t in e.args:
-            r += ir_eval(t, x, P, memo)
+            r += ir_eval(t, x, P, memo, j)
     else:
-        a = [ir_eval(t, x, P, memo) for t in e.args]
+        a = [ir_eval(t, x, P, memo if j is None else {}, j) for t in e.args]
         r = {"add": lambda: a[0] + a[1], "sub": lambda: a[0] - a[1], "mul": lambda: a[0] * a[1],
              "div": lambda: a[0] / a[1], "neg": lambda: -a[0], "sqr": lambda: a[0] ** 2,
              "pown": lambda: a[0] ** e.val, "sqrt": lambda: mp.sqrt(a[0]), "exp": lambda: mp.exp(a[0]),
              "sin": lambda: mp.sin(a[0]), "cos": lambda: mp.cos(a[0])}[op]()
-    memo[k] = r
+    if j is None:
+        memo[k] = r
     return r
 
 
@@ -91,6 +101,29 @@ def test_gradient_lists_match_derivative(spec):
         want = mp.diff(fj, x[j])
         got = ir_eval(obj.grad[j], x, spec.params)
         assert abs(got - want) <= mp.mpf("1e-40") * (1 + abs(want)), j
+
+
+@pytest.mark.parametrize("spec", SPECS, ids=lambda s: s.name)
+def test_loop_templates_match_unrolled(spec):
+    """The term-parallel code lists' loop form is the same function."""
+    obj = pb.objective(spec.family, spec.d)
+    t = obj.templates
+    rng = np.random.default_rng(5)
+    for _ in range(4):
+        x = [mp.mpf(float(v)) for v in rng.uniform(spec.lo, spec.hi)]
+        v0 = ir_eval(obj.value, x, spec.params)
+        v1 = ir_eval(t["value"], x, spec.params)
+        assert abs(v0 - v1) <= mp.mpf("1e-60") * (1 + abs(v0))
+        for jj in range(spec.d):
+            g0 = ir_eval(obj.grad[jj], x, spec.params)
+            g1 = ir_eval(t["grad"], x, spec.params, {}, jj)
+            assert abs(g0 - g1) <= mp.mpf("1e-60") * (1 + abs(g0)), jj
+
+
+def test_split_code_lists_emit():
+    src = emit.module_source(pb.objective("sepcos_chain", 10), ("fast",), split=4)
+    assert "#define GPUGO_SPLIT_G 4" in src and "obj_value_g" in src and "group_sum<R, G>" in src
+    assert "for (int j = lg; j < d - 1; j += G)" in src
 
 
 def test_symbolic_diff():
