@@ -25,6 +25,7 @@ PREC = {"fp64": 0, "fp32": 1}
 ROUND = {"exact": 0, "fast": 1}
 FLAG_MEAN_VALUE = 1
 FLAG_SPLIT = 4
+FLAG_PERSIST = 8
 STATUS = {0: "running", 1: "certified", 2: "x_converged", 3: "max_iters", 4: "exhausted", 5: "overflow"}
 
 EXPORTS = [

@@ -4,6 +4,18 @@
 #ifndef GPUGO_BNB_ARGS_HPP
 #define GPUGO_BNB_ARGS_HPP
 
+#ifndef GPUGO_H
+// status values of include/gpugo.h, for code compiled without it (modules)
+enum gpugo_status {
+  GPUGO_ST_RUNNING = 0,
+  GPUGO_ST_CERTIFIED = 1,
+  GPUGO_ST_X_CONVERGED = 2,
+  GPUGO_ST_MAX_ITERS = 3,
+  GPUGO_ST_EXHAUSTED = 4,
+  GPUGO_ST_OVERFLOW = 5
+};
+#endif
+
 namespace gpugo {
 
 // Per-problem state kept in device memory for the whole solve.  One

@@ -260,11 +260,13 @@ __device__ __forceinline__ double sample_ub_g(const ChildBox &x, const double *_
   return best;
 }
 
+// blk / nblk: this workgroup and the number sharing the pass (any block
+// size up to 1024 threads)
 template <int R, bool MV>
-__device__ __forceinline__ void bound_impl_g(const BoundArgs &a) {
+__device__ __forceinline__ void bound_phase_g(const BoundArgs &a, int blk, int nblk) {
   constexpr int G = GPUGO_SPLIT_G;
   typedef IA<double, R> A;
-  __shared__ unsigned s_wcnt[4];
+  __shared__ unsigned s_wcnt[16];
   __shared__ unsigned long long s_base;
   __shared__ unsigned long long s_ubmin;
   __shared__ unsigned s_pv, s_pk;
@@ -282,7 +284,8 @@ __device__ __forceinline__ void bound_impl_g(const BoundArgs &a) {
     s_pk = 0;
   }
   __syncthreads();
-  for (long long base = (long long)blockIdx.x * 256; base < n_child * G; base += (long long)gridDim.x * 256) {
+  const int nw = (int)(blockDim.x >> 6), bsz = (int)blockDim.x;
+  for (long long base = (long long)blk * bsz; base < n_child * G; base += (long long)nblk * bsz) {
     const long long ci = (base + threadIdx.x) / G;
     const bool active = ci < n_child;
     long long pl = 0;
@@ -342,7 +345,8 @@ __device__ __forceinline__ void bound_impl_g(const BoundArgs &a) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      const unsigned tot = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
+      unsigned tot = 0;
+      for (int q = 0; q < nw; ++q) tot += s_wcnt[q];
       unsigned long long b = 0;
       if (tot) {
         b = atomicAdd((unsigned long long *)&st->q_count, (unsigned long long)tot);
@@ -367,9 +371,67 @@ __device__ __forceinline__ void bound_impl_g(const BoundArgs &a) {
     if (s_ubmin < *(volatile unsigned long long *)&st->ub_key) atomicMin(&st->ub_key, s_ubmin);
     if (s_pv) atomicAdd((unsigned long long *)&st->pruned_value, (unsigned long long)s_pv);
     if (s_pk) atomicAdd((unsigned long long *)&st->pruned_kkt, (unsigned long long)s_pk);
-    if (blockIdx.x == 0) {
+    if (blk == 0) {
       st->iterations += 1;
       st->bounded += n_child;
+    }
+  }
+}
+#endif
+
+#if GPUGO_SPLIT_G > 1
+// Persistent step loop: one cooperative launch of resident 1024-thread
+// workgroups runs the steps back to back (select on workgroup 0, then the
+// term-parallel split+bound and the compaction on all), with grid barriers
+// instead of kernel boundaries: no launch gaps between the step's phases
+// and no host polling.  Every exit condition is read after a barrier from
+// state no workgroup writes until the next barrier, so all workgroups leave
+// at the same step.
+template <int R, bool MV>
+__device__ __forceinline__ void persist_impl(const PersistArgs &p) {
+  const int blk = blockIdx.x, nblk = p.nblocks;
+  DevState *st = p.ba.st;
+  for (long long k = 0; k < p.max_steps; ++k) {
+    const long long step = p.step0 + k;
+    const int cur = (int)(step & 1);
+    if (step > 0 && blk == 0) {
+      SelectArgs sa = p.sa;
+      sa.q_lb = p.q_lb[cur];
+      sa.q_par = p.q_par[cur];
+      sa.q_pos = p.q_pos[cur];
+      select_body(sa);
+    }
+    if (!grid_barrier(p.bar, nblk)) {
+      if (threadIdx.x == 0) atomicOr(p.err, 1);
+      return;
+    }
+    const int status = __hip_atomic_load(&st->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int cfinal = __hip_atomic_load(&st->compact_final, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (status != GPUGO_ST_RUNNING && !cfinal) return;
+    if (status == GPUGO_ST_RUNNING) {
+      BoundArgs ba = p.ba;
+      ba.q_lb = p.q_lb[cur];
+      ba.q_par = p.q_par[cur];
+      ba.q_pos = p.q_pos[cur];
+      bound_phase_g<R, MV>(ba, blk, nblk);
+    }
+    if (!grid_barrier(p.bar, nblk)) {
+      if (threadIdx.x == 0) atomicOr(p.err, 1);
+      return;
+    }
+    CompactArgs ca;
+    ca.st = st;
+    ca.lb_in = p.q_lb[cur];
+    ca.par_in = p.q_par[cur];
+    ca.pos_in = p.q_pos[cur];
+    ca.lb_out = p.q_lb[cur ^ 1];
+    ca.par_out = p.q_par[cur ^ 1];
+    ca.pos_out = p.q_pos[cur ^ 1];
+    ca.cap = p.cap;
+    compact_body(ca, blk, nblk);
+    if (!grid_barrier(p.bar, nblk)) {
+      if (threadIdx.x == 0) atomicOr(p.err, 1);
+      return;
     }
   }
 }
@@ -434,19 +496,31 @@ extern "C" __global__ __launch_bounds__(64) void gpugo_eval_exact(gpugo::EvalArg
 }
 #endif
 #if GPUGO_SPLIT_G > 1 && GPUGO_WITH_EXACT
+extern "C" __global__ __launch_bounds__(1024) void gpugo_persist_exact_g(gpugo::PersistArgs p) {
+  gpugo::persist_impl<gpugo::ROUND_EXACT, false>(p);
+}
+extern "C" __global__ __launch_bounds__(1024) void gpugo_persist_exact_mv_g(gpugo::PersistArgs p) {
+  gpugo::persist_impl<gpugo::ROUND_EXACT, true>(p);
+}
 extern "C" __global__ __launch_bounds__(256) void gpugo_bound_exact_g(gpugo::BoundArgs a) {
-  gpugo::bound_impl_g<gpugo::ROUND_EXACT, false>(a);
+  gpugo::bound_phase_g<gpugo::ROUND_EXACT, false>(a, blockIdx.x, gridDim.x);
 }
 extern "C" __global__ __launch_bounds__(256) void gpugo_bound_exact_mv_g(gpugo::BoundArgs a) {
-  gpugo::bound_impl_g<gpugo::ROUND_EXACT, true>(a);
+  gpugo::bound_phase_g<gpugo::ROUND_EXACT, true>(a, blockIdx.x, gridDim.x);
 }
 #endif
 #if GPUGO_SPLIT_G > 1 && GPUGO_WITH_FAST
+extern "C" __global__ __launch_bounds__(1024) void gpugo_persist_fast_g(gpugo::PersistArgs p) {
+  gpugo::persist_impl<gpugo::ROUND_FAST, false>(p);
+}
+extern "C" __global__ __launch_bounds__(1024) void gpugo_persist_fast_mv_g(gpugo::PersistArgs p) {
+  gpugo::persist_impl<gpugo::ROUND_FAST, true>(p);
+}
 extern "C" __global__ __launch_bounds__(256) void gpugo_bound_fast_g(gpugo::BoundArgs a) {
-  gpugo::bound_impl_g<gpugo::ROUND_FAST, false>(a);
+  gpugo::bound_phase_g<gpugo::ROUND_FAST, false>(a, blockIdx.x, gridDim.x);
 }
 extern "C" __global__ __launch_bounds__(256) void gpugo_bound_fast_mv_g(gpugo::BoundArgs a) {
-  gpugo::bound_impl_g<gpugo::ROUND_FAST, true>(a);
+  gpugo::bound_phase_g<gpugo::ROUND_FAST, true>(a, blockIdx.x, gridDim.x);
 }
 #endif
 #if GPUGO_WITH_F32 && GPUGO_WITH_EXACT

@@ -1,0 +1,685 @@
+// bnb_step.hpp -- the objective-independent kernels of one branch-and-bound
+// step: select (the K smallest boxes, stop tests, new parent records),
+// compact (list prune) and the root initialisation.  Included by
+// libgpugo.so and, as text, by every objective module (gpugo_amd.emit), so a
+// step's three launches come from one code object (switching code objects
+// between dependent launches costs several microseconds each).
+//
+// Reference correspondence (paths relative to GPUGO_1.1/):
+//   select_body   list prune + argmin pop + completeRegion + cycle update
+//                 (1-Ackley/ackley50.py:1154-1220, :1270-1370, :820-948),
+//                 generalised to the K smallest boxes per step
+//   compact_body  constructFlagArr + mask compaction (:768-810, :1217-1242)
+#pragma once
+
+namespace gpugo {
+
+// ---------------------------------------------------------------------------
+// block helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned wave_incl_scan(unsigned v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    unsigned t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// exclusive scan over the block; sh holds >= nwaves+1 entries
+__device__ unsigned block_excl_scan(unsigned v, unsigned *sh, unsigned *total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  unsigned inc = wave_incl_scan(v);
+  if (lane == 63) sh[w] = inc;
+  __syncthreads();
+  if (w == 0) {
+    unsigned x = lane < nw ? sh[lane] : 0u;
+    unsigned xi = wave_incl_scan(x);
+    if (lane < nw) sh[lane] = xi - x;
+    if (lane == nw - 1) sh[nw] = xi;
+  }
+  __syncthreads();
+  unsigned r = sh[w] + inc - v;
+  *total = sh[nw];
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// decode (round-to-nearest, one fma per bound, top child ends at U)
+// 1-Ackley/ackley50.py:565-593 (device) and :820-948 (host)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void decode_box(const double *plo, const double *phi, const double *pw, int win, int d, int wid, int c,
+                           unsigned pos, double *lo, double *hi) {
+  for (int j = 0; j < d; ++j) {
+    lo[j] = plo[j];
+    hi[j] = phi[j];
+  }
+  unsigned r = pos;
+  for (int k = 0; k < c; ++k) {
+    int j = win + k;
+    if (j >= d) j -= d;
+    unsigned q = r / (unsigned)wid, dg = r - q * (unsigned)wid;
+    r = q;
+    lo[j] = __builtin_fma(pw[j], (double)dg, plo[j]);
+    hi[j] = (dg + 1 == (unsigned)wid) ? phi[j] : __builtin_fma(pw[j], (double)(dg + 1), plo[j]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// select: the K smallest (lb, parent, pos) keys of the live queue, exact,
+// in one workgroup (radix select on 128-bit keys, 12-bit digits), then the
+// stop tests and materialisation of the selected boxes as new parents.
+// ---------------------------------------------------------------------------
+#define SEL_T 1024
+#define SEL_BINS 4096
+#define SEL_FINAL 1024
+#define SEL_KMAX 1024
+#define SEL_U 8
+#define GPUGO_DMAX_SELECT 4096
+
+typedef unsigned __int128 u128;
+
+struct SelectArgs {
+  DevState *st;
+  double *q_lb;
+  unsigned *q_par, *q_pos;
+  double *plo, *phi, *pw;
+  int *pwin;
+  double *out_lo, *out_hi, *out_lb;
+  long long out_cap, p_cap;
+  int d, wid, c, k_select;
+  double tol_f, tol_x;
+  long long max_iters;   // > 0: stop with GPUGO_ST_MAX_ITERS on the device (batch slots)
+  int pstride;           // doubles per parent record (d, or the batch's dmax)
+  int pad0;
+};
+
+__device__ __forceinline__ u128 qkey(const double *lb, const unsigned *par, const unsigned *pos, long long i) {
+  return ((u128)order_key(lb[i]) << 64) | ((u128)par[i] << 32) | (u128)pos[i];
+}
+
+// bin of a lower bound in [lmin, lmax]: monotone non-decreasing in lb
+__device__ __forceinline__ unsigned value_bin(double lb, double lmin, double scale) {
+  double t = (lb - lmin) * scale;
+  int b = (int)t;
+  return (unsigned)(b < 0 ? 0 : (b >= SEL_BINS ? SEL_BINS - 1 : b));
+}
+
+// Radix refinement on the 128-bit keys (12-bit digits) until the bin that
+// holds the K-th smallest key has at most SEL_FINAL entries or fills the
+// remaining slots exactly; then keys below that bin go to skey/sidx and the
+// bin's keys to ckey/cidx.  Used when the value histogram cannot separate
+// the keys (many equal lower bounds).
+__device__ void radix_gather(const SelectArgs &a, long long N, int K, unsigned *hist, u128 *skey, unsigned *sidx,
+                             u128 *ckey, unsigned *cidx, unsigned *scan_sh, u128 &s_pfx, int &s_pbits,
+                             unsigned &s_krem, unsigned &s_region, unsigned &s_b, unsigned &s_cb, unsigned &s_ns,
+                             unsigned &s_nc) {
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    s_pfx = 0;
+    s_pbits = 0;
+    s_krem = (unsigned)K;
+  }
+  __syncthreads();
+  for (;;) {
+    const int pbits = s_pbits;
+    const int db = 128 - pbits < 12 ? 128 - pbits : 12;
+    const u128 pfx = s_pfx;
+    const unsigned krem = s_krem;
+    const int sh = 128 - pbits - db;
+    for (int b = tid; b < SEL_BINS; b += SEL_T) hist[b] = 0;
+    __syncthreads();
+    for (long long i = tid; i < N; i += SEL_T) {
+      u128 k = qkey(a.q_lb, a.q_par, a.q_pos, i);
+      bool m = (pbits == 0) || ((k >> (128 - pbits)) == (pfx >> (128 - pbits)));
+      if (m) atomicAdd(&hist[(unsigned)(k >> sh) & ((1u << db) - 1u)], 1u);
+    }
+    __syncthreads();
+    unsigned h[4], ts = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      h[q] = hist[4 * tid + q];
+      ts += h[q];
+    }
+    unsigned tot;
+    unsigned cum = block_excl_scan(ts, scan_sh, &tot);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (cum < krem && krem <= cum + h[q]) {
+        s_b = 4 * tid + q;
+        s_cb = cum;
+        s_region = h[q];
+      }
+      cum += h[q];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      s_krem = krem - s_cb;
+      s_pfx = pfx | ((u128)s_b << sh);
+      s_pbits = pbits + db;
+    }
+    __syncthreads();
+    if (s_region == s_krem || s_region <= SEL_FINAL || s_pbits >= 128) break;
+  }
+  if (tid == 0) {
+    s_ns = 0;
+    s_nc = 0;
+  }
+  __syncthreads();
+  const int shp = 128 - s_pbits;
+  const u128 ptop = s_pfx >> shp;
+  for (long long i = tid; i < N; i += SEL_T) {
+    u128 k = qkey(a.q_lb, a.q_par, a.q_pos, i);
+    u128 top = k >> shp;
+    if (top < ptop) {
+      unsigned slot = atomicAdd(&s_ns, 1u);
+      if (slot < SEL_KMAX) {
+        skey[slot] = k;
+        sidx[slot] = (unsigned)i;
+      }
+    } else if (top == ptop) {
+      unsigned slot = atomicAdd(&s_nc, 1u);
+      if (slot < SEL_FINAL) {
+        ckey[slot] = k;
+        cidx[slot] = (unsigned)i;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__device__ void select_body(const SelectArgs &a) {
+  __shared__ unsigned hist[SEL_BINS];
+  __shared__ u128 skey[SEL_KMAX];
+  __shared__ unsigned sidx[SEL_KMAX];
+  __shared__ u128 ckey[SEL_FINAL];
+  __shared__ unsigned cidx[SEL_FINAL];
+  __shared__ unsigned scan_sh[SEL_T / 64 + 1];
+  __shared__ u128 s_pfx;
+  __shared__ int s_pbits;
+  __shared__ unsigned s_krem, s_region, s_b, s_cb, s_ns, s_nc;
+  __shared__ unsigned long long s_outmin;
+
+  DevState *st = a.st;
+  const int tid = threadIdx.x;
+  if (st->status != GPUGO_ST_RUNNING) {
+    if (tid == 0) st->compact_final = 0;
+    return;
+  }
+  if (st->fresh) {
+    // a batch slot's first step splits the root record the host wrote
+    __syncthreads();
+    if (tid == 0) st->fresh = 0;
+    return;
+  }
+  if (st->overflow || (a.max_iters > 0 && st->iterations >= a.max_iters)) {
+    // stop; the enclosure is [smallest live lower bound, ub] (the queue is
+    // compacted, so it holds no consumed entries), none after an overflow
+    const long long N = st->compact_count;
+    if (tid == 0) s_outmin = order_key(__builtin_huge_val());
+    __syncthreads();
+    unsigned long long mk_ = order_key(__builtin_huge_val());
+    for (long long i = tid; i < N; i += SEL_T) {
+      const unsigned long long k = order_key(a.q_lb[i]);
+      mk_ = k < mk_ ? k : mk_;
+    }
+    atomicMin(&s_outmin, mk_);
+    __syncthreads();
+    if (tid == 0) {
+      const bool ovf = st->overflow != 0;
+      double lbq = key_value(s_outmin);
+      if (st->out_min_lb < lbq) lbq = st->out_min_lb;
+      if (ovf || (N == 0 && st->n_out == 0)) lbq = -__builtin_huge_val();
+      st->lb_global = lbq;
+      st->width = rnd<double, ROUND_EXACT>::add_up(key_value(st->ub_key), -lbq);
+      st->status = ovf ? GPUGO_ST_OVERFLOW : GPUGO_ST_MAX_ITERS;
+      st->q_count = N;
+      st->compact_count = 0;
+      st->compact_final = 1;
+    }
+    return;
+  }
+  const long long N = st->compact_count;
+  const double ub = key_value(st->ub_key);
+  if (N == 0) {
+    if (tid == 0) {
+      bool have_out = st->n_out > 0;
+      st->status = have_out ? GPUGO_ST_X_CONVERGED : GPUGO_ST_EXHAUSTED;
+      st->lb_global = have_out ? st->out_min_lb : -__builtin_huge_val();
+      st->width = rnd<double, ROUND_EXACT>::add_up(ub, -st->lb_global);
+      st->q_count = 0;
+      st->compact_count = 0;
+      st->compact_final = 1;
+    }
+    return;
+  }
+  const int K = (int)(N < (long long)a.k_select ? N : (long long)a.k_select);
+  if (N <= K) {
+    for (long long i = tid; i < N; i += SEL_T) {
+      skey[i] = qkey(a.q_lb, a.q_par, a.q_pos, i);
+      sidx[i] = (unsigned)i;
+    }
+    __syncthreads();
+  } else {
+    // Fast path: one histogram pass over the live range [lbmin, lbmax]
+    // (tracked by the compaction).  The bin map is monotone in lb, so every
+    // key in a bin below the boundary bin is smaller than every key in it.
+    bool fast_done = false;
+    const double lmin = key_value(st->lbmin_key), lmax = key_value(st->lbmax_key);
+    const double span = lmax - lmin;
+    if (lmin > -__builtin_huge_val() && lmax < __builtin_huge_val() && span > 0) {
+      const double scale = (double)SEL_BINS / span;
+      for (int b = tid; b < SEL_BINS; b += SEL_T) hist[b] = 0;
+      __syncthreads();
+      // SEL_U loads in flight per thread: one workgroup's pass over the
+      // queue is latency-bound otherwise
+      for (long long i0 = tid; i0 < N; i0 += (long long)SEL_T * SEL_U) {
+        double v[SEL_U];
+#pragma unroll
+        for (int u = 0; u < SEL_U; ++u) {
+          const long long i = i0 + (long long)u * SEL_T;
+          v[u] = i < N ? a.q_lb[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < SEL_U; ++u)
+          if (i0 + (long long)u * SEL_T < N) atomicAdd(&hist[value_bin(v[u], lmin, scale)], 1u);
+      }
+      __syncthreads();
+      unsigned h[4], ts = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        h[q] = hist[4 * tid + q];
+        ts += h[q];
+      }
+      unsigned tot;
+      unsigned cum = block_excl_scan(ts, scan_sh, &tot);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (cum < (unsigned)K && (unsigned)K <= cum + h[q]) {
+          s_b = 4 * tid + q;
+          s_cb = cum;
+          s_region = h[q];
+        }
+        cum += h[q];
+      }
+      __syncthreads();
+      if (s_region <= SEL_FINAL || s_region == (unsigned)K - s_cb) {
+        if (tid == 0) {
+          s_ns = 0;
+          s_nc = 0;
+          s_krem = (unsigned)K - s_cb;
+        }
+        __syncthreads();
+        const unsigned bb = s_b;
+        for (long long i0 = tid; i0 < N; i0 += (long long)SEL_T * SEL_U) {
+          double v[SEL_U];
+#pragma unroll
+          for (int u = 0; u < SEL_U; ++u) {
+            const long long i = i0 + (long long)u * SEL_T;
+            v[u] = i < N ? a.q_lb[i] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < SEL_U; ++u) {
+            const long long i = i0 + (long long)u * SEL_T;
+            if (i >= N) continue;
+            const unsigned bin = value_bin(v[u], lmin, scale);
+            if (bin < bb) {
+              unsigned slot = atomicAdd(&s_ns, 1u);
+              if (slot < SEL_KMAX) {
+                skey[slot] = qkey(a.q_lb, a.q_par, a.q_pos, i);
+                sidx[slot] = (unsigned)i;
+              }
+            } else if (bin == bb) {
+              unsigned slot = atomicAdd(&s_nc, 1u);
+              if (slot < SEL_FINAL) {
+                ckey[slot] = qkey(a.q_lb, a.q_par, a.q_pos, i);
+                cidx[slot] = (unsigned)i;
+              }
+            }
+          }
+        }
+        __syncthreads();
+        fast_done = true;
+      }
+    }
+    if (!fast_done) radix_gather(a, N, K, hist, skey, sidx, ckey, cidx, scan_sh, s_pfx, s_pbits, s_krem, s_region,
+                                 s_b, s_cb, s_ns, s_nc);
+    // the s_krem smallest candidates fill the remaining slots (rank by
+    // counting: keys are unique)
+    const unsigned ns = s_ns, nc = s_nc, krem = s_krem;
+    if (tid < (int)nc) {
+      const u128 mine = ckey[tid];
+      unsigned r = 0;
+      for (unsigned j = 0; j < nc; ++j) r += ckey[j] < mine;
+      if (r < krem) {
+        skey[ns + r] = mine;
+        sidx[ns + r] = cidx[tid];
+      }
+    }
+    __syncthreads();
+  }
+  // deterministic order of the selection: rank by counting into ckey/cidx
+  {
+    u128 mine = 0;
+    unsigned myidx = 0, r = 0;
+    if (tid < K) {
+      mine = skey[tid];
+      myidx = sidx[tid];
+      for (int j = 0; j < K; ++j) r += skey[j] < mine;
+    }
+    __syncthreads();
+    if (tid < K) {
+      ckey[r] = mine;
+      cidx[r] = myidx;
+    }
+    __syncthreads();
+    if (tid < K) {
+      skey[tid] = ckey[tid];
+      sidx[tid] = cidx[tid];
+    }
+    __syncthreads();
+  }
+  // stop test: certified f-enclosure
+  const double lb0 = a.q_lb[sidx[0]];
+  const double lbg = lb0 < st->out_min_lb ? lb0 : st->out_min_lb;
+  if (a.tol_f > 0) {
+    double w = rnd<double, ROUND_EXACT>::add_up(ub, -lbg);
+    if (w <= a.tol_f) {
+      __syncthreads();
+      if (tid == 0) {
+        st->lb_global = lbg;
+        st->width = w;
+        st->status = GPUGO_ST_CERTIFIED;
+        st->q_count = N;
+        st->compact_count = 0;
+        st->compact_final = 1;
+      }
+      return;
+    }
+  }
+
+  // materialise: x-width output rule, then new parent records
+  int is_out = 0;
+  unsigned idx = 0, par = 0, pos = 0;
+  double elb = 0;
+  const int d = a.d;
+  if (tid < K) {
+    idx = sidx[tid];
+    par = a.q_par[idx];
+    pos = a.q_pos[idx];
+    elb = a.q_lb[idx];
+    if (a.tol_x > 0) {
+      const double *plo = a.plo + (long long)par * a.pstride, *phi = a.phi + (long long)par * a.pstride;
+      const double *pw = a.pw + (long long)par * a.pstride;
+      int win = a.pwin[par];
+      unsigned r = pos;
+      // window coordinates shrink; recompute the max over the child
+      double mwin = 0, mrest = 0;
+      for (int j = 0; j < d; ++j) {
+        int k = j - win;
+        if (k < 0) k += d;
+        if (k >= a.c) {
+          double w = phi[j] - plo[j];
+          if (w > mrest) mrest = w;
+        }
+      }
+      for (int k = 0; k < a.c; ++k) {
+        int j = win + k;
+        if (j >= d) j -= d;
+        unsigned q = r / (unsigned)a.wid, dg = r - q * (unsigned)a.wid;
+        r = q;
+        double lo = __builtin_fma(pw[j], (double)dg, plo[j]);
+        double hi = (dg + 1 == (unsigned)a.wid) ? phi[j] : __builtin_fma(pw[j], (double)(dg + 1), plo[j]);
+        if (hi - lo > mwin) mwin = hi - lo;
+      }
+      double m = mwin > mrest ? mwin : mrest;
+      is_out = m < a.tol_x;
+    }
+  }
+  unsigned n_split, n_outs;
+  const unsigned split_rank = block_excl_scan((tid < K && !is_out) ? 1u : 0u, scan_sh, &n_split);
+  const unsigned out_rank = block_excl_scan((tid < K && is_out) ? 1u : 0u, scan_sh, &n_outs);
+  const long long pbase = st->parents_used;
+  const long long obase = st->n_out;
+  if (pbase + (long long)n_split > a.p_cap) {
+    if (tid == 0) {
+      st->overflow |= 2;
+      st->status = GPUGO_ST_OVERFLOW;
+      st->q_count = N;
+      st->compact_count = 0;
+      st->compact_final = 1;
+    }
+    return;
+  }
+  if (tid == 0) s_outmin = order_key(__builtin_huge_val());
+  __syncthreads();
+  if (tid < K) {
+    const double *plo = a.plo + (long long)par * a.pstride, *phi = a.phi + (long long)par * a.pstride;
+    const double *pw = a.pw + (long long)par * a.pstride;
+    const int win = a.pwin[par];
+    if (!is_out) {
+      const long long np = pbase + split_rank;
+      double *nlo = a.plo + np * a.pstride, *nhi = a.phi + np * a.pstride, *npw = a.pw + np * a.pstride;
+      decode_box(plo, phi, pw, win, d, a.wid, a.c, pos, nlo, nhi);
+      for (int j = 0; j < d; ++j) npw[j] = (nhi[j] - nlo[j]) / (double)a.wid;
+      int nw = win + a.c;
+      while (nw >= d) nw -= d;
+      a.pwin[np] = nw;
+    } else {
+      const long long o = obase + out_rank;
+      if (o < a.out_cap) {
+        decode_box(plo, phi, pw, win, d, a.wid, a.c, pos, a.out_lo + o * d, a.out_hi + o * d);
+        a.out_lb[o] = elb;
+      } else {
+        atomicOr(&st->overflow, 4);
+      }
+      atomicMin(&s_outmin, order_key(elb));
+    }
+    a.q_lb[idx] = __builtin_nan("");
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double om = key_value(s_outmin);
+    if (om < st->out_min_lb) st->out_min_lb = om;
+    st->overflow &= ~4;  // output-arena overflow keeps the bound (out_min_lb) rigorous
+    st->lb_global = lbg;
+    st->first_parent = pbase;
+    st->parents_used = pbase + n_split;
+    st->n_split = (int)n_split;
+    st->n_out = obase + n_outs;
+    st->ub_snap = ub;
+    st->q_count = N;
+    st->compact_count = 0;
+    st->lbmin_key = ~0ull;
+    st->lbmax_key = 0ull;
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// compact: keep entries with lb <= ub (consumed entries are NaN)
+// ---------------------------------------------------------------------------
+struct CompactArgs {
+  DevState *st;
+  const double *lb_in;
+  const unsigned *par_in, *pos_in;
+  double *lb_out;
+  unsigned *par_out, *pos_out;
+  long long cap;
+};
+
+// blk / nblk: this workgroup's index and the number of workgroups sharing
+// the pass (any block size up to 1024 threads)
+__device__ __forceinline__ void compact_body(const CompactArgs &a, int blk, int nblk) {
+  __shared__ unsigned wcnt[16];
+  __shared__ unsigned long long s_base;
+  DevState *st = a.st;
+  // after a stop, exactly one more pass runs so the live set lands in the
+  // buffer that st->compactions designates
+  if (st->status != GPUGO_ST_RUNNING && !st->compact_final) return;
+  __shared__ unsigned long long s_kmin, s_kmax;
+  long long n = st->q_count;
+  if (n > a.cap) n = a.cap;
+  const double ub = key_value(st->ub_key);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // range of the kept lower bounds (order keys; feeds the next select)
+  unsigned long long kmin = ~0ull, kmax = 0ull;
+  if (threadIdx.x == 0) {
+    s_kmin = ~0ull;
+    s_kmax = 0ull;
+  }
+  const int nw = (int)(blockDim.x >> 6), bsz = (int)blockDim.x;
+  for (long long base = (long long)blk * bsz; base < n; base += (long long)nblk * bsz) {
+    long long i = base + threadIdx.x;
+    double lb = 0;
+    bool keep = false;
+    if (i < n) {
+      lb = a.lb_in[i];
+      keep = lb <= ub;
+      if (keep) {
+        const unsigned long long k = order_key(lb);
+        kmin = k < kmin ? k : kmin;
+        kmax = k > kmax ? k : kmax;
+      }
+    }
+    unsigned long long mask = __ballot(keep);
+    unsigned pre = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+    if (lane == 0) wcnt[w] = __popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned t = 0;
+      for (int q = 0; q < nw; ++q) t += wcnt[q];
+      s_base = t ? atomicAdd((unsigned long long *)&st->compact_count, (unsigned long long)t) : 0ull;
+    }
+    __syncthreads();
+    unsigned off = pre;
+    for (int q = 0; q < w; ++q) off += wcnt[q];
+    if (keep) {
+      long long o = (long long)s_base + off;
+      a.lb_out[o] = lb;
+      a.par_out[o] = a.par_in[i];
+      a.pos_out[o] = a.pos_in[i];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long t0 = __shfl_xor(kmin, o, 64), t1 = __shfl_xor(kmax, o, 64);
+    kmin = t0 < kmin ? t0 : kmin;
+    kmax = t1 > kmax ? t1 : kmax;
+  }
+  if (lane == 0 && kmin <= kmax) {
+    atomicMin(&s_kmin, kmin);
+    atomicMax(&s_kmax, kmax);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_kmin <= s_kmax) {
+    atomicMin(&st->lbmin_key, s_kmin);
+    atomicMax(&st->lbmax_key, s_kmax);
+  }
+  if (blk == 0 && threadIdx.x == 0) {
+    atomicMax((unsigned long long *)&st->queue_peak, (unsigned long long)n);
+    st->compactions += 1;
+  }
+}
+
+
+
+// Barrier over all workgroups of a cooperative launch (every workgroup is
+// resident).  bar[0] counts arrivals, bar[1] is the generation.  The wait is
+// bounded: after ~2^24 sleeps it gives up and returns false (the caller
+// stops and reports the error), so a mistake cannot hang the device.
+// Thread 0 fences before arriving (release) and after the wait (acquire):
+// the per-CU L1 and the per-XCD L2 are not coherent across the device, and
+// a device-scope fence writes back / invalidates them, which is expensive
+// on MI355X (8 XCDs), so one fence per workgroup and barrier, not per wave.
+__device__ __forceinline__ bool grid_barrier(unsigned *bar, unsigned nblocks) {
+  __shared__ int s_ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    const unsigned g = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    if (atomicAdd(&bar[0], 1u) == nblocks - 1u) {
+      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      unsigned spin = 0;
+      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spin > (1u << 24)) {
+          ok = 0;
+          break;
+        }
+      }
+    }
+    __threadfence();
+    s_ok = ok;
+  }
+  __syncthreads();
+  return s_ok != 0;
+}
+
+// Arguments of the persistent step loop (a cooperative launch that runs
+// select -> split+bound -> compact with grid barriers until the problem
+// stops or max_steps steps ran).  The queue pointers of `ba`, `sa` are
+// replaced per step by buffer (step & 1) of q_lb/q_par/q_pos.
+struct PersistArgs {
+  BoundArgs ba;
+  SelectArgs sa;
+  double *q_lb[2];
+  unsigned *q_par[2], *q_pos[2];
+  unsigned *bar;          // grid barrier state, zero at launch
+  int *err;               // set to 1 if a barrier wait gave up
+  long long step0;        // steps run before this launch (0: the first splits the root)
+  long long max_steps;
+  long long cap;          // queue capacity
+  int nblocks, pad;
+};
+
+// Root of a solve: parent record 0 is the initial box, one box to split.
+struct RootArgs {
+  DevState *st;
+  double *plo, *phi, *pw;
+  int *pwin;
+  const double *lo, *hi;
+  double init_ub;
+  int d, wid;
+};
+
+__device__ __forceinline__ void root_body(const RootArgs &a) {
+  for (int j = threadIdx.x; j < a.d; j += blockDim.x) {
+    a.plo[j] = a.lo[j];
+    a.phi[j] = a.hi[j];
+    a.pw[j] = (a.hi[j] - a.lo[j]) / (double)a.wid;
+  }
+  if (threadIdx.x == 0) {
+    a.pwin[0] = 0;
+    DevState s;
+    __builtin_memset(&s, 0, sizeof s);
+    s.ub_key = order_key(a.init_ub);
+    s.ub_snap = a.init_ub;
+    s.lb_global = -__builtin_huge_val();
+    s.out_min_lb = __builtin_huge_val();
+    s.width = __builtin_huge_val();
+    s.parents_used = 1;
+    s.first_parent = 0;
+    s.n_split = 1;
+    s.status = GPUGO_ST_RUNNING;
+    s.lbmin_key = ~0ull;
+    s.lbmax_key = 0ull;
+    *a.st = s;
+  }
+}
+
+}  // namespace gpugo
+
+#ifdef GPUGO_MODULE_STEP
+extern "C" __global__ __launch_bounds__(SEL_T) void gpugo_select(gpugo::SelectArgs a) { gpugo::select_body(a); }
+extern "C" __global__ __launch_bounds__(256) void gpugo_compact(gpugo::CompactArgs a) {
+  gpugo::compact_body(a, blockIdx.x, gridDim.x);
+}
+extern "C" __global__ __launch_bounds__(64) void gpugo_root(gpugo::RootArgs a) { gpugo::root_body(a); }
+#endif

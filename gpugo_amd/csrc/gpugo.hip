@@ -201,6 +201,11 @@ struct gpugo_module_s {
   hipFunction_t bound_g[2] = {nullptr, nullptr};     // term-parallel variants (split modules)
   hipFunction_t bound_mv_g[2] = {nullptr, nullptr};
   int split_g = 0;                                    // lanes per box of those variants
+  // the step's select / compact / root kernels compiled into the module
+  // (used instead of the library's copies: no code-object switch per launch)
+  hipFunction_t select_fn = nullptr, compact_fn = nullptr, root_fn = nullptr;
+  hipFunction_t persist[2] = {nullptr, nullptr};     // persistent step loops (split modules)
+  hipFunction_t persist_mv[2] = {nullptr, nullptr};
   int dim = 0, nparams = 0, abi = 0;
   bool f32 = false;                // has the FP32 value code list (precision F32)
   std::vector<signed char> psign;  // declared coefficient signs
@@ -270,6 +275,12 @@ static int module_finish(gpugo_module_t m) {
   m->abi = h[2];
   m->f32 = h[4] != 0;
   m->split_g = h[5];
+  if (hipModuleGetFunction(&m->select_fn, m->mod, "gpugo_select") != hipSuccess ||
+      hipModuleGetFunction(&m->compact_fn, m->mod, "gpugo_compact") != hipSuccess ||
+      hipModuleGetFunction(&m->root_fn, m->mod, "gpugo_root") != hipSuccess) {
+    (void)hipGetLastError();
+    m->select_fn = m->compact_fn = m->root_fn = nullptr;
+  }
   if (m->split_g > 1) {
     if (h[3] & 1) {
       HIPCHECK(hipModuleGetFunction(&m->bound_g[0], m->mod, "gpugo_bound_exact_g"));
@@ -278,6 +289,16 @@ static int module_finish(gpugo_module_t m) {
     if (h[3] & 2) {
       HIPCHECK(hipModuleGetFunction(&m->bound_g[1], m->mod, "gpugo_bound_fast_g"));
       HIPCHECK(hipModuleGetFunction(&m->bound_mv_g[1], m->mod, "gpugo_bound_fast_mv_g"));
+    }
+    const char *pn[2][2] = {{"gpugo_persist_exact_g", "gpugo_persist_exact_mv_g"},
+                            {"gpugo_persist_fast_g", "gpugo_persist_fast_mv_g"}};
+    for (int r = 0; r < 2; ++r) {
+      if (!(h[3] & (1 << r))) continue;
+      if (hipModuleGetFunction(&m->persist[r], m->mod, pn[r][0]) != hipSuccess ||
+          hipModuleGetFunction(&m->persist_mv[r], m->mod, pn[r][1]) != hipSuccess) {
+        (void)hipGetLastError();
+        m->persist[r] = m->persist_mv[r] = nullptr;
+      }
     }
   }
   if (m->f32 && (h[3] & 1)) HIPCHECK(hipModuleGetFunction(&m->eval32[0], m->mod, "gpugo_eval32_exact"));
@@ -550,467 +571,11 @@ extern "C" int gpugo_eval_boxes_f32(gpugo_module_t m, int32_t d, const double *p
   return GPUGO_OK;
 }
 
-// ---------------------------------------------------------------------------
-// block helpers (wave64)
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ unsigned wave_incl_scan(unsigned v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    unsigned t = __shfl_up(v, o, 64);
-    if (lane >= o) v += t;
-  }
-  return v;
-}
-
-// exclusive scan over the block; sh holds >= nwaves+1 entries
-__device__ unsigned block_excl_scan(unsigned v, unsigned *sh, unsigned *total) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  unsigned inc = wave_incl_scan(v);
-  if (lane == 63) sh[w] = inc;
-  __syncthreads();
-  if (w == 0) {
-    unsigned x = lane < nw ? sh[lane] : 0u;
-    unsigned xi = wave_incl_scan(x);
-    if (lane < nw) sh[lane] = xi - x;
-    if (lane == nw - 1) sh[nw] = xi;
-  }
-  __syncthreads();
-  unsigned r = sh[w] + inc - v;
-  *total = sh[nw];
-  __syncthreads();
-  return r;
-}
-
-// ---------------------------------------------------------------------------
-// decode (round-to-nearest, one fma per bound, top child ends at U)
-// 1-Ackley/ackley50.py:565-593 (device) and :820-948 (host)
-// ---------------------------------------------------------------------------
-__device__ void decode_box(const double *plo, const double *phi, const double *pw, int win, int d, int wid, int c,
-                           unsigned pos, double *lo, double *hi) {
-  for (int j = 0; j < d; ++j) {
-    lo[j] = plo[j];
-    hi[j] = phi[j];
-  }
-  unsigned r = pos;
-  for (int k = 0; k < c; ++k) {
-    int j = win + k;
-    if (j >= d) j -= d;
-    unsigned q = r / (unsigned)wid, dg = r - q * (unsigned)wid;
-    r = q;
-    lo[j] = __builtin_fma(pw[j], (double)dg, plo[j]);
-    hi[j] = (dg + 1 == (unsigned)wid) ? phi[j] : __builtin_fma(pw[j], (double)(dg + 1), plo[j]);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// select: the K smallest (lb, parent, pos) keys of the live queue, exact,
-// in one workgroup (radix select on 128-bit keys, 12-bit digits), then the
-// stop tests and materialisation of the selected boxes as new parents.
-// ---------------------------------------------------------------------------
-#define SEL_T 1024
-#define SEL_BINS 4096
-#define SEL_FINAL 1024
-#define SEL_KMAX 1024
-#define GPUGO_DMAX_SELECT 4096
-
-typedef unsigned __int128 u128;
-
-struct SelectArgs {
-  DevState *st;
-  double *q_lb;
-  unsigned *q_par, *q_pos;
-  double *plo, *phi, *pw;
-  int *pwin;
-  double *out_lo, *out_hi, *out_lb;
-  long long out_cap, p_cap;
-  int d, wid, c, k_select;
-  double tol_f, tol_x;
-  long long max_iters;   // > 0: stop with GPUGO_ST_MAX_ITERS on the device (batch slots)
-  int pstride;           // doubles per parent record (d, or the batch's dmax)
-  int pad0;
-};
-
-__device__ __forceinline__ u128 qkey(const double *lb, const unsigned *par, const unsigned *pos, long long i) {
-  return ((u128)order_key(lb[i]) << 64) | ((u128)par[i] << 32) | (u128)pos[i];
-}
-
-// bin of a lower bound in [lmin, lmax]: monotone non-decreasing in lb
-__device__ __forceinline__ unsigned value_bin(double lb, double lmin, double scale) {
-  double t = (lb - lmin) * scale;
-  int b = (int)t;
-  return (unsigned)(b < 0 ? 0 : (b >= SEL_BINS ? SEL_BINS - 1 : b));
-}
-
-// Radix refinement on the 128-bit keys (12-bit digits) until the bin that
-// holds the K-th smallest key has at most SEL_FINAL entries or fills the
-// remaining slots exactly; then keys below that bin go to skey/sidx and the
-// bin's keys to ckey/cidx.  Used when the value histogram cannot separate
-// the keys (many equal lower bounds).
-__device__ void radix_gather(const SelectArgs &a, long long N, int K, unsigned *hist, u128 *skey, unsigned *sidx,
-                             u128 *ckey, unsigned *cidx, unsigned *scan_sh, u128 &s_pfx, int &s_pbits,
-                             unsigned &s_krem, unsigned &s_region, unsigned &s_b, unsigned &s_cb, unsigned &s_ns,
-                             unsigned &s_nc) {
-  const int tid = threadIdx.x;
-  if (tid == 0) {
-    s_pfx = 0;
-    s_pbits = 0;
-    s_krem = (unsigned)K;
-  }
-  __syncthreads();
-  for (;;) {
-    const int pbits = s_pbits;
-    const int db = 128 - pbits < 12 ? 128 - pbits : 12;
-    const u128 pfx = s_pfx;
-    const unsigned krem = s_krem;
-    const int sh = 128 - pbits - db;
-    for (int b = tid; b < SEL_BINS; b += SEL_T) hist[b] = 0;
-    __syncthreads();
-    for (long long i = tid; i < N; i += SEL_T) {
-      u128 k = qkey(a.q_lb, a.q_par, a.q_pos, i);
-      bool m = (pbits == 0) || ((k >> (128 - pbits)) == (pfx >> (128 - pbits)));
-      if (m) atomicAdd(&hist[(unsigned)(k >> sh) & ((1u << db) - 1u)], 1u);
-    }
-    __syncthreads();
-    unsigned h[4], ts = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      h[q] = hist[4 * tid + q];
-      ts += h[q];
-    }
-    unsigned tot;
-    unsigned cum = block_excl_scan(ts, scan_sh, &tot);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (cum < krem && krem <= cum + h[q]) {
-        s_b = 4 * tid + q;
-        s_cb = cum;
-        s_region = h[q];
-      }
-      cum += h[q];
-    }
-    __syncthreads();
-    if (tid == 0) {
-      s_krem = krem - s_cb;
-      s_pfx = pfx | ((u128)s_b << sh);
-      s_pbits = pbits + db;
-    }
-    __syncthreads();
-    if (s_region == s_krem || s_region <= SEL_FINAL || s_pbits >= 128) break;
-  }
-  if (tid == 0) {
-    s_ns = 0;
-    s_nc = 0;
-  }
-  __syncthreads();
-  const int shp = 128 - s_pbits;
-  const u128 ptop = s_pfx >> shp;
-  for (long long i = tid; i < N; i += SEL_T) {
-    u128 k = qkey(a.q_lb, a.q_par, a.q_pos, i);
-    u128 top = k >> shp;
-    if (top < ptop) {
-      unsigned slot = atomicAdd(&s_ns, 1u);
-      if (slot < SEL_KMAX) {
-        skey[slot] = k;
-        sidx[slot] = (unsigned)i;
-      }
-    } else if (top == ptop) {
-      unsigned slot = atomicAdd(&s_nc, 1u);
-      if (slot < SEL_FINAL) {
-        ckey[slot] = k;
-        cidx[slot] = (unsigned)i;
-      }
-    }
-  }
-  __syncthreads();
-}
-
-__device__ void select_body(const SelectArgs &a) {
-  __shared__ unsigned hist[SEL_BINS];
-  __shared__ u128 skey[SEL_KMAX];
-  __shared__ unsigned sidx[SEL_KMAX];
-  __shared__ u128 ckey[SEL_FINAL];
-  __shared__ unsigned cidx[SEL_FINAL];
-  __shared__ unsigned scan_sh[SEL_T / 64 + 1];
-  __shared__ u128 s_pfx;
-  __shared__ int s_pbits;
-  __shared__ unsigned s_krem, s_region, s_b, s_cb, s_ns, s_nc;
-  __shared__ unsigned long long s_outmin;
-
-  DevState *st = a.st;
-  const int tid = threadIdx.x;
-  if (st->status != GPUGO_ST_RUNNING) {
-    if (tid == 0) st->compact_final = 0;
-    return;
-  }
-  if (st->fresh) {
-    // a batch slot's first step splits the root record the host wrote
-    __syncthreads();
-    if (tid == 0) st->fresh = 0;
-    return;
-  }
-  if (st->overflow || (a.max_iters > 0 && st->iterations >= a.max_iters)) {
-    // stop; the enclosure is [smallest live lower bound, ub] (the queue is
-    // compacted, so it holds no consumed entries), none after an overflow
-    const long long N = st->compact_count;
-    if (tid == 0) s_outmin = order_key(__builtin_huge_val());
-    __syncthreads();
-    unsigned long long mk_ = order_key(__builtin_huge_val());
-    for (long long i = tid; i < N; i += SEL_T) {
-      const unsigned long long k = order_key(a.q_lb[i]);
-      mk_ = k < mk_ ? k : mk_;
-    }
-    atomicMin(&s_outmin, mk_);
-    __syncthreads();
-    if (tid == 0) {
-      const bool ovf = st->overflow != 0;
-      double lbq = key_value(s_outmin);
-      if (st->out_min_lb < lbq) lbq = st->out_min_lb;
-      if (ovf || (N == 0 && st->n_out == 0)) lbq = -__builtin_huge_val();
-      st->lb_global = lbq;
-      st->width = rnd<double, ROUND_EXACT>::add_up(key_value(st->ub_key), -lbq);
-      st->status = ovf ? GPUGO_ST_OVERFLOW : GPUGO_ST_MAX_ITERS;
-      st->q_count = N;
-      st->compact_count = 0;
-      st->compact_final = 1;
-    }
-    return;
-  }
-  const long long N = st->compact_count;
-  const double ub = key_value(st->ub_key);
-  if (N == 0) {
-    if (tid == 0) {
-      bool have_out = st->n_out > 0;
-      st->status = have_out ? GPUGO_ST_X_CONVERGED : GPUGO_ST_EXHAUSTED;
-      st->lb_global = have_out ? st->out_min_lb : -__builtin_huge_val();
-      st->width = rnd<double, ROUND_EXACT>::add_up(ub, -st->lb_global);
-      st->q_count = 0;
-      st->compact_count = 0;
-      st->compact_final = 1;
-    }
-    return;
-  }
-  const int K = (int)(N < (long long)a.k_select ? N : (long long)a.k_select);
-  if (N <= K) {
-    for (long long i = tid; i < N; i += SEL_T) {
-      skey[i] = qkey(a.q_lb, a.q_par, a.q_pos, i);
-      sidx[i] = (unsigned)i;
-    }
-    __syncthreads();
-  } else {
-    // Fast path: one histogram pass over the live range [lbmin, lbmax]
-    // (tracked by the compaction).  The bin map is monotone in lb, so every
-    // key in a bin below the boundary bin is smaller than every key in it.
-    bool fast_done = false;
-    const double lmin = key_value(st->lbmin_key), lmax = key_value(st->lbmax_key);
-    const double span = lmax - lmin;
-    if (lmin > -__builtin_huge_val() && lmax < __builtin_huge_val() && span > 0) {
-      const double scale = (double)SEL_BINS / span;
-      for (int b = tid; b < SEL_BINS; b += SEL_T) hist[b] = 0;
-      __syncthreads();
-      for (long long i = tid; i < N; i += SEL_T)
-        atomicAdd(&hist[value_bin(a.q_lb[i], lmin, scale)], 1u);
-      __syncthreads();
-      unsigned h[4], ts = 0;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        h[q] = hist[4 * tid + q];
-        ts += h[q];
-      }
-      unsigned tot;
-      unsigned cum = block_excl_scan(ts, scan_sh, &tot);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (cum < (unsigned)K && (unsigned)K <= cum + h[q]) {
-          s_b = 4 * tid + q;
-          s_cb = cum;
-          s_region = h[q];
-        }
-        cum += h[q];
-      }
-      __syncthreads();
-      if (s_region <= SEL_FINAL || s_region == (unsigned)K - s_cb) {
-        if (tid == 0) {
-          s_ns = 0;
-          s_nc = 0;
-          s_krem = (unsigned)K - s_cb;
-        }
-        __syncthreads();
-        const unsigned bb = s_b;
-        for (long long i = tid; i < N; i += SEL_T) {
-          const unsigned bin = value_bin(a.q_lb[i], lmin, scale);
-          if (bin < bb) {
-            unsigned slot = atomicAdd(&s_ns, 1u);
-            if (slot < SEL_KMAX) {
-              skey[slot] = qkey(a.q_lb, a.q_par, a.q_pos, i);
-              sidx[slot] = (unsigned)i;
-            }
-          } else if (bin == bb) {
-            unsigned slot = atomicAdd(&s_nc, 1u);
-            if (slot < SEL_FINAL) {
-              ckey[slot] = qkey(a.q_lb, a.q_par, a.q_pos, i);
-              cidx[slot] = (unsigned)i;
-            }
-          }
-        }
-        __syncthreads();
-        fast_done = true;
-      }
-    }
-    if (!fast_done) radix_gather(a, N, K, hist, skey, sidx, ckey, cidx, scan_sh, s_pfx, s_pbits, s_krem, s_region,
-                                 s_b, s_cb, s_ns, s_nc);
-    // the s_krem smallest candidates fill the remaining slots (rank by
-    // counting: keys are unique)
-    const unsigned ns = s_ns, nc = s_nc, krem = s_krem;
-    if (tid < (int)nc) {
-      const u128 mine = ckey[tid];
-      unsigned r = 0;
-      for (unsigned j = 0; j < nc; ++j) r += ckey[j] < mine;
-      if (r < krem) {
-        skey[ns + r] = mine;
-        sidx[ns + r] = cidx[tid];
-      }
-    }
-    __syncthreads();
-  }
-  // deterministic order of the selection: rank by counting into ckey/cidx
-  {
-    u128 mine = 0;
-    unsigned myidx = 0, r = 0;
-    if (tid < K) {
-      mine = skey[tid];
-      myidx = sidx[tid];
-      for (int j = 0; j < K; ++j) r += skey[j] < mine;
-    }
-    __syncthreads();
-    if (tid < K) {
-      ckey[r] = mine;
-      cidx[r] = myidx;
-    }
-    __syncthreads();
-    if (tid < K) {
-      skey[tid] = ckey[tid];
-      sidx[tid] = cidx[tid];
-    }
-    __syncthreads();
-  }
-  // stop test: certified f-enclosure
-  const double lb0 = a.q_lb[sidx[0]];
-  const double lbg = lb0 < st->out_min_lb ? lb0 : st->out_min_lb;
-  if (a.tol_f > 0) {
-    double w = rnd<double, ROUND_EXACT>::add_up(ub, -lbg);
-    if (w <= a.tol_f) {
-      __syncthreads();
-      if (tid == 0) {
-        st->lb_global = lbg;
-        st->width = w;
-        st->status = GPUGO_ST_CERTIFIED;
-        st->q_count = N;
-        st->compact_count = 0;
-        st->compact_final = 1;
-      }
-      return;
-    }
-  }
-
-  // materialise: x-width output rule, then new parent records
-  int is_out = 0;
-  unsigned idx = 0, par = 0, pos = 0;
-  double elb = 0;
-  const int d = a.d;
-  if (tid < K) {
-    idx = sidx[tid];
-    par = a.q_par[idx];
-    pos = a.q_pos[idx];
-    elb = a.q_lb[idx];
-    if (a.tol_x > 0) {
-      const double *plo = a.plo + (long long)par * a.pstride, *phi = a.phi + (long long)par * a.pstride;
-      const double *pw = a.pw + (long long)par * a.pstride;
-      int win = a.pwin[par];
-      unsigned r = pos;
-      // window coordinates shrink; recompute the max over the child
-      double mwin = 0, mrest = 0;
-      for (int j = 0; j < d; ++j) {
-        int k = j - win;
-        if (k < 0) k += d;
-        if (k >= a.c) {
-          double w = phi[j] - plo[j];
-          if (w > mrest) mrest = w;
-        }
-      }
-      for (int k = 0; k < a.c; ++k) {
-        int j = win + k;
-        if (j >= d) j -= d;
-        unsigned q = r / (unsigned)a.wid, dg = r - q * (unsigned)a.wid;
-        r = q;
-        double lo = __builtin_fma(pw[j], (double)dg, plo[j]);
-        double hi = (dg + 1 == (unsigned)a.wid) ? phi[j] : __builtin_fma(pw[j], (double)(dg + 1), plo[j]);
-        if (hi - lo > mwin) mwin = hi - lo;
-      }
-      double m = mwin > mrest ? mwin : mrest;
-      is_out = m < a.tol_x;
-    }
-  }
-  unsigned n_split, n_outs;
-  const unsigned split_rank = block_excl_scan((tid < K && !is_out) ? 1u : 0u, scan_sh, &n_split);
-  const unsigned out_rank = block_excl_scan((tid < K && is_out) ? 1u : 0u, scan_sh, &n_outs);
-  const long long pbase = st->parents_used;
-  const long long obase = st->n_out;
-  if (pbase + (long long)n_split > a.p_cap) {
-    if (tid == 0) {
-      st->overflow |= 2;
-      st->status = GPUGO_ST_OVERFLOW;
-      st->q_count = N;
-      st->compact_count = 0;
-      st->compact_final = 1;
-    }
-    return;
-  }
-  if (tid == 0) s_outmin = order_key(__builtin_huge_val());
-  __syncthreads();
-  if (tid < K) {
-    const double *plo = a.plo + (long long)par * a.pstride, *phi = a.phi + (long long)par * a.pstride;
-    const double *pw = a.pw + (long long)par * a.pstride;
-    const int win = a.pwin[par];
-    if (!is_out) {
-      const long long np = pbase + split_rank;
-      double *nlo = a.plo + np * a.pstride, *nhi = a.phi + np * a.pstride, *npw = a.pw + np * a.pstride;
-      decode_box(plo, phi, pw, win, d, a.wid, a.c, pos, nlo, nhi);
-      for (int j = 0; j < d; ++j) npw[j] = (nhi[j] - nlo[j]) / (double)a.wid;
-      int nw = win + a.c;
-      while (nw >= d) nw -= d;
-      a.pwin[np] = nw;
-    } else {
-      const long long o = obase + out_rank;
-      if (o < a.out_cap) {
-        decode_box(plo, phi, pw, win, d, a.wid, a.c, pos, a.out_lo + o * d, a.out_hi + o * d);
-        a.out_lb[o] = elb;
-      } else {
-        atomicOr(&st->overflow, 4);
-      }
-      atomicMin(&s_outmin, order_key(elb));
-    }
-    a.q_lb[idx] = __builtin_nan("");
-  }
-  __syncthreads();
-  if (tid == 0) {
-    double om = key_value(s_outmin);
-    if (om < st->out_min_lb) st->out_min_lb = om;
-    st->overflow &= ~4;  // output-arena overflow keeps the bound (out_min_lb) rigorous
-    st->lb_global = lbg;
-    st->first_parent = pbase;
-    st->parents_used = pbase + n_split;
-    st->n_split = (int)n_split;
-    st->n_out = obase + n_outs;
-    st->ub_snap = ub;
-    st->q_count = N;
-    st->compact_count = 0;
-    st->lbmin_key = ~0ull;
-    st->lbmax_key = 0ull;
-  }
-}
+#include "bnb_step.hpp"
 
 __global__ __launch_bounds__(SEL_T) void k_select(SelectArgs a) { select_body(a); }
+__global__ __launch_bounds__(256) void k_compact(CompactArgs a) { compact_body(a, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(64) void k_root(RootArgs a) { root_body(a); }
 
 // ---------------------------------------------------------------------------
 // batches: one workgroup per slot for select and compaction
@@ -1221,90 +786,6 @@ __global__ __launch_bounds__(64) void kb_admit(BatchArgs b, const char *recs, lo
 }
 
 // ---------------------------------------------------------------------------
-// compact: keep entries with lb <= ub (consumed entries are NaN)
-// ---------------------------------------------------------------------------
-struct CompactArgs {
-  DevState *st;
-  const double *lb_in;
-  const unsigned *par_in, *pos_in;
-  double *lb_out;
-  unsigned *par_out, *pos_out;
-  long long cap;
-};
-
-__global__ __launch_bounds__(256) void k_compact(CompactArgs a) {
-  __shared__ unsigned wcnt[4];
-  __shared__ unsigned long long s_base;
-  DevState *st = a.st;
-  // after a stop, exactly one more pass runs so the live set lands in the
-  // buffer that st->compactions designates
-  if (st->status != GPUGO_ST_RUNNING && !st->compact_final) return;
-  __shared__ unsigned long long s_kmin, s_kmax;
-  long long n = st->q_count;
-  if (n > a.cap) n = a.cap;
-  const double ub = key_value(st->ub_key);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  // range of the kept lower bounds (order keys; feeds the next select)
-  unsigned long long kmin = ~0ull, kmax = 0ull;
-  if (threadIdx.x == 0) {
-    s_kmin = ~0ull;
-    s_kmax = 0ull;
-  }
-  for (long long base = (long long)blockIdx.x * 256; base < n; base += (long long)gridDim.x * 256) {
-    long long i = base + threadIdx.x;
-    double lb = 0;
-    bool keep = false;
-    if (i < n) {
-      lb = a.lb_in[i];
-      keep = lb <= ub;
-      if (keep) {
-        const unsigned long long k = order_key(lb);
-        kmin = k < kmin ? k : kmin;
-        kmax = k > kmax ? k : kmax;
-      }
-    }
-    unsigned long long mask = __ballot(keep);
-    unsigned pre = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-    if (lane == 0) wcnt[w] = __popcll(mask);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned t = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
-      s_base = t ? atomicAdd((unsigned long long *)&st->compact_count, (unsigned long long)t) : 0ull;
-    }
-    __syncthreads();
-    unsigned off = pre;
-    for (int q = 0; q < w; ++q) off += wcnt[q];
-    if (keep) {
-      long long o = (long long)s_base + off;
-      a.lb_out[o] = lb;
-      a.par_out[o] = a.par_in[i];
-      a.pos_out[o] = a.pos_in[i];
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    unsigned long long t0 = __shfl_xor(kmin, o, 64), t1 = __shfl_xor(kmax, o, 64);
-    kmin = t0 < kmin ? t0 : kmin;
-    kmax = t1 > kmax ? t1 : kmax;
-  }
-  if (lane == 0 && kmin <= kmax) {
-    atomicMin(&s_kmin, kmin);
-    atomicMax(&s_kmax, kmax);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0 && s_kmin <= s_kmax) {
-    atomicMin(&st->lbmin_key, s_kmin);
-    atomicMax(&st->lbmax_key, s_kmax);
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    atomicMax((unsigned long long *)&st->queue_peak, (unsigned long long)n);
-    st->compactions += 1;
-  }
-}
-
-
-// ---------------------------------------------------------------------------
 // smallest live lower bound (used when a run stops without a certificate)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_minlb(const DevState *st, const double *lb, long long n,
@@ -1356,6 +837,8 @@ struct gpugo_problem_s {
   unsigned W = 0;
   long long q_cap = 0, p_cap = 0, out_cap = 0;
   DevBuf d_lo, d_hi, d_params, d_params32, plo, phi, pw, pwin, out_lo, out_hi, out_lb, st, scratch;
+  DevBuf bar;          // persistent step loop: grid barrier [2] and error flag
+  int persist_blocks = 0;
   Queue q[2];
   hipStream_t stream = nullptr;
   bool started = false;
@@ -1415,10 +898,14 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
     return set_err(GPUGO_E_INVALID, "unknown precision %d", desc->precision);
   if (desc->precision == GPUGO_PREC_F32 && !m->f32)
     return set_err(GPUGO_E_INVALID, "module has no FP32 code list (emit it with f32=True)");
-  if (desc->flags & ~(GPUGO_FLAG_MEAN_VALUE | GPUGO_FLAG_SPLIT))
+  if (desc->flags & ~(GPUGO_FLAG_MEAN_VALUE | GPUGO_FLAG_SPLIT | GPUGO_FLAG_PERSIST))
     return set_err(GPUGO_E_INVALID, "unknown flags 0x%x", desc->flags);
   if ((desc->flags & GPUGO_FLAG_SPLIT) && m->split_g < 2)
     return set_err(GPUGO_E_INVALID, "module has no term-parallel kernels (emit it with split=G)");
+  if ((desc->flags & GPUGO_FLAG_PERSIST) && !(desc->flags & GPUGO_FLAG_SPLIT))
+    return set_err(GPUGO_E_INVALID, "GPUGO_FLAG_PERSIST needs GPUGO_FLAG_SPLIT");
+  if ((desc->flags & GPUGO_FLAG_PERSIST) && !m->persist[desc->rounding == GPUGO_ROUND_FAST ? 1 : 0])
+    return set_err(GPUGO_E_INVALID, "module has no persistent step loop for this rounding");
   if ((desc->flags & GPUGO_FLAG_SPLIT) && desc->precision == GPUGO_PREC_F32)
     return set_err(GPUGO_E_INVALID, "the term-parallel kernels have no FP32 pruning pass");
   if (desc->rounding != GPUGO_ROUND_EXACT && desc->rounding != GPUGO_ROUND_FAST)
@@ -1471,6 +958,7 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
   PALLOC(p->out_lb, sizeof(double) * p->out_cap);
   PALLOC(p->st, sizeof(DevState));
   PALLOC(p->scratch, 64);
+  PALLOC(p->bar, 64);
   for (int b = 0; b < 2; ++b) {
     PALLOC(p->q[b].lb, sizeof(double) * qc);
     PALLOC(p->q[b].par, sizeof(unsigned) * qc);
@@ -1504,6 +992,17 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
     if ((e = hipEventCreateWithFlags(&p->poll_ev[b], hipEventDisableTiming)) != hipSuccess)
       return fail(set_err(GPUGO_E_HIP, "event: %s", hipGetErrorString(e)));
   memset(&p->hs, 0, sizeof p->hs);
+  if (desc->flags & GPUGO_FLAG_PERSIST) {
+    // every workgroup of the cooperative launch must be resident
+    hipFunction_t fn = (desc->flags & GPUGO_FLAG_MEAN_VALUE) ? m->persist_mv[desc->rounding] : m->persist[desc->rounding];
+    int occ = 0, dev = 0, cus = 0;
+    if ((e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 1024, 0)) != hipSuccess ||
+        (e = hipGetDevice(&dev)) != hipSuccess ||
+        (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+      return fail(set_err(GPUGO_E_HIP, "occupancy: %s", hipGetErrorString(e)));
+    if (occ < 1) return fail(set_err(GPUGO_E_HIP, "persistent step loop cannot be resident"));
+    p->persist_blocks = occ * cus;
+  }
   *out = p;
   return GPUGO_OK;
 }
@@ -1542,10 +1041,10 @@ extern "C" int gpugo_problem_timing(gpugo_problem_t p, int enable) {
   return GPUGO_OK;
 }
 
+// The root state is written on the device by one kernel (the initial box is
+// already resident), so a solve starts without a host round trip; the host
+// keeps the same initial state for its own bookkeeping.
 static int root_init(gpugo_problem_t p) {
-  const int d = p->desc.d;
-  std::vector<double> pw(d);
-  for (int j = 0; j < d; ++j) pw[j] = (p->hi[j] - p->lo[j]) / (double)p->desc.wid;
   DevState s;
   memset(&s, 0, sizeof s);
   s.ub_key = order_key(p->desc.init_ub);
@@ -1554,19 +1053,29 @@ static int root_init(gpugo_problem_t p) {
   s.out_min_lb = __builtin_huge_val();
   s.width = __builtin_huge_val();
   s.parents_used = 1;
-  s.first_parent = 0;
   s.n_split = 1;
   s.status = GPUGO_ST_RUNNING;
   s.lbmin_key = ~0ull;
   s.lbmax_key = 0ull;
-  int win0 = 0;
   p->hs = s;
-  HIPCHECK(hipMemcpyAsync(p->plo.p, p->lo.data(), sizeof(double) * d, hipMemcpyHostToDevice, p->stream));
-  HIPCHECK(hipMemcpyAsync(p->phi.p, p->hi.data(), sizeof(double) * d, hipMemcpyHostToDevice, p->stream));
-  HIPCHECK(hipMemcpyAsync(p->pw.p, pw.data(), sizeof(double) * d, hipMemcpyHostToDevice, p->stream));
-  HIPCHECK(hipMemcpyAsync(p->pwin.p, &win0, sizeof(int), hipMemcpyHostToDevice, p->stream));
-  HIPCHECK(hipMemcpyAsync(p->st.p, &p->hs, sizeof(DevState), hipMemcpyHostToDevice, p->stream));
-  HIPCHECK(hipStreamSynchronize(p->stream));
+  RootArgs ra;
+  ra.st = p->st.as<DevState>();
+  ra.plo = p->plo.as<double>();
+  ra.phi = p->phi.as<double>();
+  ra.pw = p->pw.as<double>();
+  ra.pwin = p->pwin.as<int>();
+  ra.lo = p->d_lo.as<double>();
+  ra.hi = p->d_hi.as<double>();
+  ra.init_ub = p->desc.init_ub;
+  ra.d = p->desc.d;
+  ra.wid = p->desc.wid;
+  if (p->m->root_fn) {
+    void *args[] = {&ra};
+    HIPCHECK(hipModuleLaunchKernel(p->m->root_fn, 1, 1, 1, 64, 1, 1, 0, p->stream, args, nullptr));
+  } else {
+    hipLaunchKernelGGL(k_root, dim3(1), dim3(64), 0, p->stream, ra);
+    HIPCHECK(hipGetLastError());
+  }
   return GPUGO_OK;
 }
 
@@ -1599,8 +1108,13 @@ static int enqueue_step(gpugo_problem_t p) {
     sa.max_iters = 0;
     sa.pstride = d;
     sa.pad0 = 0;
-    hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_T), 0, p->stream, sa);
-    HIPCHECK(hipGetLastError());
+    if (p->m->select_fn) {
+      void *sargs[] = {&sa};
+      HIPCHECK(hipModuleLaunchKernel(p->m->select_fn, 1, 1, 1, SEL_T, 1, 1, 0, p->stream, sargs, nullptr));
+    } else {
+      hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_T), 0, p->stream, sa);
+      HIPCHECK(hipGetLastError());
+    }
   }
   BoundArgs ba;
   memset(&ba, 0, sizeof ba);
@@ -1655,8 +1169,13 @@ static int enqueue_step(gpugo_problem_t p) {
   ca.par_out = B.ppar();
   ca.pos_out = B.ppos();
   ca.cap = p->q_cap;
-  hipLaunchKernelGGL(k_compact, dim3(512), dim3(256), 0, p->stream, ca);
-  HIPCHECK(hipGetLastError());
+  if (p->m->compact_fn) {
+    void *cargs[] = {&ca};
+    HIPCHECK(hipModuleLaunchKernel(p->m->compact_fn, 512, 1, 1, 256, 1, 1, 0, p->stream, cargs, nullptr));
+  } else {
+    hipLaunchKernelGGL(k_compact, dim3(512), dim3(256), 0, p->stream, ca);
+    HIPCHECK(hipGetLastError());
+  }
   ++p->steps_enqueued;
   return GPUGO_OK;
 }
@@ -1709,6 +1228,87 @@ static int fill_result(gpugo_problem_t p, gpugo_result *res, double ms, bool max
   return GPUGO_OK;
 }
 
+// Up to `iters` steps in one cooperative launch of the persistent step
+// loop (it stops by itself when the problem stops).  With timing on, the
+// launch is bracketed by events and counted as the bound time.
+static int persist_steps(gpugo_problem_t p, int64_t iters) {
+  const int d = p->desc.d;
+  const bool mv = (p->desc.flags & GPUGO_FLAG_MEAN_VALUE) != 0;
+  hipFunction_t fn = mv ? p->m->persist_mv[p->desc.rounding] : p->m->persist[p->desc.rounding];
+  DevState *st = p->st.as<DevState>();
+  PersistArgs pa;
+  memset(&pa, 0, sizeof pa);
+  BoundArgs &ba = pa.ba;
+  ba.plo = p->plo.as<double>();
+  ba.phi = p->phi.as<double>();
+  ba.pw = p->pw.as<double>();
+  ba.pwin = p->pwin.as<int>();
+  ba.params = p->d_params.as<double>();
+  ba.dom_lo = p->d_lo.as<double>();
+  ba.dom_hi = p->d_hi.as<double>();
+  ba.st = st;
+  ba.q_cap = p->q_cap;
+  ba.d = d;
+  ba.wid = p->desc.wid;
+  ba.c = p->c;
+  ba.n_samples = p->desc.n_samples;
+  ba.W = p->W;
+  ba.inv_wid = 1.0 / (double)p->desc.wid;
+  ba.half_inv_wid = 0.5 / (double)p->desc.wid;
+  ba.flags = p->desc.flags;
+  ba.params32 = p->d_params32.p;
+  SelectArgs &sa = pa.sa;
+  sa.st = st;
+  sa.plo = p->plo.as<double>();
+  sa.phi = p->phi.as<double>();
+  sa.pw = p->pw.as<double>();
+  sa.pwin = p->pwin.as<int>();
+  sa.out_lo = p->out_lo.as<double>();
+  sa.out_hi = p->out_hi.as<double>();
+  sa.out_lb = p->out_lb.as<double>();
+  sa.out_cap = p->out_cap;
+  sa.p_cap = p->p_cap;
+  sa.d = d;
+  sa.wid = p->desc.wid;
+  sa.c = p->c;
+  sa.k_select = p->desc.k_select;
+  sa.tol_f = p->desc.tol_f;
+  sa.tol_x = p->desc.tol_x;
+  sa.max_iters = 0;
+  sa.pstride = d;
+  for (int b = 0; b < 2; ++b) {
+    pa.q_lb[b] = p->q[b].plb();
+    pa.q_par[b] = p->q[b].ppar();
+    pa.q_pos[b] = p->q[b].ppos();
+  }
+  pa.bar = p->bar.as<unsigned>();
+  pa.err = (int *)(p->bar.as<unsigned>() + 2);
+  pa.step0 = p->steps_enqueued;
+  pa.max_steps = iters;
+  pa.cap = p->q_cap;
+  pa.nblocks = p->persist_blocks;
+  HIPCHECK(hipMemsetAsync(p->bar.p, 0, 16, p->stream));
+  if (p->timing) {
+    while (p->tev.size() < p->tev_used + 2) {
+      hipEvent_t ev;
+      HIPCHECK(hipEventCreate(&ev));
+      p->tev.push_back(ev);
+    }
+    HIPCHECK(hipEventRecord(p->tev[p->tev_used++], p->stream));
+  }
+  void *args[] = {&pa};
+  HIPCHECK(hipModuleLaunchCooperativeKernel(fn, (unsigned)pa.nblocks, 1, 1, 1024, 1, 1, 0, p->stream, args));
+  if (p->timing) HIPCHECK(hipEventRecord(p->tev[p->tev_used++], p->stream));
+  int err = 0;
+  HIPCHECK(hipMemcpyAsync(&err, pa.err, sizeof err, hipMemcpyDeviceToHost, p->stream));
+  HIPCHECK(hipStreamSynchronize(p->stream));
+  if (err) return set_err(GPUGO_E_HIP, "persistent step loop: a grid barrier wait gave up");
+  p->steps_enqueued += iters;  // a launch that returns running ran all of them
+  // the steps this launch ran: the device counts them in iterations; the
+  // parity of the next launch's live buffer follows the compactions count
+  return GPUGO_OK;
+}
+
 static int run_steps(gpugo_problem_t p, int64_t iters, gpugo_result *res, bool solve) {
   if (!p || !res) return set_err(GPUGO_E_INVALID, "null pointer");
   auto t0 = std::chrono::steady_clock::now();
@@ -1718,13 +1318,16 @@ static int run_steps(gpugo_problem_t p, int64_t iters, gpugo_result *res, bool s
     p->started = true;
   }
   const long long it0 = p->hs.iterations;
-  if (p->hs.status == GPUGO_ST_RUNNING) {
+  if (p->hs.status == GPUGO_ST_RUNNING && (p->desc.flags & GPUGO_FLAG_PERSIST) && iters > 0) {
+    int rc = persist_steps(p, iters);
+    if (rc) return rc;
+  } else if (p->hs.status == GPUGO_ST_RUNNING) {
     // Steps are enqueued in batches; after each batch the device state is
     // copied into a pinned slot.  The host waits only for the previous
     // batch's copy, so it stays one batch ahead of the device; steps
     // enqueued after a stop are no-ops on the device.  Exactly `iters`
     // steps are enqueued at most.
-    const int64_t B = 4;
+    const int64_t B = 2;
     int64_t done = 0;
     int slot = 0, pending = 0;
     DevState *st = p->st.as<DevState>();

@@ -610,6 +610,9 @@ def module_source(obj: Objective, roundings=("fast",), f32: bool = False, split:
         parts.append(value_list_f32(obj))
     if split:
         parts.append(split_lists(obj))
+    # select / compact / root of a step, so all of a step's launches use
+    # this code object (and the persistent step loop can call them)
+    parts += ["#define GPUGO_MODULE_STEP 1", _read("bnb_step.hpp")]
     parts.append(_read("bnb_kernel.hpp"))
     return "\n".join(parts)
 

@@ -151,13 +151,15 @@ class Problem:
                  grid: Sequence[int] = (4, 10), k_select: int = 1, n_samples: int = 1,
                  init_ub: float = np.inf, rounding: str = "fast", max_queue: int = 0,
                  max_parents: int = 0, max_iters: int = 0, module: Optional[Module] = None,
-                 mean_value: bool = False, precision: str = "fp64", split: int = 0):
+                 mean_value: bool = False, precision: str = "fp64", split: int = 0,
+                 persistent: bool = False):
         """precision "fp32": children are first bounded in FP32 intervals
         (outward, one ulp per operation) and only those that survive get the
         FP64 value, gradient and sample; the enclosure is certified in FP64.
         split = G > 1: term-parallel kernels, G lanes share each child (for
         steps too small to fill the GPU; needs the objective's loop
-        templates)."""
+        templates).  persistent (with split): the steps of a solve run in
+        one cooperative launch with grid barriers between their phases."""
         if precision not in _lib.PREC:
             raise ValueError(f"precision must be one of {sorted(_lib.PREC)}")
         self.module = module or get_module(obj, roundings=(rounding,), f32=precision == "fp32", split=split)
@@ -167,7 +169,8 @@ class Problem:
         self.desc = _lib.ProblemDesc(
             d=obj.d, wid=int(wid), cycle=int(cycle), k_select=int(k_select), n_samples=int(n_samples),
             precision=_lib.PREC[precision], rounding=_lib.ROUND[rounding],
-            flags=(_lib.FLAG_MEAN_VALUE if mean_value else 0) | (_lib.FLAG_SPLIT if split else 0),
+            flags=((_lib.FLAG_MEAN_VALUE if mean_value else 0) | (_lib.FLAG_SPLIT if split else 0)
+                   | (_lib.FLAG_PERSIST if persistent else 0)),
             lo=ptr(self._lo), hi=ptr(self._hi),
             params=ptr(self._params), n_params=self._params.size, tol_f=float(tol), tol_x=float(tol_x),
             init_ub=float(init_ub), max_queue=int(max_queue), max_parents=int(max_parents),

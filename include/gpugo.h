@@ -154,6 +154,12 @@ int gpugo_eval_boxes_f32(gpugo_module_t m, int32_t d, const double *params, int6
                                     GPUGO_PREC_F32.  Certified enclosures are as
                                     rigorous; bits differ from the unsplit form
                                     (sums associate differently).               */
+#define GPUGO_FLAG_PERSIST    8  /* with GPUGO_FLAG_SPLIT: run the steps of a
+                                    gpugo_step/gpugo_solve call in one cooperative
+                                    launch (select, split+bound and compaction
+                                    separated by grid barriers on resident
+                                    workgroups; no launch gaps, no host polling).
+                                    Same results as GPUGO_FLAG_SPLIT alone.     */
 
 /* ---- problems -------------------------------------------------------------
  * One box-constrained minimisation.  Replaces the module-level constants and
