@@ -224,8 +224,11 @@ class Problem:
 def minimize(objective: Objective, lo, hi, tol: float = 1e-6, grid: Sequence[int] = (4, 10),
              n_samples: int = 1, init_ub: float = np.inf, params=None, k_select: int = 1,
              tol_x: float = 0.0, rounding: str = "fast", max_iters: int = 0, return_boxes: int = 0,
-             **kw) -> dict:
-    """Enclose the global minimum of `objective` over the box [lo, hi]."""
+             combine_tol: float = 0.0, **kw) -> dict:
+    """Enclose the global minimum of `objective` over the box [lo, hi].
+    return_boxes > 0 also returns up to that many live (or, with tol_x,
+    output) boxes; combine_tol > 0 merges them into roots
+    (gpugo_amd.results.roots, the reference's combine/connectivity)."""
     params = np.zeros(0) if params is None else params
     with Problem(objective, params, lo, hi, tol=tol, tol_x=tol_x, grid=grid, k_select=k_select,
                  n_samples=n_samples, init_ub=init_ub, rounding=rounding, max_iters=max_iters, **kw) as p:
@@ -234,4 +237,8 @@ def minimize(objective: Objective, lo, hi, tol: float = 1e-6, grid: Sequence[int
             blo, bhi, blb, n = p.boxes(return_boxes)
             res["boxes"] = (blo, bhi, blb)
             res["n_boxes"] = n
+            if combine_tol > 0 and len(blo):
+                from .results import roots
+                rlo, rhi, owner = roots(blo, bhi, combine_tol)
+                res["roots"] = (rlo, rhi, owner)
     return res

@@ -117,6 +117,13 @@ def test_x_width_rule(gpu):
     assert np.all(ohi - olo < 1e-3)
     assert any(np.all(olo[i] <= 0) and np.all(ohi[i] >= 0) for i in range(len(olb)))
     assert r["lb"] <= 0.0 <= r["ub"]
+    # the output boxes around the one minimiser merge into one root
+    # (combine/connectivity, 11-Rosenbrock/rosenbrock50.py:926-1049)
+    from gpugo_amd.results import record, roots
+    rlo, rhi, owner = roots(olo, ohi, 1e-6)
+    assert len(rlo) == 1 and np.all(rlo[0] <= 0) and np.all(rhi[0] >= 0)
+    rec = record(r, (olo, ohi, olb), tol=1e-6)
+    assert rec["status"] == "x_converged" and len(rec["roots"]["lo"]) == 1
 
 
 def test_queue_overflow_is_reported(gpu):
