@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
-"""Benchmark: time to a certified enclosure of configuration 1.
+"""Benchmark: time to a certified enclosure of the d = 10 sepcos_chain problem.
 
-Workload (configs[1] of the benchmark set): FP64 interval branch and bound
+Workload: FP64 interval branch and bound
 on f(x) = sum_i a_i x_i^2 + b_i (1 - cos(c_i x_i)) + sum_{i<9} w_i (x_{i+1} -
 x_i^2)^2 + (1 - x_i)^2, d = 10, seeded coefficients (seed 1), box
 [-5, 10]^10, tol 1e-6 on the width of the value enclosure.
@@ -42,7 +42,7 @@ SCHEDULES = {
 }
 
 
-TRAFFIC_FILE = "profiles/traffic_config1.json"
+TRAFFIC_FILE = "profiles/traffic_chain10.json"
 
 
 def measured_traffic(kernel):
@@ -177,10 +177,10 @@ def leg_fp32(G, spec, obj, sched, args):
 
 
 def leg_c2(G, pb, steps=5):
-    """configs[2]-shaped workload (d = 50): GPU time to certified enclosure
+    """ackley_q d = 50, box [-32,32]^50, tol 1e-6: GPU time to certified enclosure
     against the oracle on all host cores with the same schedule."""
     from oracle import oracle as O
-    spec = pb.config2()
+    spec = pb.ackley50()
     obj = pb.objective(spec.family, spec.d)
     grid, k = (3, 12), 1
     p = G.Problem(obj, spec.params, spec.lo, spec.hi, tol=spec.tol, grid=grid, k_select=k, max_iters=10000)
@@ -213,7 +213,7 @@ def leg_c2(G, pb, steps=5):
         r32 = p.solve()
         t32.append(time.perf_counter() - t0)
     p.close()
-    return {"workload": "config2: ackley_q d=50, box [-32,32]^50, tol 1e-6, seed 2",
+    return {"workload": "ackley50: ackley_q d=50, box [-32,32]^50, tol 1e-6, seed 2",
             "fp32_prune": {"time_to_certified_s": statistics.median(t32), "lb": r32["lb"], "ub": r32["ub"],
                            "width": r32["width"], "certified": r32["status_name"] == "certified",
                            "note": "the first step has no incumbent, so nothing is ruled out in FP32 there"},
@@ -249,7 +249,8 @@ def run_batch(Batch, probs, **kw):
 
 
 def leg_batch(G, pb, n, seq_sample):
-    """n independent problems in one batch (schedules from the
+    """n independent problems (families sepcos / sepcos_chain / ackley_q, d
+    log-uniform in [2,32]) in one batch (schedules from the
     tuned table) against the same problems solved one at a time with the
     single-problem solver (same schedules and capacities)."""
     from gpugo_amd.batch import Batch, BatchProblem, get_batch_module
@@ -292,7 +293,7 @@ def leg_batch(G, pb, n, seq_sample):
     same = sum(same_result(seq[t], res[t]) for t in seq)
     seq_cert = sum(r["status_name"] == "certified" for r in seq.values())
     widths = [r["width"] for r in res.values() if r["status_name"] == "certified"]
-    return {"workload": f"configs[3]: {n} problems (families C0-C2, d log-uniform [2,32], box [-h,h]^d, "
+    return {"workload": f"{n} problems (sepcos / sepcos_chain / ackley_q, d log-uniform [2,32], box [-h,h]^d, "
                         "h~U[2,32], tol in {1e-4,1e-6}), seeds [3,k]",
             "slots": n, "steps_per_launch": 8, "graph": True, "max_queue": qcap, "max_parents": pcap,
             "max_iters": maxit, "time_s": el, "problems_per_s": n / el, "certified": cert,
@@ -307,7 +308,7 @@ def leg_batch(G, pb, n, seq_sample):
 
 
 def leg_served(pb, lams=(100.0, 1000.0), seconds=3.0):
-    """Poisson arrivals, continuous batching, p50/p99 latency."""
+    """Served workload: Poisson arrivals, continuous batching, p50/p99 latency."""
     from gpugo_amd.serve import Server, latency_summary
     out = {}
     with Server(mem_fraction=0.5) as srv:
@@ -326,7 +327,7 @@ def leg_served(pb, lams=(100.0, 1000.0), seconds=3.0):
                                            for k in sorted({r.result["status_name"] for r in done})}})
             out[f"lambda_{int(lam)}"] = summ
     out["slots_per_dmax_class"] = {str(k): v for k, v in slots.items()}
-    out["workload"] = ("configs[4]: families C0-C2, d log-uniform [2,100], tol log-uniform [1e-8,1e-4], "
+    out["workload"] = ("sepcos / sepcos_chain / ackley_q, d log-uniform [2,100], tol log-uniform [1e-8,1e-4], "
                        "box [-h,h]^d h~U[2,32], seeds [4,k]; arrivals exponential(1/lambda), seed 40")
     return out
 
@@ -338,13 +339,13 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(problems shard by independent boxes/problems, see DESIGN.md)", file=sys.stderr)
+              "(a problem shards by independent boxes, a batch by independent problems)", file=sys.stderr)
         sys.exit(2)
     import gpugo_amd as G
     from gpugo_amd import emit, problems as pb
 
     legs = set(args.legs.split(","))
-    spec = pb.config1()
+    spec = pb.chain10()
     sched = SCHEDULES[args.schedule]
     obj = pb.objective(spec.family, spec.d)
     G.init(0)
@@ -395,7 +396,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded coefficients)",
-        "config": {"workload": "config1: sepcos_chain d=10, box [-5,10]^10, tol 1e-6, seed 1",
+        "config": {"workload": "chain10: sepcos_chain d=10, box [-5,10]^10, tol 1e-6, seed 1",
                    "schedule": args.schedule, "grid": list(sched["grid"]),
                    "k_select": sched["k_select"], "n_samples": 1, "rounding": args.rounding,
                    "mean_value": sched["mean_value"], "lanes_per_child": sched["split"] or 1},

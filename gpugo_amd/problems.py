@@ -177,26 +177,26 @@ def draw_sepcos(rng, d):
     return a, b, c
 
 
-def config0(seed: int = 0) -> ProblemSpec:
+def sepcos2(seed: int = 0) -> ProblemSpec:
     """d=2 separable, box [-5.12,5.12]^2, 64x64 grid, tol 1e-6; f* = 0 at x = 0."""
     rng = np.random.default_rng(seed)
     d = 2
     a, b, c = draw_sepcos(rng, d)
     return ProblemSpec("sepcos", d, np.concatenate([a, b, c]), np.full(d, -5.12), np.full(d, 5.12),
-                       1e-6, f_star=0.0, name="C0 sepcos d=2", grid=(64, 2), k_select=1)
+                       1e-6, f_star=0.0, name="sepcos d=2", grid=(64, 2), k_select=1)
 
 
-def config1(seed: int = 1) -> ProblemSpec:
+def chain10(seed: int = 1) -> ProblemSpec:
     """d=10 separable + chain, box [-5,10]^10, reference grid 4^10, tol 1e-6."""
     rng = np.random.default_rng(seed)
     d = 10
     a, b, c = draw_sepcos(rng, d)
     w = rng.uniform(1.0, 100.0, d - 1)
     return ProblemSpec("sepcos_chain", d, np.concatenate([a, b, c, w]), np.full(d, -5.0),
-                       np.full(d, 10.0), 1e-6, name="C1 sepcos_chain d=10", grid=(4, 10), k_select=1)
+                       np.full(d, 10.0), 1e-6, name="sepcos_chain d=10", grid=(4, 10), k_select=1)
 
 
-def config2(seed: int = 2, d: int = 50) -> ProblemSpec:
+def ackley50(seed: int = 2, d: int = 50) -> ProblemSpec:
     """d=50 Ackley-type + sum q x^4/50, box [-32,32]^50, tol 1e-6; f* = 0 at x = 0."""
     rng = np.random.default_rng(seed)
     A = rng.uniform(10.0, 30.0)
@@ -204,12 +204,12 @@ def config2(seed: int = 2, d: int = 50) -> ProblemSpec:
     C = rng.uniform(math.pi, 3.0 * math.pi)
     q = rng.uniform(0.0, 0.1, d)
     return ProblemSpec("ackley_q", d, np.concatenate([[A, B, C], q]), np.full(d, -32.0),
-                       np.full(d, 32.0), 1e-6, f_star=0.0, name=f"C2 ackley_q d={d}", grid=(3, 12),
+                       np.full(d, 32.0), 1e-6, f_star=0.0, name=f"ackley_q d={d}", grid=(3, 12),
                        k_select=1)
 
 
 def _family_params(rng, family: str, d: int) -> np.ndarray:
-    """Coefficients of a family drawn as in the C0-C2 workloads."""
+    """Coefficients of a family drawn as in the single-problem workloads."""
     if family == "sepcos":
         return np.concatenate(draw_sepcos(rng, d))
     if family == "sepcos_chain":

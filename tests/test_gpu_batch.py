@@ -92,7 +92,7 @@ def test_rational_exact_matches_oracle(gpu, bmod, oracle):
 
 
 def test_max_iters_and_capacity(gpu, bmod):
-    spec = pb.config2(d=6)
+    spec = pb.ackley50(d=6)
     p = BatchProblem.from_spec(spec, grid=(2, 6), k_select=8, max_iters=5, tag=1)
     with Batch(slots=2, module=bmod, rounding="fast") as b:
         r = b.solve_all([p])[1]

@@ -37,7 +37,7 @@ def test_rational_objective_bit_identical(gpu, oracle):
         assert np.array_equal(g[k], o[k]), k
 
 
-@pytest.mark.parametrize("spec", [pb.config0(), pb.config1(), pb.config2(d=6), pb.config2(d=50),
+@pytest.mark.parametrize("spec", [pb.sepcos2(), pb.chain10(), pb.ackley50(d=6), pb.ackley50(d=50),
                                   pb.rastrigin(4), pb.ackley(5)], ids=lambda s: s.name)
 @pytest.mark.parametrize("rounding", ["exact", "fast"])
 def test_enclosures_contain_truth_and_match_oracle(gpu, oracle, spec, rounding):
@@ -64,7 +64,7 @@ def test_enclosures_contain_truth_and_match_oracle(gpu, oracle, spec, rounding):
 
 
 def test_point_boxes_closed_form(gpu, oracle):
-    for spec in (pb.config0(), pb.config2(d=50), pb.rastrigin(7)):
+    for spec in (pb.sepcos2(), pb.ackley50(d=50), pb.rastrigin(7)):
         z = np.zeros((1, spec.d))
         m = gpu.get_module(pb.objective(spec.family, spec.d), roundings=("fast",))
         g = m.eval_boxes(spec.params, z, z, spec.lo, spec.hi, 1, rounding="fast")

@@ -15,12 +15,12 @@ KEYS = ("lb", "ub", "width", "iterations", "boxes_bounded", "queue_size", "statu
 
 
 @pytest.mark.parametrize("spec,grid,k,mv,rounding", [
-    (pb.config0(), (2, 2), 16, False, "fast"),
+    (pb.sepcos2(), (2, 2), 16, False, "fast"),
     (pb.rosenbrock(4), (2, 4), 64, True, "fast"),
     (pb.rosenbrock(4), (2, 4), 64, False, "exact"),
     (pb.ackley(4), (3, 4), 32, True, "fast"),
-    (pb.config1(), (4, 5), 32, True, "fast"),
-    (pb.config2(), (3, 12), 1, False, "fast"),
+    (pb.chain10(), (4, 5), 32, True, "fast"),
+    (pb.ackley50(), (3, 12), 1, False, "fast"),
 ], ids=lambda v: getattr(v, "name", str(v)))
 def test_persistent_equals_launches(gpu, spec, grid, k, mv, rounding):
     obj = pb.objective(spec.family, spec.d)

@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 FAM = {"sepcos": 0, "sepcos_chain": 1, "ackley_q": 2}
 
 
-@pytest.mark.parametrize("spec", [pb.config0(), pb.rosenbrock(4), pb.config1(), pb.config2(d=6)],
+@pytest.mark.parametrize("spec", [pb.sepcos2(), pb.rosenbrock(4), pb.chain10(), pb.ackley50(d=6)],
                          ids=lambda s: s.name)
 @pytest.mark.parametrize("rounding", ["fast"])
 def test_f32_enclosure_contains_true_values(gpu, oracle, spec, rounding):
@@ -52,11 +52,11 @@ def test_f32_enclosure_contains_true_values(gpu, oracle, spec, rounding):
 
 
 @pytest.mark.parametrize("spec,grid,k,mv", [
-    (pb.config0(), (2, 2), 16, False),
+    (pb.sepcos2(), (2, 2), 16, False),
     (pb.rosenbrock(4), (2, 4), 64, False),
-    (pb.config1(), (4, 5), 256, True),
-    (pb.config1(), (4, 5), 256, False),
-    (pb.config2(), (3, 12), 1, False),
+    (pb.chain10(), (4, 5), 256, True),
+    (pb.chain10(), (4, 5), 256, False),
+    (pb.ackley50(), (3, 12), 1, False),
 ], ids=lambda v: getattr(v, "name", str(v)))
 def test_f32_prune_fp64_certify(gpu, oracle, spec, grid, k, mv):
     obj = pb.objective(spec.family, spec.d)

@@ -51,13 +51,13 @@ def test_rational_path_identical_to_oracle(gpu, oracle, grid, k):
 
 
 @pytest.mark.parametrize("spec,grid,k", [
-    (pb.config0(), (64, 2), 1),
-    (pb.config0(), (2, 2), 64),
-    (pb.config1(), (2, 5), 256),
+    (pb.sepcos2(), (64, 2), 1),
+    (pb.sepcos2(), (2, 2), 64),
+    (pb.chain10(), (2, 5), 256),
     (pb.rastrigin(3), (2, 3), 64),
     (pb.rosenbrock(4), (2, 4), 64),
     (pb.ackley(4), (2, 4), 64),
-    (pb.config2(d=50), (3, 12), 1),
+    (pb.ackley50(d=50), (3, 12), 1),
 ], ids=lambda v: getattr(v, "name", str(v)))
 @pytest.mark.parametrize("rounding", ["fast", "exact"])
 def test_certified_enclosure(gpu, oracle, spec, grid, k, rounding):
@@ -77,7 +77,7 @@ def test_certified_enclosure(gpu, oracle, spec, grid, k, rounding):
         assert max(r["lb"], o["lb"]) <= min(r["ub"], o["ub"])
 
 
-@pytest.mark.parametrize("spec,grid,k", [(pb.rosenbrock(4), (2, 4), 16), (pb.config1(), (2, 5), 256)],
+@pytest.mark.parametrize("spec,grid,k", [(pb.rosenbrock(4), (2, 4), 16), (pb.chain10(), (2, 5), 256)],
                          ids=lambda v: getattr(v, "name", str(v)))
 def test_mean_value_mode_matches_oracle(gpu, oracle, spec, grid, k):
     """The optional mean-value lower bound (GPUGO_FLAG_MEAN_VALUE) against the
@@ -99,7 +99,7 @@ def test_mean_value_mode_matches_oracle(gpu, oracle, spec, grid, k):
 
 
 def test_deterministic(gpu):
-    spec = pb.config1()
+    spec = pb.chain10()
     out = []
     for _ in range(2):
         with gpu_problem(gpu, spec, tol=1e-6, grid=(2, 5), k_select=128) as p:
@@ -127,7 +127,7 @@ def test_x_width_rule(gpu):
 
 
 def test_queue_overflow_is_reported(gpu):
-    spec = pb.config1()
+    spec = pb.chain10()
     with gpu_problem(gpu, spec, tol=1e-6, grid=(4, 10), k_select=1, max_queue=16) as p:
         with pytest.raises(GpugoError, match="CAPACITY"):
             p.solve()
@@ -136,7 +136,7 @@ def test_queue_overflow_is_reported(gpu):
 def test_reference_grid_matches_oracle_first_steps(gpu, oracle):
     """Config 1 on the reference schedule (wid 4, 10 coordinates, one box per
     step): same incumbent and lower bound as the oracle within slack."""
-    spec = pb.config1()
+    spec = pb.chain10()
     with gpu_problem(gpu, spec, tol=1e-6, grid=(4, 10), k_select=1, rounding="exact") as p:
         r = p.step(3)
     o, _ = oracle.solve(1, spec.params, spec.lo, spec.hi, tol_f=1e-6, wid=4, cycle=10, k_select=1, max_iters=3)

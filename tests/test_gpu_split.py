@@ -18,14 +18,14 @@ FAM = {"sepcos": 0, "sepcos_chain": 1, "ackley_q": 2}
 
 
 @pytest.mark.parametrize("spec,grid,k,mv,rounding", [
-    (pb.config0(), (2, 2), 16, False, "fast"),
+    (pb.sepcos2(), (2, 2), 16, False, "fast"),
     (pb.rastrigin(3), (3, 3), 32, True, "fast"),
     (pb.rosenbrock(4), (2, 4), 64, False, "fast"),
     (pb.rosenbrock(4), (2, 4), 64, True, "exact"),
     (pb.ackley(4), (3, 4), 32, True, "fast"),
-    (pb.config1(), (4, 5), 32, True, "fast"),
-    (pb.config1(), (4, 5), 64, False, "fast"),
-    (pb.config2(), (3, 12), 1, False, "fast"),
+    (pb.chain10(), (4, 5), 32, True, "fast"),
+    (pb.chain10(), (4, 5), 64, False, "fast"),
+    (pb.ackley50(), (3, 12), 1, False, "fast"),
 ], ids=lambda v: getattr(v, "name", str(v)))
 def test_split_certifies(gpu, oracle, spec, grid, k, mv, rounding):
     obj = pb.objective(spec.family, spec.d)

@@ -71,7 +71,7 @@ def ir_eval(e, x, P, memo=None, j=None):
     return r
 
 
-SPECS = [pb.config0(), pb.config1(), pb.config2(d=7), pb.rastrigin(3), pb.rosenbrock(4), pb.ackley(4)]
+SPECS = [pb.sepcos2(), pb.chain10(), pb.ackley50(d=7), pb.rastrigin(3), pb.rosenbrock(4), pb.ackley(4)]
 
 
 @pytest.mark.parametrize("spec", SPECS, ids=lambda s: s.name)
@@ -142,7 +142,7 @@ def test_symbolic_diff():
 
 
 def test_emitter_lowering_rules():
-    spec = pb.config2(d=3)
+    spec = pb.ackley50(d=3)
     src = emit.code_lists(pb.objective(spec.family, spec.d))
     # A + e enters as an interval constant plus a scalar; 1/d as a positive divisor
     assert "mk<double>(0x1.5bf0a8b145769p+1, 0x1.5bf0a8b14576ap+1)" in src

@@ -2,7 +2,7 @@
 
 End-to-end checks against closed forms and independent witnesses: the
 certified enclosure [lb, ub] must contain f* (0 for the separable and
-Ackley-type families at x = 0; for config 1 a local minimum found by
+Ackley-type families at x = 0; for the d = 10 chain a local minimum found by
 scipy from many starts and evaluated in mpmath is a witness upper bound),
 and must be no wider than the tolerance.  The reference's x-width output
 rule must return boxes that contain the minimiser.
@@ -23,7 +23,7 @@ def solve(oracle, spec, **kw):
 
 
 def test_config0_certified_contains_zero(oracle):
-    spec = pb.config0()
+    spec = pb.sepcos2()
     for grid, k in [((64, 2), 1), ((2, 2), 16)]:
         r, _ = solve(oracle, spec, tol_f=1e-6, wid=grid[0], cycle=grid[1], k_select=k, max_iters=20000)
         assert r["status"] == 1
@@ -32,7 +32,7 @@ def test_config0_certified_contains_zero(oracle):
 
 
 def test_config1_certified_contains_witness(oracle):
-    spec = pb.config1()
+    spec = pb.chain10()
     r, _ = solve(oracle, spec, tol_f=1e-6, wid=2, cycle=5, k_select=256, max_iters=5000)
     assert r["status"] == 1 and r["ub"] - r["lb"] <= 1e-6
     # witness: best of many local minimisations, value in mpmath
@@ -49,7 +49,7 @@ def test_config1_certified_contains_witness(oracle):
     assert r["ub"] <= float(fw) + 1e-6           # the enclosure reaches the best witness
 
 
-@pytest.mark.parametrize("spec", [pb.rastrigin(2), pb.rosenbrock(3), pb.ackley(3), pb.config2(d=4)],
+@pytest.mark.parametrize("spec", [pb.rastrigin(2), pb.rosenbrock(3), pb.ackley(3), pb.ackley50(d=4)],
                          ids=lambda s: s.name)
 def test_reference_objectives_f_star_zero(oracle, spec):
     r, _ = solve(oracle, spec, tol_f=1e-6, wid=2, cycle=spec.d, k_select=64, max_iters=20000)
@@ -58,7 +58,7 @@ def test_reference_objectives_f_star_zero(oracle, spec):
 
 
 def test_k_select_changes_path_not_result(oracle):
-    spec = pb.config0()
+    spec = pb.sepcos2()
     a, _ = solve(oracle, spec, tol_f=1e-6, wid=4, cycle=2, k_select=1, max_iters=20000)
     b, _ = solve(oracle, spec, tol_f=1e-6, wid=4, cycle=2, k_select=32, max_iters=20000)
     assert a["status"] == b["status"] == 1
@@ -79,7 +79,7 @@ def test_x_width_rule_outputs_contain_minimiser(oracle):
 
 
 def test_deterministic(oracle):
-    spec = pb.config1()
+    spec = pb.chain10()
     a, _ = solve(oracle, spec, tol_f=1e-6, wid=2, cycle=5, k_select=64, max_iters=40, threads=8)
     b, _ = solve(oracle, spec, tol_f=1e-6, wid=2, cycle=5, k_select=64, max_iters=40, threads=3)
     for k in ("lb", "ub", "iterations", "boxes_bounded", "pruned_value", "pruned_kkt", "queue_size"):

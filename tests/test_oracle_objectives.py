@@ -64,7 +64,7 @@ def random_boxes(rng, spec, n, max_frac=0.3):
     return np.maximum(c - w, spec.lo), np.minimum(c + w, spec.hi)
 
 
-SPECS = [pb.config0(), pb.config1(), pb.config2(d=6), pb.rastrigin(3), pb.rosenbrock(4), pb.ackley(5)]
+SPECS = [pb.sepcos2(), pb.chain10(), pb.ackley50(d=6), pb.rastrigin(3), pb.rosenbrock(4), pb.ackley(5)]
 
 
 @pytest.mark.parametrize("spec", SPECS, ids=lambda s: s.name)
@@ -106,7 +106,7 @@ def test_reference_objectives_are_family_members(oracle, name, spec):
         assert mp.mpf(r["flo"][0]) - tol <= fr <= mp.mpf(r["fhi"][0]) + tol
 
 
-@pytest.mark.parametrize("spec", [pb.config0(), pb.config2(d=50), pb.rastrigin(4), pb.ackley(8)],
+@pytest.mark.parametrize("spec", [pb.sepcos2(), pb.ackley50(d=50), pb.rastrigin(4), pb.ackley(8)],
                          ids=lambda s: s.name)
 def test_closed_form_minimum_zero(oracle, spec):
     z = np.zeros((1, spec.d))
@@ -121,7 +121,7 @@ def test_closed_form_minimum_zero(oracle, spec):
 
 
 def test_kkt_prunes_monotone_boxes(oracle):
-    spec = pb.config0()
+    spec = pb.sepcos2()
     # far from 0 the quadratic dominates: gradient has constant sign; a box
     # touching the initial boundary on the low side must not be pruned when
     # f increases away from that boundary
