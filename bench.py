@@ -167,7 +167,15 @@ def leg_fp32(G, spec, obj, sched, args):
     t64, r64s = timed("fp64")
     t32, r32s = timed("fp32")
     r, r64 = r32s[-1], r64s[-1]
+    # the FP64 host oracle on the same workload and schedule
+    from oracle import oracle as O
+    O.build()
+    o, _ = O.solve(1, spec.params, spec.lo, spec.hi, tol_f=spec.tol, wid=sched["grid"][0], cycle=sched["grid"][1],
+                   k_select=sched["k_select"], max_iters=100000, threads=os.cpu_count() or 1,
+                   mean_value=sched["mean_value"])
     return {"precision": "fp32 prune, fp64 certify", "time_to_certified_s": t32, "fp64_time_s": t64,
+            "oracle_fp64": {"lb": o["lb"], "ub": o["ub"], "width": o["ub"] - o["lb"], "certified": o["status"] == 1},
+            "intersects_oracle": max(r["lb"], o["lb"]) <= min(r["ub"], o["ub"]),
             "speedup_vs_fp64": t64 / t32,
             "certified": all(x["status_name"] == "certified" for x in r32s),
             "lb": r["lb"], "ub": r["ub"], "width": r["width"], "iterations": r["iterations"],
@@ -216,6 +224,7 @@ def leg_c2(G, pb, steps=5):
     return {"workload": "ackley50: ackley_q d=50, box [-32,32]^50, tol 1e-6, seed 2",
             "fp32_prune": {"time_to_certified_s": statistics.median(t32), "lb": r32["lb"], "ub": r32["ub"],
                            "width": r32["width"], "certified": r32["status_name"] == "certified",
+                           "intersects_oracle": max(r32["lb"], o["lb"]) <= min(r32["ub"], o["ub"]),
                            "note": "the first step has no incumbent, so nothing is ruled out in FP32 there"},
             "grid": list(grid), "k_select": k, "dtype": "f64",
             "time_to_certified_s": gpu, "certified": all(x["status_name"] == "certified" for x in res),
