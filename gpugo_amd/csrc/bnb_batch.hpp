@@ -172,7 +172,8 @@ extern "C" __global__ void gpugo_batch_info(int *out) {
   out[3] = (GPUGO_WITH_EXACT ? 1 : 0) | (GPUGO_WITH_FAST ? 2 : 0) | (GPUGO_BATCH_MV ? 4 : 0);
   out[4] = GPUGO_PMAX;
   const int np[] = {GPUGO_FAM_NPARAMS};
-  for (int i = 0; i < GPUGO_NFAM; ++i) out[5 + i] = np[i];
+  // the library reads 64 ints; it rejects modules with more than 59 families
+  for (int i = 0; i < GPUGO_NFAM && i < 59; ++i) out[5 + i] = np[i];
 }
 // declared coefficient signs, GPUGO_PMAX entries per family
 extern "C" __global__ void gpugo_batch_param_signs(signed char *out) {

@@ -190,6 +190,134 @@ __device__ void radix_gather(const SelectArgs &a, long long N, int K, unsigned *
   __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// parent-record collection.  Records are appended, one per split box, and
+// referenced by the queue's q_par; when the arena has no room for a step's
+// new records, the records that no live entry references are dropped: the
+// kept ones move down in index order and q_par is rewritten, so the
+// (lb, parent, pos) order of the queue is unchanged and the solve goes on
+// exactly as with a larger arena.  Works in chunks of GC_CHUNK records with
+// a bitmap of referenced records and its word prefix counts in LDS (`bits`,
+// `pre`: GC_WORDS words each).  Returns the number of records kept and
+// remaps `par` (a parent index held in a register, or ~0u).
+// ---------------------------------------------------------------------------
+#define GC_WORDS SEL_BINS
+#define GC_CHUNK (GC_WORDS * 32)
+#define GC_U 4
+
+__device__ __forceinline__ unsigned gc_rank(const unsigned *bits, const unsigned *pre, unsigned r) {
+  return pre[r >> 5] + (unsigned)__builtin_popcount(bits[r >> 5] & ((1u << (r & 31)) - 1u));
+}
+
+__device__ long long collect_parents(const SelectArgs &a, long long N, long long used, unsigned *bits, unsigned *pre,
+                                     unsigned *scan_sh, unsigned &par) {
+  static_assert(GC_WORDS == 4 * SEL_T, "four bitmap words per thread");
+  const int tid = threadIdx.x;
+  const unsigned d = (unsigned)a.d;
+  long long kept = 0;
+  for (long long c0 = 0; c0 < used; c0 += GC_CHUNK) {
+    const unsigned cn = (unsigned)(used - c0 < GC_CHUNK ? used - c0 : GC_CHUNK);
+    for (int w = tid; w < GC_WORDS; w += SEL_T) bits[w] = 0u;
+    __syncthreads();
+    for (long long i0 = tid; i0 < N; i0 += (long long)SEL_T * SEL_U) {
+      unsigned v[SEL_U];
+#pragma unroll
+      for (int u = 0; u < SEL_U; ++u) {
+        const long long i = i0 + (long long)u * SEL_T;
+        v[u] = i < N ? a.q_par[i] : 0xffffffffu;
+      }
+#pragma unroll
+      for (int u = 0; u < SEL_U; ++u) {
+        const long long r = (long long)v[u] - c0;
+        if (r >= 0 && r < (long long)cn) atomicOr(&bits[r >> 5], 1u << (r & 31));
+      }
+    }
+    __syncthreads();
+    unsigned h[4], ts = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      h[q] = (unsigned)__builtin_popcount(bits[4 * tid + q]);
+      ts += h[q];
+    }
+    unsigned tot;
+    unsigned cum = block_excl_scan(ts, scan_sh, &tot);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      pre[4 * tid + q] = cum;
+      cum += h[q];
+    }
+    __syncthreads();
+    // A kept record moves to an index at or below its own, and the flat
+    // position r * pstride + j grows with the element number r * d + j, so
+    // each range of elements is read whole before it is written and no
+    // write reaches a later range.
+    const unsigned ne = cn * d;
+    for (unsigned e0 = 0; e0 < ne; e0 += SEL_T * GC_U) {
+      double vl[GC_U], vh[GC_U], vw[GC_U];
+      long long dst[GC_U];
+#pragma unroll
+      for (int u = 0; u < GC_U; ++u) {
+        const unsigned e = e0 + (unsigned)u * SEL_T + tid;
+        dst[u] = -1;
+        if (e < ne) {
+          const unsigned r = e / d, j = e - r * d;
+          if ((bits[r >> 5] >> (r & 31)) & 1u) {
+            const long long np = kept + gc_rank(bits, pre, r), op = c0 + r;
+            if (np != op) {
+              const long long s = op * a.pstride + j;
+              vl[u] = a.plo[s];
+              vh[u] = a.phi[s];
+              vw[u] = a.pw[s];
+              dst[u] = np * a.pstride + j;
+            }
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < GC_U; ++u)
+        if (dst[u] >= 0) {
+          a.plo[dst[u]] = vl[u];
+          a.phi[dst[u]] = vh[u];
+          a.pw[dst[u]] = vw[u];
+        }
+    }
+    for (unsigned r0 = 0; r0 < cn; r0 += SEL_T * GC_U) {
+      int vn[GC_U];
+      long long dst[GC_U];
+#pragma unroll
+      for (int u = 0; u < GC_U; ++u) {
+        const unsigned r = r0 + (unsigned)u * SEL_T + tid;
+        dst[u] = -1;
+        if (r < cn && ((bits[r >> 5] >> (r & 31)) & 1u)) {
+          const long long np = kept + gc_rank(bits, pre, r), op = c0 + r;
+          if (np != op) {
+            vn[u] = a.pwin[op];
+            dst[u] = np;
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < GC_U; ++u)
+        if (dst[u] >= 0) a.pwin[dst[u]] = vn[u];
+    }
+    // every referenced index is below c0 + cn and maps at or below itself,
+    // so rewritten entries are never taken up again by a later chunk
+    for (long long i = tid; i < N; i += SEL_T) {
+      const long long r = (long long)a.q_par[i] - c0;
+      if (r >= 0 && r < (long long)cn) a.q_par[i] = (unsigned)(kept + gc_rank(bits, pre, (unsigned)r));
+    }
+    if (par != 0xffffffffu) {
+      const long long r = (long long)par - c0;
+      if (r >= 0 && r < (long long)cn) par = (unsigned)(kept + gc_rank(bits, pre, (unsigned)r));
+    }
+    kept += tot;
+    __syncthreads();
+  }
+  return kept;
+}
+
 __device__ void select_body(const SelectArgs &a) {
   __shared__ unsigned hist[SEL_BINS];
   __shared__ u128 skey[SEL_KMAX];
@@ -441,9 +569,18 @@ __device__ void select_body(const SelectArgs &a) {
   unsigned n_split, n_outs;
   const unsigned split_rank = block_excl_scan((tid < K && !is_out) ? 1u : 0u, scan_sh, &n_split);
   const unsigned out_rank = block_excl_scan((tid < K && is_out) ? 1u : 0u, scan_sh, &n_outs);
-  const long long pbase = st->parents_used;
+  long long pbase = st->parents_used, room = a.p_cap;
   const long long obase = st->n_out;
-  if (pbase + (long long)n_split > a.p_cap) {
+  if (pbase + (long long)n_split > room) {
+    // arena full: drop the records no queue entry references (hist and skey
+    // are free from here on); overflow if the kept ones and this step's
+    // leave less than an eighth of the arena, so collections stay rare
+    unsigned gpar = tid < K ? par : 0xffffffffu;
+    pbase = collect_parents(a, N, pbase, hist, reinterpret_cast<unsigned *>(skey), scan_sh, gpar);
+    if (tid < K) par = gpar;
+    room = a.p_cap - (a.p_cap >> 3);
+  }
+  if (pbase + (long long)n_split > room) {
     if (tid == 0) {
       st->overflow |= 2;
       st->status = GPUGO_ST_OVERFLOW;

@@ -624,7 +624,6 @@ __global__ __launch_bounds__(SEL_T) void kb_select(BatchArgs b) {
   select_body(sa);
 }
 
-// The slot's queue is compacted by one workgroup, keeping the order.
 // The slot's queue is compacted by KB_CB workgroups (grid-stride over the
 // queue, one atomic per workgroup and chunk for the output position); the
 // last workgroup of the slot to finish switches the live buffer.

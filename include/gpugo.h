@@ -256,7 +256,7 @@ typedef struct {
   int32_t use_graph;         /* 1: capture the steps of a launch in a graph  */
   int64_t max_queue;         /* live boxes per slot (0: 65536)               */
   int64_t max_parents;       /* parent records per slot (0: 16384)           */
-  int32_t bound_blocks;      /* persistent split+bound grid (0: 4 per CU)    */
+  int32_t bound_blocks;      /* persistent split+bound grid (0: 8 per CU)    */
   int32_t reserved;
 } gpugo_batch_desc;
 
