@@ -309,11 +309,18 @@ def leg_batch(G, pb, n, seq_sample):
             "certified_per_s": cert / el,
             "status_counts": {k: sum(r["status_name"] == k for r in res.values())
                               for k in sorted({r["status_name"] for r in res.values()})},
+            "overflow_kinds": overflow_kinds(res.values()),
             "width_max_certified": max(widths) if widths else None,
             "boxes_bounded": sum(r["boxes_bounded"] for r in res.values()),
             "sequential": {"problems": len(sample), "time_s": t_seq, "problems_per_s": len(sample) / t_seq,
                            "certified": seq_cert, "identical_to_batch": same},
             "speedup_problems_per_s": (n / el) / (len(sample) / t_seq)}
+
+
+def overflow_kinds(results):
+    """Overflowed problems by the capacity that ran out."""
+    over = [r.get("overflow", 0) for r in results if r["status_name"] == "overflow"]
+    return {"box_queue": sum(1 for o in over if o & 1), "parent_arena": sum(1 for o in over if o & 2)}
 
 
 def leg_served(pb, lams=(100.0, 1000.0), seconds=3.0):
@@ -333,7 +340,8 @@ def leg_served(pb, lams=(100.0, 1000.0), seconds=3.0):
             summ.update({"lambda_per_s": lam, "requests": n, "wall_s": el,
                          "throughput_per_s": len(done) / el,
                          "status_counts": {k: sum(r.result["status_name"] == k for r in done)
-                                           for k in sorted({r.result["status_name"] for r in done})}})
+                                           for k in sorted({r.result["status_name"] for r in done})},
+                         "overflow_kinds": overflow_kinds(r.result for r in done)})
             out[f"lambda_{int(lam)}"] = summ
     out["slots_per_dmax_class"] = {str(k): v for k, v in slots.items()}
     out["workload"] = ("sepcos / sepcos_chain / ackley_q, d log-uniform [2,100], tol log-uniform [1e-8,1e-4], "

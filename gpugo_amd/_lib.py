@@ -68,7 +68,7 @@ class BatchDesc(ctypes.Structure):
 class Result(ctypes.Structure):
     _fields_ = [
         ("lb", ctypes.c_double), ("ub", ctypes.c_double), ("width", ctypes.c_double),
-        ("status", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("status", ctypes.c_int32), ("overflow", ctypes.c_int32),
         ("iterations", ctypes.c_int64), ("boxes_bounded", ctypes.c_int64),
         ("pruned_value", ctypes.c_int64), ("pruned_kkt", ctypes.c_int64),
         ("queue_size", ctypes.c_int64), ("queue_peak", ctypes.c_int64),

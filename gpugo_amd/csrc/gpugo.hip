@@ -1209,12 +1209,17 @@ static int fill_result(gpugo_problem_t p, gpugo_result *res, double ms, bool max
   res->lb = lb;
   res->ub = ub;
   {
-    // ub - lb rounded up on the host: nextafter of the nearest difference
+    // ub - lb rounded up, as the device's stop test rounds it: the nearest
+    // difference, one ulp up when its rounding error (two-sum) is positive
     double w = ub - lb;
-    if (std::isfinite(w) && w > -__builtin_huge_val()) w = std::nextafter(w, __builtin_huge_val());
+    if (std::isfinite(w)) {
+      const double bv = w - ub, err = (ub - (w - bv)) + (-lb - bv);
+      if (err > 0) w = std::nextafter(w, __builtin_huge_val());
+    }
     res->width = (s.status == GPUGO_ST_CERTIFIED || s.status == GPUGO_ST_X_CONVERGED) ? s.width : w;
   }
   res->status = s.status;
+  res->overflow = s.overflow & 3;
   if (s.status == GPUGO_ST_RUNNING && max_iters_hit) res->status = GPUGO_ST_MAX_ITERS;
   res->iterations = s.iterations;
   res->boxes_bounded = s.bounded;
@@ -1807,6 +1812,7 @@ static void batch_result(const DevState &s, gpugo_result *r, double ms) {
   r->ub = key_value(s.ub_key);
   r->width = s.width;
   r->status = s.status;
+  r->overflow = s.overflow & 3;
   r->iterations = s.iterations;
   r->boxes_bounded = s.bounded;
   r->pruned_value = s.pruned_value;

@@ -199,7 +199,8 @@ typedef struct {
   double  ub;               /* best upper bound found (incumbent)            */
   double  width;            /* ub - lb, rounded up                           */
   int32_t status;           /* enum gpugo_status                             */
-  int32_t reserved0;
+  int32_t overflow;         /* with GPUGO_ST_OVERFLOW: 1 box queue full,
+                               2 parent arena full                          */
   int64_t iterations;       /* steps taken                                   */
   int64_t boxes_bounded;    /* children that received a value enclosure      */
   int64_t pruned_value;     /* children pruned by lb > ub                    */

@@ -24,7 +24,7 @@ class GpugoResult(ctypes.Structure):
     """Mirror of gpugo_result (include/gpugo.h)."""
     _fields_ = [
         ("lb", ctypes.c_double), ("ub", ctypes.c_double), ("width", ctypes.c_double),
-        ("status", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("status", ctypes.c_int32), ("overflow", ctypes.c_int32),
         ("iterations", ctypes.c_int64), ("boxes_bounded", ctypes.c_int64),
         ("pruned_value", ctypes.c_int64), ("pruned_kkt", ctypes.c_int64),
         ("queue_size", ctypes.c_int64), ("queue_peak", ctypes.c_int64),

@@ -66,6 +66,23 @@ def test_batch_equals_single(gpu, bmod):
                 assert r["lb"] <= 0.0 <= r["ub"]
 
 
+def test_max_iters_equals_single(gpu, bmod):
+    """A slot stopped by its step limit reports what gpugo_solve reports
+    after as many steps: the smallest live lower bound and ub - lb rounded
+    up, bit for bit."""
+    probs = mixed_problems(6, seed=5)
+    for p in probs:
+        p.max_iters = 3 + p.tag
+    with Batch(slots=8, module=bmod, rounding="fast", steps_per_launch=4) as b:
+        res = b.solve_all(probs)
+    for p in probs:
+        s = single(gpu, p)
+        assert res[p.tag]["status_name"] == s["status_name"], p.tag
+        for key in KEYS:
+            assert res[p.tag][key] == s[key], (p.tag, key, res[p.tag][key], s[key])
+    assert sum(r["status_name"] == "max_iters" for r in res.values()) >= 3
+
+
 def test_alone_equals_mix_continuous(gpu, bmod):
     """Fewer slots than problems: slots are refilled while others run."""
     probs = mixed_problems(30, seed=1)
