@@ -65,9 +65,10 @@ def parse():
     ap.add_argument("--rounding", choices=["fast", "exact"], default="fast")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--legs", default="fp32,c2,batch,served",
+    ap.add_argument("--legs", default="fp32,c2,large_d,batch,served",
                     help="comma list besides the headline: fp32 (headline workload in the FP32-prune form), "
-                         "c2 (d=50), batch (4096 problems), served (Poisson); 'none' for the headline only")
+                         "c2 (d=50), large_d (windowed form, d=1000), batch (4096 problems), "
+                         "served (Poisson); 'none' for the headline only")
     ap.add_argument("--batch-n", type=int, default=4096)
     ap.add_argument("--seq-sample", type=int, default=256,
                     help="problems of the batch solved one at a time for the sequential baseline")
@@ -235,6 +236,75 @@ def leg_c2(G, pb, steps=5):
                              "sample": "one complete solve, same workload and schedule",
                              "certified": o["status"] == 1, "lb": o["lb"], "ub": o["ub"]},
             "speedup_vs_cpu": tc / gpu}
+
+
+def leg_large_d(G, names=("levy", "rosenbrock", "fu"), d=1000, reps=2, cpu_seconds=2.0):
+    """The windowed (large-d) form on reference objectives at d = 1000 (the
+    reference's boxes and initial upper bounds, 3^12 children of one box
+    per step, 10 diagonal samples, tol 1e-6): time to a certified enclosure,
+    boxes bounded per second and the roofline of the windowed split+bound
+    kernel.  CPU beside it: the oracle (the reference method, every
+    coordinate of every child) on Rosenbrock d = 1000 (the one of these it
+    restates), one step of 3^8 children from the root, all host threads:
+    its boxes/s and the time it would need for the GPU's boxes."""
+    from gpugo_amd import emit, families as F, problems as pb
+    from oracle import oracle as O
+    grid, k, S = (3, 12), 1, 10
+    out = {"workload": f"reference objectives d={d}, reference boxes and initial upper bounds, grid 3^12, "
+                       f"one box per step, {S} diagonal samples, tol 1e-6, windowed code lists"}
+    cpu_rate = None
+    for name in names:
+        s = F.spec(name, d)
+        obj = F.objective_loops(name, d)
+        m = G.get_module(obj, roundings=("exact", "fast"), windowed=True)
+        p = G.Problem(obj, [], s.lo, s.hi, tol=1e-6, grid=grid, k_select=k, n_samples=S,
+                      init_ub=s.extra["init_ub"], module=m, windowed=True, max_queue=1 << 26, max_iters=20000)
+        p.solve()
+        p.timing(True)
+        times, res = [], []
+        for _ in range(reps):
+            p.reset()
+            t0 = time.perf_counter()
+            res.append(p.solve())
+            times.append(time.perf_counter() - t0)
+        p.timing(False)
+        p.close()
+        r = res[-1]
+        fl = emit.window_flop_counts(obj, grid[1])
+        flops = sum(x["boxes_bounded"] * fl["value"]
+                    + (x["boxes_bounded"] - x["pruned_value"]) * (fl["grad"] + S * fl["sample"]) for x in res)
+        bound_ms = sum(x["bound_ms"] for x in res)
+        launches = sum(x["bound_launches"] for x in res)
+        achieved = flops / (bound_ms * 1e-3) / 1e12 if bound_ms > 0 else 0.0
+        t = statistics.median(times)
+        leg = {"time_to_certified_s": t, "certified": all(x["status_name"] == "certified" for x in res),
+               "lb": r["lb"], "ub": r["ub"], "width": r["width"], "f_star": s.f_star,
+               "contains_f_star": r["lb"] <= s.f_star <= r["ub"], "iterations": r["iterations"],
+               "boxes_bounded": r["boxes_bounded"], "boxes_bounded_per_s": r["boxes_bounded"] / t,
+               "queue_peak": r["queue_peak"],
+               "roofline": {"bound": "valu", "kernel": "gpugo_bound_fast (windowed)", "achieved": achieved,
+                            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                            "flop_per_child_value": fl["value"], "avg_launch_ms": bound_ms / max(launches, 1),
+                            "launches": launches}}
+        if name == "rosenbrock":
+            spec = pb.rosenbrock(d)
+            threads = os.cpu_count() or 1
+            O.build()
+            t0 = time.perf_counter()
+            nsteps, boxes = 0, 0
+            while nsteps == 0 or time.perf_counter() - t0 < cpu_seconds:
+                o, _ = O.solve(1, spec.params, spec.lo, spec.hi, tol_f=1e-6, wid=3, cycle=8, k_select=1,
+                               n_samples=S, max_iters=1, threads=threads)
+                boxes += o["boxes_bounded"]
+                nsteps += 1
+            cpu_rate = boxes / (time.perf_counter() - t0)
+            leg["cpu_baseline"] = {"boxes_bounded_per_s": cpu_rate, "cores": threads, "kind": "port",
+                                   "sample": f"{nsteps} x one step of 3^8 children from the root, d={d}, "
+                                             f"{S} samples, every coordinate of every child",
+                                   "est_time_for_gpu_boxes_s": r["boxes_bounded"] / cpu_rate,
+                                   "speedup_vs_cpu_est": r["boxes_bounded"] / cpu_rate / t}
+        out[name] = leg
+    return out
 
 
 def run_batch(Batch, probs, **kw):
@@ -440,6 +510,9 @@ def main():
     if "c2" in legs:
         print("# leg c2", file=sys.stderr, flush=True)
         out["c2"] = leg_c2(G, pb)
+    if "large_d" in legs:
+        print("# leg large_d", file=sys.stderr, flush=True)
+        out["large_d"] = leg_large_d(G)
     if "batch" in legs:
         print("# leg batch", file=sys.stderr, flush=True)
         out["batch"] = leg_batch(G, pb, args.batch_n, args.seq_sample)

@@ -35,7 +35,7 @@ EXPORTS = [
     "gpugo_step", "gpugo_solve", "gpugo_boxes_copy", "gpugo_problem_destroy", "gpugo_problem_reset",
     "gpugo_problem_timing", "gpugo_batch_bytes", "gpugo_mem_info", "gpugo_batch_create",
     "gpugo_batch_admit", "gpugo_batch_run", "gpugo_batch_sync", "gpugo_batch_results",
-    "gpugo_batch_counts", "gpugo_batch_destroy", "gpugo_eval_boxes_f32",
+    "gpugo_batch_counts", "gpugo_batch_destroy", "gpugo_eval_boxes_f32", "gpugo_window_ranges",
 ]
 
 
@@ -131,6 +131,7 @@ def load():
     lib.gpugo_batch_counts.argtypes = [vp, I64, I64, I64]
     lib.gpugo_batch_destroy.argtypes = [vp]
     lib.gpugo_batch_destroy.restype = None
+    lib.gpugo_window_ranges.argtypes = [ctypes.c_int64, I32, I32]
     _lib = lib
     return lib
 

@@ -16,6 +16,10 @@ enum gpugo_status {
 };
 #endif
 
+// layout version of the structs below; modules report it and the library
+// refuses a module built against another layout
+#define GPUGO_MODULE_ABI_V 3
+
 namespace gpugo {
 
 // Per-problem state kept in device memory for the whole solve.  One
@@ -65,6 +69,8 @@ struct BoundArgs {
   int flags;                     // GPUGO_FLAG_* of the problem (+ FLAG_F32_PRUNE)
   double inv_wid, half_inv_wid;  // exact digit extraction: q = floor(r*inv + half)
   const void *params32;          // coefficients as outward FP32 intervals (iv<float>)
+  const double *prest;           // windowed modules: rest sums per parent record, [p][2][nrest] intervals
+  int nrest, pad1;
 };
 
 // BoundArgs.flags bit of the FP32 pruning pass (problem precision F32)
@@ -123,6 +129,46 @@ struct AdmitHead {
   SlotInfo info;
   DevState st;
 };
+
+// Terms j in [0, nt) of a loop whose term j uses the coordinates j + k,
+// k in [klo, khi], and that meet the split window {(win + t) mod d : t < c}
+// (c <= d): at most two ranges [lo, hi) (the window may wrap past d),
+// merged when they overlap.  Returns the number of ranges.  The windowed
+// code lists use the same ranges for the rest sums of a parent record
+// (terms outside them) and for its children (terms inside them).
+__host__ __device__ inline int win_ranges(int win, int c, int d, int klo, int khi, int nt, int *lo, int *hi) {
+  int n = 0;
+  const int e0 = win + c < d ? win + c : d;  // coordinates [win, e0) and [0, e1)
+  const int e1 = win + c - d;
+  int l = win - khi, h = e0 - klo;
+  l = l < 0 ? 0 : l;
+  h = h > nt ? nt : h;
+  if (l < h) {
+    lo[n] = l;
+    hi[n] = h;
+    ++n;
+  }
+  if (e1 > 0) {
+    int l2 = -khi, h2 = e1 - klo;
+    l2 = l2 < 0 ? 0 : l2;
+    h2 = h2 > nt ? nt : h2;
+    if (l2 < h2) {
+      if (n == 1 && l2 <= hi[0] && lo[0] <= h2) {
+        lo[0] = l2 < lo[0] ? l2 : lo[0];
+        hi[0] = h2 > hi[0] ? h2 : hi[0];
+      } else {
+        lo[n] = l2;
+        hi[n] = h2;
+        ++n;
+      }
+    }
+  }
+  return n;
+}
+
+__host__ __device__ inline bool in_ranges(int j, int n, const int *lo, const int *hi) {
+  return (n > 0 && j >= lo[0] && j < hi[0]) || (n > 1 && j >= lo[1] && j < hi[1]);
+}
 
 __host__ __device__ inline unsigned long long order_key(double x) {
   unsigned long long b;

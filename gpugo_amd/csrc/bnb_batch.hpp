@@ -13,9 +13,6 @@
 // 602-766), on the slot's own queue, parent records and incumbent.
 #pragma once
 
-#ifndef GPUGO_MODULE_ABI_V
-#define GPUGO_MODULE_ABI_V 2
-#endif
 
 namespace gpugo {
 

@@ -116,6 +116,13 @@ __device__ __forceinline__ bool group_any(bool p) {
   return ((m >> base) & gm) != 0ull;
 }
 
+// A box given by its bounds arrays (a parent record, or a row of explicit boxes).
+struct RowBox {
+  const double *__restrict__ lo;
+  const double *__restrict__ hi;
+  __device__ __forceinline__ iv<double> operator[](int j) const { return mk<double>(lo[j], hi[j]); }
+};
+
 __device__ __forceinline__ ChildBox child_box(const double *plo, const double *phi, const double *pw, int win, int d,
                                               int c, unsigned wid, double inv_wid, double half_inv_wid, unsigned pos) {
   ChildBox x;

@@ -11,7 +11,6 @@
 // (GPUGO_1.1/1-Ackley/ackley50.py:602-766).
 #pragma once
 
-#define GPUGO_MODULE_ABI_V 2
 #ifndef GPUGO_WITH_F32
 #define GPUGO_WITH_F32 0
 #endif

@@ -67,6 +67,25 @@ __device__ __forceinline__ void decode_box(const double *plo, const double *phi,
   }
 }
 
+// Coordinate j of child `pos` of a parent record: the same arithmetic as
+// decode_box for one coordinate (window coordinates (win + k) mod d, k < c,
+// take base-wid digit k of pos).
+__device__ __forceinline__ void child_coord(const double *plo, const double *phi, const double *pw, int win, int d,
+                                            int wid, int c, unsigned pos, int j, double &lo, double &hi) {
+  int k = j - win;
+  if (k < 0) k += d;
+  if (k < c) {
+    unsigned r = pos;
+    for (int t = 0; t < k; ++t) r /= (unsigned)wid;
+    const unsigned dg = r % (unsigned)wid;
+    lo = __builtin_fma(pw[j], (double)dg, plo[j]);
+    hi = (dg + 1 == (unsigned)wid) ? phi[j] : __builtin_fma(pw[j], (double)(dg + 1), plo[j]);
+  } else {
+    lo = plo[j];
+    hi = phi[j];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // select: the K smallest (lb, parent, pos) keys of the live queue, exact,
 // in one workgroup (radix select on 128-bit keys, 12-bit digits), then the
@@ -77,7 +96,7 @@ __device__ __forceinline__ void decode_box(const double *plo, const double *phi,
 #define SEL_FINAL 1024
 #define SEL_KMAX 1024
 #define SEL_U 8
-#define GPUGO_DMAX_SELECT 4096
+#define GPUGO_DMAX_SELECT (1 << 17)
 
 typedef unsigned __int128 u128;
 
@@ -93,11 +112,18 @@ struct SelectArgs {
   double tol_f, tol_x;
   long long max_iters;   // > 0: stop with GPUGO_ST_MAX_ITERS on the device (batch slots)
   int pstride;           // doubles per parent record (d, or the batch's dmax)
-  int pad0;
+  int ext_n;             // doubles of per-record payload in pext (moved with the records)
+  double *pext;          // [p_cap][ext_n] (windowed modules: rest sums), or null
+  unsigned tie_xor;      // 0: equal bounds go oldest parent first; ~0u: newest first
+  int pad1;
 };
 
-__device__ __forceinline__ u128 qkey(const double *lb, const unsigned *par, const unsigned *pos, long long i) {
-  return ((u128)order_key(lb[i]) << 64) | ((u128)par[i] << 32) | (u128)pos[i];
+// Selection key (lb, parent, pos), unique per box.  Parent records are
+// numbered in creation order (collection keeps the order), so with
+// tie_xor = ~0u equal bounds go to the newest parent first: depth-first
+// among ties, which large-d problems with many equal bounds need.
+__device__ __forceinline__ u128 qkey(const SelectArgs &a, long long i) {
+  return ((u128)order_key(a.q_lb[i]) << 64) | ((u128)(a.q_par[i] ^ a.tie_xor) << 32) | (u128)a.q_pos[i];
 }
 
 // bin of a lower bound in [lmin, lmax]: monotone non-decreasing in lb
@@ -132,7 +158,7 @@ __device__ void radix_gather(const SelectArgs &a, long long N, int K, unsigned *
     for (int b = tid; b < SEL_BINS; b += SEL_T) hist[b] = 0;
     __syncthreads();
     for (long long i = tid; i < N; i += SEL_T) {
-      u128 k = qkey(a.q_lb, a.q_par, a.q_pos, i);
+      u128 k = qkey(a, i);
       bool m = (pbits == 0) || ((k >> (128 - pbits)) == (pfx >> (128 - pbits)));
       if (m) atomicAdd(&hist[(unsigned)(k >> sh) & ((1u << db) - 1u)], 1u);
     }
@@ -171,7 +197,7 @@ __device__ void radix_gather(const SelectArgs &a, long long N, int K, unsigned *
   const int shp = 128 - s_pbits;
   const u128 ptop = s_pfx >> shp;
   for (long long i = tid; i < N; i += SEL_T) {
-    u128 k = qkey(a.q_lb, a.q_par, a.q_pos, i);
+    u128 k = qkey(a, i);
     u128 top = k >> shp;
     if (top < ptop) {
       unsigned slot = atomicAdd(&s_ns, 1u);
@@ -302,6 +328,31 @@ __device__ long long collect_parents(const SelectArgs &a, long long N, long long
       for (int u = 0; u < GC_U; ++u)
         if (dst[u] >= 0) a.pwin[dst[u]] = vn[u];
     }
+    // per-record payload (ext_n doubles each), in the same element order
+    const unsigned nx = cn * (unsigned)a.ext_n;
+    for (unsigned e0 = 0; e0 < nx; e0 += SEL_T * GC_U) {
+      double vx[GC_U];
+      long long dst[GC_U];
+#pragma unroll
+      for (int u = 0; u < GC_U; ++u) {
+        const unsigned e = e0 + (unsigned)u * SEL_T + tid;
+        dst[u] = -1;
+        if (e < nx) {
+          const unsigned r = e / (unsigned)a.ext_n, k = e - r * (unsigned)a.ext_n;
+          if ((bits[r >> 5] >> (r & 31)) & 1u) {
+            const long long np = kept + gc_rank(bits, pre, r), op = c0 + r;
+            if (np != op) {
+              vx[u] = a.pext[op * a.ext_n + k];
+              dst[u] = np * a.ext_n + k;
+            }
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < GC_U; ++u)
+        if (dst[u] >= 0) a.pext[dst[u]] = vx[u];
+    }
     // every referenced index is below c0 + cn and maps at or below itself,
     // so rewritten entries are never taken up again by a later chunk
     for (long long i = tid; i < N; i += SEL_T) {
@@ -386,7 +437,7 @@ __device__ void select_body(const SelectArgs &a) {
   const int K = (int)(N < (long long)a.k_select ? N : (long long)a.k_select);
   if (N <= K) {
     for (long long i = tid; i < N; i += SEL_T) {
-      skey[i] = qkey(a.q_lb, a.q_par, a.q_pos, i);
+      skey[i] = qkey(a, i);
       sidx[i] = (unsigned)i;
     }
     __syncthreads();
@@ -456,13 +507,13 @@ __device__ void select_body(const SelectArgs &a) {
             if (bin < bb) {
               unsigned slot = atomicAdd(&s_ns, 1u);
               if (slot < SEL_KMAX) {
-                skey[slot] = qkey(a.q_lb, a.q_par, a.q_pos, i);
+                skey[slot] = qkey(a, i);
                 sidx[slot] = (unsigned)i;
               }
             } else if (bin == bb) {
               unsigned slot = atomicAdd(&s_nc, 1u);
               if (slot < SEL_FINAL) {
-                ckey[slot] = qkey(a.q_lb, a.q_par, a.q_pos, i);
+                ckey[slot] = qkey(a, i);
                 cidx[slot] = (unsigned)i;
               }
             }
@@ -528,41 +579,48 @@ __device__ void select_body(const SelectArgs &a) {
     }
   }
 
-  // materialise: x-width output rule, then new parent records
+  // materialise: x-width output rule, then new parent records.  The
+  // selected boxes' coordinates are handled one (box, coordinate) pair per
+  // thread, so the cost does not grow with d per thread.  Per-box data
+  // lives in the ranking buffers, which are free from here on.
+  static_assert(sizeof(ckey) >= SEL_KMAX * (8 + 4 + 4) && SEL_FINAL >= SEL_KMAX, "per-box scratch fits in ckey");
+  unsigned long long *bw = reinterpret_cast<unsigned long long *>(ckey);  // [K] max child width (bits)
+  unsigned *bpar = reinterpret_cast<unsigned *>(bw + SEL_KMAX);            // [K] parent record
+  unsigned *bpos = bpar + SEL_KMAX;                                         // [K] child index
+  unsigned *bdst = cidx;                                                    // [K] destination
   int is_out = 0;
   unsigned idx = 0, par = 0, pos = 0;
   double elb = 0;
   const int d = a.d;
+  const long long kd = (long long)K * d;
   if (tid < K) {
     idx = sidx[tid];
     par = a.q_par[idx];
     pos = a.q_pos[idx];
     elb = a.q_lb[idx];
-    if (a.tol_x > 0) {
-      const double *plo = a.plo + (long long)par * a.pstride, *phi = a.phi + (long long)par * a.pstride;
-      const double *pw = a.pw + (long long)par * a.pstride;
-      int win = a.pwin[par];
-      unsigned r = pos;
-      // window coordinates shrink; recompute the max over the child
-      double mwin = 0, mrest = 0;
-      for (int j = 0; j < d; ++j) {
-        int k = j - win;
-        if (k < 0) k += d;
-        if (k >= a.c) {
-          double w = phi[j] - plo[j];
-          if (w > mrest) mrest = w;
-        }
-      }
-      for (int k = 0; k < a.c; ++k) {
-        int j = win + k;
-        if (j >= d) j -= d;
-        unsigned q = r / (unsigned)a.wid, dg = r - q * (unsigned)a.wid;
-        r = q;
-        double lo = __builtin_fma(pw[j], (double)dg, plo[j]);
-        double hi = (dg + 1 == (unsigned)a.wid) ? phi[j] : __builtin_fma(pw[j], (double)(dg + 1), plo[j]);
-        if (hi - lo > mwin) mwin = hi - lo;
-      }
-      double m = mwin > mrest ? mwin : mrest;
+    bpar[tid] = par;
+    bpos[tid] = pos;
+    bw[tid] = 0ull;
+  }
+  __syncthreads();
+  if (a.tol_x > 0) {
+    // max coordinate width of each selected child (widths are >= 0, so
+    // their bit patterns order like the values)
+    for (long long e = tid; e < kd; e += SEL_T) {
+      const int b = (int)(e / d), j = (int)(e - (long long)b * d);
+      const long long rec = (long long)bpar[b] * a.pstride;
+      double lo, hi;
+      child_coord(a.plo + rec, a.phi + rec, a.pw + rec, a.pwin[bpar[b]], d, a.wid, a.c, bpos[b], j, lo, hi);
+      const double w = hi - lo;
+      unsigned long long wb;
+      __builtin_memcpy(&wb, &w, 8);
+      atomicMax(&bw[b], wb);
+    }
+    __syncthreads();
+    if (tid < K) {
+      double m;
+      const unsigned long long wb = bw[tid];
+      __builtin_memcpy(&m, &wb, 8);
       is_out = m < a.tol_x;
     }
   }
@@ -591,27 +649,43 @@ __device__ void select_body(const SelectArgs &a) {
     return;
   }
   if (tid == 0) s_outmin = order_key(__builtin_huge_val());
-  __syncthreads();
+  unsigned *bwin = hist;  // [K] window start of each box's parent (hist is free)
   if (tid < K) {
-    const double *plo = a.plo + (long long)par * a.pstride, *phi = a.phi + (long long)par * a.pstride;
-    const double *pw = a.pw + (long long)par * a.pstride;
-    const int win = a.pwin[par];
+    bpar[tid] = par;  // remapped if the arena was collected
+    bwin[tid] = (unsigned)a.pwin[par];
+    bdst[tid] = is_out ? (0x80000000u | out_rank) : split_rank;
+  }
+  __syncthreads();
+  // new parent records (split boxes) and output boxes (x-width rule): their
+  // sources are kept records below pbase, their destinations at or above it
+  for (long long e = tid; e < kd; e += SEL_T) {
+    const int b = (int)(e / d), j = (int)(e - (long long)b * d);
+    const long long rec = (long long)bpar[b] * a.pstride;
+    double lo, hi;
+    child_coord(a.plo + rec, a.phi + rec, a.pw + rec, (int)bwin[b], d, a.wid, a.c, bpos[b], j, lo, hi);
+    const unsigned dst = bdst[b];
+    if (!(dst & 0x80000000u)) {
+      const long long np = (pbase + dst) * a.pstride + j;
+      a.plo[np] = lo;
+      a.phi[np] = hi;
+      a.pw[np] = (hi - lo) / (double)a.wid;
+    } else {
+      const long long o = obase + (dst & 0x7fffffffu);
+      if (o < a.out_cap) {
+        a.out_lo[o * d + j] = lo;
+        a.out_hi[o * d + j] = hi;
+      }
+    }
+  }
+  if (tid < K) {
     if (!is_out) {
-      const long long np = pbase + split_rank;
-      double *nlo = a.plo + np * a.pstride, *nhi = a.phi + np * a.pstride, *npw = a.pw + np * a.pstride;
-      decode_box(plo, phi, pw, win, d, a.wid, a.c, pos, nlo, nhi);
-      for (int j = 0; j < d; ++j) npw[j] = (nhi[j] - nlo[j]) / (double)a.wid;
-      int nw = win + a.c;
+      int nw = (int)bwin[tid] + a.c;
       while (nw >= d) nw -= d;
-      a.pwin[np] = nw;
+      a.pwin[pbase + split_rank] = nw;
     } else {
       const long long o = obase + out_rank;
-      if (o < a.out_cap) {
-        decode_box(plo, phi, pw, win, d, a.wid, a.c, pos, a.out_lo + o * d, a.out_hi + o * d);
-        a.out_lb[o] = elb;
-      } else {
-        atomicOr(&st->overflow, 4);
-      }
+      if (o < a.out_cap) a.out_lb[o] = elb;
+      else atomicOr(&st->overflow, 4);
       atomicMin(&s_outmin, order_key(elb));
     }
     a.q_lb[idx] = __builtin_nan("");

@@ -1,0 +1,243 @@
+// bnb_window.hpp -- kernels of a windowed objective module (large d).
+// Included (as text, by gpugo_amd.emit.window_module_source) after ia.hpp,
+// bnb_args.hpp, bnb_box.hpp, the windowed code lists (obj_rest,
+// obj_value_w, obj_kkt_w), the loop-form code lists (obj_value_g,
+// obj_grad_g, obj_kkt_g) and bnb_step.hpp.
+//
+// A step splits a parent record in the c coordinates of its window
+// (win + t) mod d, t < c (1-Ackley/ackley50.py:482-594); every child
+// shares the parent's other d - c coordinates.  So each template sum of the
+// objective is enclosed as
+//   rest (the terms that do not touch the window, over the parent record,
+//         summed once per parent by gpugo_prep_*)
+//   + the terms that do (over the child)
+// and a child costs O(c) terms instead of O(d).  The reference evaluates all
+// d coordinates for every child (6-Levy/levy10000.py:160-232).  Rest sums
+// are also kept at the parent's S diagonal sample points, for the samples
+// of the incumbent (the reference's numSamples diagonal points, :235-380).
+//
+// Reference kernels replaced: sampling and ruleOutGPU
+// (GPUGO_1.1/6-Levy/levy10000.py:686-850).
+#pragma once
+
+namespace gpugo {
+
+// Rest sums of the parent records split in this step: one workgroup per
+// record, threads over the terms, a butterfly in each wavefront and a
+// fixed-order sum over the wavefronts (deterministic).  Per record
+// (1 + S) * NREST intervals: [0, NREST) over the box, then NREST at each of
+// its S diagonal sample points (DiagPoint t = 0 .. S-1).
+template <int R>
+__device__ __forceinline__ void prep_impl(const BoundArgs &a) {
+  typedef IA<double, R> A;
+  __shared__ iv<double> part[16];
+  const DevState *st = a.st;
+  if (st->status != 0) return;
+  const int b = blockIdx.x;
+  if (b >= st->n_split) return;
+  const long long p = st->first_parent + b;
+  const int S = a.n_samples, win = a.pwin[p];
+  const RowBox x{a.plo + p * GPUGO_D, a.phi + p * GPUGO_D};
+  iv<double> *out = (iv<double> *)(a.prest) + p * (long long)((1 + S) * GPUGO_NREST);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
+  for (int k = 0; k <= S; ++k) {
+    for (int s = 0; s < GPUGO_NREST; ++s) {
+      iv<double> v = k == 0 ? obj_rest<R>(s, x, a.params, a.d, win, a.c, tid, (int)blockDim.x)
+                            : obj_rest<R>(s, DiagPoint<RowBox>{x, k - 1, S}, a.params, a.d, win, a.c, tid,
+                                          (int)blockDim.x);
+      v = group_sum<R, 64>(v);
+      if (lane == 0) part[w] = v;
+      __syncthreads();
+      if (tid == 0) {
+        iv<double> t = part[0];
+        for (int q = 1; q < nw; ++q) t = A::add(t, part[q]);
+        out[k * GPUGO_NREST + s] = t;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Split+bound with windowed code lists: value test, then for survivors the
+// monotonicity test and the diagonal samples; survivors are appended to the
+// queue (one atomic per workgroup), the incumbent is lowered by one atomic
+// per workgroup (same protocol as bound_impl in bnb_kernel.hpp).
+template <int R>
+__device__ __forceinline__ void bound_w_impl(const BoundArgs &a) {
+  __shared__ unsigned s_wcnt[4];
+  __shared__ unsigned long long s_base;
+  __shared__ unsigned long long s_ubmin;
+  __shared__ unsigned s_pv, s_pk;
+  DevState *st = a.st;
+  if (st->status != 0) return;
+  const long long n_child = (long long)st->n_split * (long long)a.W;
+  if (n_child == 0) return;
+  const long long first = st->first_parent;
+  const double ub0 = st->ub_snap;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const double invW = 1.0 / (double)a.W, halfInvW = 0.5 / (double)a.W;
+  if (threadIdx.x == 0) {
+    s_ubmin = order_key(__builtin_huge_val());
+    s_pv = 0;
+    s_pk = 0;
+  }
+  __syncthreads();
+  for (long long base = (long long)blockIdx.x * 256; base < n_child; base += (long long)gridDim.x * 256) {
+    const long long t = base + threadIdx.x;
+    long long pl = 0;
+    unsigned pos = 0;
+    double lb = __builtin_huge_val(), su = __builtin_huge_val();
+    bool keep = false, pv = false, pk = false;
+    if (t < n_child) {
+      pl = (long long)__builtin_fma((double)t, invW, halfInvW);
+      pos = (unsigned)(t - pl * (long long)a.W);
+      const long long p = first + pl;
+      const int win = a.pwin[p];
+      const ChildBox x = child_box(a.plo + p * GPUGO_D, a.phi + p * GPUGO_D, a.pw + p * GPUGO_D, win, GPUGO_D, a.c,
+                                   (unsigned)a.wid, a.inv_wid, a.half_inv_wid, pos);
+      const int S = a.n_samples;
+      const iv<double> *rest = (const iv<double> *)(a.prest) + p * (long long)((1 + S) * GPUGO_NREST);
+      const iv<double> f = obj_value_w<R>(x, a.params, a.d, rest, win, a.c);
+      lb = f.lo;
+      if (!(lb == lb)) lb = -__builtin_huge_val();
+      if (lb <= ub0) {
+        keep = !obj_kkt_w<R>(x, a.params, a.dom_lo, a.dom_hi, a.d, rest, win, a.c);
+        pk = !keep;
+        // S points on the child's diagonal: window coordinates from the
+        // child, the others from the parent's same sample point (its rest)
+        for (int ts = 0; ts < S; ++ts) {
+          const DiagPoint<ChildBox> xs{x, ts, S};
+          const double u = obj_value_w<R>(xs, a.params, a.d, rest + (1 + ts) * GPUGO_NREST, win, a.c).hi;
+          su = u < su ? u : su;
+        }
+      } else {
+        pv = true;
+      }
+    }
+    const unsigned long long mask = __ballot(keep);
+    const unsigned pre =
+        __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+    if (lane == 0) s_wcnt[w] = (unsigned)__popcll(mask);
+    double m = su;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      double t2 = __shfl_xor(m, o, 64);
+      m = t2 < m ? t2 : m;
+    }
+    const unsigned long long pvm = __ballot(pv), pkm = __ballot(pk);
+    if (lane == 0) {
+      if (m < __builtin_huge_val()) atomicMin(&s_ubmin, order_key(m));
+      if (pvm) atomicAdd(&s_pv, (unsigned)__popcll(pvm));
+      if (pkm) atomicAdd(&s_pk, (unsigned)__popcll(pkm));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const unsigned tot = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
+      unsigned long long b = 0;
+      if (tot) {
+        b = atomicAdd((unsigned long long *)&st->q_count, (unsigned long long)tot);
+        if (b + tot > (unsigned long long)a.q_cap) atomicOr(&st->overflow, 1);
+      }
+      s_base = b;
+    }
+    __syncthreads();
+    if (keep) {
+      unsigned off = pre;
+      for (int q = 0; q < w; ++q) off += s_wcnt[q];
+      const unsigned long long o = s_base + off;
+      if (o < (unsigned long long)a.q_cap) {
+        a.q_lb[o] = lb;
+        a.q_par[o] = (unsigned)(first + pl);
+        a.q_pos[o] = pos;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (s_ubmin < *(volatile unsigned long long *)&st->ub_key) atomicMin(&st->ub_key, s_ubmin);
+    if (s_pv) atomicAdd((unsigned long long *)&st->pruned_value, (unsigned long long)s_pv);
+    if (s_pk) atomicAdd((unsigned long long *)&st->pruned_kkt, (unsigned long long)s_pk);
+    if (blockIdx.x == 0) {
+      st->iterations += 1;
+      st->bounded += n_child;
+    }
+  }
+}
+
+// Gradient sink writing component j of box i to explicit arrays.
+struct RowSink {
+  double *glo, *ghi;
+  struct Ref {
+    double *lo, *hi;
+    __device__ __forceinline__ void operator=(const iv<double> &g) const {
+      *lo = g.lo;
+      *hi = g.hi;
+    }
+  };
+  __device__ __forceinline__ Ref operator[](int j) const { return Ref{glo + j, ghi + j}; }
+};
+
+// Explicit boxes (parity hook): the loop-form code lists over all d
+// coordinates (one lane per box), the same functions the windowed lists
+// split into rest + window terms.
+template <int R>
+__device__ __forceinline__ void eval_w_impl(const EvalArgs &a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const RowBox x{a.lo + i * GPUGO_D, a.hi + i * GPUGO_D};
+  if (a.flo) {
+    const iv<double> f = obj_value_g<R, 1>(x, a.params, a.d, 0);
+    a.flo[i] = f.lo;
+    a.fhi[i] = f.hi;
+  }
+  if (a.glo) obj_grad_g<R, 1>(x, a.params, RowSink{a.glo + i * GPUGO_D, a.ghi + i * GPUGO_D}, a.d, 0);
+  if (a.kkt) a.kkt[i] = obj_kkt_g<R, 1>(x, a.params, a.dom_lo, a.dom_hi, a.d, 0) ? 1 : 0;
+  if (a.ub) {
+    double best = __builtin_huge_val();
+    for (int t = 0; t < a.n_samples; ++t) {
+      const DiagPoint<RowBox> xs{x, t, a.n_samples};
+      const iv<double> f = obj_value_g<R, 1>(xs, a.params, a.d, 0);
+      if (f.hi < best) best = f.hi;
+    }
+    a.ub[i] = best;
+  }
+}
+
+}  // namespace gpugo
+
+#if GPUGO_WITH_EXACT
+extern "C" __global__ __launch_bounds__(256) void gpugo_bound_exact(gpugo::BoundArgs a) {
+  gpugo::bound_w_impl<gpugo::ROUND_EXACT>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void gpugo_prep_exact(gpugo::BoundArgs a) {
+  gpugo::prep_impl<gpugo::ROUND_EXACT>(a);
+}
+extern "C" __global__ __launch_bounds__(64) void gpugo_eval_exact(gpugo::EvalArgs a) {
+  gpugo::eval_w_impl<gpugo::ROUND_EXACT>(a);
+}
+#endif
+#if GPUGO_WITH_FAST
+extern "C" __global__ __launch_bounds__(256) void gpugo_bound_fast(gpugo::BoundArgs a) {
+  gpugo::bound_w_impl<gpugo::ROUND_FAST>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void gpugo_prep_fast(gpugo::BoundArgs a) {
+  gpugo::prep_impl<gpugo::ROUND_FAST>(a);
+}
+extern "C" __global__ __launch_bounds__(64) void gpugo_eval_fast(gpugo::EvalArgs a) {
+  gpugo::eval_w_impl<gpugo::ROUND_FAST>(a);
+}
+#endif
+extern "C" __global__ void gpugo_param_signs(signed char *out) {
+  const signed char s[] = {GPUGO_PSIGN};
+  for (int i = threadIdx.x; i < GPUGO_NPARAMS; i += blockDim.x) out[i] = s[i];
+}
+// out: [D, NPARAMS, ABI, rounding variants, f32, split G, rest slots (windowed)]
+extern "C" __global__ void gpugo_info(int *out) {
+  out[0] = GPUGO_D;
+  out[1] = GPUGO_NPARAMS;
+  out[2] = GPUGO_MODULE_ABI_V;
+  out[3] = (GPUGO_WITH_EXACT ? 1 : 0) | (GPUGO_WITH_FAST ? 2 : 0);
+  out[4] = 0;
+  out[5] = 0;
+  out[6] = GPUGO_NREST;
+}

@@ -12,6 +12,7 @@
 //   module kernel    sampling + ruleOutGPU (1-Ackley/ackley50.py:602-766)
 //   k_compact        constructFlagArr + mask compaction (:768-810, :1217-1242)
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <hip/hiprtc.h>
 
 #include <algorithm>
@@ -206,6 +207,9 @@ struct gpugo_module_s {
   hipFunction_t select_fn = nullptr, compact_fn = nullptr, root_fn = nullptr;
   hipFunction_t persist[2] = {nullptr, nullptr};     // persistent step loops (split modules)
   hipFunction_t persist_mv[2] = {nullptr, nullptr};
+  // windowed modules (large d): rest-sum kernel per rounding, rest slots
+  hipFunction_t prep[2] = {nullptr, nullptr};
+  int nrest = 0;
   int dim = 0, nparams = 0, abi = 0;
   bool f32 = false;                // has the FP32 value code list (precision F32)
   std::vector<signed char> psign;  // declared coefficient signs
@@ -218,7 +222,7 @@ struct gpugo_module_s {
   std::vector<signed char> fam_psign;  // [nfam][pmax]
 };
 
-#define GPUGO_MODULE_ABI 2
+#define GPUGO_MODULE_ABI GPUGO_MODULE_ABI_V
 
 static int batch_module_finish(gpugo_module_t m, hipFunction_t info) {
   m->batch = true;
@@ -275,6 +279,7 @@ static int module_finish(gpugo_module_t m) {
   m->abi = h[2];
   m->f32 = h[4] != 0;
   m->split_g = h[5];
+  m->nrest = h[6];
   if (hipModuleGetFunction(&m->select_fn, m->mod, "gpugo_select") != hipSuccess ||
       hipModuleGetFunction(&m->compact_fn, m->mod, "gpugo_compact") != hipSuccess ||
       hipModuleGetFunction(&m->root_fn, m->mod, "gpugo_root") != hipSuccess) {
@@ -305,15 +310,22 @@ static int module_finish(gpugo_module_t m) {
   if (m->f32 && (h[3] & 2)) HIPCHECK(hipModuleGetFunction(&m->eval32[1], m->mod, "gpugo_eval32_fast"));
   if (m->abi != GPUGO_MODULE_ABI)
     return set_err(GPUGO_E_INVALID, "module ABI %d, library expects %d", m->abi, GPUGO_MODULE_ABI);
-  if (h[3] & 1) {
-    HIPCHECK(hipModuleGetFunction(&m->bound[0], m->mod, "gpugo_bound_exact"));
-    HIPCHECK(hipModuleGetFunction(&m->bound_mv[0], m->mod, "gpugo_bound_exact_mv"));
-    HIPCHECK(hipModuleGetFunction(&m->eval[0], m->mod, "gpugo_eval_exact"));
-  }
-  if (h[3] & 2) {
-    HIPCHECK(hipModuleGetFunction(&m->bound[1], m->mod, "gpugo_bound_fast"));
-    HIPCHECK(hipModuleGetFunction(&m->bound_mv[1], m->mod, "gpugo_bound_fast_mv"));
-    HIPCHECK(hipModuleGetFunction(&m->eval[1], m->mod, "gpugo_eval_fast"));
+  const char *rname[2] = {"exact", "fast"};
+  for (int r = 0; r < 2; ++r) {
+    if (!(h[3] & (1 << r))) continue;
+    char nm[64];
+    snprintf(nm, sizeof nm, "gpugo_bound_%s", rname[r]);
+    HIPCHECK(hipModuleGetFunction(&m->bound[r], m->mod, nm));
+    snprintf(nm, sizeof nm, "gpugo_eval_%s", rname[r]);
+    HIPCHECK(hipModuleGetFunction(&m->eval[r], m->mod, nm));
+    if (m->nrest > 0) {
+      // windowed module: rest sums per parent record, no mean-value variant
+      snprintf(nm, sizeof nm, "gpugo_prep_%s", rname[r]);
+      HIPCHECK(hipModuleGetFunction(&m->prep[r], m->mod, nm));
+    } else {
+      snprintf(nm, sizeof nm, "gpugo_bound_%s_mv", rname[r]);
+      HIPCHECK(hipModuleGetFunction(&m->bound_mv[r], m->mod, nm));
+    }
   }
   // declared coefficient signs
   m->psign.assign(m->nparams, 0);
@@ -431,6 +443,26 @@ extern "C" void gpugo_module_destroy(gpugo_module_t m) {
   if (!m) return;
   if (m->mod) (void)hipModuleUnload(m->mod);
   delete m;
+}
+
+extern "C" int gpugo_window_ranges(int64_t n, const int32_t *args, int32_t *out) {
+  if (n < 0 || (n > 0 && (!args || !out))) return set_err(GPUGO_E_INVALID, "null pointer or negative n");
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t *a = args + 6 * i;
+    const int win = a[0], c = a[1], d = a[2], klo = a[3], khi = a[4], nt = a[5];
+    if (d < 1 || c < 1 || c > d || win < 0 || win >= d || klo > khi || nt < 0)
+      return set_err(GPUGO_E_INVALID, "row %lld: need 0 <= win < d, 1 <= c <= d, klo <= khi, nt >= 0",
+                     (long long)i);
+    int lo[2] = {0, 0}, hi[2] = {0, 0};
+    const int r = win_ranges(win, c, d, klo, khi, nt, lo, hi);
+    int32_t *o = out + 5 * i;
+    o[0] = r;
+    o[1] = lo[0];
+    o[2] = hi[0];
+    o[3] = lo[1];
+    o[4] = hi[1];
+  }
+  return GPUGO_OK;
 }
 
 static int check_module_problem(gpugo_module_t m, int32_t d, const double *params, int64_t n_params) {
@@ -620,7 +652,9 @@ __global__ __launch_bounds__(SEL_T) void kb_select(BatchArgs b) {
   sa.tol_x = 0.0;
   sa.max_iters = I.max_iters;
   sa.pstride = b.dmax;
-  sa.pad0 = 0;
+  sa.ext_n = 0;
+  sa.pext = nullptr;
+  sa.tie_xor = 0u;
   select_body(sa);
 }
 
@@ -837,6 +871,8 @@ struct gpugo_problem_s {
   long long q_cap = 0, p_cap = 0, out_cap = 0;
   DevBuf d_lo, d_hi, d_params, d_params32, plo, phi, pw, pwin, out_lo, out_hi, out_lb, st, scratch;
   DevBuf bar;          // persistent step loop: grid barrier [2] and error flag
+  DevBuf pext;         // windowed modules: rest sums per parent record [p_cap][ext_n]
+  int ext_n = 0;
   int persist_blocks = 0;
   Queue q[2];
   hipStream_t stream = nullptr;
@@ -911,6 +947,12 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
     return set_err(GPUGO_E_INVALID, "unknown rounding %d", desc->rounding);
   if (!m->bound[desc->rounding])
     return set_err(GPUGO_E_INVALID, "module has no %s variant", desc->rounding ? "fast" : "exact");
+  if (m->nrest > 0) {
+    // windowed (large-d) module: natural-extension bound, midpoint sample
+    if (desc->flags & (GPUGO_FLAG_MEAN_VALUE | GPUGO_FLAG_SPLIT | GPUGO_FLAG_PERSIST))
+      return set_err(GPUGO_E_INVALID, "windowed modules take no mean-value, split or persistent flags");
+    if (desc->precision != GPUGO_PREC_F64) return set_err(GPUGO_E_INVALID, "windowed modules are FP64 only");
+  }
   if (std::isnan(desc->tol_f) || std::isnan(desc->tol_x) || std::isnan(desc->init_ub))
     return set_err(GPUGO_E_DOMAIN, "NaN tolerance or init_ub");
   if (!(desc->tol_f > 0) && !(desc->tol_x > 0)) return set_err(GPUGO_E_INVALID, "set tol_f or tol_x");
@@ -930,10 +972,15 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
   }
   long long qc = desc->max_queue > 0 ? desc->max_queue : (1LL << 24);
   if (qc > 0xfffffff0LL) qc = 0xfffffff0LL;
-  long long pc = desc->max_parents > 0 ? desc->max_parents : (1LL << 20);
+  // windowed modules: (1 + n_samples) * nrest rest-sum intervals per record
+  p->ext_n = m->nrest > 0 ? 2 * m->nrest * (1 + desc->n_samples) : 0;
+  // default arenas: parent records up to 2^20 or 8 GiB, output boxes up to
+  // 2^16 or 2^24 coordinates
+  const long long rec_bytes = 24LL * d + 4 + 8LL * p->ext_n;
+  long long pc = desc->max_parents > 0 ? desc->max_parents : std::min(1LL << 20, (8LL << 30) / rec_bytes);
   p->q_cap = qc;
   p->p_cap = pc;
-  p->out_cap = 1 << 16;
+  p->out_cap = std::max(64LL, std::min(1LL << 16, (1LL << 24) / d));
   if (p->desc.max_iters <= 0) p->desc.max_iters = 1LL << 40;
   auto fail = [&](int code) {
     delete p;
@@ -958,6 +1005,7 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
   PALLOC(p->st, sizeof(DevState));
   PALLOC(p->scratch, 64);
   PALLOC(p->bar, 64);
+  if (p->ext_n > 0) PALLOC(p->pext, sizeof(double) * p->ext_n * pc);
   for (int b = 0; b < 2; ++b) {
     PALLOC(p->q[b].lb, sizeof(double) * qc);
     PALLOC(p->q[b].par, sizeof(unsigned) * qc);
@@ -1106,7 +1154,9 @@ static int enqueue_step(gpugo_problem_t p) {
     sa.tol_x = p->desc.tol_x;
     sa.max_iters = 0;
     sa.pstride = d;
-    sa.pad0 = 0;
+    sa.ext_n = p->ext_n;
+    sa.pext = p->pext.as<double>();
+    sa.tie_xor = p->m->nrest > 0 ? ~0u : 0u;  // windowed (large-d): newest parent first among ties
     if (p->m->select_fn) {
       void *sargs[] = {&sa};
       HIPCHECK(hipModuleLaunchKernel(p->m->select_fn, 1, 1, 1, SEL_T, 1, 1, 0, p->stream, sargs, nullptr));
@@ -1138,6 +1188,14 @@ static int enqueue_step(gpugo_problem_t p) {
   ba.half_inv_wid = 0.5 / (double)p->desc.wid;
   ba.flags = p->desc.flags | (p->desc.precision == GPUGO_PREC_F32 ? GPUGO_FLAG_F32_PRUNE : 0);
   ba.params32 = p->d_params32.p;
+  ba.prest = p->pext.as<double>();
+  ba.nrest = p->m->nrest;
+  if (p->m->nrest > 0) {
+    // rest sums of the parent records this step splits (one workgroup each)
+    const unsigned nb = p->steps_enqueued == 0 ? 1u : (unsigned)p->desc.k_select;
+    void *args[] = {&ba};
+    HIPCHECK(hipModuleLaunchKernel(p->m->prep[p->desc.rounding], nb, 1, 1, 256, 1, 1, 0, p->stream, args, nullptr));
+  }
   {
     const bool split = (p->desc.flags & GPUGO_FLAG_SPLIT) != 0;
     long long maxk = p->steps_enqueued == 0 ? 1 : p->desc.k_select;
@@ -1145,19 +1203,23 @@ static int enqueue_step(gpugo_problem_t p) {
     long long blocks = (nthreads + 255) / 256;
     if (blocks > 16384) blocks = 16384;
     void *args[] = {&ba};
+    const bool mv = (p->desc.flags & GPUGO_FLAG_MEAN_VALUE) != 0;
+    hipFunction_t fn = split ? (mv ? p->m->bound_mv_g[p->desc.rounding] : p->m->bound_g[p->desc.rounding])
+                             : (mv ? p->m->bound_mv[p->desc.rounding] : p->m->bound[p->desc.rounding]);
     if (p->timing) {
+      // events taken from the kernel's own dispatch (start and end of its
+      // execution), not recorded around it: the launch gap is excluded
       while (p->tev.size() < p->tev_used + 2) {
         hipEvent_t ev;
         HIPCHECK(hipEventCreate(&ev));
         p->tev.push_back(ev);
       }
-      HIPCHECK(hipEventRecord(p->tev[p->tev_used++], p->stream));
+      hipEvent_t e0 = p->tev[p->tev_used++], e1 = p->tev[p->tev_used++];
+      HIPCHECK(hipExtModuleLaunchKernel(fn, (uint32_t)(blocks * 256), 1, 1, 256, 1, 1, 0, p->stream, args, nullptr,
+                                        e0, e1, 0));
+    } else {
+      HIPCHECK(hipModuleLaunchKernel(fn, (unsigned)blocks, 1, 1, 256, 1, 1, 0, p->stream, args, nullptr));
     }
-    const bool mv = (p->desc.flags & GPUGO_FLAG_MEAN_VALUE) != 0;
-    hipFunction_t fn = split ? (mv ? p->m->bound_mv_g[p->desc.rounding] : p->m->bound_g[p->desc.rounding])
-                             : (mv ? p->m->bound_mv[p->desc.rounding] : p->m->bound[p->desc.rounding]);
-    HIPCHECK(hipModuleLaunchKernel(fn, (unsigned)blocks, 1, 1, 256, 1, 1, 0, p->stream, args, nullptr));
-    if (p->timing) HIPCHECK(hipEventRecord(p->tev[p->tev_used++], p->stream));
   }
   CompactArgs ca;
   ca.st = st;
@@ -1280,6 +1342,9 @@ static int persist_steps(gpugo_problem_t p, int64_t iters) {
   sa.tol_x = p->desc.tol_x;
   sa.max_iters = 0;
   sa.pstride = d;
+  sa.ext_n = 0;
+  sa.pext = nullptr;
+  sa.tie_xor = 0u;
   for (int b = 0; b < 2; ++b) {
     pa.q_lb[b] = p->q[b].plb();
     pa.q_par[b] = p->q[b].ppar();

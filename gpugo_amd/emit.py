@@ -228,7 +228,7 @@ class _FuncG(_Func):
         if op == "tsum":
             return self.loop_sum(e)
         if op == "twhen":
-            cond = {"j+1<d": "j + 1 < d", "j>0": "j > 0"}[e.val]
+            cond = {"j+1<d": "j + 1 < d", "j>0": "j > 0", "j==0": "j == 0", "j+1==d": "j + 1 == d"}[e.val]
             name = f"t{self.n}"
             self.n += 1
             self.lines.append(f"  iv<{self.ty}> {name} = mk<{self.ty}>(0, 0);")
@@ -316,6 +316,201 @@ def split_lists(obj: Objective) -> str:
     out.append("}")
     out.append("}  // namespace gpugo")
     return "\n".join(out) + "\n"
+
+
+# ---------------------------------------------------------------------------
+# windowed code lists (large d): a child differs from its parent record only
+# in the c coordinates of the parent's split window, so each template sum is
+# (rest over the parent) + (terms that touch the window, over the child)
+# ---------------------------------------------------------------------------
+def _walk(e: Expr, fn, seen=None):
+    if seen is None:
+        seen = set()
+    if id(e) in seen:
+        return
+    seen.add(id(e))
+    fn(e)
+    for a in e.args:
+        _walk(a, fn, seen)
+
+
+def _tsums(*exprs) -> List[Expr]:
+    """Distinct template sums of the expressions, in order of appearance."""
+    out, keys = [], set()
+
+    def fn(e):
+        if e.op == "tsum" and e.key() not in keys:
+            keys.add(e.key())
+            out.append(e)
+    for e in exprs:
+        _walk(wrap(e), fn)
+    return out
+
+
+def _x_offsets(body: Expr, what: str) -> Tuple[int, int]:
+    """[min, max] of b over the coordinates X(J + b) of a loop body; the
+    windowed form needs unit stride.  No coordinate: (0, 0)."""
+    offs = []
+
+    def fn(e):
+        if e.op == "var":
+            if not isinstance(e.val, tuple):
+                raise ValueError(f"{what}: fixed coordinate X[{e.val}] inside a loop body is not windowable")
+            if e.val[1] != 1:
+                raise ValueError(f"{what}: coordinate index {e.val[1]}*j{e.val[2]:+d} is not windowable")
+            offs.append(e.val[2])
+    _walk(body, fn)
+    return (min(offs), max(offs)) if offs else (0, 0)
+
+
+def _check_local_body(body: Expr, what: str):
+    """A loop body may not use coordinate-dependent values computed outside
+    its own term (they would differ between a parent and its children)."""
+    def fn(e):
+        if e.op == "tsum":
+            raise ValueError(f"{what}: nested template sums are not windowable")
+        if not has_loop_index(e) and e.depends_on_x():
+            raise ValueError(f"{what}: loop-invariant coordinate term {e!r} is not windowable")
+    _walk(body, fn)
+
+
+class _FuncW(_FuncG):
+    """Windowed lowering: a TSum becomes rest[s] (its terms that do not
+    touch the window, enclosed over the parent record once per parent) plus
+    the terms j in the window's ranges, evaluated over the child."""
+
+    def __init__(self, slots: Dict[tuple, int]):
+        super().__init__()
+        self.slots = slots
+
+    def loop_sum(self, e: Expr) -> Tuple[str, str]:
+        s = self.slots[e.key()]
+        acc = f"t{self.n}"
+        self.n += 1
+        self.lines.append(f"  iv<double> {acc} = rest[{s}];")
+        self.lines.append(f"  {{ int rlo[2], rhi[2]; const int nr = wranges_{s}(win, c, d, rlo, rhi);")
+        self.lines.append("    for (int r = 0; r < nr; ++r)")
+        self.lines.append(f"      for (int j = rlo[r]; j < rhi[r]; ++j) {acc} = A::add({acc}, wterm_{s}<R>(x, P, d, j)); }}")
+        return ("i", acc)
+
+
+def window_lists(obj: Objective) -> str:
+    """Windowed code lists from the objective's loop templates:
+      wterm_s<R>(x, P, d, j)   term j of template sum s
+      wranges_s(win, c, d, lo, hi)  its terms that touch the window
+      obj_rest<R>(s, x, P, d, win, c, t, nt)  thread t of nt: partial sum
+          of sum s's other terms over x (the parent box or one of its
+          diagonal sample points)
+      obj_value_w<R>(x, P, d, rest, win, c)   value over a child
+      obj_kkt_w<R>(x, P, dlo, dhi, d, rest, win, c)  monotonicity test
+    The monotonicity test covers the gradient components the window can
+    change: all of them if the gradient template uses a template sum,
+    else those whose coordinates meet the window (the others equal the
+    parent's, which did not prune it)."""
+    t = obj.templates
+    if not t or "value" not in t or "grad" not in t:
+        raise ValueError(f"objective {obj.name} has no loop templates")
+    value, grad = wrap(t["value"]), wrap(t["grad"])
+    sums = _tsums(value, grad)
+    slots = {e.key(): i for i, e in enumerate(sums)}
+    S = len(sums)
+    signs = param_signs(obj)
+    out = [f"#define GPUGO_D {obj.d}", f"#define GPUGO_NPARAMS {obj.n_params}",
+           "#define GPUGO_PSIGN " + (", ".join(str(v) for v in signs) if signs else "0"),
+           f"#define GPUGO_NREST {max(S, 1)}", "namespace gpugo {"]
+    for s, e in enumerate(sums):
+        body = e.args[0]
+        _check_local_body(body, f"template sum {s}")
+        klo, khi = _x_offsets(body, f"template sum {s}")
+        f = _FuncG()
+        f.in_loop = True
+        code = f.as_iv(f.lower(body))
+        out.append(f"template <int R, class XS>\n__device__ __forceinline__ iv<double> wterm_{s}("
+                   "const XS &x, const double *__restrict__ P, int d, int j) {")
+        out.append("  typedef IA<double, R> A;")
+        out += f.lines
+        out.append(f"  return {code};")
+        out.append("}")
+        nt = "d" if e.val == 0 else (f"d + {e.val}" if e.val > 0 else f"d - {-e.val}")
+        out.append(f"__device__ __forceinline__ int wranges_{s}(int win, int c, int d, int *lo, int *hi) {{\n"
+                   f"  return win_ranges(win, c, d, {klo}, {khi}, {nt}, lo, hi);\n}}")
+    # rest sums of a parent record: slot s over an accessor (the box, or
+    # one of its diagonal sample points)
+    out.append("template <int R, class XS>\n__device__ __forceinline__ iv<double> obj_rest("
+               "int s, const XS &x, const double *__restrict__ P, int d, int win, int c, int t, int nt) {")
+    out.append("  typedef IA<double, R> A;")
+    out.append("  iv<double> acc = mk<double>(0.0, 0.0);")
+    out.append("  int rlo[2], rhi[2];")
+    out.append("  switch (s) {")
+    for s_, e in enumerate(sums):
+        nt = "d" if e.val == 0 else (f"d + {e.val}" if e.val > 0 else f"d - {-e.val}")
+        out.append(f"  case {s_}: {{ const int nr = wranges_{s_}(win, c, d, rlo, rhi);")
+        out.append(f"    for (int j = t; j < {nt}; j += nt)")
+        out.append(f"      if (!in_ranges(j, nr, rlo, rhi)) acc = A::add(acc, wterm_{s_}<R>(x, P, d, j));")
+        out.append("    break; }")
+    out.append("  default: break;")
+    out.append("  }")
+    out.append("  return acc;")
+    out.append("}")
+    # value over a child
+    f = _FuncW(slots)
+    v = f.lower(value)
+    out.append("template <int R, class XS>\n__device__ __forceinline__ iv<double> obj_value_w("
+               "const XS &x, const double *__restrict__ P, int d, const iv<double> *__restrict__ rest, "
+               "int win, int c) {")
+    out.append("  typedef IA<double, R> A;")
+    out += f.lines
+    out.append(f"  return {f.as_iv(v)};")
+    out.append("}")
+    # monotonicity test
+    f = _FuncW(slots)
+    global_grad = bool(_tsums(grad))
+    f.hoist(grad)
+    if global_grad:
+        f.lines.append("  for (int j = 0; j < d; ++j) {")
+    else:
+        gklo, gkhi = _x_offsets(grad, "gradient template")
+        f.lines.append(f"  int glo[2], ghi[2]; const int ng = win_ranges(win, c, d, {gklo}, {gkhi}, d, glo, ghi);")
+        f.lines.append("  for (int r = 0; r < ng; ++r)")
+        f.lines.append("  for (int j = glo[r]; j < ghi[r]; ++j) {")
+    saved = dict(f.memo)
+    f.in_loop = True
+    code = f.as_iv(f.lower(grad))
+    f.in_loop = False
+    f.memo = saved
+    f.lines += [f"    const iv<double> gj = {code};",
+                "    if (gj.lo > 0.0 && x[j].lo != dlo[j]) return true;",
+                "    if (gj.hi < 0.0 && x[j].hi != dhi[j]) return true;",
+                "  }"]
+    out.append("template <int R, class XS>\n__device__ __forceinline__ bool obj_kkt_w("
+               "const XS &x, const double *__restrict__ P, const double *__restrict__ dlo, "
+               "const double *__restrict__ dhi, int d, const iv<double> *__restrict__ rest, int win, int c) {")
+    out.append("  typedef IA<double, R> A;")
+    out += f.lines
+    out.append("  return false;")
+    out.append("}")
+    out.append("}  // namespace gpugo")
+    return "\n".join(out) + "\n"
+
+
+def window_module_source(obj: Objective, roundings=("fast",)) -> str:
+    """hipRTC source of a windowed (large-d) module: loop-form code lists
+    only (no per-coordinate unrolling, so the code does not grow with d),
+    the windowed split+bound kernel and the per-parent rest-sum kernel."""
+    bad = set(roundings) - {"exact", "fast"}
+    if bad or not roundings:
+        raise ValueError(f"roundings must be a non-empty subset of ('exact', 'fast'), got {roundings}")
+    return "\n".join([
+        "// generated by gpugo_amd.emit (windowed form) for objective " + obj.name,
+        f"#define GPUGO_WITH_EXACT {int('exact' in roundings)}",
+        f"#define GPUGO_WITH_FAST {int('fast' in roundings)}",
+        "#define GPUGO_WINDOWED 1",
+        _read("ia.hpp"), _read("bnb_args.hpp"), _read("bnb_box.hpp"),
+        window_lists(obj),
+        split_lists(obj),
+        "#define GPUGO_MODULE_STEP 1", _read("bnb_step.hpp"),
+        _read("bnb_window.hpp"),
+    ])
 
 
 # ---------------------------------------------------------------------------
@@ -425,9 +620,14 @@ def param_signs(obj: Objective) -> List[int]:
             return
         seen.add(k)
         if e.op == "param" and e.val2:
-            if signs[e.val] not in (0, e.val2):
-                raise ValueError(f"P[{e.val}] declared with both signs")
-            signs[e.val] = e.val2
+            # P(a*J + b) inside a loop template: the indices of j in [0, d)
+            idxs = ([e.val[1] * j + e.val[2] for j in range(obj.d)] if isinstance(e.val, tuple) else [e.val])
+            for i in idxs:
+                if not 0 <= i < obj.n_params:
+                    continue
+                if signs[i] not in (0, e.val2):
+                    raise ValueError(f"P[{i}] declared with both signs")
+                signs[i] = e.val2
         for a in e.args:
             walk(a)
 
@@ -534,7 +734,10 @@ _FLOP = {"add": 2, "sub": 2, "neg": 0, "scale": 2, "addc": 2, "rsubc": 2, "divc"
          "mul": 8, "div": 10, "sqrt": 2, "exp": 40, "sin": 40, "cos": 40}
 
 
-def _count(exprs) -> int:
+def _count(exprs, tsum_leaf: bool = False) -> int:
+    """FLOP of the expressions (each distinct subexpression once); with
+    tsum_leaf, template sums count as leaves (their terms are counted by the
+    caller)."""
     seen = set()
     total = 0
 
@@ -544,10 +747,12 @@ def _count(exprs) -> int:
         if k in seen:
             return
         seen.add(k)
+        if tsum_leaf and e.op == "tsum":
+            return
         for a in e.args:
             walk(a)
         op = e.op
-        if op in ("var", "const", "param", "iconst", "guard") or e.is_scalar():
+        if op in ("var", "const", "param", "iconst", "guard", "twhen", "dim") or e.is_scalar():
             return
         if op == "gsum":
             total += _FLOP["add"] * (len(e.args) - 1)
@@ -577,6 +782,33 @@ def flop_counts(obj: Objective) -> dict:
     sample evaluation (the value code list on a point)."""
     v = _count([obj.value])
     g = _count(gradient(obj))
+    return {"value": v, "grad": g, "sample": v}
+
+
+def window_flop_counts(obj: Objective, c: int) -> dict:
+    """Algorithmic FLOP per child of the windowed code lists at a nominal
+    window (c coordinates away from the ends): value = the non-sum part plus,
+    per template sum, its terms that touch the window (c + stencil span) and
+    their additions; grad = the monotonicity test over the components the
+    window changes (c + span, or all d for a gradient that uses a template
+    sum); sample = the value code list at one point.  The per-parent rest
+    sums (O(d) once per parent record) are not included."""
+    t = obj.templates
+    value, grad = wrap(t["value"]), wrap(t["grad"])
+
+    def sums_cost(e):
+        total = 0
+        for ts in _tsums(e):
+            klo, khi = _x_offsets(ts.args[0], "template sum")
+            total += min(c + khi - klo, obj.d + ts.val) * (_count([ts.args[0]]) + _FLOP["add"])
+        return total
+    v = _count([value], tsum_leaf=True) + sums_cost(value)
+    if _tsums(grad):
+        ncomp, gsum = obj.d, sums_cost(grad)
+    else:
+        gklo, gkhi = _x_offsets(grad, "gradient template")
+        ncomp, gsum = min(c + gkhi - gklo, obj.d), 0
+    g = gsum + ncomp * _count([grad], tsum_leaf=True)
     return {"value": v, "grad": g, "sample": v}
 
 

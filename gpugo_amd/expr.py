@@ -174,10 +174,13 @@ def TSum(body, off: int = 0) -> Expr:
     return Expr("tsum", (wrap(body),), int(off))
 
 
+TWHEN_CONDS = ("j+1<d", "j>0", "j==0", "j+1==d")
+
+
 def TWhen(cond: str, body) -> Expr:
     """Inside a template: `body` where the condition on j holds, else 0.
-    cond is "j+1<d" or "j>0"."""
-    if cond not in ("j+1<d", "j>0"):
+    cond is one of TWHEN_CONDS ("j+1<d", "j>0", "j==0", "j+1==d")."""
+    if cond not in TWHEN_CONDS:
         raise ValueError(f"unknown template condition {cond!r}")
     return Expr("twhen", (wrap(body),), cond)
 

@@ -19,7 +19,7 @@ from typing import Callable, Dict, List
 
 import numpy as np
 
-from .expr import E, PI, Const, Objective, X, cos, exp, fsum, sin, sqr, sqrt
+from .expr import E, PI, Const, J, Objective, TSum, TWhen, X, cos, exp, fsum, sin, sqr, sqrt
 from .problems import ProblemSpec
 
 
@@ -111,6 +111,76 @@ def _rosenbrock(d):
     return fsum(sqr(X(i + 1) - sqr(X(i))) * 100.0 + sqr(X(i) - 1.0) for i in range(d - 1))
 
 
+# Loop templates (value with template sums, one gradient component in J) of
+# the families whose terms are sums: the same functions as the unrolled
+# code lists above (checked in 256-bit arithmetic by the tests), used by
+# the term-parallel and the windowed (large-d) code lists.  Gradients are
+# derived by hand from the definitions.
+def _t_ackley(d):
+    s1 = TSum(sqr(X(J)))
+    s2 = TSum(cos(X(J) * (PI * 2.0)))
+    r = sqrt(s1 / float(d))
+    t1 = exp(r * -0.02)
+    t2 = exp(s2 / float(d))
+    value = ((t1 * -20.0 - t2) + 20.0) + E
+    k1 = (((t1 / r) * 0.02) * 20.0) / float(d)
+    k2 = (t2 * (PI * 2.0)) / float(d)
+    return {"value": value, "grad": k1 * X(J) + k2 * sin(X(J) * (PI * 2.0))}
+
+
+def _t_belegundu(d):
+    s = TSum(sqr(X(J) - 5.0))
+    r = sqrt(s)
+    return {"value": s * 0.1 - cos(r * 5.0),
+            "grad": (X(J) - 5.0) * 0.2 + ((sin(r * 5.0) * 5.0) / r) * (X(J) - 5.0)}
+
+
+def _t_breiman(d):
+    return {"value": TSum(cos(X(J) * (PI * 5.0))) * -0.1 + TSum(sqr(X(J))),
+            "grad": (sin(X(J) * (PI * 5.0)) * (PI * 5.0)) * 0.1 + X(J) * 2.0}
+
+
+def _t_fu(d):
+    t = sqr(X(J) - 0.9)
+    body = (sqr(sin(t * 7.0)) * 8.0 + sqr(sin(t * 14.0)) * 6.0) + t
+    return {"value": TSum(body) + 1.0,
+            "grad": ((X(J) - 0.9) * 2.0) * ((sin(t * 14.0) * 56.0 + sin(t * 28.0) * 84.0) + 1.0)}
+
+
+def _t_levy(d):
+    q, t3 = PI * 0.25, PI * 0.75
+    head = sqr(sin(X(0) * q + t3)) * 10.0
+    body = sqr((X(J) - 1.0) * 0.25) * (sqr(sin(X(J + 1) * q + t3)) * 10.0 + 1.0)
+    tail = sqr((X(d - 1) - 1.0) * 0.25)
+    # d/dx 10 sin^2(pi/4 x + 3pi/4) = (5pi/2) sin(pi/2 x + 3pi/2)
+    s2 = sin(X(J) * (PI * 0.5) + PI * 1.5) * (PI * 2.5)
+    g = ((TWhen("j==0", s2)
+          + TWhen("j+1<d", ((X(J) - 1.0) * 0.125) * (sqr(sin(X(J + 1) * q + t3)) * 10.0 + 1.0)))
+         + TWhen("j>0", sqr((X(J - 1) - 1.0) * 0.25) * s2)) + TWhen("j+1==d", (X(J) - 1.0) * 0.125)
+    return {"value": (PI / float(d)) * ((head + TSum(body, off=-1)) + tail), "grad": (PI / float(d)) * g}
+
+
+def _t_rastrigin(d):
+    return {"value": TSum(sqr(X(J)) - cos(X(J) * (PI * 2.0)) * 10.0) + 10.0 * d,
+            "grad": X(J) * 2.0 + sin(X(J) * (PI * 2.0)) * (PI * 20.0)}
+
+
+def _t_salomon(d):
+    r = sqrt(TSum(sqr(X(J))))
+    return {"value": (1.0 - cos(r * (PI * 2.0))) + r * 0.1,
+            "grad": ((sin(r * (PI * 2.0)) * (PI * 2.0) + 0.1) / r) * X(J)}
+
+
+def _t_rosenbrock(d):
+    return {"value": TSum(sqr(X(J + 1) - sqr(X(J))) * 100.0 + sqr(X(J) - 1.0), off=-1),
+            "grad": TWhen("j+1<d", (X(J) * (X(J + 1) - sqr(X(J)))) * -400.0 + (X(J) - 1.0) * 2.0)
+            + TWhen("j>0", (X(J) - sqr(X(J - 1))) * 200.0)}
+
+
+TEMPLATES = {"ackley": _t_ackley, "belegundu": _t_belegundu, "breiman": _t_breiman, "fu": _t_fu,
+             "levy": _t_levy, "rastrigin": _t_rastrigin, "salomon": _t_salomon, "rosenbrock": _t_rosenbrock}
+
+
 @dataclass
 class RefFamily:
     build: Callable[[int], "Expr"]
@@ -149,9 +219,21 @@ REFERENCE: Dict[str, RefFamily] = {
 
 
 def objective(name: str, d: int) -> Objective:
-    """The reference objective `name` in d dimensions (no coefficients)."""
+    """The reference objective `name` in d dimensions (no coefficients),
+    with its loop templates where the family has them."""
     fam = REFERENCE[name]
-    return Objective(d, 0, fam.build(d), None, name=f"ref_{name}_d{d}")
+    tpl = TEMPLATES[name](d) if name in TEMPLATES else None
+    return Objective(d, 0, fam.build(d), None, name=f"ref_{name}_d{d}", templates=tpl)
+
+
+def objective_loops(name: str, d: int) -> Objective:
+    """The reference objective `name` in loop form only (no unrolled code
+    lists; for the windowed large-d modules, whose size does not grow with d).
+    The unrolled value is not built: its `value` is the template."""
+    if name not in TEMPLATES:
+        raise ValueError(f"{name} has no loop templates (product families are not windowable)")
+    tpl = TEMPLATES[name](d)
+    return Objective(d, 0, tpl["value"], None, name=f"ref_{name}_loops_d{d}", templates=tpl)
 
 
 def spec(name: str, d: int, tol: float = 1e-6) -> ProblemSpec:

@@ -131,6 +131,13 @@ def objective(family: str, d: int) -> Objective:
     return _build(family, d, d > GUARDED_D)
 
 
+def objective_loops(family: str, d: int) -> Objective:
+    """A family in loop form only (its templates; no unrolled code lists),
+    for the windowed large-d modules."""
+    tpl = _templates(family, d)
+    return Objective(d, n_params(family, d), tpl["value"], None, name=f"{family}_loops_d{d}", templates=tpl)
+
+
 def objective_dynamic(family: str, dmax: int) -> Objective:
     """Variable-dimension form (runtime d <= dmax) used by batches."""
     return _build(family, dmax, True)

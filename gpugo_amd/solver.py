@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import GpugoError, check, f64, ptr
-from .emit import module_source, source_key
+from .emit import module_source, source_key, window_module_source
 from .expr import Objective
 
 _CACHE_DIR = os.environ.get("GPUGO_CACHE_DIR",
@@ -31,10 +31,20 @@ _modules = {}
 _mod_lock = threading.Lock()
 
 
+def _source(obj: Objective, roundings, f32: bool, split: int, windowed: bool) -> str:
+    if windowed:
+        if f32 or split:
+            raise ValueError("windowed modules have no FP32 or term-parallel kernels")
+        return window_module_source(obj, tuple(roundings))
+    return module_source(obj, tuple(roundings), f32, split)
+
+
 def rtc_compile_to_cache(obj: Objective, options: str = "", roundings=("fast",), f32: bool = False,
-                         split: int = 0) -> str:
-    """Compile an objective module with hipRTC (no GPU needed) into the cache."""
-    return compile_source_to_cache(module_source(obj, tuple(roundings), f32, split), obj.name, options)
+                         split: int = 0, windowed: bool = False) -> str:
+    """Compile an objective module with hipRTC (no GPU needed) into the cache.
+    windowed: the large-d form (loop templates only; a child costs the
+    terms its split window touches)."""
+    return compile_source_to_cache(_source(obj, roundings, f32, split, windowed), obj.name, options)
 
 
 def compile_source_to_cache(src: str, name: str, options: str = "") -> str:
@@ -61,14 +71,15 @@ class Module:
     """A compiled objective (one hipRTC code object loaded on the GPU)."""
 
     def __init__(self, obj: Objective, options: str = "", roundings=("fast",), f32: bool = False,
-                 split: int = 0):
+                 split: int = 0, windowed: bool = False):
         _lib.init()
         lib = _lib.load()
         self.obj = obj
         self.roundings = tuple(roundings)
         self.f32 = bool(f32)
         self.split = int(split)
-        path = rtc_compile_to_cache(obj, options, self.roundings, self.f32, self.split)
+        self.windowed = bool(windowed)
+        path = rtc_compile_to_cache(obj, options, self.roundings, self.f32, self.split, self.windowed)
         with open(path, "rb") as fh:
             code = fh.read()
         h = ctypes.c_void_p()
@@ -119,13 +130,13 @@ def eval_boxes_f32(module: "Module", params, lo, hi, rounding: str = "fast"):
 
 
 def get_module(obj: Objective, options: str = "", roundings=("fast",), f32: bool = False,
-               split: int = 0) -> Module:
+               split: int = 0, windowed: bool = False) -> Module:
     roundings = tuple(sorted(set(roundings)))
-    key = source_key(module_source(obj, roundings, f32, split), options)
+    key = source_key(_source(obj, roundings, f32, split, windowed), options)
     with _mod_lock:
         m = _modules.get(key)
         if m is None:
-            m = Module(obj, options, roundings, f32, split)
+            m = Module(obj, options, roundings, f32, split, windowed)
             _modules[key] = m
         return m
 
@@ -144,6 +155,17 @@ def ia_eval(op: str, alo, ahi, blo=None, bhi=None, s: float = 0.0, prec: str = "
     return olo, ohi
 
 
+def window_ranges(rows) -> np.ndarray:
+    """The windowed modules' rule (host copy of the device function): rows
+    of (win, c, d, klo, khi, nt) -> (n, lo0, hi0, lo1, hi1), the ranges of
+    terms j in [0, nt) whose coordinates j + k, k in [klo, khi], meet the
+    window {(win + t) mod d : t < c}.  Needs no GPU."""
+    a = np.ascontiguousarray(np.atleast_2d(rows), dtype=np.int32)
+    out = np.zeros((a.shape[0], 5), dtype=np.int32)
+    check(_lib.load().gpugo_window_ranges(a.shape[0], ptr(a, ctypes.c_int32), ptr(out, ctypes.c_int32)))
+    return out
+
+
 class Problem:
     """One device-resident branch-and-bound solve."""
 
@@ -152,17 +174,21 @@ class Problem:
                  init_ub: float = np.inf, rounding: str = "fast", max_queue: int = 0,
                  max_parents: int = 0, max_iters: int = 0, module: Optional[Module] = None,
                  mean_value: bool = False, precision: str = "fp64", split: int = 0,
-                 persistent: bool = False):
+                 persistent: bool = False, windowed: bool = False):
         """precision "fp32": children are first bounded in FP32 intervals
         (outward, one ulp per operation) and only those that survive get the
         FP64 value, gradient and sample; the enclosure is certified in FP64.
         split = G > 1: term-parallel kernels, G lanes share each child (for
         steps too small to fill the GPU; needs the objective's loop
         templates).  persistent (with split): the steps of a solve run in
-        one cooperative launch with grid barriers between their phases."""
+        one cooperative launch with grid barriers between their phases.
+        windowed: the large-d form (loop templates only; each template sum
+        is kept per parent record outside its split window, so a child
+        costs the terms its window touches; FP64, midpoint sample)."""
         if precision not in _lib.PREC:
             raise ValueError(f"precision must be one of {sorted(_lib.PREC)}")
-        self.module = module or get_module(obj, roundings=(rounding,), f32=precision == "fp32", split=split)
+        self.module = module or get_module(obj, roundings=(rounding,), f32=precision == "fp32", split=split,
+                                           windowed=windowed)
         self.obj = obj
         self._lo, self._hi, self._params = f64(lo), f64(hi), f64(params)
         wid, cycle = grid

@@ -141,6 +141,19 @@ int gpugo_eval_boxes_f32(gpugo_module_t m, int32_t d, const double *params, int6
                          int64_t n, const double *lo, const double *hi, int32_t rounding,
                          double *flo, double *fhi);
 
+/* Windowed (large-d) modules: built from an objective's loop templates, they
+ * keep each template sum per parent record outside the parent's split window
+ * (win + t) mod d, t < c, so a child costs the terms the window touches
+ * rather than all d (the reference evaluates all d coordinates per child,
+ * 6-Levy/levy10000.py:160-232); the rest sums are also kept at the parent's
+ * n_samples diagonal points for the incumbent samples.  Problems on such a
+ * module take no flags and FP64 precision.  This is the test hook of the window rule
+ * shared by the device code: for each row of args (win, c, d, klo, khi, nt:
+ * terms j in [0, nt) using coordinates j + k, k in [klo, khi]) it writes
+ * (n, lo0, hi0, lo1, hi1), the n <= 2 ranges [lo, hi) of terms whose
+ * coordinates meet the window.  Runs on the host; needs no GPU. */
+int gpugo_window_ranges(int64_t n, const int32_t *args, int32_t *out);
+
 /* gpugo_problem_desc.flags */
 #define GPUGO_FLAG_MEAN_VALUE 1  /* also bound each child by the mean-value form
                                     f(c) + sum_j G_j (X_j - c_j), c the midpoint,

@@ -92,3 +92,29 @@ def test_closed_form_minimum(name):
     rng = np.random.default_rng(0)
     vals = [f_ref(name, rng.uniform(fam.lo, fam.hi, d)) for _ in range(300)]
     assert min(vals) >= fam.f_star(d) - 1e-12
+
+
+@pytest.mark.parametrize("name", sorted(F.TEMPLATES))
+@pytest.mark.parametrize("d", [2, 3, 6])
+def test_loop_templates_match_formula(name, d):
+    """The loop templates (term-parallel and windowed code lists) are the
+    same function as the definition, and their gradient component j is
+    mpmath's derivative of the definition (256-bit)."""
+    tpl = F.TEMPLATES[name](d)
+    fam = F.REFERENCE[name]
+    rng = np.random.default_rng(10 + d)
+    for _ in range(4):
+        # away from the kinks where a sqrt of a sum of squares vanishes
+        xf = rng.uniform(fam.lo, fam.hi, d)
+        x = [mp.mpf(float(v)) for v in xf]
+        want = f_ref(name, x)
+        got = ir_eval(tpl["value"], x, [])
+        assert abs(got - want) <= mp.mpf("1e-60") * (1 + abs(want)), (name, xf)
+        for j in range(d):
+            def fj(s, j=j):
+                y = list(x)
+                y[j] = s
+                return f_ref(name, y)
+            dw = mp.diff(fj, x[j])
+            dg = ir_eval(tpl["grad"], x, [], {}, j)
+            assert abs(dg - dw) <= mp.mpf("1e-40") * (1 + abs(dw)), (name, j, xf)

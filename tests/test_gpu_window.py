@@ -1,0 +1,167 @@
+"""Windowed (large-d) modules on the GPU.
+
+A windowed module is built from an objective's loop templates only; each
+template sum is enclosed as (its terms outside the parent record's split
+window, summed once per parent over the parent box and at its diagonal
+sample points) + (the terms the window touches, over the child).  The
+reference evaluates every coordinate for every child
+(GPUGO_1.1/6-Levy/levy10000.py:160-380).  Bar:
+  - explicit boxes through the windowed module's loop-form code lists:
+    enclosures contain the 256-bit values and derivatives of the definitions
+    (tests/test_families.py), both roundings; for the rational Rosenbrock in
+    EXACT rounding the value enclosure is bit-identical to the unrolled
+    module's (the same left fold);
+  - solves at d = 3 with cycle 2 (windows wrap past d): certified, contain
+    the closed-form f*, and intersect the unrolled module's certified
+    enclosure under the same schedule, with 1 and 10 diagonal samples;
+  - the coefficient families at d = 6 (cycle 4, wrapping): certified
+    enclosures intersect the oracle's under the same schedule;
+  - two solves are identical (deterministic);
+  - with a parent arena small enough that records are collected (their rest
+    sums move with them) the solve equals the default arena's;
+  - d = 50 for every windowable reference family and d = 1000 for Levy,
+    Rosenbrock and Fu (the reference's levy1000.py etc. boxes and initial
+    upper bounds, 10 diagonal samples, 3^12 children per step, one box per
+    step): certified enclosures of the closed-form f*.
+"""
+import mpmath as mp
+import numpy as np
+import pytest
+
+from gpugo_amd import families as F, problems as pb
+from tests.test_families import f_ref
+
+pytestmark = pytest.mark.gpu
+WIN = sorted(F.TEMPLATES)
+RES_KEYS = ("lb", "ub", "width", "iterations", "boxes_bounded", "queue_size", "status")
+
+
+def wmod(G, obj):
+    return G.get_module(obj, roundings=("exact", "fast"), windowed=True)
+
+
+def ref_solve(G, name, d, grid, k, S, module, windowed, tol=1e-6, **kw):
+    s = F.spec(name, d, tol=tol)
+    obj = F.objective_loops(name, d) if windowed else F.objective(name, d)
+    print(f"# solve {name} d={d} grid={grid} K={k} S={S} windowed={windowed}", flush=True)
+    return G.minimize(obj, s.lo, s.hi, tol=tol, grid=grid, k_select=k, n_samples=S,
+                      init_ub=s.extra["init_ub"], module=module, rounding="fast", windowed=windowed, **kw), s
+
+
+@pytest.mark.parametrize("name", WIN)
+@pytest.mark.parametrize("rounding", ["exact", "fast"])
+def test_window_eval_contains_truth(gpu, name, rounding):
+    d = 3
+    fam = F.REFERENCE[name]
+    m = wmod(gpu, F.objective_loops(name, d))
+    rng = np.random.default_rng(21)
+    n = 32
+    span = fam.hi - fam.lo
+    lo = fam.lo + rng.uniform(0, 1, (n, d)) * span
+    w = rng.choice([1e-7, 1e-3, 0.05, 0.5], (n, 1)) * span
+    hi = np.minimum(lo + rng.uniform(0, 1, (n, d)) * w, fam.hi)
+    g = m.eval_boxes(np.zeros(0), lo, hi, np.full(d, fam.lo), np.full(d, fam.hi), 1, rounding=rounding)
+    mp.mp.prec = 256
+    for i in range(n):
+        for _ in range(3):
+            x = lo[i] + rng.uniform(0, 1, d) * (hi[i] - lo[i])
+            t = f_ref(name, x)
+            assert g["flo"][i] <= t <= g["fhi"][i], (name, i, g["flo"][i], t, g["fhi"][i])
+        if i % 4 == 0:
+            x = lo[i] + 0.5 * (hi[i] - lo[i])
+            for j in range(d):
+                def fj(s, j=j):
+                    y = [mp.mpf(float(v)) for v in x]
+                    y[j] = s
+                    return f_ref(name, y)
+                dj = mp.diff(fj, mp.mpf(float(x[j])))
+                assert g["glo"][i, j] - 1e-30 <= dj <= g["ghi"][i, j] + 1e-30, (name, i, j)
+
+
+def test_window_eval_rational_bit_identical(gpu):
+    d = 3
+    fam = F.REFERENCE["rosenbrock"]
+    mw = wmod(gpu, F.objective_loops("rosenbrock", d))
+    mu = gpu.get_module(F.objective("rosenbrock", d), roundings=("exact", "fast"))
+    rng = np.random.default_rng(22)
+    lo = rng.uniform(fam.lo, fam.hi, (64, d))
+    hi = np.minimum(lo + rng.uniform(0, 5, (64, d)), fam.hi)
+    a = mw.eval_boxes(np.zeros(0), lo, hi, rounding="exact", want=("f",))
+    b = mu.eval_boxes(np.zeros(0), lo, hi, rounding="exact", want=("f",))
+    assert np.array_equal(a["flo"], b["flo"]) and np.array_equal(a["fhi"], b["fhi"])
+
+
+@pytest.mark.parametrize("name", WIN)
+@pytest.mark.parametrize("S", [1, 10])
+def test_window_solve_matches_unrolled(gpu, name, S):
+    d, grid, k = 3, (3, 2), 16
+    rw, s = ref_solve(gpu, name, d, grid, k, S, wmod(gpu, F.objective_loops(name, d)), True,
+                      max_iters=20000, max_queue=1 << 22)
+    ru, _ = ref_solve(gpu, name, d, grid, k, S, gpu.get_module(F.objective(name, d), roundings=("exact", "fast")),
+                      False, max_iters=20000, max_queue=1 << 22)
+    for r in (rw, ru):
+        assert r["status_name"] == "certified", (name, r)
+        assert r["lb"] <= s.f_star <= r["ub"] and r["ub"] - r["lb"] <= 1e-6, (name, r)
+    assert max(rw["lb"], ru["lb"]) <= min(rw["ub"], ru["ub"])
+
+
+@pytest.mark.parametrize("family", pb.FAMILIES)
+def test_window_coefficient_families_match_oracle(gpu, oracle, family):
+    d, grid, k = 6, (3, 4), 16
+    rng = np.random.default_rng([7, d])
+    params = pb._family_params(rng, family, d)
+    lo, hi = np.full(d, -4.0), np.full(d, 4.0)
+    obj = pb.objective_loops(family, d)
+    print(f"# solve {family} d={d} windowed", flush=True)
+    r = gpu.minimize(obj, lo, hi, tol=1e-6, grid=grid, k_select=k, params=params, module=wmod(gpu, obj),
+                     windowed=True, max_iters=20000, max_queue=1 << 22)
+    fam = {"sepcos": 0, "sepcos_chain": 1, "ackley_q": 2}[family]
+    o, _ = oracle.solve(fam, params, lo, hi, tol_f=1e-6, wid=grid[0], cycle=grid[1], k_select=k, max_iters=20000)
+    assert r["status_name"] == "certified" and o["status"] == 1, (r, o)
+    assert max(r["lb"], o["lb"]) <= min(r["ub"], o["ub"]), (r, o)
+    if family != "sepcos_chain":
+        assert r["lb"] <= 0.0 <= r["ub"]
+
+
+def test_window_deterministic(gpu):
+    m = wmod(gpu, F.objective_loops("levy", 50))
+    a, _ = ref_solve(gpu, "levy", 50, (3, 12), 1, 10, m, True, max_queue=1 << 24)
+    b, _ = ref_solve(gpu, "levy", 50, (3, 12), 1, 10, m, True, max_queue=1 << 24)
+    assert all(a[key] == b[key] for key in RES_KEYS)
+
+
+def test_window_small_arena_equals_default(gpu):
+    """Collection of parent records moves their rest sums: for some arena
+    that is outgrown (records written > arena) without overflow the result
+    equals the default arena's; smaller arenas may report overflow."""
+    d, grid = 6, (2, 2)
+    m = wmod(gpu, F.objective_loops("rastrigin", d))
+    base, _ = ref_solve(gpu, "rastrigin", d, grid, 1, 1, m, True, tol=1e-9, max_iters=20000)
+    assert base["status_name"] == "certified"
+    collected = 0
+    for arena in (16, 24, 32, 48, 64, 96):
+        r, _ = ref_solve(gpu, "rastrigin", d, grid, 1, 1, m, True, tol=1e-9, max_iters=20000, max_parents=arena)
+        if r["status_name"] == "overflow":
+            assert r["overflow"] & 2
+            continue
+        assert all(r[key] == base[key] for key in RES_KEYS), (arena, r, base)
+        collected += r["iterations"] + 1 > arena
+    assert collected, "no arena size exercised the collection"
+
+
+@pytest.mark.parametrize("name", [n for n in WIN if n != "breiman"])
+def test_window_d50_certifies(gpu, name):
+    d = 50
+    r, s = ref_solve(gpu, name, d, (3, 12), 1, 10, wmod(gpu, F.objective_loops(name, d)), True,
+                     max_iters=20000, max_queue=1 << 24)
+    assert r["status_name"] == "certified", (name, r)
+    assert r["lb"] <= s.f_star <= r["ub"] and r["ub"] - r["lb"] <= 1e-6, (name, r)
+
+
+@pytest.mark.parametrize("name", ["levy", "rosenbrock", "fu"])
+def test_window_d1000_certifies(gpu, name):
+    d = 1000
+    r, s = ref_solve(gpu, name, d, (3, 12), 1, 10, wmod(gpu, F.objective_loops(name, d)), True,
+                     max_iters=20000, max_queue=1 << 26)
+    assert r["status_name"] == "certified", (name, r)
+    assert r["lb"] <= s.f_star <= r["ub"] and r["ub"] - r["lb"] <= 1e-6, (name, r)

@@ -45,7 +45,7 @@ def ir_eval(e, x, P, memo=None, j=None):
         for jj in range(len(x) + e.val):
             r += ir_eval(e.args[0], x, P, {}, jj)
     elif op == "twhen":
-        ok = (j + 1 < len(x)) if e.val == "j+1<d" else (j > 0)
+        ok = {"j+1<d": j + 1 < len(x), "j>0": j > 0, "j==0": j == 0, "j+1==d": j + 1 == len(x)}[e.val]
         r = ir_eval(e.args[0], x, P, {}, j) if ok else mp.mpf(0)
     elif op == "const":
         r = mp.mpf(e.val)
@@ -193,3 +193,52 @@ def test_rtc_compile_without_gpu(tmp_path, monkeypatch):
     path = S.rtc_compile_to_cache(pb.objective("sepcos", 2), roundings=("fast",))
     data = open(path, "rb").read()
     assert len(data) > 1000 and data[:4] == b"\x7fELF"
+
+
+def test_window_ranges_exact():
+    """The window rule of the windowed (large-d) modules, run from the same
+    source on the host: the ranges hold exactly the terms j in [0, nt) whose
+    coordinates j + k (k in [klo, khi]) meet the window {(win + t) mod d},
+    brute force over every window of small d (wrapping ones included)."""
+    from gpugo_amd.solver import window_ranges
+    rows, want = [], []
+    for d in range(1, 13):
+        for c in range(1, d + 1):
+            for win in range(d):
+                W = {(win + t) % d for t in range(c)}
+                for klo, khi in ((0, 0), (0, 1), (-1, 1), (-1, 0), (-2, 3)):
+                    for nt in {d, max(d - 1, 0), d + 1}:
+                        rows.append((win, c, d, klo, khi, nt))
+                        want.append({j for j in range(nt) if any(j + k in W for k in range(klo, khi + 1))})
+    out = window_ranges(rows)
+    for row, o, w in zip(rows, out, want):
+        n = o[0]
+        assert 0 <= n <= 2
+        got = [set(range(o[1 + 2 * r], o[2 + 2 * r])) for r in range(n)]
+        assert all(g for g in got), (row, o)
+        if n == 2:
+            assert not (got[0] & got[1]), (row, o)
+        assert set().union(*got) == w, (row, o, sorted(w))
+
+
+def test_window_ranges_rejects_bad_rows():
+    from gpugo_amd.solver import window_ranges
+    with pytest.raises(_lib.GpugoError):
+        window_ranges([(5, 2, 5, 0, 0, 5)])  # win >= d
+    with pytest.raises(_lib.GpugoError):
+        window_ranges([(0, 6, 5, 0, 0, 5)])  # c > d
+
+
+def test_window_module_emits():
+    """Windowed module text: loop-form code lists only (no per-coordinate
+    unrolling), rest sums per template sum, the window test on the gradient
+    components the window changes (all of them for a global gradient)."""
+    from gpugo_amd import families as F
+    src = emit.window_module_source(F.objective_loops("levy", 1000), ("fast",))
+    assert "#define GPUGO_WINDOWED 1" in src and "#define GPUGO_NREST 1" in src
+    assert "x[999]" in src and "x[998]" not in src           # tail term only, no unrolled sum
+    assert "win_ranges(win, c, d, -1, 1, d, glo, ghi)" in src  # local gradient: window components
+    src = emit.window_module_source(F.objective_loops("ackley", 1000), ("fast",))
+    assert "#define GPUGO_NREST 2" in src and "for (int j = 0; j < d; ++j) {" in src
+    with pytest.raises(ValueError):
+        F.objective_loops("griewank", 10)
