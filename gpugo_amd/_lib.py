@@ -22,7 +22,7 @@ _ERR_NAMES = {E_INVALID: "INVALID", E_DOMAIN: "DOMAIN", E_NOMEM: "NOMEM", E_COMP
 OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "neg": 4, "scale": 5, "addc": 6, "rsubc": 7,
        "divc": 8, "sqr": 9, "pown": 10, "sqrt": 11, "exp": 12, "sin": 13, "cos": 14}
 PREC = {"fp64": 0, "fp32": 1}
-ROUND = {"exact": 0, "fast": 1}
+ROUND = {"exact": 0, "fast": 1, "digits15": 2}
 FLAG_MEAN_VALUE = 1
 FLAG_SPLIT = 4
 FLAG_PERSIST = 8

@@ -14,6 +14,9 @@
 #ifndef GPUGO_WITH_F32
 #define GPUGO_WITH_F32 0
 #endif
+#ifndef GPUGO_WITH_DIGITS15
+#define GPUGO_WITH_DIGITS15 0
+#endif
 #ifndef GPUGO_SPLIT_G
 #define GPUGO_SPLIT_G 0
 #endif
@@ -427,6 +430,8 @@ __device__ __forceinline__ void persist_impl(const PersistArgs &p) {
     ca.par_out = p.q_par[cur ^ 1];
     ca.pos_out = p.q_pos[cur ^ 1];
     ca.cap = p.cap;
+    ca.poll = nullptr;
+    ca.pad = 0;
     compact_body(ca, blk, nblk);
     if (!grid_barrier(p.bar, nblk)) {
       if (threadIdx.x == 0) atomicOr(p.err, 1);
@@ -532,6 +537,18 @@ extern "C" __global__ __launch_bounds__(64) void gpugo_eval32_fast(gpugo::EvalAr
   gpugo::eval32_impl<gpugo::ROUND_FAST>(a);
 }
 #endif
+#if GPUGO_WITH_DIGITS15
+// the IA of the paper's results: 15-significant-digit outward rounding
+extern "C" __global__ __launch_bounds__(256) void gpugo_bound_digits15(gpugo::BoundArgs a) {
+  gpugo::bound_impl<gpugo::ROUND_DIGITS15, false>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void gpugo_bound_digits15_mv(gpugo::BoundArgs a) {
+  gpugo::bound_impl<gpugo::ROUND_DIGITS15, true>(a);
+}
+extern "C" __global__ __launch_bounds__(64) void gpugo_eval_digits15(gpugo::EvalArgs a) {
+  gpugo::eval_impl<gpugo::ROUND_DIGITS15>(a);
+}
+#endif
 #if GPUGO_WITH_FAST
 extern "C" __global__ __launch_bounds__(256) void gpugo_bound_fast(gpugo::BoundArgs a) {
   gpugo::bound_impl<gpugo::ROUND_FAST, false>(a);
@@ -553,7 +570,7 @@ extern "C" __global__ void gpugo_info(int *out) {
   out[0] = GPUGO_D;
   out[1] = GPUGO_NPARAMS;
   out[2] = GPUGO_MODULE_ABI_V;
-  out[3] = (GPUGO_WITH_EXACT ? 1 : 0) | (GPUGO_WITH_FAST ? 2 : 0);
+  out[3] = (GPUGO_WITH_EXACT ? 1 : 0) | (GPUGO_WITH_FAST ? 2 : 0) | (GPUGO_WITH_DIGITS15 ? 4 : 0);
   out[4] = GPUGO_WITH_F32;
   out[5] = GPUGO_SPLIT_G;
 }

@@ -719,6 +719,8 @@ struct CompactArgs {
   double *lb_out;
   unsigned *par_out, *pos_out;
   long long cap;
+  int *poll;             // or null: receives the problem's status (host-mapped pinned memory)
+  int pad;
 };
 
 // blk / nblk: this workgroup's index and the number of workgroups sharing
@@ -727,6 +729,9 @@ __device__ __forceinline__ void compact_body(const CompactArgs &a, int blk, int 
   __shared__ unsigned wcnt[16];
   __shared__ unsigned long long s_base;
   DevState *st = a.st;
+  // the host polls the status here instead of copying the state back
+  if (a.poll && blk == 0 && threadIdx.x == 0)
+    __hip_atomic_store(a.poll, st->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   // after a stop, exactly one more pass runs so the live set lands in the
   // buffer that st->compactions designates
   if (st->status != GPUGO_ST_RUNNING && !st->compact_final) return;
@@ -887,7 +892,39 @@ __device__ __forceinline__ void root_body(const RootArgs &a) {
 
 }  // namespace gpugo
 
+namespace gpugo {
+// Compaction of this step fused with the select of the next one: every
+// workgroup compacts its share; the last to finish (ticket in st->cblocks,
+// each workgroup fences its writes before taking one, the last fences
+// before reading) selects from the compacted queue.  Saves one launch and
+// its gap per step.  `poll` (or null) receives the status after the select.
+__device__ __forceinline__ void compact_select_body(const CompactArgs &ca, const SelectArgs &sa, int *poll) {
+  __shared__ int s_last;
+  compact_body(ca, blockIdx.x, gridDim.x);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const int t = atomicAdd(&ca.st->cblocks, 1);
+    s_last = t == (int)gridDim.x - 1;
+    if (s_last) {
+      ca.st->cblocks = 0;
+      __threadfence();
+    }
+  }
+  __syncthreads();
+  if (!s_last) return;
+  select_body(sa);
+  __syncthreads();
+  if (poll && threadIdx.x == 0)
+    __hip_atomic_store(poll, ca.st->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+}  // namespace gpugo
+
 #ifdef GPUGO_MODULE_STEP
+extern "C" __global__ __launch_bounds__(SEL_T) void gpugo_compact_select(gpugo::CompactArgs ca, gpugo::SelectArgs sa,
+                                                                         int *poll) {
+  gpugo::compact_select_body(ca, sa, poll);
+}
 extern "C" __global__ __launch_bounds__(SEL_T) void gpugo_select(gpugo::SelectArgs a) { gpugo::select_body(a); }
 extern "C" __global__ __launch_bounds__(256) void gpugo_compact(gpugo::CompactArgs a) {
   gpugo::compact_body(a, blockIdx.x, gridDim.x);

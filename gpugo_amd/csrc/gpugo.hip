@@ -140,8 +140,10 @@ extern "C" int gpugo_ia_eval(int op, int prec, int rounding, int64_t n, const do
                              const double *blo, const double *bhi, double s, double *olo, double *ohi) {
   if (op < 0 || op >= GPUGO_OP_COUNT) return set_err(GPUGO_E_INVALID, "unknown op %d", op);
   if (prec != GPUGO_PREC_F64 && prec != GPUGO_PREC_F32) return set_err(GPUGO_E_INVALID, "unknown precision %d", prec);
-  if (rounding != GPUGO_ROUND_EXACT && rounding != GPUGO_ROUND_FAST)
+  if (rounding != GPUGO_ROUND_EXACT && rounding != GPUGO_ROUND_FAST && rounding != GPUGO_ROUND_DIGITS15)
     return set_err(GPUGO_E_INVALID, "unknown rounding %d", rounding);
+  if (rounding == GPUGO_ROUND_DIGITS15 && prec != GPUGO_PREC_F64)
+    return set_err(GPUGO_E_INVALID, "15-digit rounding is FP64 only");
   if (n < 0) return set_err(GPUGO_E_INVALID, "negative n");
   if (n == 0) return GPUGO_OK;
   if (!alo || !ahi || !olo || !ohi) return set_err(GPUGO_E_INVALID, "null pointer");
@@ -176,7 +178,9 @@ extern "C" int gpugo_ia_eval(int op, int prec, int rounding, int64_t n, const do
   dim3 grid((unsigned)((n + 255) / 256)), block(256);
   const double *a0 = d_alo.as<double>(), *a1 = d_ahi.as<double>();
   double *o0 = d_olo.as<double>(), *o1 = d_ohi.as<double>();
-  if (prec == GPUGO_PREC_F64 && rounding == GPUGO_ROUND_EXACT)
+  if (rounding == GPUGO_ROUND_DIGITS15)
+    hipLaunchKernelGGL((k_ia_eval<double, ROUND_DIGITS15>), grid, block, 0, 0, op, (long long)n, a0, a1, pb_lo, pb_hi, s, o0, o1);
+  else if (prec == GPUGO_PREC_F64 && rounding == GPUGO_ROUND_EXACT)
     hipLaunchKernelGGL((k_ia_eval<double, ROUND_EXACT>), grid, block, 0, 0, op, (long long)n, a0, a1, pb_lo, pb_hi, s, o0, o1);
   else if (prec == GPUGO_PREC_F64)
     hipLaunchKernelGGL((k_ia_eval<double, ROUND_FAST>), grid, block, 0, 0, op, (long long)n, a0, a1, pb_lo, pb_hi, s, o0, o1);
@@ -195,9 +199,9 @@ extern "C" int gpugo_ia_eval(int op, int prec, int rounding, int64_t n, const do
 // ---------------------------------------------------------------------------
 struct gpugo_module_s {
   hipModule_t mod = nullptr;
-  hipFunction_t bound[2] = {nullptr, nullptr};  // [rounding]
-  hipFunction_t bound_mv[2] = {nullptr, nullptr};  // [rounding], mean-value variant
-  hipFunction_t eval[2] = {nullptr, nullptr};
+  hipFunction_t bound[3] = {nullptr, nullptr, nullptr};     // [rounding]
+  hipFunction_t bound_mv[3] = {nullptr, nullptr, nullptr};  // [rounding], mean-value variant
+  hipFunction_t eval[3] = {nullptr, nullptr, nullptr};
   hipFunction_t eval32[2] = {nullptr, nullptr};  // FP32 value (precision F32 modules)
   hipFunction_t bound_g[2] = {nullptr, nullptr};     // term-parallel variants (split modules)
   hipFunction_t bound_mv_g[2] = {nullptr, nullptr};
@@ -205,10 +209,11 @@ struct gpugo_module_s {
   // the step's select / compact / root kernels compiled into the module
   // (used instead of the library's copies: no code-object switch per launch)
   hipFunction_t select_fn = nullptr, compact_fn = nullptr, root_fn = nullptr;
+  hipFunction_t compact_select_fn = nullptr;  // compaction fused with the next step's select
   hipFunction_t persist[2] = {nullptr, nullptr};     // persistent step loops (split modules)
   hipFunction_t persist_mv[2] = {nullptr, nullptr};
   // windowed modules (large d): rest-sum kernel per rounding, rest slots
-  hipFunction_t prep[2] = {nullptr, nullptr};
+  hipFunction_t prep[3] = {nullptr, nullptr, nullptr};
   int nrest = 0;
   int dim = 0, nparams = 0, abi = 0;
   bool f32 = false;                // has the FP32 value code list (precision F32)
@@ -282,9 +287,10 @@ static int module_finish(gpugo_module_t m) {
   m->nrest = h[6];
   if (hipModuleGetFunction(&m->select_fn, m->mod, "gpugo_select") != hipSuccess ||
       hipModuleGetFunction(&m->compact_fn, m->mod, "gpugo_compact") != hipSuccess ||
-      hipModuleGetFunction(&m->root_fn, m->mod, "gpugo_root") != hipSuccess) {
+      hipModuleGetFunction(&m->root_fn, m->mod, "gpugo_root") != hipSuccess ||
+      hipModuleGetFunction(&m->compact_select_fn, m->mod, "gpugo_compact_select") != hipSuccess) {
     (void)hipGetLastError();
-    m->select_fn = m->compact_fn = m->root_fn = nullptr;
+    m->select_fn = m->compact_fn = m->root_fn = m->compact_select_fn = nullptr;
   }
   if (m->split_g > 1) {
     if (h[3] & 1) {
@@ -310,8 +316,8 @@ static int module_finish(gpugo_module_t m) {
   if (m->f32 && (h[3] & 2)) HIPCHECK(hipModuleGetFunction(&m->eval32[1], m->mod, "gpugo_eval32_fast"));
   if (m->abi != GPUGO_MODULE_ABI)
     return set_err(GPUGO_E_INVALID, "module ABI %d, library expects %d", m->abi, GPUGO_MODULE_ABI);
-  const char *rname[2] = {"exact", "fast"};
-  for (int r = 0; r < 2; ++r) {
+  const char *rname[3] = {"exact", "fast", "digits15"};
+  for (int r = 0; r < 3; ++r) {
     if (!(h[3] & (1 << r))) continue;
     char nm[64];
     snprintf(nm, sizeof nm, "gpugo_bound_%s", rname[r]);
@@ -488,9 +494,9 @@ extern "C" int gpugo_eval_boxes(gpugo_module_t m, int32_t d, const double *param
                                 double *ghi, int32_t *kkt, double *ub) {
   int rc = check_module_problem(m, d, params, n_params);
   if (rc) return rc;
-  if (rounding != GPUGO_ROUND_EXACT && rounding != GPUGO_ROUND_FAST)
+  if (rounding != GPUGO_ROUND_EXACT && rounding != GPUGO_ROUND_FAST && rounding != GPUGO_ROUND_DIGITS15)
     return set_err(GPUGO_E_INVALID, "unknown rounding %d", rounding);
-  if (!m->eval[rounding]) return set_err(GPUGO_E_INVALID, "module has no %s variant", rounding ? "fast" : "exact");
+  if (!m->eval[rounding]) return set_err(GPUGO_E_INVALID, "module has no variant for rounding %d", rounding);
   if (n < 0) return set_err(GPUGO_E_INVALID, "negative n");
   if (n == 0) return GPUGO_OK;
   if (!lo || !hi) return set_err(GPUGO_E_INVALID, "null box pointer");
@@ -877,10 +883,12 @@ struct gpugo_problem_s {
   Queue q[2];
   hipStream_t stream = nullptr;
   bool started = false;
+  bool selected_ahead = false;  // the last compaction also ran the next step's select
   long long steps_enqueued = 0;
   DevState hs;  // last state read back
   // state polling: pinned copies of DevState and an event per slot
-  DevState *hpoll = nullptr;
+  int *hpoll = nullptr;   // [2] pinned, written by the compaction of a batch's last step
+  int *dpoll = nullptr;   // its device address
   hipEvent_t poll_ev[2] = {nullptr, nullptr};
   // optional timing of the split+bound launches
   bool timing = false;
@@ -939,14 +947,18 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
     return set_err(GPUGO_E_INVALID, "module has no term-parallel kernels (emit it with split=G)");
   if ((desc->flags & GPUGO_FLAG_PERSIST) && !(desc->flags & GPUGO_FLAG_SPLIT))
     return set_err(GPUGO_E_INVALID, "GPUGO_FLAG_PERSIST needs GPUGO_FLAG_SPLIT");
+  if (desc->rounding == GPUGO_ROUND_DIGITS15 &&
+      (desc->flags & (GPUGO_FLAG_SPLIT | GPUGO_FLAG_PERSIST) || desc->precision != GPUGO_PREC_F64))
+    return set_err(GPUGO_E_INVALID, "15-digit rounding has no split, persistent or FP32 form");
   if ((desc->flags & GPUGO_FLAG_PERSIST) && !m->persist[desc->rounding == GPUGO_ROUND_FAST ? 1 : 0])
     return set_err(GPUGO_E_INVALID, "module has no persistent step loop for this rounding");
   if ((desc->flags & GPUGO_FLAG_SPLIT) && desc->precision == GPUGO_PREC_F32)
     return set_err(GPUGO_E_INVALID, "the term-parallel kernels have no FP32 pruning pass");
-  if (desc->rounding != GPUGO_ROUND_EXACT && desc->rounding != GPUGO_ROUND_FAST)
+  if (desc->rounding != GPUGO_ROUND_EXACT && desc->rounding != GPUGO_ROUND_FAST &&
+      desc->rounding != GPUGO_ROUND_DIGITS15)
     return set_err(GPUGO_E_INVALID, "unknown rounding %d", desc->rounding);
   if (!m->bound[desc->rounding])
-    return set_err(GPUGO_E_INVALID, "module has no %s variant", desc->rounding ? "fast" : "exact");
+    return set_err(GPUGO_E_INVALID, "module has no variant for rounding %d", desc->rounding);
   if (m->nrest > 0) {
     // windowed (large-d) module: natural-extension bound, midpoint sample
     if (desc->flags & (GPUGO_FLAG_MEAN_VALUE | GPUGO_FLAG_SPLIT | GPUGO_FLAG_PERSIST))
@@ -1033,8 +1045,10 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
         hipSuccess)
       return fail(set_err(GPUGO_E_HIP, "upload: %s", hipGetErrorString(e)));
   }
-  if ((e = hipHostMalloc((void **)&p->hpoll, 2 * sizeof(DevState), hipHostMallocDefault)) != hipSuccess)
+  if ((e = hipHostMalloc((void **)&p->hpoll, 2 * sizeof(int), hipHostMallocDefault)) != hipSuccess)
     return fail(set_err(GPUGO_E_NOMEM, "pinned poll buffer: %s", hipGetErrorString(e)));
+  if ((e = hipHostGetDevicePointer((void **)&p->dpoll, p->hpoll, 0)) != hipSuccess)
+    return fail(set_err(GPUGO_E_HIP, "poll buffer device address: %s", hipGetErrorString(e)));
   for (int b = 0; b < 2; ++b)
     if ((e = hipEventCreateWithFlags(&p->poll_ev[b], hipEventDisableTiming)) != hipSuccess)
       return fail(set_err(GPUGO_E_HIP, "event: %s", hipGetErrorString(e)));
@@ -1071,6 +1085,7 @@ extern "C" int gpugo_problem_reset(gpugo_problem_t p) {
   if (!p) return set_err(GPUGO_E_INVALID, "null problem");
   HIPCHECK(hipStreamSynchronize(p->stream));
   p->started = false;
+  p->selected_ahead = false;
   p->steps_enqueued = 0;
   p->tev_used = 0;
   p->bound_ms = 0;
@@ -1126,37 +1141,49 @@ static int root_init(gpugo_problem_t p) {
   return GPUGO_OK;
 }
 
-static int enqueue_step(gpugo_problem_t p) {
+// Select of the step that reads queue buffer q (the K smallest boxes,
+// stop tests, new parent records).
+static SelectArgs select_args(gpugo_problem_t p, const Queue &q) {
+  SelectArgs sa;
+  memset(&sa, 0, sizeof sa);
+  sa.st = p->st.as<DevState>();
+  sa.q_lb = q.plb();
+  sa.q_par = q.ppar();
+  sa.q_pos = q.ppos();
+  sa.plo = p->plo.as<double>();
+  sa.phi = p->phi.as<double>();
+  sa.pw = p->pw.as<double>();
+  sa.pwin = p->pwin.as<int>();
+  sa.out_lo = p->out_lo.as<double>();
+  sa.out_hi = p->out_hi.as<double>();
+  sa.out_lb = p->out_lb.as<double>();
+  sa.out_cap = p->out_cap;
+  sa.p_cap = p->p_cap;
+  sa.d = p->desc.d;
+  sa.wid = p->desc.wid;
+  sa.c = p->c;
+  sa.k_select = p->desc.k_select;
+  sa.tol_f = p->desc.tol_f;
+  sa.tol_x = p->desc.tol_x;
+  sa.max_iters = 0;
+  sa.pstride = p->desc.d;
+  sa.ext_n = p->ext_n;
+  sa.pext = p->pext.as<double>();
+  sa.tie_xor = p->m->nrest > 0 ? ~0u : 0u;  // windowed (large-d): newest parent first among ties
+  return sa;
+}
+
+// One step: select (unless the previous compaction ran it), the windowed
+// modules' rest sums, split+bound, compact.  poll: pinned status word
+// written after the step (or null).  more: another step follows in this
+// call, so the compaction may run its select.
+static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
   const int d = p->desc.d;
   const int cur = (int)(p->steps_enqueued & 1);
   Queue &A = p->q[cur], &B = p->q[cur ^ 1];
   DevState *st = p->st.as<DevState>();
-  if (p->steps_enqueued > 0) {
-    SelectArgs sa;
-    sa.st = st;
-    sa.q_lb = A.plb();
-    sa.q_par = A.ppar();
-    sa.q_pos = A.ppos();
-    sa.plo = p->plo.as<double>();
-    sa.phi = p->phi.as<double>();
-    sa.pw = p->pw.as<double>();
-    sa.pwin = p->pwin.as<int>();
-    sa.out_lo = p->out_lo.as<double>();
-    sa.out_hi = p->out_hi.as<double>();
-    sa.out_lb = p->out_lb.as<double>();
-    sa.out_cap = p->out_cap;
-    sa.p_cap = p->p_cap;
-    sa.d = d;
-    sa.wid = p->desc.wid;
-    sa.c = p->c;
-    sa.k_select = p->desc.k_select;
-    sa.tol_f = p->desc.tol_f;
-    sa.tol_x = p->desc.tol_x;
-    sa.max_iters = 0;
-    sa.pstride = d;
-    sa.ext_n = p->ext_n;
-    sa.pext = p->pext.as<double>();
-    sa.tie_xor = p->m->nrest > 0 ? ~0u : 0u;  // windowed (large-d): newest parent first among ties
+  if (p->steps_enqueued > 0 && !p->selected_ahead) {
+    SelectArgs sa = select_args(p, A);
     if (p->m->select_fn) {
       void *sargs[] = {&sa};
       HIPCHECK(hipModuleLaunchKernel(p->m->select_fn, 1, 1, 1, SEL_T, 1, 1, 0, p->stream, sargs, nullptr));
@@ -1165,6 +1192,7 @@ static int enqueue_step(gpugo_problem_t p) {
       HIPCHECK(hipGetLastError());
     }
   }
+  p->selected_ahead = false;
   BoundArgs ba;
   memset(&ba, 0, sizeof ba);
   ba.plo = p->plo.as<double>();
@@ -1230,7 +1258,16 @@ static int enqueue_step(gpugo_problem_t p) {
   ca.par_out = B.ppar();
   ca.pos_out = B.ppos();
   ca.cap = p->q_cap;
-  if (p->m->compact_fn) {
+  ca.poll = poll;
+  ca.pad = 0;
+  if (more && p->m->compact_select_fn) {
+    // 128 workgroups of 1024 threads compact; the last one selects from B
+    SelectArgs sa = select_args(p, B);
+    ca.poll = nullptr;
+    void *fargs[] = {&ca, &sa, &poll};
+    HIPCHECK(hipModuleLaunchKernel(p->m->compact_select_fn, 128, 1, 1, SEL_T, 1, 1, 0, p->stream, fargs, nullptr));
+    p->selected_ahead = true;
+  } else if (p->m->compact_fn) {
     void *cargs[] = {&ca};
     HIPCHECK(hipModuleLaunchKernel(p->m->compact_fn, 512, 1, 1, 256, 1, 1, 0, p->stream, cargs, nullptr));
   } else {
@@ -1391,30 +1428,29 @@ static int run_steps(gpugo_problem_t p, int64_t iters, gpugo_result *res, bool s
     int rc = persist_steps(p, iters);
     if (rc) return rc;
   } else if (p->hs.status == GPUGO_ST_RUNNING) {
-    // Steps are enqueued in batches; after each batch the device state is
-    // copied into a pinned slot.  The host waits only for the previous
-    // batch's copy, so it stays one batch ahead of the device; steps
+    // Steps are enqueued in batches; the compaction of a batch's last step
+    // writes the status into a pinned slot (no copy kernel).  The host waits
+    // only for the previous batch, so it stays one batch ahead of the device; steps
     // enqueued after a stop are no-ops on the device.  Exactly `iters`
     // steps are enqueued at most.
     const int64_t B = 2;
     int64_t done = 0;
     int slot = 0, pending = 0;
-    DevState *st = p->st.as<DevState>();
     while (done < iters) {
       int64_t batch = B < iters - done ? B : iters - done;
+      p->hpoll[slot] = GPUGO_ST_RUNNING;
       for (int64_t i = 0; i < batch; ++i) {
-        int rc = enqueue_step(p);
+        int rc = enqueue_step(p, i + 1 == batch ? p->dpoll + slot : nullptr, done + i + 1 < iters);
         if (rc) return rc;
       }
       done += batch;
-      HIPCHECK(hipMemcpyAsync(&p->hpoll[slot], st, sizeof(DevState), hipMemcpyDeviceToHost, p->stream));
       HIPCHECK(hipEventRecord(p->poll_ev[slot], p->stream));
       ++pending;
       if (pending == 2) {
         const int old = slot ^ 1;
         HIPCHECK(hipEventSynchronize(p->poll_ev[old]));
         --pending;
-        if (p->hpoll[old].status != GPUGO_ST_RUNNING) break;
+        if (__atomic_load_n(&p->hpoll[old], __ATOMIC_ACQUIRE) != GPUGO_ST_RUNNING) break;
       }
       slot ^= 1;
     }

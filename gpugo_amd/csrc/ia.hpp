@@ -8,7 +8,7 @@
 // Replaces the device functions of the reference interval module
 // ("Updated Interval Arithmetic Operations/IA.py":57-570).  CDNA has no
 // per-instruction rounding mode, so directed rounding is built from
-// round-to-nearest with one of two policies:
+// round-to-nearest with one of three policies:
 //
 //   EXACT  error-free transformations: TwoSum (Knuth) for +/-, the fma
 //          residual for * / and sqrt.  The sign of the exact error decides
@@ -25,6 +25,15 @@
 //          successor in rounding to nearest", BIT 49 (2009)).  At most one
 //          ulp wider than EXACT per operation.
 //
+//   DIGITS15  (FP64 only) the semantics of the IA behind the paper's results
+//          (1-Ackley/IAAckley.py:106-141, outRound with n = 15): every
+//          result is widened to the enclosing decimal of 15 significant
+//          digits.  Each bound is the EXACT bound, lowered (raised) further
+//          to floor (ceil) of the nearest result at 15 significant digits,
+//          so it is never narrower than EXACT and stays rigorous where the
+//          decimal scaling itself rounds.  Magnitudes below 1e-292 widen to
+//          0 (lower bound of a positive value) or to -/+1e-292.
+//
 // exp/sin/cos use the device library's round-to-nearest functions and are
 // widened outward by W_TRANS ulps on the bit pattern, following the shape of
 // IA.py:363-570 (nearest value, then integer steps); the bound of the
@@ -38,7 +47,7 @@
 
 namespace gpugo {
 
-enum { ROUND_EXACT = 0, ROUND_FAST = 1 };
+enum { ROUND_EXACT = 0, ROUND_FAST = 1, ROUND_DIGITS15 = 2 };
 
 template <typename T> struct fp;
 
@@ -196,7 +205,66 @@ template <typename T> struct rnd<T, ROUND_FAST> {
 // ---------------------------------------------------------------------------
 // Intervals.
 // ---------------------------------------------------------------------------
+// 15-significant-digit outward rounding of a double (DIGITS15): the largest
+// (smallest) decimal of 15 significant digits <= x (>= x), computed as
+// floor(x * 10^(14-e)) / 10^(14-e), e = floor(log10 |x|), the shape of
+// 1-Ackley/IAAckley.py:106-141; callers take the min (max) with a rigorous
+// bound, which absorbs the rounding of this scaling.
+__device__ __forceinline__ double digits15(double x, bool up) {
+  if (!(x == x) || x == 0.0 || __builtin_isinf(x)) return x;
+  const double ax = __builtin_fabs(x);
+  if (ax < 1e-292) return (x > 0.0) == up ? (up ? 1e-292 : -1e-292) : 0.0;
+  const int e = (int)__builtin_floor(::log10(ax));
+  // 10^k exactly for 0 <= k <= 22 (the reference's integer power), else pow
+  const int k = 14 - e;
+  double sc = 1.0;
+  if (k >= 0 && k <= 22) {
+    for (int i = 0; i < k; ++i) sc *= 10.0;
+  } else {
+    sc = ::pow(10.0, (double)k);
+  }
+  const double q = ax * sc;
+  // magnitude rounded away from 0 when the bound moves away from 0
+  const bool away = (x > 0.0) == up;
+  const double m = (away ? __builtin_ceil(q) : __builtin_floor(q)) / sc;
+  return x > 0.0 ? m : -m;
+}
+
+template <typename T> struct rnd<T, ROUND_DIGITS15> {
+  typedef rnd<T, ROUND_EXACT> X;
+  __device__ static __forceinline__ T dn(T x, T r) { return min_<T>(digits15(x, false), r); }
+  __device__ static __forceinline__ T up(T x, T r) { return max_<T>(digits15(x, true), r); }
+  __device__ static __forceinline__ T add_dn(T a, T b) { return dn(a + b, X::add_dn(a, b)); }
+  __device__ static __forceinline__ T add_up(T a, T b) { return up(a + b, X::add_up(a, b)); }
+  __device__ static __forceinline__ T sub_dn(T a, T b) { return dn(a - b, X::sub_dn(a, b)); }
+  __device__ static __forceinline__ T sub_up(T a, T b) { return up(a - b, X::sub_up(a, b)); }
+  __device__ static __forceinline__ T mul_dn(T a, T b) { return dn(a * b, X::mul_dn(a, b)); }
+  __device__ static __forceinline__ T mul_up(T a, T b) { return up(a * b, X::mul_up(a, b)); }
+  __device__ static __forceinline__ T div_dn(T a, T b) { return dn(a / b, X::div_dn(a, b)); }
+  __device__ static __forceinline__ T div_up(T a, T b) { return up(a / b, X::div_up(a, b)); }
+  __device__ static __forceinline__ T sqrt_dn(T a) { return dn(__builtin_sqrt(a), X::sqrt_dn(a)); }
+  __device__ static __forceinline__ T sqrt_up(T a) { return up(__builtin_sqrt(a), X::sqrt_up(a)); }
+  // outward rounding of an already rigorous bound (transcendental results)
+  __device__ static __forceinline__ T out_dn(T r) { return dn(r, r); }
+  __device__ static __forceinline__ T out_up(T r) { return up(r, r); }
+};
+
 template <typename T> struct iv { T lo, hi; };
+
+// rigorous bounds of transcendental functions pass through the policy's
+// out_dn/out_up where it has them (DIGITS15), unchanged otherwise
+template <class RR, class = void> struct has_out { static constexpr bool value = false; };
+template <class RR> struct has_out<RR, decltype((void)RR::out_dn(0.0), void())> {
+  static constexpr bool value = true;
+};
+template <class RR, typename T> __device__ __forceinline__ T out_dn(T x) {
+  if constexpr (has_out<RR>::value) return RR::out_dn(x);
+  else return x;
+}
+template <class RR, typename T> __device__ __forceinline__ T out_up(T x) {
+  if constexpr (has_out<RR>::value) return RR::out_up(x);
+  else return x;
+}
 
 template <typename T> __device__ __forceinline__ iv<T> mk(T lo, T hi) { iv<T> r; r.lo = lo; r.hi = hi; return r; }
 template <typename T> __device__ __forceinline__ iv<T> point(T x) { return mk<T>(x, x); }
@@ -336,8 +404,8 @@ template <typename T, int R> struct IA {
 
   // out of line: keeps unrolled code lists (and hipRTC compile time) small
   __device__ __attribute__((noinline)) static V exp(V a) {
-    T l = max_<T>(widen_dn(fp<T>::exp_(a.lo)), T(0));
-    return mk<T>(l, widen_up(fp<T>::exp_(a.hi)));
+    T l = max_<T>(out_dn<RR>(widen_dn(fp<T>::exp_(a.lo))), T(0));
+    return mk<T>(l, out_up<RR>(widen_up(fp<T>::exp_(a.hi))));
   }
 
   // shift 0: cos (extrema at m*pi), shift 1: sin (extrema at pi/2 + m*pi);
@@ -371,8 +439,14 @@ template <typename T, int R> struct IA {
     }
     return mk<T>(l, h);
   }
-  __device__ static __forceinline__ V cos(V a) { return trig<0>(a); }
-  __device__ static __forceinline__ V sin(V a) { return trig<1>(a); }
+  __device__ static __forceinline__ V cos(V a) {
+    const V r = trig<0>(a);
+    return mk<T>(out_dn<RR>(r.lo), out_up<RR>(r.hi));
+  }
+  __device__ static __forceinline__ V sin(V a) {
+    const V r = trig<1>(a);
+    return mk<T>(out_dn<RR>(r.lo), out_up<RR>(r.hi));
+  }
 };
 
 }  // namespace gpugo

@@ -824,13 +824,14 @@ def module_source(obj: Objective, roundings=("fast",), f32: bool = False, split:
     the objective's loop templates)."""
     if split and (split < 2 or split > 64 or split & (split - 1)):
         raise ValueError("split must be a power of two in [2, 64]")
-    bad = set(roundings) - {"exact", "fast"}
+    bad = set(roundings) - {"exact", "fast", "digits15"}
     if bad or not roundings:
-        raise ValueError(f"roundings must be a non-empty subset of ('exact', 'fast'), got {roundings}")
+        raise ValueError(f"roundings must be a non-empty subset of ('exact', 'fast', 'digits15'), got {roundings}")
     parts = [
         "// generated by gpugo_amd.emit for objective " + obj.name,
         f"#define GPUGO_WITH_EXACT {int('exact' in roundings)}",
         f"#define GPUGO_WITH_FAST {int('fast' in roundings)}",
+        f"#define GPUGO_WITH_DIGITS15 {int('digits15' in roundings)}",
         f"#define GPUGO_WITH_F32 {int(bool(f32))}",
         f"#define GPUGO_SPLIT_G {int(split)}",
         _read("ia.hpp"),

@@ -78,8 +78,14 @@ enum gpugo_prec {
 enum gpugo_round {
   GPUGO_ROUND_EXACT = 0, /* +,-,*,/,sqrt give the true round-down/round-up
                             results (bit-identical to IEEE directed rounding) */
-  GPUGO_ROUND_FAST  = 1  /* round-to-nearest then one-ulp outward step:
+  GPUGO_ROUND_FAST  = 1, /* round-to-nearest then one-ulp outward step:
                             rigorous, at most one ulp wider per operation     */
+  GPUGO_ROUND_DIGITS15 = 2 /* FP64 only: the IA of the paper's results, each
+                            result widened to the enclosing decimal of 15
+                            significant digits (outRound, n = 15,
+                            1-Ackley/IAAckley.py:106-141); never narrower
+                            than EXACT.  gpugo_ia_eval and single-problem
+                            modules (no split, persistent or FP32 form)       */
 };
 
 /* ---- library ----------------------------------------------------------- */

@@ -168,3 +168,27 @@ def solve(family: int, params, lo, hi, *, tol_f=1e-6, tol_x=0.0, wid=4, cycle=10
     r = res.as_dict()
     n = min(res.n_out, out_max)
     return r, (olo[:n], ohi[:n], olb[:n])
+
+
+def digits15(x: float, up: bool) -> float:
+    """outRound of the IA behind the paper's results, n = 15
+    (GPUGO_1.1/1-Ackley/IAAckley.py:106-141, ru/rd :56-104): with
+    k = 14 - floor(log10 |x|), the bound away from zero is
+    ceil(|x| * 10^k) / 10^k and the bound towards zero floor(|x| * 10^k) / 10^k,
+    in IEEE double arithmetic (Python floats) as the reference computes it;
+    10^k is a double power here (the reference's integer power overflows
+    int64 for k > 18); magnitudes below 1e-292 widen to 0 (towards zero) or
+    to -/+1e-292 (away from zero), the guard of 10-Zabinsky/IASinusoidal.py:
+    109-175.  Like the reference it can land an ulp inside x where |x| * 10^k
+    rounds onto an integer; the GPU mode keeps its bound no narrower than
+    the directed-rounding bound."""
+    import math
+    if x != x or x == 0.0 or math.isinf(x):
+        return x
+    ax = abs(x)
+    if ax < 1e-292:
+        return (1e-292 if up else -1e-292) if (x > 0) == up else 0.0
+    sc = 10.0 ** (14 - math.floor(math.log10(ax)))
+    q = ax * sc
+    m = (math.ceil(q) if (x > 0) == up else math.floor(q)) / sc
+    return m if x > 0 else -m

@@ -132,3 +132,72 @@ def test_domain_errors(gpu):
         gpu.ia_eval("divc", [1.0], [2.0], s=0.0)
     lo, hi = gpu.ia_eval("add", [], [], [], [])
     assert lo.size == 0
+
+
+@pytest.mark.parametrize("op", RATIONAL)
+def test_digits15_mode(gpu, oracle, op):
+    """GPUGO_ROUND_DIGITS15 (the IA of the paper's results): each bound
+    contains the true directed-rounding bound w (the oracle's EXACT result)
+    and is the 15-significant-digit rounding of the nearest result, as
+    outRound(lower1 + lower2, ...) rounds it (1-Ackley/IAAckley.py:106-141),
+    kept no narrower than w.  The nearest result is w or the double one ulp
+    inside it, so the checker's restatement of outRound (oracle.digits15) of
+    either, held no narrower than w, is the expected bound: within one
+    15-digit step everywhere, and equal for at least 90% of the bounds where
+    the reference's power of ten 10^k (k = 14 - floor(log10 |w|)) is an exact
+    integer (0 <= k <= 18; beyond, the reference's int64 power overflows and
+    the device's and the host's pow may round differently)."""
+    from oracle.oracle import digits15
+    rng = np.random.default_rng(200 + OPC[op])
+    alo, ahi, blo, bhi, s = inputs(op, rng)
+    want = oracle.ia_eval(OPC[op], alo, ahi, blo, bhi, s)
+    got = gpu.ia_eval(op, alo, ahi, blo, bhi, s, rounding="digits15")
+    assert np.all(got[0] <= want[0]) and np.all(got[1] >= want[1])
+    if op == "neg":
+        return  # exact: nothing is rounded
+    if op == "pown":
+        # repeated multiplication: each step rounds to 15 digits
+        fin = np.isfinite(want[0]) & np.isfinite(want[1])
+        # (plus the documented widening of magnitudes below 1e-292)
+        tol = 8 * 1e-14 * np.maximum(np.abs(want[0]), np.abs(want[1])) + 2e-292
+        with np.errstate(invalid="ignore"):  # inf - inf outside `fin`
+            dlo, dhi = want[0] - got[0], got[1] - want[1]
+        assert np.all(dlo[fin] <= tol[fin]) and np.all(dhi[fin] <= tol[fin])
+        return
+    same = total = 0
+    for i in range(len(alo)):
+        for g, w, up in ((got[0][i], want[0][i], False), (got[1][i], want[1][i], True)):
+            if not np.isfinite(w):
+                continue
+            w = float(w)
+            inside = math.nextafter(w, -math.inf if up else math.inf)
+            refs = [max(digits15(v, up), w) if up else min(digits15(v, up), w) for v in (w, inside)]
+            assert min(abs(g - r) for r in refs) <= 1e-14 * abs(w) + 2 * math.ulp(w), (op, i, w, g, refs)
+            if w != 0.0 and 0 <= 14 - math.floor(math.log10(abs(w))) <= 18:
+                total += 1
+                same += g in refs
+    assert total > 0.2 * 2 * len(alo) and same >= 0.9 * total, (same, total)
+
+
+@pytest.mark.parametrize("fn", ["exp", "sin", "cos"])
+def test_digits15_transcendentals_contain(gpu, fn):
+    rng = np.random.default_rng(300)
+    lo = rng.uniform(-20, 20, 400)
+    hi = lo + rng.uniform(0, 2, 400)
+    g = gpu.ia_eval(fn, lo, hi, rounding="digits15")
+    f = gpu.ia_eval(fn, lo, hi, rounding="exact")
+    assert np.all(g[0] <= f[0]) and np.all(g[1] >= f[1])
+
+
+def test_digits15_solve_certifies(gpu, oracle):
+    """A solve in the paper's IA: certified enclosure of f* = 0 (sepcos d=2)
+    that intersects the oracle's (true directed rounding)."""
+    from gpugo_amd import problems as pb
+    spec = pb.sepcos2()
+    obj = pb.objective(spec.family, spec.d)
+    m = gpu.get_module(obj, roundings=("digits15",))
+    r = gpu.minimize(obj, spec.lo, spec.hi, tol=1e-6, grid=(2, 2), k_select=16, params=spec.params,
+                     rounding="digits15", module=m, max_iters=10000)
+    o, _ = oracle.solve(0, spec.params, spec.lo, spec.hi, tol_f=1e-6, wid=2, cycle=2, k_select=16, max_iters=10000)
+    assert r["status_name"] == "certified" and r["lb"] <= 0.0 <= r["ub"] and r["ub"] - r["lb"] <= 1e-6
+    assert max(r["lb"], o["lb"]) <= min(r["ub"], o["ub"])

@@ -144,3 +144,23 @@ def test_reference_grid_matches_oracle_first_steps(gpu, oracle):
     assert abs(r["ub"] - o["ub"]) <= 1e-12 * abs(o["ub"])
     assert abs(r["lb"] - o["lb"]) <= 1e-12 * (1 + abs(o["lb"]))
     assert abs(r["queue_size"] - o["queue_size"]) <= max(2, 0.001 * o["queue_size"])
+
+
+@pytest.mark.parametrize("spec,grid,k,kw", [
+    (pb.rosenbrock(4), (2, 4), 16, {"rounding": "exact"}),
+    (pb.sepcos2(), (2, 2), 16, {}),
+    (pb.chain10(), (4, 5), 32, {"mean_value": True, "split": 4}),
+], ids=lambda v: getattr(v, "name", None))
+def test_fused_select_matches_separate(gpu, spec, grid, k, kw):
+    """A solve runs each next step's select inside the compaction kernel;
+    one step per call never does (the call's last step keeps them apart).
+    Both give the same solve."""
+    with gpu_problem(gpu, spec, tol=1e-6, grid=grid, k_select=k, **kw) as p:
+        a = p.solve()
+    with gpu_problem(gpu, spec, tol=1e-6, grid=grid, k_select=k, **kw) as p:
+        for _ in range(100000):
+            b = p.step(1)
+            if b["status_name"] != "running":
+                break
+    for key in ("lb", "ub", "width", "iterations", "boxes_bounded", "pruned_value", "pruned_kkt", "status"):
+        assert a[key] == b[key], (key, a[key], b[key])

@@ -177,3 +177,31 @@ def test_checker_catches_nearest_rounding():
     wrong = a + b  # nearest, not round-down
     mismatches = sum(wrong[i] != rd(F(a[i]) + F(b[i])) for i in range(200))
     assert mismatches > 50
+
+
+def test_digits15_restatement():
+    """The checker's restatement of outRound (the IA behind the paper's
+    results, 1-Ackley/IAAckley.py:106-141) on closed-form cases: decimals of
+    15 significant digits as doubles, tiny magnitudes widened to 0 or
+    -/+1e-292; on random values the bounds lie within 4 ulps of a decimal of
+    15 significant digits (the reference scales by inexact powers of ten) and
+    within one ulp of the outside of x (the scaling may land an ulp inside)."""
+    from oracle.oracle import digits15 as g
+    assert g(-2.5, False) == -2.5 and g(-2.5, True) == -2.5 and g(0.0, False) == 0.0
+    assert g(math.pi, False) == 3.14159265358979 and g(math.pi, True) == 3.1415926535898
+    assert g(-math.pi, False) == -3.1415926535898 and g(-math.pi, True) == -3.14159265358979
+    assert g(1 / 3, True) == 0.333333333333334 and g(1 / 3, False) == 0.333333333333333
+    assert g(2 / 3, True) == 0.666666666666667 and g(2 / 3, False) == 0.666666666666666
+    assert g(123456789012345678.0, False) == 123456789012345000.0
+    assert g(123456789012345678.0, True) == 123456789012346000.0
+    assert g(1e-300, False) == 0.0 and g(1e-300, True) == 1e-292 and g(-1e-300, False) == -1e-292
+    from decimal import Decimal
+    rng = np.random.default_rng(3)
+    for x in np.concatenate([rng.standard_normal(200) * 10.0 ** rng.integers(-200, 200, 200), [1e23, 9.999e22]]):
+        x = float(x)
+        lo, hi = g(x, False), g(x, True)
+        assert lo <= math.nextafter(x, math.inf) and hi >= math.nextafter(x, -math.inf)
+        for b in (lo, hi):   # a 15-digit decimal, up to the scaling's rounding
+            d = Decimal(b)
+            near = float(d.quantize(Decimal(1).scaleb(d.adjusted() - 14)))
+            assert abs(near - b) <= 4 * math.ulp(b), (x, b, near)
