@@ -832,6 +832,7 @@ def module_source(obj: Objective, roundings=("fast",), f32: bool = False, split:
         f"#define GPUGO_WITH_EXACT {int('exact' in roundings)}",
         f"#define GPUGO_WITH_FAST {int('fast' in roundings)}",
         f"#define GPUGO_WITH_DIGITS15 {int('digits15' in roundings)}",
+        f"#define GPUGO_ACCESSOR_D {obj.d}",
         f"#define GPUGO_WITH_F32 {int(bool(f32))}",
         f"#define GPUGO_SPLIT_G {int(split)}",
         _read("ia.hpp"),
