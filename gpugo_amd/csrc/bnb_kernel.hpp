@@ -431,7 +431,8 @@ __device__ __forceinline__ void persist_impl(const PersistArgs &p) {
     ca.pos_out = p.q_pos[cur ^ 1];
     ca.cap = p.cap;
     ca.poll = nullptr;
-    ca.pad = 0;
+    ca.tie_xor = 0u;
+    ca.wmin = nullptr;
     compact_body(ca, blk, nblk);
     if (!grid_barrier(p.bar, nblk)) {
       if (threadIdx.x == 0) atomicOr(p.err, 1);

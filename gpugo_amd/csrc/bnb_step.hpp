@@ -115,7 +115,8 @@ struct SelectArgs {
   int ext_n;             // doubles of per-record payload in pext (moved with the records)
   double *pext;          // [p_cap][ext_n] (windowed modules: rest sums), or null
   unsigned tie_xor;      // 0: equal bounds go oldest parent first; ~0u: newest first
-  int pad1;
+  int nwmin;             // entries of wmin
+  const unsigned long long *wmin;  // K = 1: the compaction's per-workgroup smallest keys, or null
 };
 
 // Selection key (lb, parent, pos), unique per box.  Parent records are
@@ -441,6 +442,34 @@ __device__ void select_body(const SelectArgs &a) {
       sidx[i] = (unsigned)i;
     }
     __syncthreads();
+  } else if (K == 1 && a.wmin) {
+    // one box per step: the smallest of the compaction workgroups' smallest
+    // keys (the same unique key order as the passes below), no queue pass
+    if (tid < 64) {
+      unsigned long long bh = ~0ull, bl = ~0ull, bo = 0ull;
+      for (int q = tid; q < a.nwmin; q += 64) {
+        const unsigned long long h = a.wmin[3 * q], l = a.wmin[3 * q + 1];
+        if (h < bh || (h == bh && l < bl)) {
+          bh = h;
+          bl = l;
+          bo = a.wmin[3 * q + 2];
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long th = __shfl_xor(bh, o, 64), tl = __shfl_xor(bl, o, 64), to = __shfl_xor(bo, o, 64);
+        if (th < bh || (th == bh && tl < bl)) {
+          bh = th;
+          bl = tl;
+          bo = to;
+        }
+      }
+      if (tid == 0) {
+        skey[0] = ((u128)bh << 64) | (u128)bl;
+        sidx[0] = (unsigned)bo;
+      }
+    }
+    __syncthreads();
   } else {
     // Fast path: one histogram pass over the live range [lbmin, lbmax]
     // (tracked by the compaction).  The bin map is monotone in lb, so every
@@ -720,7 +749,8 @@ struct CompactArgs {
   unsigned *par_out, *pos_out;
   long long cap;
   int *poll;             // or null: receives the problem's status (host-mapped pinned memory)
-  int pad;
+  unsigned tie_xor;      // key order of the next select (SelectArgs::tie_xor)
+  unsigned long long *wmin;  // or null: [3 * nblk] smallest kept key (hi, lo) and its output index per workgroup
 };
 
 // blk / nblk: this workgroup's index and the number of workgroups sharing
@@ -747,6 +777,9 @@ __device__ __forceinline__ void compact_body(const CompactArgs &a, int blk, int 
     s_kmax = 0ull;
   }
   const int nw = (int)(blockDim.x >> 6), bsz = (int)blockDim.x;
+  // smallest kept key (order_key(lb), (parent ^ tie_xor) << 32 | pos) and its
+  // output index, for a one-box select that needs no pass over the queue
+  unsigned long long mh = ~0ull, ml = ~0ull, mo = 0ull;
   for (long long base = (long long)blk * bsz; base < n; base += (long long)nblk * bsz) {
     long long i = base + threadIdx.x;
     double lb = 0;
@@ -774,11 +807,49 @@ __device__ __forceinline__ void compact_body(const CompactArgs &a, int blk, int 
     for (int q = 0; q < w; ++q) off += wcnt[q];
     if (keep) {
       long long o = (long long)s_base + off;
+      const unsigned par = a.par_in[i], pos = a.pos_in[i];
       a.lb_out[o] = lb;
-      a.par_out[o] = a.par_in[i];
-      a.pos_out[o] = a.pos_in[i];
+      a.par_out[o] = par;
+      a.pos_out[o] = pos;
+      if (a.wmin) {
+        const unsigned long long kh = order_key(lb), kl = ((unsigned long long)(par ^ a.tie_xor) << 32) | pos;
+        if (kh < mh || (kh == mh && kl < ml)) {
+          mh = kh;
+          ml = kl;
+          mo = (unsigned long long)o;
+        }
+      }
     }
     __syncthreads();
+  }
+  if (a.wmin) {
+    __shared__ unsigned long long s_mk[16][3];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long th = __shfl_xor(mh, o, 64), tl = __shfl_xor(ml, o, 64), to = __shfl_xor(mo, o, 64);
+      if (th < mh || (th == mh && tl < ml)) {
+        mh = th;
+        ml = tl;
+        mo = to;
+      }
+    }
+    if (lane == 0) {
+      s_mk[w][0] = mh;
+      s_mk[w][1] = ml;
+      s_mk[w][2] = mo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int q = 1; q < nw; ++q)
+        if (s_mk[q][0] < mh || (s_mk[q][0] == mh && s_mk[q][1] < ml)) {
+          mh = s_mk[q][0];
+          ml = s_mk[q][1];
+          mo = s_mk[q][2];
+        }
+      a.wmin[3 * blk] = mh;
+      a.wmin[3 * blk + 1] = ml;
+      a.wmin[3 * blk + 2] = mo;
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {

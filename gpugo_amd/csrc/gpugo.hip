@@ -661,6 +661,8 @@ __global__ __launch_bounds__(SEL_T) void kb_select(BatchArgs b) {
   sa.ext_n = 0;
   sa.pext = nullptr;
   sa.tie_xor = 0u;
+  sa.nwmin = 0;
+  sa.wmin = nullptr;
   select_body(sa);
 }
 
@@ -861,6 +863,12 @@ __global__ __launch_bounds__(256) void k_materialize(const double *plo, const do
 // ---------------------------------------------------------------------------
 // problems
 // ---------------------------------------------------------------------------
+// Workgroups of the two compaction launches; each leaves its smallest key
+// in p->wmin for a one-box select.
+#define GPUGO_COMPACT_WG 512   // gpugo_compact, 256 threads
+#define GPUGO_FUSED_WG 128     // gpugo_compact_select, 1024 threads
+#define GPUGO_WMIN_MAX GPUGO_COMPACT_WG
+
 struct Queue {
   DevBuf lb, par, pos;
   double *plb() const { return lb.as<double>(); }
@@ -878,6 +886,7 @@ struct gpugo_problem_s {
   DevBuf d_lo, d_hi, d_params, d_params32, plo, phi, pw, pwin, out_lo, out_hi, out_lb, st, scratch;
   DevBuf bar;          // persistent step loop: grid barrier [2] and error flag
   DevBuf pext;         // windowed modules: rest sums per parent record [p_cap][ext_n]
+  DevBuf wmin;         // k_select 1: per-compaction-workgroup smallest key [GPUGO_WMIN_MAX][3]
   int ext_n = 0;
   int persist_blocks = 0;
   Queue q[2];
@@ -1018,6 +1027,7 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
   PALLOC(p->scratch, 64);
   PALLOC(p->bar, 64);
   if (p->ext_n > 0) PALLOC(p->pext, sizeof(double) * p->ext_n * pc);
+  PALLOC(p->wmin, sizeof(unsigned long long) * 3 * GPUGO_WMIN_MAX);
   for (int b = 0; b < 2; ++b) {
     PALLOC(p->q[b].lb, sizeof(double) * qc);
     PALLOC(p->q[b].par, sizeof(unsigned) * qc);
@@ -1142,8 +1152,9 @@ static int root_init(gpugo_problem_t p) {
 }
 
 // Select of the step that reads queue buffer q (the K smallest boxes,
-// stop tests, new parent records).
-static SelectArgs select_args(gpugo_problem_t p, const Queue &q) {
+// stop tests, new parent records); nwg: workgroups of the compaction
+// that wrote q.
+static SelectArgs select_args(gpugo_problem_t p, const Queue &q, int nwg) {
   SelectArgs sa;
   memset(&sa, 0, sizeof sa);
   sa.st = p->st.as<DevState>();
@@ -1170,6 +1181,9 @@ static SelectArgs select_args(gpugo_problem_t p, const Queue &q) {
   sa.ext_n = p->ext_n;
   sa.pext = p->pext.as<double>();
   sa.tie_xor = p->m->nrest > 0 ? ~0u : 0u;  // windowed (large-d): newest parent first among ties
+  // one box per step: the previous compaction's per-workgroup smallest keys
+  sa.wmin = p->desc.k_select == 1 ? p->wmin.as<unsigned long long>() : nullptr;
+  sa.nwmin = sa.wmin ? nwg : 0;
   return sa;
 }
 
@@ -1183,7 +1197,7 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
   Queue &A = p->q[cur], &B = p->q[cur ^ 1];
   DevState *st = p->st.as<DevState>();
   if (p->steps_enqueued > 0 && !p->selected_ahead) {
-    SelectArgs sa = select_args(p, A);
+    SelectArgs sa = select_args(p, A, GPUGO_COMPACT_WG);  // after a plain compaction
     if (p->m->select_fn) {
       void *sargs[] = {&sa};
       HIPCHECK(hipModuleLaunchKernel(p->m->select_fn, 1, 1, 1, SEL_T, 1, 1, 0, p->stream, sargs, nullptr));
@@ -1259,19 +1273,21 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
   ca.pos_out = B.ppos();
   ca.cap = p->q_cap;
   ca.poll = poll;
-  ca.pad = 0;
+  ca.tie_xor = p->m->nrest > 0 ? ~0u : 0u;
+  ca.wmin = p->desc.k_select == 1 ? p->wmin.as<unsigned long long>() : nullptr;
   if (more && p->m->compact_select_fn) {
     // 128 workgroups of 1024 threads compact; the last one selects from B
-    SelectArgs sa = select_args(p, B);
+    SelectArgs sa = select_args(p, B, GPUGO_FUSED_WG);
     ca.poll = nullptr;
     void *fargs[] = {&ca, &sa, &poll};
-    HIPCHECK(hipModuleLaunchKernel(p->m->compact_select_fn, 128, 1, 1, SEL_T, 1, 1, 0, p->stream, fargs, nullptr));
+    HIPCHECK(hipModuleLaunchKernel(p->m->compact_select_fn, GPUGO_FUSED_WG, 1, 1, SEL_T, 1, 1, 0, p->stream, fargs,
+                                   nullptr));
     p->selected_ahead = true;
   } else if (p->m->compact_fn) {
     void *cargs[] = {&ca};
-    HIPCHECK(hipModuleLaunchKernel(p->m->compact_fn, 512, 1, 1, 256, 1, 1, 0, p->stream, cargs, nullptr));
+    HIPCHECK(hipModuleLaunchKernel(p->m->compact_fn, GPUGO_COMPACT_WG, 1, 1, 256, 1, 1, 0, p->stream, cargs, nullptr));
   } else {
-    hipLaunchKernelGGL(k_compact, dim3(512), dim3(256), 0, p->stream, ca);
+    hipLaunchKernelGGL(k_compact, dim3(GPUGO_COMPACT_WG), dim3(256), 0, p->stream, ca);
     HIPCHECK(hipGetLastError());
   }
   ++p->steps_enqueued;
@@ -1382,6 +1398,8 @@ static int persist_steps(gpugo_problem_t p, int64_t iters) {
   sa.ext_n = 0;
   sa.pext = nullptr;
   sa.tie_xor = 0u;
+  sa.nwmin = 0;
+  sa.wmin = nullptr;
   for (int b = 0; b < 2; ++b) {
     pa.q_lb[b] = p->q[b].plb();
     pa.q_par[b] = p->q[b].ppar();
