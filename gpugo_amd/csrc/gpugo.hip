@@ -969,7 +969,7 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
   if (!m->bound[desc->rounding])
     return set_err(GPUGO_E_INVALID, "module has no variant for rounding %d", desc->rounding);
   if (m->nrest > 0) {
-    // windowed (large-d) module: natural-extension bound, midpoint sample
+    // windowed (large-d) module: natural-extension bound, diagonal samples
     if (desc->flags & (GPUGO_FLAG_MEAN_VALUE | GPUGO_FLAG_SPLIT | GPUGO_FLAG_PERSIST))
       return set_err(GPUGO_E_INVALID, "windowed modules take no mean-value, split or persistent flags");
     if (desc->precision != GPUGO_PREC_F64) return set_err(GPUGO_E_INVALID, "windowed modules are FP64 only");

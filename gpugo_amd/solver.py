@@ -184,7 +184,7 @@ class Problem:
         one cooperative launch with grid barriers between their phases.
         windowed: the large-d form (loop templates only; each template sum
         is kept per parent record outside its split window, so a child
-        costs the terms its window touches; FP64, midpoint sample)."""
+        costs the terms its window touches; FP64, any n_samples)."""
         if precision not in _lib.PREC:
             raise ValueError(f"precision must be one of {sorted(_lib.PREC)}")
         self.module = module or get_module(obj, roundings=(rounding,), f32=precision == "fp32", split=split,
