@@ -1055,7 +1055,10 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
         hipSuccess)
       return fail(set_err(GPUGO_E_HIP, "upload: %s", hipGetErrorString(e)));
   }
-  if ((e = hipHostMalloc((void **)&p->hpoll, 2 * sizeof(int), hipHostMallocDefault)) != hipSuccess)
+  // coherent (fine-grained): the compaction's system-scope store of the
+  // status reaches host memory without depending on HIP_HOST_COHERENT
+  if ((e = hipHostMalloc((void **)&p->hpoll, 2 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent)) !=
+      hipSuccess)
     return fail(set_err(GPUGO_E_NOMEM, "pinned poll buffer: %s", hipGetErrorString(e)));
   if ((e = hipHostGetDevicePointer((void **)&p->dpoll, p->hpoll, 0)) != hipSuccess)
     return fail(set_err(GPUGO_E_HIP, "poll buffer device address: %s", hipGetErrorString(e)));
