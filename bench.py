@@ -66,10 +66,10 @@ def parse():
     ap.add_argument("--rounding", choices=["fast", "exact"], default="fast")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--legs", default="fp32,c2,large_d,batch,served",
+    ap.add_argument("--legs", default="fp32,c2,large_d,xl_d,batch,served",
                     help="comma list besides the headline: fp32 (headline workload in the FP32-prune form), "
-                         "c2 (d=50), large_d (windowed form, d=1000), batch (4096 problems), "
-                         "served (Poisson); 'none' for the headline only")
+                         "c2 (d=50), large_d (windowed form, d=1000 and 2000), xl_d (Levy d=10000, one solve), "
+                         "batch (4096 problems), served (Poisson); 'none' for the headline only")
     ap.add_argument("--batch-n", type=int, default=4096)
     ap.add_argument("--seq-sample", type=int, default=256,
                     help="problems of the batch solved one at a time for the sequential baseline")
@@ -243,9 +243,11 @@ def leg_c2(G, pb, steps=5):
 
 
 LARGE_D = (("levy", 1000), ("rosenbrock", 1000), ("fu", 1000), ("rastrigin", 1000), ("levy", 2000))
+# the reference's levy10000.py dimension: one solve (~30 s), a queue of up to 2^31 boxes
+XL_D = (("levy", 10000),)
 
 
-def leg_large_d(G, cases=LARGE_D, reps=2, cpu_seconds=2.0):
+def leg_large_d(G, cases=LARGE_D, reps=2, cpu_seconds=2.0, max_queue=1 << 28, warm=True):
     """The windowed (large-d) form on reference objectives at d = 1000 and
     2000 (the reference's boxes and initial upper bounds, 3^12 children of
     one box per step, 10 diagonal samples, tol 1e-6, a queue of up to 2^28
@@ -266,8 +268,9 @@ def leg_large_d(G, cases=LARGE_D, reps=2, cpu_seconds=2.0):
         obj = F.objective_loops(name, d)
         m = G.get_module(obj, roundings=("exact", "fast"), windowed=True)
         p = G.Problem(obj, [], s.lo, s.hi, tol=1e-6, grid=grid, k_select=k, n_samples=S,
-                      init_ub=s.extra["init_ub"], module=m, windowed=True, max_queue=1 << 28, max_iters=20000)
-        p.solve()
+                      init_ub=s.extra["init_ub"], module=m, windowed=True, max_queue=max_queue, max_iters=40000)
+        if warm:
+            p.solve()
         p.timing(True)
         times, res = [], []
         for _ in range(reps):
@@ -531,6 +534,9 @@ def main():
     if "large_d" in legs:
         print("# leg large_d", file=sys.stderr, flush=True)
         out["large_d"] = leg_large_d(G)
+    if "xl_d" in legs:
+        print("# leg xl_d", file=sys.stderr, flush=True)
+        out["xl_d"] = leg_large_d(G, XL_D, reps=1, max_queue=1 << 31, warm=False)
     if "batch" in legs:
         print("# leg batch", file=sys.stderr, flush=True)
         out["batch"] = leg_batch(G, pb, args.batch_n, args.seq_sample)
