@@ -248,10 +248,10 @@ XL_D = (("levy", 10000),)
 
 
 def leg_large_d(G, cases=LARGE_D, reps=2, cpu_seconds=2.0, max_queue=1 << 28, warm=True):
-    """The windowed (large-d) form on reference objectives at d = 1000 and
-    2000 (the reference's boxes and initial upper bounds, 3^12 children of
-    one box per step, 10 diagonal samples, tol 1e-6, a queue of up to 2^28
-    boxes): time to a certified enclosure,
+    """The windowed (large-d) form on reference objectives at d = 1000 to
+    10,000 (the reference's boxes and initial upper bounds, 3^12 children of
+    one box per step, 10 diagonal samples, tol 1e-6, a queue of up to
+    max_queue boxes): time to a certified enclosure,
     boxes bounded per second and the roofline of the windowed split+bound
     kernel.  CPU beside it: the oracle (the reference method, every
     coordinate of every child) on Rosenbrock d = 1000 (the one of these it
@@ -260,8 +260,10 @@ def leg_large_d(G, cases=LARGE_D, reps=2, cpu_seconds=2.0, max_queue=1 << 28, wa
     from gpugo_amd import emit, families as F, problems as pb
     from oracle import oracle as O
     grid, k, S = (3, 12), 1, 10
-    out = {"workload": "reference objectives d=1000/2000, reference boxes and initial upper bounds, grid 3^12, "
-                       f"one box per step, {S} diagonal samples, tol 1e-6, windowed code lists"}
+    dims = "/".join(sorted({str(d) for _, d in cases}, key=int))
+    out = {"workload": f"reference objectives d={dims}, reference boxes and initial upper bounds, grid 3^12, "
+                       f"one box per step, {S} diagonal samples, tol 1e-6, windowed code lists, "
+                       f"queue up to 2^{max_queue.bit_length() - 1} boxes"}
     cpu_rate = None
     for name, d in cases:
         s = F.spec(name, d)
