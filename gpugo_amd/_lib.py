@@ -26,6 +26,7 @@ ROUND = {"exact": 0, "fast": 1, "digits15": 2}
 FLAG_MEAN_VALUE = 1
 FLAG_SPLIT = 4
 FLAG_PERSIST = 8
+FLAG_EAGER_COMPACT = 16
 STATUS = {0: "running", 1: "certified", 2: "x_converged", 3: "max_iters", 4: "exhausted", 5: "overflow"}
 
 EXPORTS = [

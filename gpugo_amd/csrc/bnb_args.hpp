@@ -18,7 +18,7 @@ enum gpugo_status {
 
 // layout version of the structs below; modules report it and the library
 // refuses a module built against another layout
-#define GPUGO_MODULE_ABI_V 3
+#define GPUGO_MODULE_ABI_V 4
 
 namespace gpugo {
 
@@ -51,6 +51,9 @@ struct DevState {
   int compact_final;             // 1: the stop step's compaction still has to run
   int fresh;                     // batch slots: 1 until the root step ran
   int cblocks;                   // batch slots: compaction workgroups finished this step
+  // one box per step with stale entries left in place between compactions:
+  long long dirty_lo;            // chunk minima from this queue index up are stale
+  long long consumed;            // queue index the last select consumed, or -1
 };
 
 // Arguments of the module kernel gpugo_bound_{exact,fast}.

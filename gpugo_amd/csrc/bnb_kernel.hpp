@@ -433,6 +433,9 @@ __device__ __forceinline__ void persist_impl(const PersistArgs &p) {
     ca.poll = nullptr;
     ca.tie_xor = 0u;
     ca.wmin = nullptr;
+    ca.cmin = nullptr;
+    ca.chunk_log2 = 0;
+    ca.pad_c = 0;
     compact_body(ca, blk, nblk);
     if (!grid_barrier(p.bar, nblk)) {
       if (threadIdx.x == 0) atomicOr(p.err, 1);

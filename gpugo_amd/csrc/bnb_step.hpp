@@ -117,6 +117,10 @@ struct SelectArgs {
   unsigned tie_xor;      // 0: equal bounds go oldest parent first; ~0u: newest first
   int nwmin;             // entries of wmin
   const unsigned long long *wmin;  // K = 1: the compaction's per-workgroup smallest keys, or null
+  int lazy;              // K = 1 over a queue that keeps consumed (NaN) and ruled-out (lb > ub)
+                         // entries between compactions: wmin always gives the box, stale
+                         // entries are skipped, and the chunks to rescan are recorded
+  int pad_lazy;
 };
 
 // Selection key (lb, parent, pos), unique per box.  Parent records are
@@ -237,7 +241,7 @@ __device__ __forceinline__ unsigned gc_rank(const unsigned *bits, const unsigned
 }
 
 __device__ long long collect_parents(const SelectArgs &a, long long N, long long used, unsigned *bits, unsigned *pre,
-                                     unsigned *scan_sh, unsigned &par) {
+                                     unsigned *scan_sh, unsigned &par, double ub) {
   static_assert(GC_WORDS == 4 * SEL_T, "four bitmap words per thread");
   const int tid = threadIdx.x;
   const unsigned d = (unsigned)a.d;
@@ -251,7 +255,9 @@ __device__ long long collect_parents(const SelectArgs &a, long long N, long long
 #pragma unroll
       for (int u = 0; u < SEL_U; ++u) {
         const long long i = i0 + (long long)u * SEL_T;
-        v[u] = i < N ? a.q_par[i] : 0xffffffffu;
+        // a lazy queue's stale entries (NaN or lb > ub: never selected
+        // again) reference nothing; their q_par is left as it is
+        v[u] = (i < N && (!a.lazy || a.q_lb[i] <= ub)) ? a.q_par[i] : 0xffffffffu;
       }
 #pragma unroll
       for (int u = 0; u < SEL_U; ++u) {
@@ -357,6 +363,7 @@ __device__ long long collect_parents(const SelectArgs &a, long long N, long long
     // every referenced index is below c0 + cn and maps at or below itself,
     // so rewritten entries are never taken up again by a later chunk
     for (long long i = tid; i < N; i += SEL_T) {
+      if (a.lazy && !(a.q_lb[i] <= ub)) continue;
       const long long r = (long long)a.q_par[i] - c0;
       if (r >= 0 && r < (long long)cn) a.q_par[i] = (unsigned)(kept + gc_rank(bits, pre, (unsigned)r));
     }
@@ -436,7 +443,7 @@ __device__ void select_body(const SelectArgs &a) {
     return;
   }
   const int K = (int)(N < (long long)a.k_select ? N : (long long)a.k_select);
-  if (N <= K) {
+  if (N <= K && !a.lazy) {
     for (long long i = tid; i < N; i += SEL_T) {
       skey[i] = qkey(a, i);
       sidx[i] = (unsigned)i;
@@ -470,6 +477,23 @@ __device__ void select_body(const SelectArgs &a) {
       }
     }
     __syncthreads();
+    if (a.lazy) {
+      // every entry is stale (ruled out since, or consumed): the live set is
+      // empty, as in the N == 0 case above
+      const unsigned long long kh = (unsigned long long)(skey[0] >> 64);
+      if (kh == ~0ull || !(key_value(kh) <= ub)) {
+        if (tid == 0) {
+          bool have_out = st->n_out > 0;
+          st->status = have_out ? GPUGO_ST_X_CONVERGED : GPUGO_ST_EXHAUSTED;
+          st->lb_global = have_out ? st->out_min_lb : -__builtin_huge_val();
+          st->width = rnd<double, ROUND_EXACT>::add_up(ub, -st->lb_global);
+          st->q_count = 0;
+          st->compact_count = 0;
+          st->compact_final = 1;
+        }
+        return;
+      }
+    }
   } else {
     // Fast path: one histogram pass over the live range [lbmin, lbmax]
     // (tracked by the compaction).  The bin map is monotone in lb, so every
@@ -658,12 +682,13 @@ __device__ void select_body(const SelectArgs &a) {
   const unsigned out_rank = block_excl_scan((tid < K && is_out) ? 1u : 0u, scan_sh, &n_outs);
   long long pbase = st->parents_used, room = a.p_cap;
   const long long obase = st->n_out;
-  if (pbase + (long long)n_split > room) {
+  const bool collected = pbase + (long long)n_split > room;
+  if (collected) {
     // arena full: drop the records no queue entry references (hist and skey
     // are free from here on); overflow if the kept ones and this step's
     // leave less than an eighth of the arena, so collections stay rare
     unsigned gpar = tid < K ? par : 0xffffffffu;
-    pbase = collect_parents(a, N, pbase, hist, reinterpret_cast<unsigned *>(skey), scan_sh, gpar);
+    pbase = collect_parents(a, N, pbase, hist, reinterpret_cast<unsigned *>(skey), scan_sh, gpar, ub);
     if (tid < K) par = gpar;
     room = a.p_cap - (a.p_cap >> 3);
   }
@@ -734,6 +759,12 @@ __device__ void select_body(const SelectArgs &a) {
     st->compact_count = 0;
     st->lbmin_key = ~0ull;
     st->lbmax_key = 0ull;
+    if (a.lazy) {
+      // this step's appends start at N; a collection renumbered parent
+      // records, which are part of the keys, so every chunk minimum is stale
+      st->dirty_lo = collected ? 0 : (N < st->dirty_lo ? N : st->dirty_lo);
+      st->consumed = idx;
+    }
   }
 }
 
@@ -751,6 +782,9 @@ struct CompactArgs {
   int *poll;             // or null: receives the problem's status (host-mapped pinned memory)
   unsigned tie_xor;      // key order of the next select (SelectArgs::tie_xor)
   unsigned long long *wmin;  // or null: [3 * nblk] smallest kept key (hi, lo) and its output index per workgroup
+  unsigned long long *cmin;  // or null: chunk minima of a lazy queue, [3 * chunks] key (hi, lo) and index
+  int chunk_log2;            // entries per chunk: 1 << chunk_log2
+  int pad_c;
 };
 
 // blk / nblk: this workgroup's index and the number of workgroups sharing
@@ -869,6 +903,108 @@ __device__ __forceinline__ void compact_body(const CompactArgs &a, int blk, int 
   if (blk == 0 && threadIdx.x == 0) {
     atomicMax((unsigned long long *)&st->queue_peak, (unsigned long long)n);
     st->compactions += 1;
+    st->dirty_lo = 0;  // entries moved: chunk minima (if kept) are stale
+  }
+}
+
+// Smaller of two keys (hi, lo) with their indices.
+__device__ __forceinline__ void key_min3(unsigned long long &h, unsigned long long &l, unsigned long long &o,
+                                         unsigned long long th, unsigned long long tl, unsigned long long to) {
+  if (th < h || (th == h && tl < l)) {
+    h = th;
+    l = tl;
+    o = to;
+  }
+}
+
+// Workgroup minimum of per-thread keys (any block size up to 1024); the
+// result is valid in thread 0.
+__device__ __forceinline__ void block_min3(unsigned long long &h, unsigned long long &l, unsigned long long &o) {
+  __shared__ unsigned long long s_k[16][3];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1)
+    key_min3(h, l, o, __shfl_xor(h, s, 64), __shfl_xor(l, s, 64), __shfl_xor(o, s, 64));
+  __syncthreads();
+  if (lane == 0) {
+    s_k[w][0] = h;
+    s_k[w][1] = l;
+    s_k[w][2] = o;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int q = 1; q < nw; ++q) key_min3(h, l, o, s_k[q][0], s_k[q][1], s_k[q][2]);
+}
+
+// ---------------------------------------------------------------------------
+// lazy step tail (one box per step): instead of compacting, refresh the
+// minima of the queue chunks that changed since the last select -- the
+// chunks from st->dirty_lo up (this step's appends) and the chunk of the
+// consumed entry -- keeping entries with lb <= ub, then reduce each
+// workgroup's chunks (chunk ch belongs to workgroup ch mod nblk) into
+// wmin[blk] for the select.  Consumed and ruled-out entries stay in place
+// until the next compaction; they are never the minimum of a live queue
+// (their keys are NaN or above ub, every live key is below).  After a stop
+// the pass is the stop step's compaction, as compact_body.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void lazy_chunks_body(const CompactArgs &a, int blk, int nblk) {
+  __shared__ unsigned long long s_best[3];
+  DevState *st = a.st;
+  if (st->status != GPUGO_ST_RUNNING) {
+    compact_body(a, blk, nblk);
+    return;
+  }
+  if (a.poll && blk == 0 && threadIdx.x == 0)
+    __hip_atomic_store(a.poll, st->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  long long n = st->q_count;
+  if (n > a.cap) n = a.cap;
+  const double ub = key_value(st->ub_key);
+  const int sh = a.chunk_log2;
+  const long long nch = (n + (1ll << sh) - 1) >> sh;
+  const long long dlo = st->dirty_lo, cons = st->consumed;
+  const long long cd = (dlo < n ? dlo : n) >> sh;       // first chunk with appends
+  const long long cc = (cons >= 0 && cons < n) ? cons >> sh : -1;
+  if (threadIdx.x == 0) {
+    s_best[0] = ~0ull;
+    s_best[1] = ~0ull;
+    s_best[2] = 0ull;
+  }
+  for (long long ch = blk; ch < nch; ch += nblk) {
+    if (ch < cd && ch != cc) continue;
+    const long long e0 = ch << sh, e1 = (e0 + (1ll << sh) < n) ? e0 + (1ll << sh) : n;
+    unsigned long long h = ~0ull, l = ~0ull, o = 0ull;
+    for (long long i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
+      const double lb = a.lb_in[i];
+      if (!(lb <= ub)) continue;
+      const unsigned long long kh = order_key(lb);
+      if (kh > h) continue;
+      const unsigned long long kl = ((unsigned long long)(a.par_in[i] ^ a.tie_xor) << 32) | a.pos_in[i];
+      key_min3(h, l, o, kh, kl, (unsigned long long)i);
+    }
+    block_min3(h, l, o);
+    if (threadIdx.x == 0) {
+      a.cmin[3 * ch] = h;
+      a.cmin[3 * ch + 1] = l;
+      a.cmin[3 * ch + 2] = o;
+      key_min3(s_best[0], s_best[1], s_best[2], h, l, o);
+    }
+  }
+  // the workgroup's other chunks are unchanged: their minima from earlier passes
+  unsigned long long h = ~0ull, l = ~0ull, o = 0ull;
+  for (long long ch = blk + (long long)threadIdx.x * nblk; ch < nch; ch += (long long)blockDim.x * nblk) {
+    if (ch >= cd || ch == cc) continue;
+    key_min3(h, l, o, a.cmin[3 * ch], a.cmin[3 * ch + 1], a.cmin[3 * ch + 2]);
+  }
+  block_min3(h, l, o);
+  if (threadIdx.x == 0) {
+    key_min3(h, l, o, s_best[0], s_best[1], s_best[2]);
+    a.wmin[3 * blk] = h;
+    a.wmin[3 * blk + 1] = l;
+    a.wmin[3 * blk + 2] = o;
+    if (blk == 0) {
+      st->compact_count = n;
+      atomicMax((unsigned long long *)&st->queue_peak, (unsigned long long)n);
+    }
   }
 }
 
@@ -971,7 +1107,8 @@ namespace gpugo {
 // its gap per step.  `poll` (or null) receives the status after the select.
 __device__ __forceinline__ void compact_select_body(const CompactArgs &ca, const SelectArgs &sa, int *poll) {
   __shared__ int s_last;
-  compact_body(ca, blockIdx.x, gridDim.x);
+  if (ca.cmin) lazy_chunks_body(ca, blockIdx.x, gridDim.x);
+  else compact_body(ca, blockIdx.x, gridDim.x);
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
@@ -979,6 +1116,11 @@ __device__ __forceinline__ void compact_select_body(const CompactArgs &ca, const
     s_last = t == (int)gridDim.x - 1;
     if (s_last) {
       ca.st->cblocks = 0;
+      if (ca.cmin) {
+        // every workgroup has read these; all chunk minima are current now
+        ca.st->dirty_lo = 0x7fffffffffffffffll;
+        ca.st->consumed = -1;
+      }
       __threadfence();
     }
   }

@@ -663,6 +663,8 @@ __global__ __launch_bounds__(SEL_T) void kb_select(BatchArgs b) {
   sa.tie_xor = 0u;
   sa.nwmin = 0;
   sa.wmin = nullptr;
+  sa.lazy = 0;
+  sa.pad_lazy = 0;
   select_body(sa);
 }
 
@@ -867,7 +869,11 @@ __global__ __launch_bounds__(256) void k_materialize(const double *plo, const do
 // in p->wmin for a one-box select.
 #define GPUGO_COMPACT_WG 512   // gpugo_compact, 256 threads
 #define GPUGO_FUSED_WG 128     // gpugo_compact_select, 1024 threads
+#define GPUGO_LAZY_WG 512      // gpugo_compact_select as the lazy tail, 1024 threads
 #define GPUGO_WMIN_MAX GPUGO_COMPACT_WG
+#define GPUGO_CHUNK_LOG2 13    // lazy queue: 8192 entries per chunk minimum
+#define GPUGO_LAZY_MAX_EVERY 64
+static_assert(GPUGO_LAZY_WG <= GPUGO_WMIN_MAX, "wmin holds one key per lazy-tail workgroup");
 
 struct Queue {
   DevBuf lb, par, pos;
@@ -887,6 +893,14 @@ struct gpugo_problem_s {
   DevBuf bar;          // persistent step loop: grid barrier [2] and error flag
   DevBuf pext;         // windowed modules: rest sums per parent record [p_cap][ext_n]
   DevBuf wmin;         // k_select 1: per-compaction-workgroup smallest key [GPUGO_WMIN_MAX][3]
+  DevBuf cmin;         // lazy queue: smallest key per chunk of 2^GPUGO_CHUNK_LOG2 entries, [3] each
+  // One box per step: the queue is compacted every compact_every steps; the
+  // steps between leave consumed and ruled-out entries in place and refresh
+  // chunk minima instead (lazy tails).
+  bool lazy = false;
+  int compact_every = 1;
+  int since_compact = 0;
+  long long lazy_steps = 0;  // lazy tails enqueued (they keep the queue in its buffer)
   int ext_n = 0;
   int persist_blocks = 0;
   Queue q[2];
@@ -950,7 +964,7 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
     return set_err(GPUGO_E_INVALID, "unknown precision %d", desc->precision);
   if (desc->precision == GPUGO_PREC_F32 && !m->f32)
     return set_err(GPUGO_E_INVALID, "module has no FP32 code list (emit it with f32=True)");
-  if (desc->flags & ~(GPUGO_FLAG_MEAN_VALUE | GPUGO_FLAG_SPLIT | GPUGO_FLAG_PERSIST))
+  if (desc->flags & ~(GPUGO_FLAG_MEAN_VALUE | GPUGO_FLAG_SPLIT | GPUGO_FLAG_PERSIST | GPUGO_FLAG_EAGER_COMPACT))
     return set_err(GPUGO_E_INVALID, "unknown flags 0x%x", desc->flags);
   if ((desc->flags & GPUGO_FLAG_SPLIT) && m->split_g < 2)
     return set_err(GPUGO_E_INVALID, "module has no term-parallel kernels (emit it with split=G)");
@@ -1028,6 +1042,17 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
   PALLOC(p->bar, 64);
   if (p->ext_n > 0) PALLOC(p->pext, sizeof(double) * p->ext_n * pc);
   PALLOC(p->wmin, sizeof(unsigned long long) * 3 * GPUGO_WMIN_MAX);
+  if (desc->k_select == 1 && !(desc->flags & (GPUGO_FLAG_PERSIST | GPUGO_FLAG_EAGER_COMPACT))) {
+    // stale entries between compactions stay under an eighth of the queue:
+    // (compact_every - 1) steps append at most W entries each
+    const long long every = std::min((long long)GPUGO_LAZY_MAX_EVERY, qc / 8 / (long long)p->W);
+    if (every >= 2) {
+      p->lazy = true;
+      p->compact_every = (int)every;
+      const long long nch = (qc + (1LL << GPUGO_CHUNK_LOG2) - 1) >> GPUGO_CHUNK_LOG2;
+      PALLOC(p->cmin, sizeof(unsigned long long) * 3 * nch);
+    }
+  }
   for (int b = 0; b < 2; ++b) {
     PALLOC(p->q[b].lb, sizeof(double) * qc);
     PALLOC(p->q[b].par, sizeof(unsigned) * qc);
@@ -1100,6 +1125,8 @@ extern "C" int gpugo_problem_reset(gpugo_problem_t p) {
   p->started = false;
   p->selected_ahead = false;
   p->steps_enqueued = 0;
+  p->since_compact = 0;
+  p->lazy_steps = 0;
   p->tev_used = 0;
   p->bound_ms = 0;
   p->bound_launches = 0;
@@ -1187,16 +1214,20 @@ static SelectArgs select_args(gpugo_problem_t p, const Queue &q, int nwg) {
   // one box per step: the previous compaction's per-workgroup smallest keys
   sa.wmin = p->desc.k_select == 1 ? p->wmin.as<unsigned long long>() : nullptr;
   sa.nwmin = sa.wmin ? nwg : 0;
+  sa.lazy = p->lazy ? 1 : 0;
+  sa.pad_lazy = 0;
   return sa;
 }
 
 // One step: select (unless the previous compaction ran it), the windowed
-// modules' rest sums, split+bound, compact.  poll: pinned status word
-// written after the step (or null).  more: another step follows in this
-// call, so the compaction may run its select.
+// modules' rest sums, split+bound, compact (or, for a lazy queue, refresh
+// the chunk minima).  poll: pinned status word written after the step (or
+// null).  more: another step follows in this call, so the compaction may
+// run its select.  A call's last step always compacts.
 static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
   const int d = p->desc.d;
-  const int cur = (int)(p->steps_enqueued & 1);
+  // every compaction moves the queue to the other buffer, a lazy tail does not
+  const int cur = (int)((p->steps_enqueued - p->lazy_steps) & 1);
   Queue &A = p->q[cur], &B = p->q[cur ^ 1];
   DevState *st = p->st.as<DevState>();
   if (p->steps_enqueued > 0 && !p->selected_ahead) {
@@ -1278,6 +1309,27 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
   ca.poll = poll;
   ca.tie_xor = p->m->nrest > 0 ? ~0u : 0u;
   ca.wmin = p->desc.k_select == 1 ? p->wmin.as<unsigned long long>() : nullptr;
+  ca.cmin = nullptr;
+  ca.chunk_log2 = GPUGO_CHUNK_LOG2;
+  ca.pad_c = 0;
+  const bool lazy_tail = p->lazy && more && p->m->compact_select_fn && p->steps_enqueued > 0 &&
+                         p->since_compact + 1 < p->compact_every;
+  if (lazy_tail) {
+    // the queue stays in A; 512 workgroups refresh the changed chunk
+    // minima, the last one selects from A
+    ca.cmin = p->cmin.as<unsigned long long>();
+    SelectArgs sa = select_args(p, A, GPUGO_LAZY_WG);
+    ca.poll = nullptr;
+    void *fargs[] = {&ca, &sa, &poll};
+    HIPCHECK(hipModuleLaunchKernel(p->m->compact_select_fn, GPUGO_LAZY_WG, 1, 1, SEL_T, 1, 1, 0, p->stream, fargs,
+                                   nullptr));
+    p->selected_ahead = true;
+    ++p->lazy_steps;
+    ++p->since_compact;
+    ++p->steps_enqueued;
+    return GPUGO_OK;
+  }
+  p->since_compact = 0;
   if (more && p->m->compact_select_fn) {
     // 128 workgroups of 1024 threads compact; the last one selects from B
     SelectArgs sa = select_args(p, B, GPUGO_FUSED_WG);
@@ -1403,6 +1455,8 @@ static int persist_steps(gpugo_problem_t p, int64_t iters) {
   sa.tie_xor = 0u;
   sa.nwmin = 0;
   sa.wmin = nullptr;
+  sa.lazy = 0;
+  sa.pad_lazy = 0;
   for (int b = 0; b < 2; ++b) {
     pa.q_lb[b] = p->q[b].plb();
     pa.q_par[b] = p->q[b].ppar();

@@ -179,6 +179,17 @@ int gpugo_window_ranges(int64_t n, const int32_t *args, int32_t *out);
                                     separated by grid barriers on resident
                                     workgroups; no launch gaps, no host polling).
                                     Same results as GPUGO_FLAG_SPLIT alone.     */
+#define GPUGO_FLAG_EAGER_COMPACT 16 /* k_select 1: compact the box queue after
+                                    every step.  By default a one-box-per-step
+                                    problem compacts every M steps (M <= 64,
+                                    M * wid^cycle <= max_queue / 8); the steps
+                                    between leave consumed and ruled-out
+                                    entries in place and find the next box
+                                    from per-chunk minimum keys.  Same boxes,
+                                    bounds and results either way; queue_peak
+                                    then counts the stale entries too, and the
+                                    queue can fill up to M * wid^cycle entries
+                                    sooner.                                    */
 
 /* ---- problems -------------------------------------------------------------
  * One box-constrained minimisation.  Replaces the module-level constants and

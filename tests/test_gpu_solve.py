@@ -164,3 +164,42 @@ def test_fused_select_matches_separate(gpu, spec, grid, k, kw):
                 break
     for key in ("lb", "ub", "width", "iterations", "boxes_bounded", "pruned_value", "pruned_kkt", "status"):
         assert a[key] == b[key], (key, a[key], b[key])
+
+
+LAZY_KEYS = ("lb", "ub", "width", "iterations", "boxes_bounded", "pruned_value", "pruned_kkt", "status",
+             "queue_size", "n_out")
+
+
+@pytest.mark.parametrize("spec,grid,kw", [
+    (pb.sepcos2(), (64, 2), {}),
+    (pb.sepcos2(), (64, 2), {"max_queue": 1 << 16}),   # compaction every 2 steps
+    (pb.chain10(), (2, 5), {"rounding": "exact"}),
+    (pb.rosenbrock(4), (4, 3), {"rounding": "exact", "mean_value": True}),
+    (pb.rastrigin(2), (4, 2), {"tol_x": 1e-3, "tol": 0.0}),  # x-width outputs
+    (pb.chain10(), (2, 5), {"max_parents": 64}),        # parent records collected
+], ids=lambda v: getattr(v, "name", None))
+def test_lazy_queue_matches_eager(gpu, spec, grid, kw):
+    """One box per step: the default queue (compacted every few steps,
+    per-chunk minimum keys between, consumed and ruled-out entries left in
+    place) selects the same boxes as compaction after every step, so the
+    solve is the same; its queue_peak also counts the stale entries."""
+    kw = dict(kw)
+    tol = kw.pop("tol", 1e-6)
+    out = []
+    for eager in (False, True):
+        with gpu_problem(gpu, spec, tol=tol, grid=grid, k_select=1, max_iters=20000, eager_compact=eager,
+                         **kw) as p:
+            try:
+                out.append(p.solve())
+            except GpugoError as e:
+                out.append({"error": str(e)})
+    a, b = out
+    assert ("error" in a) == ("error" in b), (a, b)
+    if "error" in a:
+        assert a["error"] == b["error"]
+        return
+    assert a["status_name"] in ("certified", "x_converged"), a
+    assert a["iterations"] > 2
+    for key in LAZY_KEYS:
+        assert a[key] == b[key], (key, a[key], b[key])
+    assert a["queue_peak"] >= b["queue_peak"]

@@ -150,6 +150,27 @@ def test_window_small_arena_equals_default(gpu):
     assert collected, "no arena size exercised the collection"
 
 
+@pytest.mark.parametrize("name,d,S,kw", [("levy", 50, 10, {}), ("rastrigin", 50, 10, {}),
+                                         ("rosenbrock", 50, 1, {"max_queue": 1 << 24}),
+                                         ("rastrigin", 6, 1, {"tol": 1e-9, "grid": (2, 2), "max_parents": 16})])
+def test_window_lazy_queue_matches_eager(gpu, name, d, S, kw):
+    """The default one-box-per-step queue (compacted every few steps, chunk
+    minima between) against compaction after every step: the same solve
+    (with a small parent arena: records are collected while stale entries
+    are in the queue)."""
+    kw = dict(kw)
+    tol, grid = kw.pop("tol", 1e-6), kw.pop("grid", (3, 12))
+    m = wmod(gpu, F.objective_loops(name, d))
+    a, s = ref_solve(gpu, name, d, grid, 1, S, m, True, tol=tol, max_iters=20000, **kw)
+    b, _ = ref_solve(gpu, name, d, grid, 1, S, m, True, tol=tol, max_iters=20000, eager_compact=True, **kw)
+    assert a["status_name"] == "certified" and a["iterations"] > 2, a
+    if "max_parents" in kw:
+        assert a["iterations"] + 1 > kw["max_parents"], "the arena was never collected"
+    for key in RES_KEYS + ("width", "pruned_value", "pruned_kkt"):
+        assert a[key] == b[key], (key, a[key], b[key])
+    assert a["queue_peak"] >= b["queue_peak"]
+
+
 @pytest.mark.parametrize("name", [n for n in WIN if n != "breiman"])
 def test_window_d50_certifies(gpu, name):
     d = 50
