@@ -33,28 +33,28 @@ __device__ __forceinline__ void prep_impl(const BoundArgs &a) {
   __shared__ iv<double> part[16];
   const DevState *st = a.st;
   if (st->status != 0) return;
-  const int b = blockIdx.x;
+  // grid (pairs, parents): workgroup (q, b) sums template s = q mod NREST at
+  // point k = q / NREST (0: the parent box, k > 0: diagonal sample k - 1)
+  const int b = blockIdx.y;
   if (b >= st->n_split) return;
+  const int S = a.n_samples;
+  const int q = blockIdx.x, k = q / GPUGO_NREST, s = q - k * GPUGO_NREST;
+  if (k > S) return;
   const long long p = st->first_parent + b;
-  const int S = a.n_samples, win = a.pwin[p];
+  const int win = a.pwin[p];
   const RowBox x{a.plo + p * GPUGO_D, a.phi + p * GPUGO_D};
   iv<double> *out = (iv<double> *)(a.prest) + p * (long long)((1 + S) * GPUGO_NREST);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
-  for (int k = 0; k <= S; ++k) {
-    for (int s = 0; s < GPUGO_NREST; ++s) {
-      iv<double> v = k == 0 ? obj_rest<R>(s, x, a.params, a.d, win, a.c, tid, (int)blockDim.x)
-                            : obj_rest<R>(s, DiagPoint<RowBox>{x, k - 1, S}, a.params, a.d, win, a.c, tid,
-                                          (int)blockDim.x);
-      v = group_sum<R, 64>(v);
-      if (lane == 0) part[w] = v;
-      __syncthreads();
-      if (tid == 0) {
-        iv<double> t = part[0];
-        for (int q = 1; q < nw; ++q) t = A::add(t, part[q]);
-        out[k * GPUGO_NREST + s] = t;
-      }
-      __syncthreads();
-    }
+  iv<double> v = k == 0 ? obj_rest<R>(s, x, a.params, a.d, win, a.c, tid, (int)blockDim.x)
+                        : obj_rest<R>(s, DiagPoint<RowBox>{x, k - 1, S}, a.params, a.d, win, a.c, tid,
+                                      (int)blockDim.x);
+  v = group_sum<R, 64>(v);
+  if (lane == 0) part[w] = v;
+  __syncthreads();
+  if (tid == 0) {
+    iv<double> t = part[0];
+    for (int i = 1; i < nw; ++i) t = A::add(t, part[i]);
+    out[k * GPUGO_NREST + s] = t;
   }
 }
 

@@ -872,7 +872,7 @@ __global__ __launch_bounds__(256) void k_materialize(const double *plo, const do
 #define GPUGO_LAZY_WG 512      // gpugo_compact_select as the lazy tail, 1024 threads
 #define GPUGO_WMIN_MAX GPUGO_COMPACT_WG
 #define GPUGO_CHUNK_LOG2 13    // lazy queue: 8192 entries per chunk minimum
-#define GPUGO_LAZY_MAX_EVERY 64
+#define GPUGO_LAZY_MAX_EVERY 256
 static_assert(GPUGO_LAZY_WG <= GPUGO_WMIN_MAX, "wmin holds one key per lazy-tail workgroup");
 
 struct Queue {
@@ -1267,10 +1267,13 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
   ba.prest = p->pext.as<double>();
   ba.nrest = p->m->nrest;
   if (p->m->nrest > 0) {
-    // rest sums of the parent records this step splits (one workgroup each)
+    // rest sums of the parent records this step splits: one workgroup per
+    // (point, template sum) pair and parent
     const unsigned nb = p->steps_enqueued == 0 ? 1u : (unsigned)p->desc.k_select;
+    const unsigned npairs = (unsigned)((1 + p->desc.n_samples) * p->m->nrest);
     void *args[] = {&ba};
-    HIPCHECK(hipModuleLaunchKernel(p->m->prep[p->desc.rounding], nb, 1, 1, 256, 1, 1, 0, p->stream, args, nullptr));
+    HIPCHECK(hipModuleLaunchKernel(p->m->prep[p->desc.rounding], npairs, nb, 1, 256, 1, 1, 0, p->stream, args,
+                                   nullptr));
   }
   {
     const bool split = (p->desc.flags & GPUGO_FLAG_SPLIT) != 0;

@@ -181,7 +181,7 @@ int gpugo_window_ranges(int64_t n, const int32_t *args, int32_t *out);
                                     Same results as GPUGO_FLAG_SPLIT alone.     */
 #define GPUGO_FLAG_EAGER_COMPACT 16 /* k_select 1: compact the box queue after
                                     every step.  By default a one-box-per-step
-                                    problem compacts every M steps (M <= 64,
+                                    problem compacts every M steps (M <= 256,
                                     M * wid^cycle <= max_queue / 8); the steps
                                     between leave consumed and ruled-out
                                     entries in place and find the next box
