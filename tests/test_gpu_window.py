@@ -20,10 +20,13 @@ reference evaluates every coordinate for every child
   - with a parent arena small enough that records are collected (their rest
     sums move with them) the solve equals the default arena's;
   - d = 50 for every windowable reference family, d = 1000 for Levy,
-    Rosenbrock, Fu and Rastrigin and d = 2000 and 5000 for Levy (the
-    reference's levy1000.py etc. boxes and initial upper bounds, 10 diagonal
-    samples, 3^12 children per step, one box per step, a queue of up to 2^29
-    boxes): certified enclosures of the closed-form f*.
+    Rosenbrock, Fu and Rastrigin and d = 2000, 5000 and 10,000 for Levy (the
+    reference's levy1000.py ... levy10000.py boxes and initial upper bounds,
+    10 diagonal samples, 3^12 children per step, one box per step, a queue of
+    up to 2^29 boxes, 2^31 at d = 10,000): certified enclosures of the
+    closed-form f*;
+  - one box per step, the default queue (compacted every few steps, chunk
+    minima between) gives the same solve as compaction after every step.
 """
 import mpmath as mp
 import numpy as np
@@ -181,9 +184,9 @@ def test_window_d50_certifies(gpu, name):
 
 
 @pytest.mark.parametrize("name,d", [("levy", 1000), ("rosenbrock", 1000), ("fu", 1000), ("rastrigin", 1000),
-                                    ("levy", 2000), ("levy", 5000)])
+                                    ("levy", 2000), ("levy", 5000), ("levy", 10000)])
 def test_window_large_d_certifies(gpu, name, d):
     r, s = ref_solve(gpu, name, d, (3, 12), 1, 10, wmod(gpu, F.objective_loops(name, d)), True,
-                     max_iters=20000, max_queue=1 << 29)
+                     max_iters=20000, max_queue=1 << (31 if d > 5000 else 29))
     assert r["status_name"] == "certified", (name, r)
     assert r["lb"] <= s.f_star <= r["ub"] and r["ub"] - r["lb"] <= 1e-6, (name, r)
