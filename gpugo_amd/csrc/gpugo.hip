@@ -1320,7 +1320,7 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
   const bool lazy_tail = p->lazy && more && p->m->compact_select_fn && p->steps_enqueued > 0 &&
                          p->since_compact + 1 < p->compact_every;
   if (lazy_tail) {
-    // the queue stays in A; 512 workgroups refresh the changed chunk
+    // the queue stays in A; 256 workgroups refresh the changed chunk
     // minima, the last one selects from A
     ca.cmin = p->cmin.as<unsigned long long>();
     SelectArgs sa = select_args(p, A, GPUGO_LAZY_WG);
