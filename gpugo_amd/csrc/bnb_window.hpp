@@ -216,6 +216,18 @@ extern "C" __global__ __launch_bounds__(64) void gpugo_eval_exact(gpugo::EvalArg
   gpugo::eval_w_impl<gpugo::ROUND_EXACT>(a);
 }
 #endif
+#if GPUGO_WITH_DIGITS15
+// the IA of the paper's results (15 significant digits, never narrower than EXACT)
+extern "C" __global__ __launch_bounds__(256) void gpugo_bound_digits15(gpugo::BoundArgs a) {
+  gpugo::bound_w_impl<gpugo::ROUND_DIGITS15>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void gpugo_prep_digits15(gpugo::BoundArgs a) {
+  gpugo::prep_impl<gpugo::ROUND_DIGITS15>(a);
+}
+extern "C" __global__ __launch_bounds__(64) void gpugo_eval_digits15(gpugo::EvalArgs a) {
+  gpugo::eval_w_impl<gpugo::ROUND_DIGITS15>(a);
+}
+#endif
 #if GPUGO_WITH_FAST
 extern "C" __global__ __launch_bounds__(256) void gpugo_bound_fast(gpugo::BoundArgs a) {
   gpugo::bound_w_impl<gpugo::ROUND_FAST>(a);
@@ -236,7 +248,7 @@ extern "C" __global__ void gpugo_info(int *out) {
   out[0] = GPUGO_D;
   out[1] = GPUGO_NPARAMS;
   out[2] = GPUGO_MODULE_ABI_V;
-  out[3] = (GPUGO_WITH_EXACT ? 1 : 0) | (GPUGO_WITH_FAST ? 2 : 0);
+  out[3] = (GPUGO_WITH_EXACT ? 1 : 0) | (GPUGO_WITH_FAST ? 2 : 0) | (GPUGO_WITH_DIGITS15 ? 4 : 0);
   out[4] = 0;
   out[5] = 0;
   out[6] = GPUGO_NREST;

@@ -497,13 +497,14 @@ def window_module_source(obj: Objective, roundings=("fast",)) -> str:
     """hipRTC source of a windowed (large-d) module: loop-form code lists
     only (no per-coordinate unrolling, so the code does not grow with d),
     the windowed split+bound kernel and the per-parent rest-sum kernel."""
-    bad = set(roundings) - {"exact", "fast"}
+    bad = set(roundings) - {"exact", "fast", "digits15"}
     if bad or not roundings:
-        raise ValueError(f"roundings must be a non-empty subset of ('exact', 'fast'), got {roundings}")
+        raise ValueError(f"roundings must be a non-empty subset of ('exact', 'fast', 'digits15'), got {roundings}")
     return "\n".join([
         "// generated by gpugo_amd.emit (windowed form) for objective " + obj.name,
         f"#define GPUGO_WITH_EXACT {int('exact' in roundings)}",
         f"#define GPUGO_WITH_FAST {int('fast' in roundings)}",
+        f"#define GPUGO_WITH_DIGITS15 {int('digits15' in roundings)}",
         "#define GPUGO_WINDOWED 1",
         _read("ia.hpp"), _read("bnb_args.hpp"), _read("bnb_box.hpp"),
         window_lists(obj),

@@ -85,7 +85,8 @@ enum gpugo_round {
                             significant digits (outRound, n = 15,
                             1-Ackley/IAAckley.py:106-141); never narrower
                             than EXACT.  gpugo_ia_eval and single-problem
-                            modules (no split, persistent or FP32 form)       */
+                            modules, unrolled or windowed (no split,
+                            persistent or FP32 form)                          */
 };
 
 /* ---- library ----------------------------------------------------------- */

@@ -82,6 +82,62 @@ def test_window_eval_contains_truth(gpu, name, rounding):
                 assert g["glo"][i, j] - 1e-30 <= dj <= g["ghi"][i, j] + 1e-30, (name, i, j)
 
 
+@pytest.mark.parametrize("name", ["ackley", "levy", "rastrigin", "rosenbrock"])
+def test_window_digits15_eval(gpu, name):
+    """The paper's 15-significant-digit IA in a windowed module: on explicit
+    boxes the value enclosure contains the 256-bit value of the definition
+    and is no narrower than the EXACT-rounding enclosure of the same module
+    form (up to an ulp-sized slack), and wider than it by at most 1e-12 of
+    the magnitudes involved."""
+    d = 3
+    fam = F.REFERENCE[name]
+    obj = F.objective_loops(name, d)
+    m15 = gpu.get_module(obj, roundings=("digits15",), windowed=True)
+    mx = wmod(gpu, obj)
+    rng = np.random.default_rng(23)
+    n = 48
+    span = fam.hi - fam.lo
+    lo = fam.lo + rng.uniform(0, 1, (n, d)) * span
+    w = rng.choice([1e-7, 1e-3, 0.05, 0.5], (n, 1)) * span
+    hi = np.minimum(lo + rng.uniform(0, 1, (n, d)) * w, fam.hi)
+    a = m15.eval_boxes(np.zeros(0), lo, hi, rounding="digits15", want=("f",))
+    b = mx.eval_boxes(np.zeros(0), lo, hi, rounding="exact", want=("f",))
+    # each operation is held no narrower than EXACT and widens by about one
+    # 15-digit step of its own result (terms of these objectives are below
+    # ~1e2 where f is small); transcendental steps evaluated at other points
+    # may nest only up to an ulp, so the comparison allows 1e-15 of slack
+    mag = np.maximum(np.abs(b["flo"]), np.abs(b["fhi"])) + 100.0
+    assert np.all(a["flo"] <= b["flo"] + 1e-15 * mag) and np.all(a["fhi"] >= b["fhi"] - 1e-15 * mag)
+    assert np.all(a["fhi"] - a["flo"] >= b["fhi"] - b["flo"] - 1e-15 * mag)
+    assert np.all(np.abs(b["flo"] - a["flo"]) <= 1e-12 * mag)
+    assert np.all(np.abs(a["fhi"] - b["fhi"]) <= 1e-12 * mag)
+    mp.mp.prec = 256
+    for i in range(n):
+        x = lo[i] + rng.uniform(0, 1, d) * (hi[i] - lo[i])
+        t = f_ref(name, x)
+        assert a["flo"][i] <= t <= a["fhi"][i], (name, i)
+
+
+@pytest.mark.parametrize("name", ["levy", "rastrigin"])
+def test_window_digits15_solve(gpu, name):
+    """A windowed solve in the 15-digit IA (d = 50, the reference's box,
+    initial upper bound and 10 samples): certified, contains f*, and its
+    enclosure intersects the FAST-rounding solve's."""
+    d = 50
+    s = F.spec(name, d)
+    obj = F.objective_loops(name, d)
+    kw = dict(tol=1e-6, grid=(3, 12), k_select=1, n_samples=10, init_ub=s.extra["init_ub"], windowed=True,
+              max_iters=20000, max_queue=1 << 24)
+    print(f"# solve {name} d={d} digits15 windowed", flush=True)
+    a = gpu.minimize(obj, s.lo, s.hi, rounding="digits15",
+                     module=gpu.get_module(obj, roundings=("digits15",), windowed=True), **kw)
+    b = gpu.minimize(obj, s.lo, s.hi, rounding="fast", module=wmod(gpu, obj), **kw)
+    for r in (a, b):
+        assert r["status_name"] == "certified", r
+        assert r["lb"] <= s.f_star <= r["ub"] and r["width"] <= 1e-6
+    assert max(a["lb"], b["lb"]) <= min(a["ub"], b["ub"])
+
+
 def test_window_eval_rational_bit_identical(gpu):
     d = 3
     fam = F.REFERENCE["rosenbrock"]
