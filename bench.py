@@ -230,7 +230,10 @@ def leg_c2(G, pb, steps=5):
             "fp32_prune": {"time_to_certified_s": statistics.median(t32), "lb": r32["lb"], "ub": r32["ub"],
                            "width": r32["width"], "certified": r32["status_name"] == "certified",
                            "intersects_oracle": max(r32["lb"], o["lb"]) <= min(r32["ub"], o["ub"]),
-                           "note": "the first step has no incumbent, so nothing is ruled out in FP32 there"},
+                           "pruned_value": r32["pruned_value"], "boxes_bounded": r32["boxes_bounded"],
+                           "speedup_vs_fp64": gpu / statistics.median(t32),
+                           "note": "one step from the root: an FP32 sample pass over all children gives "
+                                   "that step its incumbent, then the FP32 value test rules children out"},
             "grid": list(grid), "k_select": k, "dtype": "f64",
             "time_to_certified_s": gpu, "certified": all(x["status_name"] == "certified" for x in res),
             "lb": r["lb"], "ub": r["ub"], "width": r["width"], "iterations": r["iterations"],

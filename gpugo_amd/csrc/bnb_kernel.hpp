@@ -471,6 +471,62 @@ __device__ __forceinline__ void eval_impl(const EvalArgs &a) {
 }
 
 #if GPUGO_WITH_F32
+// FP32 sample pass of the first step of an FP32-prune solve, before
+// split+bound (the reference's order: sampling, 1-Ackley/ackley50.py:602-684,
+// then rule-out against the lowered incumbent, :686-766).  The first step
+// has no incumbent yet, so the FP32 value test could rule nothing out
+// there.  The FP32 value code list at each child's S diagonal points, each
+// coordinate rounded outward (the enclosure contains f at the FP64 point: a
+// rigorous upper bound of the minimum); their minimum lowers the incumbent
+// and, by the last workgroup (ticket in st->cblocks, which it leaves at 0),
+// the step's pruning bound ub_snap.
+template <int R>
+__device__ __forceinline__ void sample32_impl(const BoundArgs &a) {
+  __shared__ unsigned long long s_ubmin;
+  DevState *st = a.st;
+  if (st->status != 0) return;
+  const long long n_child = (long long)st->n_split * (long long)a.W;
+  const long long first = st->first_parent;
+  const double invW = 1.0 / (double)a.W, halfInvW = 0.5 / (double)a.W;
+  const int lane = threadIdx.x & 63;
+  if (threadIdx.x == 0) s_ubmin = order_key(__builtin_huge_val());
+  __syncthreads();
+  double m = __builtin_huge_val();
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n_child;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long pl = (long long)__builtin_fma((double)t, invW, halfInvW);
+    const unsigned pos = (unsigned)(t - pl * (long long)a.W);
+    const long long p = first + pl;
+    const ChildBox x = child_box(a.plo + p * GPUGO_D, a.phi + p * GPUGO_D, a.pw + p * GPUGO_D, a.pwin[p], GPUGO_D,
+                                 a.c, (unsigned)a.wid, a.inv_wid, a.half_inv_wid, pos);
+    for (int ts = 0; ts < a.n_samples; ++ts) {
+      const DiagPoint<ChildBox> xs{x, ts, a.n_samples};
+      const F32Box<DiagPoint<ChildBox>> xf{xs};
+      const double u = (double)obj_value32<R>(xf, (const iv<float> *)a.params32, a.d).hi;
+      m = u < m ? u : m;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t2 = __shfl_xor(m, o, 64);
+    m = t2 < m ? t2 : m;
+  }
+  if (lane == 0 && m < __builtin_huge_val()) atomicMin(&s_ubmin, order_key(m));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_ubmin < order_key(__builtin_huge_val())) atomicMin(&st->ub_key, s_ubmin);
+    __threadfence();
+    if (atomicAdd(&st->cblocks, 1) == (int)gridDim.x - 1) {
+      __threadfence();
+      const double ub = key_value(__hip_atomic_load(&st->ub_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (ub < st->ub_snap) st->ub_snap = ub;
+      st->cblocks = 0;
+    }
+  }
+}
+#endif
+
+#if GPUGO_WITH_F32
 // FP32 value enclosure of explicit boxes (parity hook of the pruning pass);
 // a.ub carries the coefficient intervals (iv<float>[n_params])
 template <int R>
@@ -535,10 +591,16 @@ extern "C" __global__ __launch_bounds__(256) void gpugo_bound_fast_mv_g(gpugo::B
 extern "C" __global__ __launch_bounds__(64) void gpugo_eval32_exact(gpugo::EvalArgs a) {
   gpugo::eval32_impl<gpugo::ROUND_EXACT>(a);
 }
+extern "C" __global__ __launch_bounds__(256) void gpugo_sample32_exact(gpugo::BoundArgs a) {
+  gpugo::sample32_impl<gpugo::ROUND_EXACT>(a);
+}
 #endif
 #if GPUGO_WITH_F32 && GPUGO_WITH_FAST
 extern "C" __global__ __launch_bounds__(64) void gpugo_eval32_fast(gpugo::EvalArgs a) {
   gpugo::eval32_impl<gpugo::ROUND_FAST>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void gpugo_sample32_fast(gpugo::BoundArgs a) {
+  gpugo::sample32_impl<gpugo::ROUND_FAST>(a);
 }
 #endif
 #if GPUGO_WITH_DIGITS15

@@ -203,6 +203,7 @@ struct gpugo_module_s {
   hipFunction_t bound_mv[3] = {nullptr, nullptr, nullptr};  // [rounding], mean-value variant
   hipFunction_t eval[3] = {nullptr, nullptr, nullptr};
   hipFunction_t eval32[2] = {nullptr, nullptr};  // FP32 value (precision F32 modules)
+  hipFunction_t sample32[2] = {nullptr, nullptr};  // FP32 sample pass of a solve's first step
   hipFunction_t bound_g[2] = {nullptr, nullptr};     // term-parallel variants (split modules)
   hipFunction_t bound_mv_g[2] = {nullptr, nullptr};
   int split_g = 0;                                    // lanes per box of those variants
@@ -312,8 +313,14 @@ static int module_finish(gpugo_module_t m) {
       }
     }
   }
-  if (m->f32 && (h[3] & 1)) HIPCHECK(hipModuleGetFunction(&m->eval32[0], m->mod, "gpugo_eval32_exact"));
-  if (m->f32 && (h[3] & 2)) HIPCHECK(hipModuleGetFunction(&m->eval32[1], m->mod, "gpugo_eval32_fast"));
+  if (m->f32 && (h[3] & 1)) {
+    HIPCHECK(hipModuleGetFunction(&m->eval32[0], m->mod, "gpugo_eval32_exact"));
+    HIPCHECK(hipModuleGetFunction(&m->sample32[0], m->mod, "gpugo_sample32_exact"));
+  }
+  if (m->f32 && (h[3] & 2)) {
+    HIPCHECK(hipModuleGetFunction(&m->eval32[1], m->mod, "gpugo_eval32_fast"));
+    HIPCHECK(hipModuleGetFunction(&m->sample32[1], m->mod, "gpugo_sample32_fast"));
+  }
   if (m->abi != GPUGO_MODULE_ABI)
     return set_err(GPUGO_E_INVALID, "module ABI %d, library expects %d", m->abi, GPUGO_MODULE_ABI);
   const char *rname[3] = {"exact", "fast", "digits15"};
@@ -1276,6 +1283,16 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
     void *args[] = {&ba};
     HIPCHECK(hipModuleLaunchKernel(p->m->prep[p->desc.rounding], npairs, nb, 1, 256, 1, 1, 0, p->stream, args,
                                    nullptr));
+  }
+  if (p->steps_enqueued == 0 && p->desc.precision == GPUGO_PREC_F32 && p->desc.rounding < 2 &&
+      p->m->sample32[p->desc.rounding]) {
+    // first step of an FP32-prune solve: sample every child in FP32 first,
+    // so the FP32 value test has an incumbent in this step too
+    long long blocks = ((long long)p->W + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    void *args[] = {&ba};
+    HIPCHECK(hipModuleLaunchKernel(p->m->sample32[p->desc.rounding], (unsigned)blocks, 1, 1, 256, 1, 1, 0, p->stream,
+                                   args, nullptr));
   }
   {
     const bool split = (p->desc.flags & GPUGO_FLAG_SPLIT) != 0;

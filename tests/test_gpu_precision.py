@@ -77,3 +77,10 @@ def test_f32_prune_fp64_certify(gpu, oracle, spec, grid, k, mv):
     assert max(r["lb"], r64["lb"]) <= min(r["ub"], r64["ub"])
     if r["iterations"] > 1:
         assert r["pruned_value"] > 0
+    if spec.d == 50:
+        # one step from the root: the FP32 sample pass gives that step an
+        # incumbent, so FP32 rules children out there (the FP64 form, which
+        # samples the children that survive by value, rules out none in its
+        # first step)
+        assert r["iterations"] == r64["iterations"] == 1
+        assert r["pruned_value"] > 0 and r64["pruned_value"] == 0
