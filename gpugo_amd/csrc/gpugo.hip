@@ -876,9 +876,7 @@ __global__ __launch_bounds__(256) void k_materialize(const double *plo, const do
 // in p->wmin for a one-box select.
 #define GPUGO_COMPACT_WG 512   // gpugo_compact, 256 threads
 #define GPUGO_FUSED_WG 128     // gpugo_compact_select, 1024 threads
-#ifndef GPUGO_LAZY_WG
 #define GPUGO_LAZY_WG 256      // gpugo_compact_select as the lazy tail, 1024 threads
-#endif
 #define GPUGO_WMIN_MAX GPUGO_COMPACT_WG
 #define GPUGO_CHUNK_LOG2 13    // lazy queue: 8192 entries per chunk minimum
 #define GPUGO_LAZY_MAX_EVERY 256
