@@ -242,3 +242,30 @@ def test_window_module_emits():
     assert "#define GPUGO_NREST 2" in src and "for (int j = 0; j < d; ++j) {" in src
     with pytest.raises(ValueError):
         F.objective_loops("griewank", 10)
+    # the 15-significant-digit IA: its kernels are instantiated on request only
+    src = emit.window_module_source(F.objective_loops("levy", 50), ("digits15",))
+    assert "#define GPUGO_WITH_DIGITS15 1" in src and "#define GPUGO_WITH_FAST 0" in src
+    assert "gpugo_bound_digits15" in src and "gpugo_prep_digits15" in src
+    with pytest.raises(ValueError):
+        emit.window_module_source(F.objective_loops("levy", 50), ("nearest",))
+
+
+def header_defines():
+    txt = open(os.path.join(ROOT, "include", "gpugo.h")).read()
+    return {m.group(1): int(m.group(2)) for m in re.finditer(r"#define\s+(GPUGO_[A-Z0-9_]+)\s+(-?\d+)", txt)}
+
+
+def test_flags_match_header():
+    """The Python flag and status constants are the header's values."""
+    h = header_defines()
+    for name in ("FLAG_MEAN_VALUE", "FLAG_SPLIT", "FLAG_PERSIST", "FLAG_EAGER_COMPACT"):
+        assert getattr(_lib, name) == h["GPUGO_" + name], name
+    assert _lib.E_INVALID == h["GPUGO_E_INVALID"] and _lib.E_DOMAIN == h["GPUGO_E_DOMAIN"]
+    flags = [h[k] for k in h if k.startswith("GPUGO_FLAG_")]
+    assert len(set(flags)) == len(flags) and all(f & (f - 1) == 0 for f in flags)  # distinct single bits
+
+
+def test_problem_create_rejects_null():
+    """gpugo_problem_create checks its pointers before any device work."""
+    lib = _lib.load()
+    assert lib.gpugo_problem_create(None, None, None) == _lib.E_INVALID
