@@ -166,6 +166,22 @@ def test_fused_select_matches_separate(gpu, spec, grid, k, kw):
         assert a[key] == b[key], (key, a[key], b[key])
 
 
+def test_unknown_flag_rejected(gpu):
+    """A description with a flag bit the header does not define is refused
+    with GPUGO_E_INVALID and a message; the eager-compaction bit is known."""
+    import ctypes
+    from gpugo_amd import _lib
+    spec = pb.sepcos2()
+    with gpu_problem(gpu, spec, tol=1e-6, grid=(2, 2), k_select=1, eager_compact=True) as p:
+        lib = _lib.load()
+        d2 = _lib.ProblemDesc.from_buffer_copy(p.desc)
+        d2.flags = 32
+        h = ctypes.c_void_p()
+        assert lib.gpugo_problem_create(p.module.handle, ctypes.byref(d2), ctypes.byref(h)) == _lib.E_INVALID
+        assert b"unknown flags" in lib.gpugo_last_error()
+        assert p.solve()["status_name"] == "certified"
+
+
 LAZY_KEYS = ("lb", "ub", "width", "iterations", "boxes_bounded", "pruned_value", "pruned_kkt", "status",
              "queue_size", "n_out")
 
