@@ -108,10 +108,20 @@ def cpu_baseline(spec, sched, seconds):
         if times[-1] > seconds:
             break
     t = statistics.median(times)
+    # the same solve on one host thread (3 solves, median)
+    t1 = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.solve(fam, spec.params, spec.lo, spec.hi, tol_f=spec.tol, wid=sched["grid"][0], cycle=sched["grid"][1],
+                k_select=sched["k_select"], n_samples=1, max_iters=100000, threads=1,
+                mean_value=sched["mean_value"])
+        t1.append(time.perf_counter() - t0)
     return {"value": t, "unit": "s", "cores": threads, "kind": "port",
             "sample": f"{len(times)} complete solves of the same workload and schedule, median",
             "boxes_bounded_per_s": boxes[0] / t, "certified": res["status"] == 1,
-            "lb": res["lb"], "ub": res["ub"]}
+            "lb": res["lb"], "ub": res["ub"],
+            "one_core": {"value": statistics.median(t1), "unit": "s", "cores": 1,
+                         "sample": "3 complete solves on one host thread, median"}}
 
 
 def cpu_reference_schedule(spec, steps=3):
