@@ -166,6 +166,32 @@ def test_fused_select_matches_separate(gpu, spec, grid, k, kw):
         assert a[key] == b[key], (key, a[key], b[key])
 
 
+def test_lazy_queue_boxes_between_calls(gpu):
+    """A call's last step compacts, so between gpugo_step calls the live
+    boxes (gpugo_boxes_copy) and counters of the default one-box-per-step
+    queue equal those of compaction after every step."""
+    spec = pb.rastrigin(3)
+    kw = dict(tol=1e-6, grid=(4, 3), k_select=1, max_iters=20000)
+
+    def rows(p):
+        lo, hi, lb, n = p.boxes(1 << 16)
+        assert n == len(lb)
+        a = np.column_stack([lb, lo, hi])
+        return a[np.lexsort(a.T[::-1])]
+
+    with gpu_problem(gpu, spec, **kw) as a, gpu_problem(gpu, spec, eager_compact=True, **kw) as b:
+        calls = 0
+        for n in (2, 2, 3, 3, 5, 8, 13):
+            ra, rb = a.step(n), b.step(n)
+            for key in ("lb", "ub", "iterations", "boxes_bounded", "queue_size", "status"):
+                assert ra[key] == rb[key], (n, key, ra[key], rb[key])
+            assert np.array_equal(rows(a), rows(b)), n
+            calls += 1
+            if ra["status_name"] != "running":
+                break
+        assert calls >= 3 and ra["status_name"] == "certified"
+
+
 def test_unknown_flag_rejected(gpu):
     """A description with a flag bit the header does not define is refused
     with GPUGO_E_INVALID and a message; the eager-compaction bit is known."""
