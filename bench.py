@@ -267,9 +267,9 @@ def leg_large_d(G, cases=LARGE_D, reps=2, cpu_seconds=2.0, max_queue=1 << 28, wa
     max_queue boxes): time to a certified enclosure,
     boxes bounded per second and the roofline of the windowed split+bound
     kernel.  CPU beside it: the oracle (the reference method, every
-    coordinate of every child) on Rosenbrock d = 1000 (the one of these it
-    restates), one step of 3^8 children from the root, all host threads:
-    its boxes/s and the time it would need for the GPU's boxes."""
+    coordinate of every child) on Rosenbrock and Levy at d = 1000 (the ones
+    of these it restates), steps of 3^8 children from the root, all host
+    threads: its boxes/s and the time it would need for the GPU's boxes."""
     from gpugo_amd import emit, families as F, problems as pb
     from oracle import oracle as O
     grid, k, S = (3, 12), 1, 10
@@ -312,14 +312,16 @@ def leg_large_d(G, cases=LARGE_D, reps=2, cpu_seconds=2.0, max_queue=1 << 28, wa
                             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                             "flop_per_child_value": fl["value"], "avg_launch_ms": bound_ms / max(launches, 1),
                             "launches": launches}}
-        if name == "rosenbrock" and d == 1000:
-            spec = pb.rosenbrock(d)
+        if name in ("rosenbrock", "levy") and d == 1000:
+            # the oracle's restatements: Rosenbrock as family 1 (a = b = 0,
+            # w = 100), Levy as family 3 (no coefficients)
+            fam, params = (1, pb.rosenbrock(d).params) if name == "rosenbrock" else (3, np.zeros(0))
             threads = os.cpu_count() or 1
             O.build()
             t0 = time.perf_counter()
             nsteps, boxes = 0, 0
             while nsteps == 0 or time.perf_counter() - t0 < cpu_seconds:
-                o, _ = O.solve(1, spec.params, spec.lo, spec.hi, tol_f=1e-6, wid=3, cycle=8, k_select=1,
+                o, _ = O.solve(fam, params, s.lo, s.hi, tol_f=1e-6, wid=3, cycle=8, k_select=1,
                                n_samples=S, max_iters=1, threads=threads)
                 boxes += o["boxes_bounded"]
                 nsteps += 1

@@ -293,6 +293,10 @@ int oracle_ia_eval(int op, int64_t n, const double *alo, const double *ahi,
  * family 2, Ackley-type:     f = -A exp(-B sqrt(mean x^2)) - exp(mean cos(C x))
  *     + A + e + sum_i q_i x_i^4 / 50, params [A, B, C, q(d)].  Ackley
  *     (1-Ackley/ackley50.py:171-212) is A=20, B=0.02, C=2pi, q=0.
+ * family 3, Levy (6-Levy/levy50.py:166-224), no params:
+ *     f = (pi/d) [10 sin^2(pi/4 x_0 + 3pi/4)
+ *                 + sum_{i<d-1} ((x_i - 1)/4)^2 (1 + 10 sin^2(pi/4 x_{i+1} + 3pi/4))
+ *                 + ((x_{d-1} - 1)/4)^2],  pi held as the interval [PI_LO, PI_HI].
  * Sums are left folds in increasing i starting from [0,0]; the GPU code
  * lists use the same order.
  * ------------------------------------------------------------------------- */
@@ -303,8 +307,45 @@ static iv fam0_term(iv x, double a, double b, double c)
   return iv_add(t1, t3);
 }
 
+static const double PI_LO = 0x1.921fb54442d18p+1, PI_HI = 0x1.921fb54442d19p+1;
+
+/* k pi x + c pi */
+static iv levy_arg(iv x, double k, double c)
+{
+  const iv pi = mk(PI_LO, PI_HI);
+  return iv_add(iv_mul(x, iv_scale(pi, k)), iv_scale(pi, c));
+}
+/* ((x - 1)/4)^2 */
+static iv levy_sq4(iv x) { return iv_sqr(iv_scale(iv_addc(x, -1.0), 0.25)); }
+/* 1 + 10 sin^2(pi/4 x + 3pi/4) */
+static iv levy_w(iv x) { return iv_addc(iv_scale(iv_sqr(iv_sin(levy_arg(x, 0.25, 0.75))), 10.0), 1.0); }
+/* d/dx 10 sin^2(pi/4 x + 3pi/4) = (5pi/2) sin(pi/2 x + 3pi/2) */
+static iv levy_dw(iv x) { return iv_mul(iv_scale(mk(PI_LO, PI_HI), 2.5), iv_sin(levy_arg(x, 0.5, 1.5))); }
+
+static iv levy_value(int d, const iv *x)
+{
+  iv s = iv_scale(iv_sqr(iv_sin(levy_arg(x[0], 0.25, 0.75))), 10.0);
+  for (int i = 0; i + 1 < d; ++i) s = iv_add(s, iv_mul(levy_sq4(x[i]), levy_w(x[i + 1])));
+  s = iv_add(s, levy_sq4(x[d - 1]));
+  return iv_mul(iv_divc(mk(PI_LO, PI_HI), (double)d), s);
+}
+
+static void levy_grad(int d, const iv *x, iv *g)
+{
+  const iv k = iv_divc(mk(PI_LO, PI_HI), (double)d);
+  for (int j = 0; j < d; ++j) {
+    iv gj = mk(0.0, 0.0);
+    if (j == 0) gj = levy_dw(x[0]);
+    if (j + 1 < d) gj = iv_add(gj, iv_mul(iv_scale(iv_addc(x[j], -1.0), 0.125), levy_w(x[j + 1])));
+    if (j > 0) gj = iv_add(gj, iv_mul(levy_sq4(x[j - 1]), levy_dw(x[j])));
+    if (j + 1 == d) gj = iv_add(gj, iv_scale(iv_addc(x[j], -1.0), 0.125));
+    g[j] = iv_mul(k, gj);
+  }
+}
+
 static iv family_value(int fam, int d, const double *P, const iv *x)
 {
+  if (fam == 3) return levy_value(d, x);
   if (fam == 0 || fam == 1) {
     const double *a = P, *b = P + d, *c = P + 2 * d;
     iv s = mk(0.0, 0.0);
@@ -339,6 +380,10 @@ static iv family_value(int fam, int d, const double *P, const iv *x)
 /* Interval gradient; g has d entries. */
 static void family_grad(int fam, int d, const double *P, const iv *x, iv *g)
 {
+  if (fam == 3) {
+    levy_grad(d, x, g);
+    return;
+  }
   if (fam == 0 || fam == 1) {
     const double *a = P, *b = P + d, *c = P + 2 * d;
     for (int j = 0; j < d; ++j) {
@@ -479,7 +524,7 @@ int oracle_eval_boxes(int fam, int d, const double *P, int64_t n,
                       double *flo, double *fhi, double *glo, double *ghi,
                       int32_t *kkt, double *ub)
 {
-  if (fam < 0 || fam > 2 || d < 1) return -1;
+  if (fam < 0 || fam > 3 || d < 1) return -1;
   int old = fegetround();
   iv *x = malloc(sizeof(iv) * d), *g = malloc(sizeof(iv) * d), *xs = malloc(sizeof(iv) * d);
   for (int64_t i = 0; i < n; ++i) {
@@ -574,7 +619,7 @@ int oracle_solve(const oracle_problem *pr, gpugo_result *res,
   const int K = pr->k_select > 0 ? pr->k_select : 1;
   const int S = pr->n_samples > 0 ? pr->n_samples : 1;
   const double Wd = ipow_count(wid, c);
-  if (fam < 0 || fam > 2 || d < 1 || wid < 2 || c < 1 || Wd > 4294967295.0) return -1;
+  if (fam < 0 || fam > 3 || d < 1 || wid < 2 || c < 1 || Wd > 4294967295.0) return -1;
   const uint32_t W = (uint32_t)Wd;
   int nthreads = pr->threads;
 #ifdef _OPENMP

@@ -183,6 +183,26 @@ def test_window_coefficient_families_match_oracle(gpu, oracle, family):
         assert r["lb"] <= 0.0 <= r["ub"]
 
 
+@pytest.mark.parametrize("d", [3, 6])
+def test_levy_matches_oracle(gpu, oracle, d):
+    """Levy (the reference's box and initial upper bound, 10 diagonal
+    samples) through the windowed and the unrolled modules against the
+    oracle's restatement (family 3) under the same schedule: all certified,
+    pairwise intersecting enclosures of f* = 0."""
+    s = F.spec("levy", d)
+    grid, k = (3, 2), 16
+    o, _ = oracle.solve(3, np.zeros(0), s.lo, s.hi, tol_f=1e-6, wid=grid[0], cycle=grid[1], k_select=k,
+                        n_samples=10, init_ub=s.extra["init_ub"], max_iters=20000)
+    rw, _ = ref_solve(gpu, "levy", d, grid, k, 10, wmod(gpu, F.objective_loops("levy", d)), True,
+                      max_iters=20000, max_queue=1 << 22)
+    ru, _ = ref_solve(gpu, "levy", d, grid, k, 10, gpu.get_module(F.objective("levy", d), roundings=("exact", "fast")),
+                      False, max_iters=20000, max_queue=1 << 22)
+    assert o["status"] == 1 and o["lb"] <= 0.0 <= o["ub"]
+    for r in (rw, ru):
+        assert r["status_name"] == "certified" and r["lb"] <= 0.0 <= r["ub"], r
+        assert max(r["lb"], o["lb"]) <= min(r["ub"], o["ub"])
+
+
 def test_window_deterministic(gpu):
     m = wmod(gpu, F.objective_loops("levy", 50))
     a, _ = ref_solve(gpu, "levy", 50, (3, 12), 1, 10, m, True, max_queue=1 << 24)

@@ -84,3 +84,12 @@ def test_deterministic(oracle):
     b, _ = solve(oracle, spec, tol_f=1e-6, wid=2, cycle=5, k_select=64, max_iters=40, threads=3)
     for k in ("lb", "ub", "iterations", "boxes_bounded", "pruned_value", "pruned_kkt", "queue_size"):
         assert a[k] == b[k], k
+
+
+def test_levy_solve_certifies(oracle):
+    """The oracle's loop on its Levy restatement (d = 3, the reference box
+    and initial upper bound, 10 diagonal samples) certifies f* = 0."""
+    d = 3
+    o, _ = oracle.solve(3, np.zeros(0), np.full(d, -10.0), np.full(d, 10.0), tol_f=1e-6, wid=3, cycle=2,
+                        k_select=16, n_samples=10, init_ub=1000.0, max_iters=20000)
+    assert o["status"] == 1 and o["lb"] <= 0.0 <= o["ub"] and o["ub"] - o["lb"] <= 1e-6

@@ -139,3 +139,37 @@ def test_kkt_prunes_monotone_boxes(oracle):
     # box touches the boundary only on the low side -> gradient < 0 prunes
     r3 = oracle.eval_boxes(0, spec.params, lo3, hi3, spec.lo, spec.hi, 1)
     assert r3["kkt"][0] == 1
+
+
+@pytest.mark.parametrize("d", [2, 3, 5])
+def test_levy_restatement_contains_truth(oracle, d):
+    """Family 3 (Levy, 6-Levy/levy50.py:166-224) against the independent
+    256-bit definition: value and gradient enclosures on random boxes of the
+    reference box [-10, 10]^d contain f and mpmath's derivative at random
+    points, the sample bound is at least f at the midpoint, and the closed-
+    form minimum f(1, ..., 1) = 0 is enclosed with a gradient containing 0."""
+    from tests.test_families import f_ref
+    rng = np.random.default_rng([12, d])
+    n = 40
+    c = rng.uniform(-10, 10, (n, d))
+    w = 20 * 0.3 * rng.uniform(0, 1, (n, d)) * rng.uniform(0, 1, (n, 1)) ** 4
+    lo, hi = np.maximum(c - w, -10.0), np.minimum(c + w, 10.0)
+    r = oracle.eval_boxes(3, np.zeros(0), lo, hi, np.full(d, -10.0), np.full(d, 10.0), 1)
+    mp.mp.prec = 256
+    for i in range(n):
+        for t in range(3):
+            x = lo[i] + rng.uniform(0, 1, d) * (hi[i] - lo[i])
+            assert mp.mpf(r["flo"][i]) <= f_ref("levy", x) <= mp.mpf(r["fhi"][i]), (i, t)
+            if t == 0:
+                for j in range(d):
+                    def fj(s, j=j):
+                        y = [mp.mpf(float(v)) for v in x]
+                        y[j] = s
+                        return f_ref("levy", y)
+                    dj = mp.diff(fj, mp.mpf(float(x[j])))
+                    assert r["glo"][i, j] - 1e-30 <= dj <= r["ghi"][i, j] + 1e-30, (i, j)
+        assert mp.mpf(r["ub"][i]) >= f_ref("levy", lo[i] + (hi[i] - lo[i]) / 2)
+    one = np.ones((1, d))
+    z = oracle.eval_boxes(3, np.zeros(0), one, one, np.full(d, -10.0), np.full(d, 10.0), 1)
+    assert z["flo"][0] <= 0.0 <= z["fhi"][0] and z["fhi"][0] < 1e-25
+    assert np.all(z["glo"][0] <= 0.0) and np.all(z["ghi"][0] >= 0.0)
