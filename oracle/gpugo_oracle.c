@@ -31,7 +31,8 @@
  *   - the code lists <fn>Value_IA / <fn>Sample_IA / <fn>Derivative_IA
  *     (1-Ackley/ackley50.py:116-479, 7-Rastrigin/rastrigin50.py:110-372,
  *     11-Rosenbrock/rosenbrock50.py:115-451) for three families that contain
- *     those objectives as special cases (see family_* below).
+ *     those objectives as special cases, and Levy's (6-Levy/levy50.py:
+ *     166-224) as a fourth (see family_* below).
  *   - the driver loop (1-Ackley/ackley50.py:1033-1380): sample children for
  *     the incumbent upper bound (sampling, :602-684), rule children out by
  *     value and by the monotonicity (KKT) test (ruleOutGPU, :686-766), prune

@@ -1,5 +1,5 @@
 """Pins the oracle's objective code lists (value, gradient, monotonicity
-test, sample bound) for the three families.
+test, sample bound) for the three coefficient families and Levy.
 
 Reference code lists: 1-Ackley/ackley50.py:116-479,
 7-Rastrigin/rastrigin50.py:110-372, 11-Rosenbrock/rosenbrock50.py:115-451.
