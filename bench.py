@@ -1,23 +1,34 @@
 #!/usr/bin/env python3
-"""Benchmark: time to a certified enclosure of the d = 10 sepcos_chain problem.
+"""Benchmark: time to a certified enclosure (tol 1e-6) of the d = 50
+Ackley-type problem, FP32-prune / FP64-certify form.
 
-Workload: FP64 interval branch and bound
-on f(x) = sum_i a_i x_i^2 + b_i (1 - cos(c_i x_i)) + sum_{i<9} w_i (x_{i+1} -
-x_i^2)^2 + (1 - x_i)^2, d = 10, seeded coefficients (seed 1), box
-[-5, 10]^10, tol 1e-6 on the width of the value enclosure.
+Headline workload (ackley50): f(x) = -A exp(-B sqrt(mean x^2)) -
+exp(mean cos(C x)) + A + e + sum_i q_i x_i^4 / 50, d = 50, coefficients
+drawn from seed 2, box [-32, 32]^50, tol 1e-6 on the width of the value
+enclosure, 3^12 children of one box per step, the midpoint as the sample
+point; children are ruled out in FP32 intervals (outward, one ulp per
+operation) and the enclosure is certified in FP64.
 
-One "step" = one complete solve, from the root box to a certified
-enclosure [lb, ub] with ub - lb <= 1e-6, with the objective module loaded
-and the problem's device buffers allocated beforehand (inputs resident in
-HBM); the timed region covers reset + solve and ends after the solve's
-final device synchronisation.  value = mean time to certified enclosure
-(seconds, lower is better).  Also reported: boxes bounded per second, the
-roofline of the split+bound kernel (algorithmic FP64 FLOP per launch over
-its HIP-event duration, measured in a second pass of the same solves so the
-events do not add to value, against the MI355X FP64 vector peak) and the CPU
-oracle on the same workload and schedule (cpu_baseline).
+One "step" = one complete solve, from the root box to a certified enclosure
+[lb, ub] with ub - lb <= 1e-6, with the objective module loaded and the
+problem's device buffers allocated beforehand (inputs resident in HBM); the
+timed region covers reset + solve and ends after the solve's final device
+synchronisation.  value = mean time to certified enclosure (seconds, lower is
+better).  With the midpoint as sample point this problem certifies in one
+step (the natural extension over the child that holds x* = 0 has lower bound
+0 up to rounding and its midpoint is x*); the c2_s10 leg runs the same
+problem with the reference's ten diagonal samples (61 steps).
 
-    python bench.py [--steps K] [--warmup W] [--schedule tuned|reference]
+Also reported: the roofline of the dominant kernel (algorithmic FLOP per
+launch over its HIP-event duration, measured in a second pass of the same
+solves so the events do not add to value), the reference method on the
+host cores (cpu_baseline: the C restatement of the reference loop, one box
+per step, natural extension, no mean-value bound), the other
+configurations (c0, c1 on its 4^10 grid, c2 in the FP64 form and with ten
+samples), the compile-inclusive (cold) time, the large-d windowed form, the
+4096-problem batch and the Poisson-served workload.
+
+    python bench.py [--steps K] [--warmup W] [--legs ...] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -34,27 +45,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "time-to-certified enclosure (tol 1e-6) + boxes bounded/s; served p50/p99 latency"
-FP64_PEAK_TFLOPS = 78.6
-SCHEDULES = {
-    # (wid, cycle) subregion grid, boxes selected per step, mean-value bound,
-    # lanes per child of the term-parallel kernels (0: one lane per child)
-    "tuned": {"grid": (4, 5), "k_select": 32, "mean_value": True, "split": 4},
-    "reference": {"grid": (4, 10), "k_select": 1, "mean_value": False, "split": 0},
-}
-
-
-TRAFFIC_FILE = "profiles/traffic_chain10.json"
-
-
-def measured_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` from the committed counter profile
-    (rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by a device copy of
-    known size); None if that profile has no row for it."""
-    try:
-        with open(os.path.join(ROOT, TRAFFIC_FILE)) as fh:
-            return json.load(fh)["kernels"][kernel]["traffic_bytes"]
-    except (OSError, KeyError, ValueError):
-        return None
+# MI355X published vector peaks (TFLOP/s)
+PEAK = {"f64": 78.6, "f32": 157.3}
+FAMILY_INDEX = {"sepcos": 0, "sepcos_chain": 1, "ackley_q": 2}
+TRAFFIC_FILE = "profiles/traffic_ackley50.json"
+# the headline schedule: the reference's one box per step, 3^12 children
+C2_GRID, C2_K = (3, 12), 1
 
 
 def parse():
@@ -62,14 +58,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--schedule", choices=sorted(SCHEDULES), default="tuned")
-    ap.add_argument("--rounding", choices=["fast", "exact"], default="fast")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--legs", default="fp32,c2,large_d,xl_d,batch,served",
-                    help="comma list besides the headline: fp32 (headline workload in the FP32-prune form), "
-                         "c2 (d=50), large_d (windowed form, d=1000 and 2000), xl_d (Levy d=10000, one solve), "
-                         "batch (4096 problems), served (Poisson); 'none' for the headline only")
+    ap.add_argument("--no-cpu", action="store_true", help="skip every CPU baseline")
+    ap.add_argument("--cpu-budget", type=float, default=20.0,
+                    help="seconds of host work per CPU baseline (complete solves when they fit)")
+    ap.add_argument("--legs", default="c2_fp64,c2_s10,c1,c1_tuned,c0,cold,large_d,xl_d,batch,served",
+                    help="comma list besides the headline; 'none' for the headline only")
     ap.add_argument("--batch-n", type=int, default=4096)
     ap.add_argument("--seq-sample", type=int, default=256,
                     help="problems of the batch solved one at a time for the sequential baseline")
@@ -89,187 +82,262 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}))
 
 
-def cpu_baseline(spec, sched, seconds):
-    """The oracle (C restatement of the reference loop) on the same workload
-    and schedule, all host threads, repeated for about `seconds`."""
-    from oracle import oracle as O
-    O.build()
-    threads = os.cpu_count() or 1
-    fam = {"sepcos": 0, "sepcos_chain": 1, "ackley_q": 2}[spec.family]
-    times, boxes, res = [], [], None
-    t_end = time.perf_counter() + seconds
-    while time.perf_counter() < t_end or len(times) < 3:
-        t0 = time.perf_counter()
-        res, _ = O.solve(fam, spec.params, spec.lo, spec.hi, tol_f=spec.tol, wid=sched["grid"][0],
-                         cycle=sched["grid"][1], k_select=sched["k_select"], n_samples=1,
-                         max_iters=100000, threads=threads, mean_value=sched["mean_value"])
-        times.append(time.perf_counter() - t0)
-        boxes.append(res["boxes_bounded"])
-        if times[-1] > seconds:
-            break
-    t = statistics.median(times)
-    # the same solve on one host thread (3 solves, median)
-    t1 = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        O.solve(fam, spec.params, spec.lo, spec.hi, tol_f=spec.tol, wid=sched["grid"][0], cycle=sched["grid"][1],
-                k_select=sched["k_select"], n_samples=1, max_iters=100000, threads=1,
-                mean_value=sched["mean_value"])
-        t1.append(time.perf_counter() - t0)
-    return {"value": t, "unit": "s", "cores": threads, "kind": "port",
-            "sample": f"{len(times)} complete solves of the same workload and schedule, median",
-            "boxes_bounded_per_s": boxes[0] / t, "certified": res["status"] == 1,
-            "lb": res["lb"], "ub": res["ub"],
-            "one_core": {"value": statistics.median(t1), "unit": "s", "cores": 1,
-                         "sample": "3 complete solves on one host thread, median"}}
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed counter profile
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by a device copy of
+    known size); None if that profile has no row for it."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as fh:
+            return json.load(fh)["kernels"][kernel]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
-def cpu_reference_schedule(spec, steps=3):
-    """Boxes/s of the oracle on the reference schedule (4^10 children, one
-    box per step), over a bounded number of steps."""
-    from oracle import oracle as O
-    threads = os.cpu_count() or 1
-    t0 = time.perf_counter()
-    res, _ = O.solve(1, spec.params, spec.lo, spec.hi, tol_f=spec.tol, wid=4, cycle=10, k_select=1,
-                     max_iters=steps, threads=threads)
-    dt = time.perf_counter() - t0
-    return {"boxes_bounded_per_s": res["boxes_bounded"] / dt, "steps": steps, "cores": threads}
+def host_threads():
+    """Host threads for the CPU baselines: the cores this process may run
+    on, at most 16 (one GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
-def bound_roofline(obj, results, mean_value=False, kernel="gpugo_bound_fast"):
-    """Algorithmic FP64 FLOP of the split+bound launches over their
-    HIP-event time: every child gets the value code list, survivors of the
-    value test also the gradient and the sample (with the mean-value bound
-    also the midpoint value and d interval products and additions)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# ---------------------------------------------------------------------------
+# GPU solves
+# ---------------------------------------------------------------------------
+def roofline(obj, results, *, precision="fp64", mean_value=False, n_samples=1, kernel="gpugo_bound_fast",
+             windowed_c=0):
+    """Algorithmic FLOP of the split+bound launches over their HIP-event
+    time (per-op weights add/sub/scalar 2, sqr 2, pown(k) 2 ceil(log2 k),
+    interval product 8, division 10, sqrt 2, exp/sin/cos 40, counted by
+    gpugo_amd.emit from the code lists): every child gets the value code list; value survivors also
+    get the gradient (the full gradient is charged: the monotonicity test
+    may stop at its first pruning component) and n_samples sample
+    evaluations (with the mean-value bound also the midpoint value and d
+    interval products and additions).  In the FP32-prune form the FP32
+    value list is charged for every child and the FP64 lists for the value
+    survivors only.  frac is therefore an upper bound of the VALU work
+    actually done over the peak."""
     from gpugo_amd import emit
-    fl = emit.flop_counts(obj)
+    fl = emit.window_flop_counts(obj, windowed_c) if windowed_c else emit.flop_counts(obj)
     boxes = sum(x["boxes_bounded"] for x in results)
     pv = sum(x["pruned_value"] for x in results)
     bound_ms = sum(x["bound_ms"] for x in results)
     launches = sum(x["bound_launches"] for x in results)
-    extra = fl["sample"] + fl["grad"] + (10 * obj.d if mean_value else 0)
-    flops = boxes * fl["value"] + (boxes - pv) * extra
+    extra = n_samples * fl["sample"] + fl["grad"] + (10 * obj.d if mean_value else 0)
+    surv = boxes - pv
+    if precision == "fp32":
+        flops = boxes * fl["value"] + surv * (fl["value"] + extra)
+    else:
+        flops = boxes * fl["value"] + surv * extra
+    dt = "f32" if precision == "fp32" else "f64"
     achieved = flops / (bound_ms * 1e-3) / 1e12 if bound_ms > 0 else 0.0
-    return {"bound": "valu", "kernel": kernel + ("_mv" if mean_value else ""),
-            "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+    return {"bound": "valu", "kernel": kernel, "achieved": achieved, "peak": PEAK[dt], "unit": "TFLOP/s",
+            "frac": achieved / PEAK[dt], "frac_is_upper_bound": True,
             "flop_per_launch": flops / max(launches, 1), "flop_per_child": flops / max(boxes, 1),
-            "avg_launch_ms": bound_ms / max(launches, 1), "launches": launches}
+            "children_per_launch": boxes / max(launches, 1),
+            "avg_launch_ms": bound_ms / max(launches, 1), "launches": launches,
+            "timing": "HIP events from each split+bound dispatch, second pass of the same solves"}
 
 
-def leg_fp32(G, spec, obj, sched, args):
-    """The headline workload in the FP32-prune / FP64-certify precision form,
-    against the FP64 form with the same schedule (one lane per child: the
-    FP32 pass has no term-parallel variant)."""
-    def timed(precision):
-        prob = G.Problem(obj, spec.params, spec.lo, spec.hi, tol=spec.tol, grid=sched["grid"],
-                         k_select=sched["k_select"], n_samples=1, rounding=args.rounding, max_iters=200000,
-                         mean_value=sched["mean_value"], precision=precision)
-        for _ in range(args.warmup):
-            prob.reset()
-            prob.solve()
-        res = []
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            prob.reset()
-            res.append(prob.solve())
-        t = (time.perf_counter() - t0) / args.steps
-        prob.close()
-        return t, res
-    t64, r64s = timed("fp64")
-    t32, r32s = timed("fp32")
-    r, r64 = r32s[-1], r64s[-1]
-    # the FP64 host oracle on the same workload and schedule
-    from oracle import oracle as O
-    O.build()
-    o, _ = O.solve(1, spec.params, spec.lo, spec.hi, tol_f=spec.tol, wid=sched["grid"][0], cycle=sched["grid"][1],
-                   k_select=sched["k_select"], max_iters=100000, threads=os.cpu_count() or 1,
-                   mean_value=sched["mean_value"])
-    return {"precision": "fp32 prune, fp64 certify", "time_to_certified_s": t32, "fp64_time_s": t64,
-            "oracle_fp64": {"lb": o["lb"], "ub": o["ub"], "width": o["ub"] - o["lb"], "certified": o["status"] == 1},
-            "intersects_oracle": max(r["lb"], o["lb"]) <= min(r["ub"], o["ub"]),
-            "speedup_vs_fp64": t64 / t32,
-            "certified": all(x["status_name"] == "certified" for x in r32s),
-            "lb": r["lb"], "ub": r["ub"], "width": r["width"], "iterations": r["iterations"],
-            "boxes_bounded": r["boxes_bounded"], "pruned_value": r["pruned_value"],
-            "fp64_form": {"lb": r64["lb"], "ub": r64["ub"], "width": r64["width"]},
-            "enclosures_intersect": max(r["lb"], r64["lb"]) <= min(r["ub"], r64["ub"])}
-
-
-def leg_c2(G, pb, steps=5):
-    """ackley_q d = 50, box [-32,32]^50, tol 1e-6: GPU time to certified enclosure
-    against the oracle on all host cores with the same schedule."""
-    from oracle import oracle as O
-    spec = pb.ackley50()
-    obj = pb.objective(spec.family, spec.d)
-    grid, k = (3, 12), 1
-    p = G.Problem(obj, spec.params, spec.lo, spec.hi, tol=spec.tol, grid=grid, k_select=k, max_iters=10000)
-    p.solve()
-    times, res = [], []
-    for _ in range(steps):
+def gpu_solves(G, obj, spec, *, steps, warmup, grid, k_select, n_samples=1, precision="fp64",
+               mean_value=False, split=0, windowed=False, module=None, init_ub=np.inf, max_iters=100000,
+               max_queue=0):
+    """`steps` complete solves after `warmup` untimed ones; then the same
+    solves again with per-launch events (roofline).  Returns timing and the
+    results of both passes."""
+    p = G.Problem(obj, spec.params, spec.lo, spec.hi, tol=spec.tol, grid=grid, k_select=k_select,
+                  n_samples=n_samples, precision=precision, mean_value=mean_value, split=split,
+                  windowed=windowed, module=module, init_ub=init_ub, max_iters=max_iters, max_queue=max_queue)
+    for _ in range(warmup):
         p.reset()
-        t0 = time.perf_counter()
         p.solve()
-        times.append(time.perf_counter() - t0)
-    p.timing(True)  # roofline pass: the same solves with dispatch events
+    times, res = [], []
+    t0 = time.perf_counter()
     for _ in range(steps):
         p.reset()
+        t1 = time.perf_counter()
         res.append(p.solve())
+        times.append(time.perf_counter() - t1)
+    total = time.perf_counter() - t0
+    p.timing(True)
+    timed = []
+    for _ in range(steps):
+        p.reset()
+        timed.append(p.solve())
     p.timing(False)
     p.close()
+    same = all(all(a[k] == b[k] for k in ("lb", "ub", "iterations", "boxes_bounded")) for a, b in zip(res, timed))
     r = res[-1]
-    O.build()
-    threads = os.cpu_count() or 1
-    t0 = time.perf_counter()
-    o, _ = O.solve(2, spec.params, spec.lo, spec.hi, tol_f=spec.tol, wid=grid[0], cycle=grid[1], k_select=k,
-                   max_iters=10000, threads=threads)
-    tc = time.perf_counter() - t0
-    gpu = statistics.median(times)
-    # precision form: FP32 prune, FP64 certify
-    p = G.Problem(obj, spec.params, spec.lo, spec.hi, tol=spec.tol, grid=grid, k_select=k, max_iters=10000,
-                  precision="fp32")
-    p.solve()
-    t32 = []
-    for _ in range(steps):
-        p.reset()
-        t0 = time.perf_counter()
-        r32 = p.solve()
-        t32.append(time.perf_counter() - t0)
-    p.close()
-    return {"workload": "ackley50: ackley_q d=50, box [-32,32]^50, tol 1e-6, seed 2",
-            "fp32_prune": {"time_to_certified_s": statistics.median(t32), "lb": r32["lb"], "ub": r32["ub"],
-                           "width": r32["width"], "certified": r32["status_name"] == "certified",
-                           "intersects_oracle": max(r32["lb"], o["lb"]) <= min(r32["ub"], o["ub"]),
-                           "pruned_value": r32["pruned_value"], "boxes_bounded": r32["boxes_bounded"],
-                           "speedup_vs_fp64": gpu / statistics.median(t32),
-                           "note": "one step from the root: an FP32 sample pass over all children gives "
-                                   "that step its incumbent, then the FP32 value test rules children out"},
-            "grid": list(grid), "k_select": k, "dtype": "f64",
-            "time_to_certified_s": gpu, "certified": all(x["status_name"] == "certified" for x in res),
+    return {"mean_s": total / steps, "median_s": statistics.median(times), "min_s": min(times),
+            "certified": all(x["status_name"] == "certified" for x in res + timed),
             "lb": r["lb"], "ub": r["ub"], "width": r["width"], "iterations": r["iterations"],
-            "boxes_bounded": r["boxes_bounded"], "boxes_bounded_per_s": r["boxes_bounded"] / gpu,
-            "roofline": bound_roofline(obj, res),
-            "cpu_baseline": {"value": tc, "unit": "s", "cores": threads, "kind": "port",
-                             "sample": "one complete solve, same workload and schedule",
-                             "certified": o["status"] == 1, "lb": o["lb"], "ub": o["ub"]},
-            "speedup_vs_cpu": tc / gpu}
+            "boxes_bounded": r["boxes_bounded"], "pruned_value": r["pruned_value"],
+            "queue_peak": r["queue_peak"], "boxes_bounded_per_s": r["boxes_bounded"] / statistics.median(times),
+            "passes_identical": same, "_timed": timed}
+
+
+def public(d):
+    return {k: v for k, v in d.items() if not k.startswith("_")}
+
+
+# ---------------------------------------------------------------------------
+# CPU: the reference method on the host cores
+# ---------------------------------------------------------------------------
+def cpu_reference(spec, grid, k_select, n_samples, budget, gpu_iterations=None, one_core=True):
+    """The oracle (C restatement of IA.py and of the reference loop: one box
+    per step, wid^cycle children, n_samples diagonal samples, natural
+    extension and monotonicity test, no mean-value bound) on the host.
+    Complete solves (median of up to 5) when one fits the budget; otherwise
+    a bounded number of steps from the root, reported as boxes/s.  The same
+    on one host thread (median of up to 3 complete solves, or steps)."""
+    from oracle import oracle as O
+    O.build()
+    fam = FAMILY_INDEX[spec.family]
+    threads = host_threads()
+
+    def run(th, max_iters):
+        t0 = time.perf_counter()
+        r, _ = O.solve(fam, spec.params, spec.lo, spec.hi, tol_f=spec.tol, wid=grid[0], cycle=grid[1],
+                       k_select=k_select, n_samples=n_samples, max_iters=max_iters, threads=th)
+        return time.perf_counter() - t0, r
+
+    def measure(th, reps):
+        # probe two steps, then decide between complete solves and a sample
+        t_probe, r_probe = run(th, 2)
+        per_box = t_probe / max(r_probe["boxes_bounded"], 1)
+        est = None
+        if gpu_iterations is not None and r_probe["status"] == 0:
+            est = per_box * r_probe["boxes_bounded"] / max(r_probe["iterations"], 1) * gpu_iterations
+        if r_probe["status"] == 1 or (est is not None and est <= budget):
+            ts, r = [], None
+            t_start = time.perf_counter()
+            while len(ts) < reps and (not ts or time.perf_counter() - t_start + ts[-1] <= budget):
+                t, r = run(th, 1 << 40)
+                ts.append(t)
+            return {"value": statistics.median(ts), "unit": "s", "cores": th, "complete": True,
+                    "sample": f"{len(ts)} complete solve(s), median", "certified": r["status"] == 1,
+                    "lb": r["lb"], "ub": r["ub"], "iterations": r["iterations"],
+                    "boxes_bounded": r["boxes_bounded"], "boxes_bounded_per_s": r["boxes_bounded"] / statistics.median(ts)}
+        # bounded sample: as many steps from the root as fit the budget
+        steps = max(2, int(budget / max(t_probe / 2, 1e-9)))
+        if gpu_iterations is not None:
+            steps = min(steps, gpu_iterations)
+        t, r = run(th, steps)
+        rate = r["boxes_bounded"] / t
+        if r["status"] == 1:  # the bounded run reached the certificate
+            return {"value": t, "unit": "s", "cores": th, "complete": True, "sample": "1 complete solve",
+                    "certified": True, "lb": r["lb"], "ub": r["ub"], "iterations": r["iterations"],
+                    "boxes_bounded": r["boxes_bounded"], "boxes_bounded_per_s": rate}
+        return {"value": None, "unit": "s", "cores": th, "complete": False,
+                "sample": f"{r['iterations']} steps from the root, then stopped", "boxes_bounded_per_s": rate,
+                "boxes_bounded": r["boxes_bounded"], "seconds": t}
+
+    out = measure(threads, 5)
+    out.update({"kind": "port", "cpu_model": cpu_model(),
+                "method": f"reference loop: one box per step, {grid[0]}^{grid[1]} children, "
+                          f"{n_samples} diagonal sample(s), natural extension + monotonicity test"})
+    if one_core:
+        out["one_core"] = measure(1, 3)
+    return out
+
+
+def with_ratio(cpu, gpu_s, gpu_boxes):
+    """Speed-up of the GPU over the host: measured if the host solve
+    completed, else estimated from its boxes/s for the GPU's boxes."""
+    if cpu.get("complete"):
+        cpu["speedup_gpu_vs_cpu"] = cpu["value"] / gpu_s
+    else:
+        est = gpu_boxes / cpu["boxes_bounded_per_s"]
+        cpu["est_time_for_gpu_boxes_s"] = est
+        cpu["speedup_gpu_vs_cpu_est"] = est / gpu_s
+    oc = cpu.get("one_core")
+    if oc:
+        oc["speedup_gpu_vs_cpu"] = (oc["value"] / gpu_s) if oc.get("complete") else \
+            (gpu_boxes / oc["boxes_bounded_per_s"]) / gpu_s
+    return cpu
+
+
+# ---------------------------------------------------------------------------
+# legs
+# ---------------------------------------------------------------------------
+def leg_config(G, pb, args, spec, grid, k, *, precisions=("fp64",), n_samples=1, mean_value=False, split=0,
+               cpu=True, kernel="gpugo_bound_fast", max_iters=100000):
+    obj = pb.objective(spec.family, spec.d)
+    out = {"workload": f"{spec.name}, box [{spec.lo[0]:g},{spec.hi[0]:g}]^{spec.d}, tol {spec.tol:g}",
+           "grid": list(grid), "k_select": k, "n_samples": n_samples, "mean_value": mean_value,
+           "lanes_per_child": split or 1}
+    first = None
+    for prec in precisions:
+        g = gpu_solves(G, obj, spec, steps=args.steps, warmup=args.warmup, grid=grid, k_select=k,
+                       n_samples=n_samples, precision=prec, mean_value=mean_value, split=split, max_iters=max_iters)
+        g["roofline"] = roofline(obj, g["_timed"], precision=prec, mean_value=mean_value, n_samples=n_samples,
+                                 kernel=kernel + ("_mv" if mean_value else "") + ("_g" if split else ""))
+        out[prec] = public(g)
+        first = first or g
+    if cpu and not args.no_cpu:
+        print(f"# cpu {spec.name} {grid} S={n_samples}", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = with_ratio(cpu_reference(spec, grid, k, n_samples, args.cpu_budget,
+                                                       gpu_iterations=first["iterations"]),
+                                         first["median_s"], first["boxes_bounded"])
+    return out
+
+
+def leg_cold(G, pb):
+    """Compile-inclusive time: hipRTC compile of the headline module from its
+    source (the on-disk cache bypassed), load, problem set-up and one solve.
+    The reference's own timer also covers its JIT (1-Ackley/ackley50.py:960,
+    :1382-1390)."""
+    import ctypes
+    from gpugo_amd import _lib, emit
+    spec = pb.ackley50()
+    obj = pb.objective(spec.family, spec.d)
+    src = emit.module_source(obj, ("fast",), True, 0)
+    lib = _lib.load()
+    t0 = time.perf_counter()
+    h = ctypes.c_void_p()
+    _lib.check(lib.gpugo_module_compile(src.encode(), b"ackley50_cold.hip", b"", ctypes.byref(h)))
+    t1 = time.perf_counter()
+
+    class _M:  # a Module-like wrapper around the freshly compiled handle
+        handle = h
+    p = G.Problem(obj, spec.params, spec.lo, spec.hi, tol=spec.tol, grid=C2_GRID, k_select=C2_K,
+                  precision="fp32", module=_M, max_iters=1000)
+    r = p.solve()
+    t2 = time.perf_counter()
+    p.close()
+    lib.gpugo_module_destroy(h)
+    return {"workload": "ackley50, FP32-prune / FP64-certify form, compiled from source",
+            "compile_s": t1 - t0, "cold_time_to_certified_s": t2 - t0, "solve_s": t2 - t1,
+            "certified": r["status_name"] == "certified"}
 
 
 LARGE_D = (("levy", 1000), ("rosenbrock", 1000), ("fu", 1000), ("rastrigin", 1000), ("levy", 2000))
-# the reference's levy10000.py dimension: one solve (~30 s), a queue of up to 2^31 boxes
+# the reference's levy10000.py dimension: one solve, a queue of up to 2^31 boxes
 XL_D = (("levy", 10000),)
 
 
-def leg_large_d(G, cases=LARGE_D, reps=2, cpu_seconds=2.0, max_queue=1 << 28, warm=True):
+def leg_large_d(G, cases=LARGE_D, reps=2, cpu_seconds=2.0, max_queue=1 << 28, warm=True, no_cpu=False):
     """The windowed (large-d) form on reference objectives at d = 1000 to
     10,000 (the reference's boxes and initial upper bounds, 3^12 children of
-    one box per step, 10 diagonal samples, tol 1e-6, a queue of up to
-    max_queue boxes): time to a certified enclosure,
-    boxes bounded per second and the roofline of the windowed split+bound
-    kernel.  CPU beside it: the oracle (the reference method, every
-    coordinate of every child) on Rosenbrock and Levy at d = 1000 (the ones
-    of these it restates), steps of 3^8 children from the root, all host
-    threads: its boxes/s and the time it would need for the GPU's boxes."""
+    one box per step, 10 diagonal samples, tol 1e-6): time to a certified
+    enclosure, boxes bounded per second and the roofline of the windowed
+    split+bound kernel.  CPU beside it: the oracle (the reference method,
+    every coordinate of every child) on Rosenbrock and Levy at d = 1000,
+    steps of 3^8 children from the root, all host threads: its boxes/s and
+    the time it would need for the GPU's boxes."""
     from gpugo_amd import emit, families as F, problems as pb
     from oracle import oracle as O
     grid, k, S = (3, 12), 1, 10
@@ -277,7 +345,6 @@ def leg_large_d(G, cases=LARGE_D, reps=2, cpu_seconds=2.0, max_queue=1 << 28, wa
     out = {"workload": f"reference objectives d={dims}, reference boxes and initial upper bounds, grid 3^12, "
                        f"one box per step, {S} diagonal samples, tol 1e-6, windowed code lists, "
                        f"queue up to 2^{max_queue.bit_length() - 1} boxes"}
-    cpu_rate = None
     for name, d in cases:
         s = F.spec(name, d)
         obj = F.objective_loops(name, d)
@@ -296,27 +363,18 @@ def leg_large_d(G, cases=LARGE_D, reps=2, cpu_seconds=2.0, max_queue=1 << 28, wa
         p.timing(False)
         p.close()
         r = res[-1]
-        fl = emit.window_flop_counts(obj, grid[1])
-        flops = sum(x["boxes_bounded"] * fl["value"]
-                    + (x["boxes_bounded"] - x["pruned_value"]) * (fl["grad"] + S * fl["sample"]) for x in res)
-        bound_ms = sum(x["bound_ms"] for x in res)
-        launches = sum(x["bound_launches"] for x in res)
-        achieved = flops / (bound_ms * 1e-3) / 1e12 if bound_ms > 0 else 0.0
         t = statistics.median(times)
+        rl = roofline(obj, res, n_samples=S, kernel="gpugo_bound_fast (windowed)", windowed_c=grid[1])
         leg = {"time_to_certified_s": t, "certified": all(x["status_name"] == "certified" for x in res),
                "lb": r["lb"], "ub": r["ub"], "width": r["width"], "f_star": s.f_star,
                "contains_f_star": r["lb"] <= s.f_star <= r["ub"], "iterations": r["iterations"],
                "boxes_bounded": r["boxes_bounded"], "boxes_bounded_per_s": r["boxes_bounded"] / t,
-               "queue_peak": r["queue_peak"],
-               "roofline": {"bound": "valu", "kernel": "gpugo_bound_fast (windowed)", "achieved": achieved,
-                            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                            "flop_per_child_value": fl["value"], "avg_launch_ms": bound_ms / max(launches, 1),
-                            "launches": launches}}
-        if name in ("rosenbrock", "levy") and d == 1000:
+               "queue_peak": r["queue_peak"], "roofline": rl}
+        if name in ("rosenbrock", "levy") and d == 1000 and not no_cpu:
             # the oracle's restatements: Rosenbrock as family 1 (a = b = 0,
             # w = 100), Levy as family 3 (no coefficients)
             fam, params = (1, pb.rosenbrock(d).params) if name == "rosenbrock" else (3, np.zeros(0))
-            threads = os.cpu_count() or 1
+            threads = host_threads()
             O.build()
             t0 = time.perf_counter()
             nsteps, boxes = 0, 0
@@ -325,12 +383,12 @@ def leg_large_d(G, cases=LARGE_D, reps=2, cpu_seconds=2.0, max_queue=1 << 28, wa
                                n_samples=S, max_iters=1, threads=threads)
                 boxes += o["boxes_bounded"]
                 nsteps += 1
-            cpu_rate = boxes / (time.perf_counter() - t0)
-            leg["cpu_baseline"] = {"boxes_bounded_per_s": cpu_rate, "cores": threads, "kind": "port",
+            rate = boxes / (time.perf_counter() - t0)
+            leg["cpu_baseline"] = {"boxes_bounded_per_s": rate, "cores": threads, "kind": "port",
                                    "sample": f"{nsteps} x one step of 3^8 children from the root, d={d}, "
                                              f"{S} samples, every coordinate of every child",
-                                   "est_time_for_gpu_boxes_s": r["boxes_bounded"] / cpu_rate,
-                                   "speedup_vs_cpu_est": r["boxes_bounded"] / cpu_rate / t}
+                                   "est_time_for_gpu_boxes_s": r["boxes_bounded"] / rate,
+                                   "speedup_vs_cpu_est": r["boxes_bounded"] / rate / t}
         out[name if d == 1000 else f"{name}{d}"] = leg
     return out
 
@@ -355,11 +413,18 @@ def run_batch(Batch, probs, **kw):
     return res, el
 
 
+def overflow_kinds(results):
+    """Overflowed problems by the capacity that ran out."""
+    over = [r.get("overflow", 0) for r in results if r["status_name"] == "overflow"]
+    return {"box_queue": sum(1 for o in over if o & 1), "parent_arena": sum(1 for o in over if o & 2)}
+
+
 def leg_batch(G, pb, n, seq_sample):
     """n independent problems (families sepcos / sepcos_chain / ackley_q, d
-    log-uniform in [2,32]) in one batch (schedules from the
-    tuned table) against the same problems solved one at a time with the
-    single-problem solver (same schedules and capacities)."""
+    log-uniform in [2,32]) in one batch (schedules from the tuned table)
+    against the same problems solved one at a time with the single-problem
+    solver (same schedules and capacities).  Throughput counts certified
+    problems on both sides."""
     from gpugo_amd.batch import Batch, BatchProblem, get_batch_module
     from gpugo_amd.schedule import schedule_for
     qcap, pcap, maxit = 1 << 18, 8192, 2000
@@ -373,7 +438,6 @@ def leg_batch(G, pb, n, seq_sample):
     run_batch(Batch, probs[:64], **kw)  # warm-up
     res, el = run_batch(Batch, probs, **kw)
     cert = sum(r["status_name"] == "certified" for r in res.values())
-    # sequential: the first seq_sample problems, one gpugo_solve each
     sample = probs[:seq_sample]
     mods = {}
     for p in sample:
@@ -411,14 +475,9 @@ def leg_batch(G, pb, n, seq_sample):
             "width_max_certified": max(widths) if widths else None,
             "boxes_bounded": sum(r["boxes_bounded"] for r in res.values()),
             "sequential": {"problems": len(sample), "time_s": t_seq, "problems_per_s": len(sample) / t_seq,
-                           "certified": seq_cert, "identical_to_batch": same},
-            "speedup_problems_per_s": (n / el) / (len(sample) / t_seq)}
-
-
-def overflow_kinds(results):
-    """Overflowed problems by the capacity that ran out."""
-    over = [r.get("overflow", 0) for r in results if r["status_name"] == "overflow"]
-    return {"box_queue": sum(1 for o in over if o & 1), "parent_arena": sum(1 for o in over if o & 2)}
+                           "certified": seq_cert, "certified_per_s": seq_cert / t_seq,
+                           "identical_to_batch": same},
+            "speedup_certified_per_s": (cert / el) / max(seq_cert / t_seq, 1e-12)}
 
 
 def leg_served(pb, lams=(100.0, 1000.0), seconds=3.0):
@@ -437,6 +496,7 @@ def leg_served(pb, lams=(100.0, 1000.0), seconds=3.0):
             summ = latency_summary(done)
             summ.update({"lambda_per_s": lam, "requests": n, "wall_s": el,
                          "throughput_per_s": len(done) / el,
+                         "certified_per_s": summ["certified"] / el,
                          "status_counts": {k: sum(r.result["status_name"] == k for r in done)
                                            for k in sorted({r.result["status_name"] for r in done})},
                          "overflow_kinds": overflow_kinds(r.result for r in done)})
@@ -457,103 +517,74 @@ def main():
               "(a problem shards by independent boxes, a batch by independent problems)", file=sys.stderr)
         sys.exit(2)
     import gpugo_amd as G
-    from gpugo_amd import emit, problems as pb
+    from gpugo_amd import problems as pb
 
     legs = set(args.legs.split(","))
-    spec = pb.chain10()
-    sched = SCHEDULES[args.schedule]
-    obj = pb.objective(spec.family, spec.d)
     G.init(0)
-    prob = G.Problem(obj, spec.params, spec.lo, spec.hi, tol=spec.tol, grid=sched["grid"],
-                     k_select=sched["k_select"], n_samples=1, rounding=args.rounding, max_iters=200000,
-                     mean_value=sched["mean_value"], split=sched["split"])
-    kernel = (f"gpugo_bound_{args.rounding}" + ("_mv" if sched["mean_value"] else "")
-              + ("_g" if sched["split"] else ""))
-    for _ in range(args.warmup):
-        prob.reset()
-        prob.solve()
-    print(f"# warmup done ({args.warmup} solves)", file=sys.stderr, flush=True)
-
-    # value: K solves without per-launch events (they add ~6 us a launch);
-    # the roofline: the same K solves again, each split+bound launch timed by
-    # events taken from its dispatch on the problem's stream
-    results = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        prob.reset()
-        results.append(prob.solve())
-    t1 = time.perf_counter()
-    per_solve = (t1 - t0) / args.steps
-    prob.timing(True)
-    timed = []
-    for _ in range(args.steps):
-        prob.reset()
-        timed.append(prob.solve())
-    prob.timing(False)
-    r = results[-1]
-    certified = all(x["status_name"] == "certified" for x in results + timed)
-    same = all(all(a[k] == b[k] for k in ("lb", "ub", "iterations", "boxes_bounded")) for a, b in zip(results, timed))
-    boxes = sum(x["boxes_bounded"] for x in timed)
-    pv = sum(x["pruned_value"] for x in timed)
-    bound_ms = sum(x["bound_ms"] for x in timed)
-    launches = sum(x["bound_launches"] for x in timed)
-
-    fl = emit.flop_counts(obj)
-    # survivors of the value test also get the sample and the gradient; with
-    # the mean-value bound they also get the midpoint value and d interval
-    # products and additions (10 FLOP each)
-    extra = fl["sample"] + fl["grad"] + (10 * spec.d if sched["mean_value"] else 0)
-    flops = boxes * fl["value"] + (boxes - pv) * extra
-    achieved = flops / (bound_ms * 1e-3) / 1e12 if bound_ms > 0 else 0.0
-
+    spec = pb.ackley50()
+    obj = pb.objective(spec.family, spec.d)
+    head = gpu_solves(G, obj, spec, steps=args.steps, warmup=args.warmup, grid=C2_GRID, k_select=C2_K,
+                      precision="fp32")
+    print(f"# headline done ({args.steps} solves)", file=sys.stderr, flush=True)
+    rl = roofline(obj, head["_timed"], precision="fp32", kernel="gpugo_bound_fast")
+    rl["traffic"] = measured_traffic("gpugo_bound_fast")
+    rl["traffic_source"] = TRAFFIC_FILE
     out = {
         "metric": METRIC,
-        "value": per_solve,
+        "value": head["mean_s"],
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": per_solve * 1e3,
+        "ms_per_step": head["mean_s"] * 1e3,
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 prune, f64 certify",
         "data": "synthetic (seeded coefficients)",
-        "config": {"workload": "chain10: sepcos_chain d=10, box [-5,10]^10, tol 1e-6, seed 1",
-                   "schedule": args.schedule, "grid": list(sched["grid"]),
-                   "k_select": sched["k_select"], "n_samples": 1, "rounding": args.rounding,
-                   "mean_value": sched["mean_value"], "lanes_per_child": sched["split"] or 1},
-        "time_to_certified_s": per_solve,
-        "boxes_bounded_per_s": sum(x["boxes_bounded"] for x in results) / (t1 - t0),
-        "boxes_bounded_per_solve": boxes // args.steps,
-        "iterations_per_solve": r["iterations"],
-        "certified": certified,
-        "lb": r["lb"], "ub": r["ub"], "width": r["width"],
-        "roofline": {"bound": "valu",
-                     "kernel": kernel,
-                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": measured_traffic(kernel),
-                     "traffic_source": TRAFFIC_FILE,
-                     "flop_per_launch": flops / max(launches, 1),
-                     "avg_launch_ms": bound_ms / max(launches, 1), "launches": launches,
-                     "timing": "a second pass of the same solves with dispatch events on each launch",
-                     "passes_identical": same},
+        "config": {"workload": "ackley50: ackley_q d=50, box [-32,32]^50, tol 1e-6, seed 2",
+                   "precision": "fp32 prune, fp64 certify", "grid": list(C2_GRID), "k_select": C2_K,
+                   "n_samples": 1, "rounding": "fast", "mean_value": False,
+                   "note": "certifies in one step: the child holding x* = 0 has natural-extension lower "
+                           "bound 0 and its midpoint is x*; see c2_s10 for the ten-sample run"},
+        "time_to_certified_s": head["mean_s"],
+        "median_s": head["median_s"],
+        "certified": head["certified"],
+        "lb": head["lb"], "ub": head["ub"], "width": head["width"], "iterations": head["iterations"],
+        "boxes_bounded": head["boxes_bounded"], "boxes_bounded_per_s": head["boxes_bounded_per_s"],
+        "passes_identical": head["passes_identical"],
+        "roofline": rl,
     }
-    if "fp32" in legs:
-        out["fp32_prune"] = leg_fp32(G, spec, obj, sched, args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(spec, sched, args.cpu_seconds)
-        out["cpu_reference_schedule"] = cpu_reference_schedule(spec)
-    if "c2" in legs:
-        print("# leg c2", file=sys.stderr, flush=True)
-        out["c2"] = leg_c2(G, pb)
+        print("# cpu baseline (headline workload)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = with_ratio(cpu_reference(spec, C2_GRID, C2_K, 1, args.cpu_budget,
+                                                       gpu_iterations=head["iterations"]),
+                                         head["mean_s"], head["boxes_bounded"])
+    if "c2_fp64" in legs:
+        print("# leg c2_fp64", file=sys.stderr, flush=True)
+        out["c2_fp64"] = leg_config(G, pb, args, spec, C2_GRID, C2_K, cpu=False)
+        out["c2_fp64"]["fp32_form_speedup"] = out["c2_fp64"]["fp64"]["mean_s"] / head["mean_s"]
+    if "c2_s10" in legs:
+        print("# leg c2_s10", file=sys.stderr, flush=True)
+        out["c2_s10"] = leg_config(G, pb, args, spec, C2_GRID, C2_K, precisions=("fp32", "fp64"), n_samples=10)
+    if "c1" in legs:
+        print("# leg c1", file=sys.stderr, flush=True)
+        out["c1"] = leg_config(G, pb, args, pb.chain10(), (4, 10), 1)
+    if "c1_tuned" in legs:
+        print("# leg c1_tuned", file=sys.stderr, flush=True)
+        out["c1_tuned"] = leg_config(G, pb, args, pb.chain10(), (4, 5), 32, mean_value=True, split=4, cpu=False)
+    if "c0" in legs:
+        print("# leg c0", file=sys.stderr, flush=True)
+        out["c0"] = leg_config(G, pb, args, pb.sepcos2(), (64, 2), 1)
+    if "cold" in legs:
+        print("# leg cold", file=sys.stderr, flush=True)
+        out["cold"] = leg_cold(G, pb)
     if "large_d" in legs:
         print("# leg large_d", file=sys.stderr, flush=True)
-        out["large_d"] = leg_large_d(G)
+        out["large_d"] = leg_large_d(G, no_cpu=args.no_cpu)
     if "xl_d" in legs:
         print("# leg xl_d", file=sys.stderr, flush=True)
-        out["xl_d"] = leg_large_d(G, XL_D, reps=1, max_queue=1 << 31, warm=False)
+        out["xl_d"] = leg_large_d(G, XL_D, reps=1, max_queue=1 << 31, warm=False, no_cpu=True)
     if "batch" in legs:
         print("# leg batch", file=sys.stderr, flush=True)
         out["batch"] = leg_batch(G, pb, args.batch_n, args.seq_sample)

@@ -20,7 +20,29 @@
 // (GPUGO_1.1/6-Levy/levy10000.py:686-850).
 #pragma once
 
+#ifndef GPUGO_WITH_F32
+#define GPUGO_WITH_F32 0
+#endif
+
 namespace gpugo {
+
+// FP32 pruning pass of the windowed form (precision F32): the child's value
+// in FP32 intervals (coordinates rounded outward, the parent's FP64 rest
+// sums rounded outward); a child whose FP32 lower bound exceeds the
+// incumbent is ruled out without FP64 work.  Rigorous: the FP32 enclosure
+// contains f over the child.
+template <int R>
+__device__ __forceinline__ bool f32_prunes_w(const BoundArgs &a, const ChildBox &x, const iv<double> *rest, int win,
+                                             double ub0) {
+#if GPUGO_WITH_F32
+  if (!(a.flags & GPUGO_FLAG_F32_PRUNE) || !(ub0 < __builtin_huge_val())) return false;
+  const F32Box<ChildBox> xf{x};
+  const iv<float> f = obj_value_w32<R>(xf, (const iv<float> *)a.params32, a.d, rest, win, a.c);
+  return (double)f.lo > ub0;
+#else
+  return false;
+#endif
+}
 
 // Rest sums of the parent records split in this step: one workgroup per
 // record, threads over the terms, a butterfly in each wavefront and a
@@ -97,6 +119,9 @@ __device__ __forceinline__ void bound_w_impl(const BoundArgs &a) {
                                    (unsigned)a.wid, a.inv_wid, a.half_inv_wid, pos);
       const int S = a.n_samples;
       const iv<double> *rest = (const iv<double> *)(a.prest) + p * (long long)((1 + S) * GPUGO_NREST);
+      if (f32_prunes_w<R>(a, x, rest, win, ub0)) {
+        pv = true;
+      } else {
       const iv<double> f = obj_value_w<R>(x, a.params, a.d, rest, win, a.c);
       lb = f.lo;
       if (!(lb == lb)) lb = -__builtin_huge_val();
@@ -112,6 +137,7 @@ __device__ __forceinline__ void bound_w_impl(const BoundArgs &a) {
         }
       } else {
         pv = true;
+      }
       }
     }
     const unsigned long long mask = __ballot(keep);
@@ -203,8 +229,97 @@ __device__ __forceinline__ void eval_w_impl(const EvalArgs &a) {
   }
 }
 
+#if GPUGO_WITH_F32
+// FP32 sample pass of the first step of an FP32-prune solve (windowed
+// form): the FP32 value at each child's S diagonal points, the window's
+// coordinates from the child and the other terms from the parent's rest sums
+// at the same sample point; their minimum lowers the incumbent and, by the
+// last workgroup (ticket in st->cblocks, left at 0), the step's pruning
+// bound ub_snap.  Same protocol as the unrolled modules' sample pass.
+template <int R>
+__device__ __forceinline__ void sample32_w_impl(const BoundArgs &a) {
+  __shared__ unsigned long long s_ubmin;
+  DevState *st = a.st;
+  if (st->status != 0) return;
+  const long long n_child = (long long)st->n_split * (long long)a.W;
+  const long long first = st->first_parent;
+  const double invW = 1.0 / (double)a.W, halfInvW = 0.5 / (double)a.W;
+  const int lane = threadIdx.x & 63, S = a.n_samples;
+  if (threadIdx.x == 0) s_ubmin = order_key(__builtin_huge_val());
+  __syncthreads();
+  double m = __builtin_huge_val();
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n_child;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long pl = (long long)__builtin_fma((double)t, invW, halfInvW);
+    const unsigned pos = (unsigned)(t - pl * (long long)a.W);
+    const long long p = first + pl;
+    const int win = a.pwin[p];
+    const ChildBox x = child_box(a.plo + p * GPUGO_D, a.phi + p * GPUGO_D, a.pw + p * GPUGO_D, win, GPUGO_D, a.c,
+                                 (unsigned)a.wid, a.inv_wid, a.half_inv_wid, pos);
+    const iv<double> *rest = (const iv<double> *)(a.prest) + p * (long long)((1 + S) * GPUGO_NREST);
+    for (int ts = 0; ts < S; ++ts) {
+      const DiagPoint<ChildBox> xs{x, ts, S};
+      const F32Box<DiagPoint<ChildBox>> xf{xs};
+      const double u =
+          (double)obj_value_w32<R>(xf, (const iv<float> *)a.params32, a.d, rest + (1 + ts) * GPUGO_NREST, win, a.c).hi;
+      m = u < m ? u : m;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t2 = __shfl_xor(m, o, 64);
+    m = t2 < m ? t2 : m;
+  }
+  if (lane == 0 && m < __builtin_huge_val()) atomicMin(&s_ubmin, order_key(m));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_ubmin < order_key(__builtin_huge_val())) atomicMin(&st->ub_key, s_ubmin);
+    __threadfence();
+    if (atomicAdd(&st->cblocks, 1) == (int)gridDim.x - 1) {
+      __threadfence();
+      const double ub = key_value(__hip_atomic_load(&st->ub_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (ub < st->ub_snap) st->ub_snap = ub;
+      st->cblocks = 0;
+    }
+  }
+}
+
+// FP32 value enclosure of explicit boxes (parity hook of the windowed
+// pruning pass): the windowed FP32 list with the window covering every
+// coordinate (c = d) and empty rest sums, so every term is evaluated over
+// the box; a.ub carries the coefficient intervals (iv<float>[n_params]).
+template <int R>
+__device__ __forceinline__ void eval32_w_impl(const EvalArgs &a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const RowBox x{a.lo + i * GPUGO_D, a.hi + i * GPUGO_D};
+  const F32Box<RowBox> xf{x};
+  iv<double> rest[GPUGO_NREST];
+  for (int s = 0; s < GPUGO_NREST; ++s) rest[s] = mk<double>(0.0, 0.0);
+  const iv<float> f = obj_value_w32<R>(xf, (const iv<float> *)a.ub, a.d, rest, 0, a.d);
+  a.flo[i] = (double)f.lo;
+  a.fhi[i] = (double)f.hi;
+}
+#endif
+
 }  // namespace gpugo
 
+#if GPUGO_WITH_F32 && GPUGO_WITH_EXACT
+extern "C" __global__ __launch_bounds__(256) void gpugo_sample32_exact(gpugo::BoundArgs a) {
+  gpugo::sample32_w_impl<gpugo::ROUND_EXACT>(a);
+}
+extern "C" __global__ __launch_bounds__(64) void gpugo_eval32_exact(gpugo::EvalArgs a) {
+  gpugo::eval32_w_impl<gpugo::ROUND_EXACT>(a);
+}
+#endif
+#if GPUGO_WITH_F32 && GPUGO_WITH_FAST
+extern "C" __global__ __launch_bounds__(256) void gpugo_sample32_fast(gpugo::BoundArgs a) {
+  gpugo::sample32_w_impl<gpugo::ROUND_FAST>(a);
+}
+extern "C" __global__ __launch_bounds__(64) void gpugo_eval32_fast(gpugo::EvalArgs a) {
+  gpugo::eval32_w_impl<gpugo::ROUND_FAST>(a);
+}
+#endif
 #if GPUGO_WITH_EXACT
 extern "C" __global__ __launch_bounds__(256) void gpugo_bound_exact(gpugo::BoundArgs a) {
   gpugo::bound_w_impl<gpugo::ROUND_EXACT>(a);
@@ -249,7 +364,7 @@ extern "C" __global__ void gpugo_info(int *out) {
   out[1] = GPUGO_NPARAMS;
   out[2] = GPUGO_MODULE_ABI_V;
   out[3] = (GPUGO_WITH_EXACT ? 1 : 0) | (GPUGO_WITH_FAST ? 2 : 0) | (GPUGO_WITH_DIGITS15 ? 4 : 0);
-  out[4] = 0;
+  out[4] = GPUGO_WITH_F32;
   out[5] = 0;
   out[6] = GPUGO_NREST;
 }

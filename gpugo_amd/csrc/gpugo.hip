@@ -993,7 +993,6 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
     // windowed (large-d) module: natural-extension bound, diagonal samples
     if (desc->flags & (GPUGO_FLAG_MEAN_VALUE | GPUGO_FLAG_SPLIT | GPUGO_FLAG_PERSIST))
       return set_err(GPUGO_E_INVALID, "windowed modules take no mean-value, split or persistent flags");
-    if (desc->precision != GPUGO_PREC_F64) return set_err(GPUGO_E_INVALID, "windowed modules are FP64 only");
   }
   if (std::isnan(desc->tol_f) || std::isnan(desc->tol_x) || std::isnan(desc->init_ub))
     return set_err(GPUGO_E_DOMAIN, "NaN tolerance or init_ub");

@@ -224,7 +224,10 @@ class _FuncG(_Func):
                 raise ValueError("loop index used outside a template loop")
             _, a, b = e.val
             idx = f"({a} * j + {b})" if b >= 0 else f"({a} * j - {-b})"
-            return ("i", f"x[{idx}]") if op == "var" else ("s", f"P[{idx}]")
+            if op == "var":
+                return ("i", f"x[{idx}]")
+            # FP32 code lists take the coefficients as outward FP32 intervals
+            return ("i" if self.ty == "float" else "s", f"P[{idx}]")
         if op == "tsum":
             return self.loop_sum(e)
         if op == "twhen":
@@ -379,19 +382,61 @@ class _FuncW(_FuncG):
     touch the window, enclosed over the parent record once per parent) plus
     the terms j in the window's ranges, evaluated over the child."""
 
-    def __init__(self, slots: Dict[tuple, int]):
-        super().__init__()
+    def __init__(self, slots: Dict[tuple, int], ty: str = "double"):
+        super().__init__(ty)
         self.slots = slots
 
     def loop_sum(self, e: Expr) -> Tuple[str, str]:
         s = self.slots[e.key()]
         acc = f"t{self.n}"
         self.n += 1
-        self.lines.append(f"  iv<double> {acc} = rest[{s}];")
+        if self.ty == "float":
+            # the FP64 rest sum rounded outward to an FP32 interval
+            self.lines.append(f"  iv<float> {acc} = mk<float>(to_dn<float>(rest[{s}].lo), to_up<float>(rest[{s}].hi));")
+            term = f"wterm32_{s}"
+        else:
+            self.lines.append(f"  iv<double> {acc} = rest[{s}];")
+            term = f"wterm_{s}"
         self.lines.append(f"  {{ int rlo[2], rhi[2]; const int nr = wranges_{s}(win, c, d, rlo, rhi);")
         self.lines.append("    for (int r = 0; r < nr; ++r)")
-        self.lines.append(f"      for (int j = rlo[r]; j < rhi[r]; ++j) {acc} = A::add({acc}, wterm_{s}<R>(x, P, d, j)); }}")
+        self.lines.append(f"      for (int j = rlo[r]; j < rhi[r]; ++j) {acc} = A::add({acc}, {term}<R>(x, P, d, j)); }}")
         return ("i", acc)
+
+
+def window_lists_f32(obj: Objective) -> str:
+    """FP32 windowed value code list of the FP32-prune form, after
+    window_lists (it uses its wranges_s):
+      wterm32_s<R>(x, P, d, j)   term j of template sum s in FP32 intervals
+      obj_value_w32<R>(x, P, d, rest, win, c)  value over a child: each FP64
+          rest sum rounded outward to FP32 plus the window's terms in FP32
+    x is an FP32 accessor (coordinates rounded outward) and P the
+    coefficients as outward FP32 intervals."""
+    t = obj.templates
+    value, grad = wrap(t["value"]), wrap(t["grad"])
+    sums = _tsums(value, grad)
+    slots = {e.key(): i for i, e in enumerate(sums)}
+    out = ["namespace gpugo {"]
+    for s, e in enumerate(sums):
+        f = _FuncG("float")
+        f.in_loop = True
+        code = f.as_iv(f.lower(e.args[0]))
+        out.append(f"template <int R, class XS>\n__device__ __forceinline__ iv<float> wterm32_{s}("
+                   "const XS &x, const iv<float> *__restrict__ P, int d, int j) {")
+        out.append("  typedef IA<float, R> A;")
+        out += f.lines
+        out.append(f"  return {code};")
+        out.append("}")
+    f = _FuncW(slots, "float")
+    v = f.lower(value)
+    out.append("template <int R, class XS>\n__device__ __forceinline__ iv<float> obj_value_w32("
+               "const XS &x, const iv<float> *__restrict__ P, int d, const iv<double> *__restrict__ rest, "
+               "int win, int c) {")
+    out.append("  typedef IA<float, R> A;")
+    out += f.lines
+    out.append(f"  return {f.as_iv(v)};")
+    out.append("}")
+    out.append("}  // namespace gpugo")
+    return "\n".join(out) + "\n"
 
 
 def window_lists(obj: Objective) -> str:
@@ -493,25 +538,31 @@ def window_lists(obj: Objective) -> str:
     return "\n".join(out) + "\n"
 
 
-def window_module_source(obj: Objective, roundings=("fast",)) -> str:
+def window_module_source(obj: Objective, roundings=("fast",), f32: bool = False) -> str:
     """hipRTC source of a windowed (large-d) module: loop-form code lists
     only (no per-coordinate unrolling, so the code does not grow with d),
-    the windowed split+bound kernel and the per-parent rest-sum kernel."""
+    the windowed split+bound kernel and the per-parent rest-sum kernel.
+    `f32` adds the FP32 windowed value list of the FP32-prune form (FAST
+    and EXACT roundings only)."""
     bad = set(roundings) - {"exact", "fast", "digits15"}
     if bad or not roundings:
         raise ValueError(f"roundings must be a non-empty subset of ('exact', 'fast', 'digits15'), got {roundings}")
-    return "\n".join([
+    if f32 and "digits15" in roundings:
+        raise ValueError("the 15-digit IA has no FP32 form")
+    parts = [
         "// generated by gpugo_amd.emit (windowed form) for objective " + obj.name,
         f"#define GPUGO_WITH_EXACT {int('exact' in roundings)}",
         f"#define GPUGO_WITH_FAST {int('fast' in roundings)}",
         f"#define GPUGO_WITH_DIGITS15 {int('digits15' in roundings)}",
+        f"#define GPUGO_WITH_F32 {int(bool(f32))}",
         "#define GPUGO_WINDOWED 1",
         _read("ia.hpp"), _read("bnb_args.hpp"), _read("bnb_box.hpp"),
         window_lists(obj),
-        split_lists(obj),
-        "#define GPUGO_MODULE_STEP 1", _read("bnb_step.hpp"),
-        _read("bnb_window.hpp"),
-    ])
+    ]
+    if f32:
+        parts.append(window_lists_f32(obj))
+    parts += [split_lists(obj), "#define GPUGO_MODULE_STEP 1", _read("bnb_step.hpp"), _read("bnb_window.hpp")]
+    return "\n".join(parts)
 
 
 # ---------------------------------------------------------------------------

@@ -33,9 +33,9 @@ _mod_lock = threading.Lock()
 
 def _source(obj: Objective, roundings, f32: bool, split: int, windowed: bool) -> str:
     if windowed:
-        if f32 or split:
-            raise ValueError("windowed modules have no FP32 or term-parallel kernels")
-        return window_module_source(obj, tuple(roundings))
+        if split:
+            raise ValueError("windowed modules have no term-parallel kernels")
+        return window_module_source(obj, tuple(roundings), f32)
     return module_source(obj, tuple(roundings), f32, split)
 
 
@@ -184,7 +184,8 @@ class Problem:
         one cooperative launch with grid barriers between their phases.
         windowed: the large-d form (loop templates only; each template sum
         is kept per parent record outside its split window, so a child
-        costs the terms its window touches; FP64, any n_samples).
+        costs the terms its window touches; any n_samples; FP64, or the
+        FP32-prune form with precision "fp32").
         eager_compact (k_select 1): compact the queue after every step
         instead of every few steps with per-chunk minimum keys between
         (same results; a test and measurement switch)."""

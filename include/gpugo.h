@@ -154,7 +154,9 @@ int gpugo_eval_boxes_f32(gpugo_module_t m, int32_t d, const double *params, int6
  * rather than all d (the reference evaluates all d coordinates per child,
  * 6-Levy/levy10000.py:160-232); the rest sums are also kept at the parent's
  * n_samples diagonal points for the incumbent samples.  Problems on such a
- * module take no flags and FP64 precision.  This is the test hook of the window rule
+ * module take no flags; precision is FP64, or GPUGO_PREC_F32 when the module
+ * carries the FP32 windowed value list (gpugo_eval_boxes_f32 then evaluates
+ * it on explicit boxes).  This is the test hook of the window rule
  * shared by the device code: for each row of args (win, c, d, klo, khi, nt:
  * terms j in [0, nt) using coordinates j + k, k in [klo, khi]) it writes
  * (n, lo0, hi0, lo1, hi1), the n <= 2 ranges [lo, hi) of terms whose

@@ -551,6 +551,26 @@ int oracle_eval_boxes(int fam, int d, const double *P, int64_t n,
   return 0;
 }
 
+/* Mean-value lower bound of n explicit boxes (the bound that
+ * ORACLE_MEAN_VALUE solves use beside the natural extension): the exported
+ * form of family_mv_lower, so tests can pin it on its own. */
+int oracle_mv_lower(int fam, int d, const double *P, int64_t n,
+                    const double *lo, const double *hi, double *out)
+{
+  if (fam < 0 || fam > 3 || d < 1) return -1;
+  int old = fegetround();
+  iv *x = malloc(sizeof(iv) * d), *g = malloc(sizeof(iv) * d), *xs = malloc(sizeof(iv) * d);
+  for (int64_t i = 0; i < n; ++i) {
+    for (int j = 0; j < d; ++j) x[j] = mk(lo[i * d + j], hi[i * d + j]);
+    fesetround(FE_UPWARD);
+    family_grad(fam, d, P, x, g);
+    out[i] = family_mv_lower(fam, d, P, x, g, xs);
+  }
+  fesetround(old);
+  free(x); free(g); free(xs);
+  return 0;
+}
+
 /* ---------------------------------------------------------------------------
  * Branch and bound.
  * ------------------------------------------------------------------------- */

@@ -25,6 +25,8 @@ int oracle_eval_boxes(int fam, int d, const double *P, int64_t n,
                       const double *dlo, const double *dhi, int S,
                       double *flo, double *fhi, double *glo, double *ghi,
                       int32_t *kkt, double *ub);
+int oracle_mv_lower(int fam, int d, const double *P, int64_t n,
+                    const double *lo, const double *hi, double *out);
 int oracle_solve(const oracle_problem *pr, gpugo_result *res,
                  double *out_lo, double *out_hi, double *out_lb, int64_t out_max);
 int64_t oracle_split_bound(int fam, int d, const double *P, int64_t nparents,

@@ -69,6 +69,7 @@ def lib():
                                        ctypes.c_int64, u32, d, d]
         _lib.oracle_eval_boxes.argtypes = [ctypes.c_int, ctypes.c_int, d, ctypes.c_int64, d, d, d, d,
                                            ctypes.c_int, d, d, d, d, i32, d]
+        _lib.oracle_mv_lower.argtypes = [ctypes.c_int, ctypes.c_int, d, ctypes.c_int64, d, d, d]
         _lib.oracle_solve.argtypes = [ctypes.POINTER(OracleProblem), ctypes.POINTER(GpugoResult),
                                       d, d, d, ctypes.c_int64]
         _lib.oracle_split_bound.restype = ctypes.c_int64
@@ -129,6 +130,19 @@ def eval_boxes(family: int, params, lo, hi, dom_lo, dom_hi, n_samples: int = 1):
     if rc:
         raise ValueError("oracle_eval_boxes: bad family/d")
     return dict(flo=flo, fhi=fhi, glo=glo, ghi=ghi, kkt=kkt, ub=ub)
+
+
+def mv_lower(family: int, params, lo, hi):
+    """Mean-value lower bound f(c) + sum_j G_j (X_j - c_j) of each box (c the
+    midpoint, G the gradient enclosure over the box)."""
+    lo, hi = _f64(lo), _f64(hi)
+    n, d = lo.shape
+    params = _f64(params)
+    out = np.empty(n)
+    rc = lib().oracle_mv_lower(family, d, _p(params), n, _p(lo), _p(hi), _p(out))
+    if rc:
+        raise ValueError("oracle_mv_lower: bad family/d")
+    return out
 
 
 def split_bound(family, params, plo, phi, pwin, wid, cycle, n_samples, dom_lo, dom_hi, ub,

@@ -18,21 +18,28 @@ pytestmark = pytest.mark.gpu
 KEYS = ("lb", "ub", "width", "status", "iterations", "boxes_bounded", "queue_size")
 
 
-def specs(n, dmax=32):
+def specs(n, dmax=32, dmin=1):
+    """The first n served-workload problems with dmin <= d <= dmax."""
     out, k = [], 0
     while len(out) < n:
         s = pb.served_problem(k)
         k += 1
-        if s.d <= dmax:
+        if dmin <= s.d <= dmax:
             out.append(s)
     return out
 
 
-def test_served_equals_alone(gpu):
-    ss = specs(24)
-    arrivals = np.arange(len(ss)) * 2e-3
-    with Server(slots={16: 3, 32: 2}, dmax_classes=(16, 32), max_iters=300) as srv:
-        done = srv.run_trace(ss, arrivals, time_limit=60)
+@pytest.mark.parametrize("classes,slots,dmin,dmax,lam", [
+    ((16, 32), {16: 3, 32: 2}, 1, 32, 500.0),
+    ((64, 100), {64: 3, 100: 2}, 33, 100, 200.0),
+], ids=["d<=32", "d33-100"])
+def test_served_equals_alone(gpu, classes, slots, dmin, dmax, lam):
+    """Poisson arrivals (the served workload's exponential gaps, seed 40)
+    into fewer slots than requests."""
+    ss = specs(24, dmax=dmax, dmin=dmin)
+    arrivals = pb.arrivals(lam, len(ss), seed=40)
+    with Server(slots=slots, dmax_classes=classes, max_iters=300) as srv:
+        done = srv.run_trace(ss, arrivals, time_limit=90)
         assert len(done) == len(ss)
         byid = {r.rid: r for r in done}
         # some request was admitted while an earlier one was still running

@@ -96,6 +96,12 @@ def test_mean_value_mode_matches_oracle(gpu, oracle, spec, grid, k):
     plain, _ = oracle.solve(FAM[spec.family], spec.params, spec.lo, spec.hi, tol_f=1e-6, wid=grid[0],
                             cycle=grid[1], k_select=k, max_iters=20000)
     assert r["iterations"] <= plain["iterations"]
+    # the plain (natural-extension) enclosure is pinned independently of the
+    # mean-value code (closed form, L-BFGS witness): both must meet it
+    assert plain["status"] == 1
+    assert max(r["lb"], plain["lb"]) <= min(r["ub"], plain["ub"])
+    if spec.f_star is not None:
+        assert r["lb"] <= spec.f_star <= r["ub"]
 
 
 def test_deterministic(gpu):
