@@ -7,7 +7,11 @@ exp(mean cos(C x)) + A + e + sum_i q_i x_i^4 / 50, d = 50, coefficients
 drawn from seed 2, box [-32, 32]^50, tol 1e-6 on the width of the value
 enclosure, 3^12 children of one box per step, the midpoint as the sample
 point; children are ruled out in FP32 intervals (outward, one ulp per
-operation) and the enclosure is certified in FP64.
+operation) and the enclosure is certified in FP64.  Windowed code lists: a
+child differs from its parent in the 12 coordinates of the split window, so
+it evaluates those terms of each sum; the other 38 are enclosed once per
+parent (the unrolled form, every coordinate of every child, is measured
+beside it in c2_forms).
 
 One "step" = one complete solve, from the root box to a certified enclosure
 [lb, ub] with ub - lb <= 1e-6, with the objective module loaded and the
@@ -51,6 +55,12 @@ FAMILY_INDEX = {"sepcos": 0, "sepcos_chain": 1, "ackley_q": 2}
 TRAFFIC_FILE = "profiles/traffic_ackley50.json"
 # the headline schedule: the reference's one box per step, 3^12 children
 C2_GRID, C2_K = (3, 12), 1
+# seconds of one-thread work per CPU baseline (a bounded sample unless a
+# complete solve fits)
+ONE_CORE_BUDGET = 10.0
+# (precision, windowed code lists): the headline form and the others measured beside it
+HEADLINE_FORM = ("fp32", True)
+OTHER_FORMS = (("fp64", True), ("fp32", False), ("fp64", False))
 
 
 def parse():
@@ -59,9 +69,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip every CPU baseline")
-    ap.add_argument("--cpu-budget", type=float, default=20.0,
+    ap.add_argument("--cpu-budget", type=float, default=40.0,
                     help="seconds of host work per CPU baseline (complete solves when they fit)")
-    ap.add_argument("--legs", default="c2_fp64,c2_s10,c1,c1_tuned,c0,cold,large_d,xl_d,batch,served",
+    ap.add_argument("--legs", default="c2_forms,c2_s10,c1,c1_tuned,c0,cold,large_d,xl_d,batch,served",
                     help="comma list besides the headline; 'none' for the headline only")
     ap.add_argument("--batch-n", type=int, default=4096)
     ap.add_argument("--seq-sample", type=int, default=256,
@@ -234,7 +244,8 @@ def cpu_reference(spec, grid, k_select, n_samples, budget, gpu_iterations=None, 
         # bounded sample: as many steps from the root as fit the budget
         steps = max(2, int(budget / max(t_probe / 2, 1e-9)))
         if gpu_iterations is not None:
-            steps = min(steps, gpu_iterations)
+            # the certificate is taken at the select after the last split step
+            steps = min(steps, gpu_iterations + 1)
         t, r = run(th, steps)
         rate = r["boxes_bounded"] / t
         if r["status"] == 1:  # the bounded run reached the certificate
@@ -250,6 +261,7 @@ def cpu_reference(spec, grid, k_select, n_samples, budget, gpu_iterations=None, 
                 "method": f"reference loop: one box per step, {grid[0]}^{grid[1]} children, "
                           f"{n_samples} diagonal sample(s), natural extension + monotonicity test"})
     if one_core:
+        budget = min(budget, ONE_CORE_BUDGET)
         out["one_core"] = measure(1, 3)
     return out
 
@@ -273,19 +285,42 @@ def with_ratio(cpu, gpu_s, gpu_boxes):
 # ---------------------------------------------------------------------------
 # legs
 # ---------------------------------------------------------------------------
-def leg_config(G, pb, args, spec, grid, k, *, precisions=("fp64",), n_samples=1, mean_value=False, split=0,
-               cpu=True, kernel="gpugo_bound_fast", max_iters=100000):
-    obj = pb.objective(spec.family, spec.d)
+def form_solves(G, pb, args, spec, grid, k, form, *, n_samples=1, mean_value=False, split=0, max_iters=100000):
+    """Solves of one form: form = (precision, windowed).  Windowed forms use
+    the loop-form module (a child costs the terms its split window touches;
+    the other terms are summed once per parent record)."""
+    prec, windowed = form
+    if windowed:
+        obj = pb.objective_loops(spec.family, spec.d)
+        module = G.get_module(obj, roundings=("fast",), f32=prec == "fp32", windowed=True)
+    else:
+        obj = pb.objective(spec.family, spec.d)
+        module = None
+    g = gpu_solves(G, obj, spec, steps=args.steps, warmup=args.warmup, grid=grid, k_select=k,
+                   n_samples=n_samples, precision=prec, mean_value=mean_value, split=split, windowed=windowed,
+                   module=module, max_iters=max_iters, max_queue=1 << 26)
+    kname = "gpugo_bound_fast" + ("_mv" if mean_value else "") + ("_g" if split else "")
+    g["roofline"] = roofline(obj, g["_timed"], precision=prec, mean_value=mean_value, n_samples=n_samples,
+                             kernel=kname + (" (windowed)" if windowed else ""),
+                             windowed_c=min(grid[1], spec.d) if windowed else 0)
+    return g
+
+
+def form_name(form):
+    prec, windowed = form
+    return f"{'windowed' if windowed else 'unrolled'}_{prec}"
+
+
+def leg_config(G, pb, args, spec, grid, k, *, forms=(("fp64", False),), n_samples=1, mean_value=False, split=0,
+               cpu=True, max_iters=100000):
     out = {"workload": f"{spec.name}, box [{spec.lo[0]:g},{spec.hi[0]:g}]^{spec.d}, tol {spec.tol:g}",
            "grid": list(grid), "k_select": k, "n_samples": n_samples, "mean_value": mean_value,
            "lanes_per_child": split or 1}
     first = None
-    for prec in precisions:
-        g = gpu_solves(G, obj, spec, steps=args.steps, warmup=args.warmup, grid=grid, k_select=k,
-                       n_samples=n_samples, precision=prec, mean_value=mean_value, split=split, max_iters=max_iters)
-        g["roofline"] = roofline(obj, g["_timed"], precision=prec, mean_value=mean_value, n_samples=n_samples,
-                                 kernel=kernel + ("_mv" if mean_value else "") + ("_g" if split else ""))
-        out[prec] = public(g)
+    for form in forms:
+        g = form_solves(G, pb, args, spec, grid, k, form, n_samples=n_samples, mean_value=mean_value, split=split,
+                        max_iters=max_iters)
+        out[form_name(form)] = public(g)
         first = first or g
     if cpu and not args.no_cpu:
         print(f"# cpu {spec.name} {grid} S={n_samples}", file=sys.stderr, flush=True)
@@ -303,8 +338,8 @@ def leg_cold(G, pb):
     import ctypes
     from gpugo_amd import _lib, emit
     spec = pb.ackley50()
-    obj = pb.objective(spec.family, spec.d)
-    src = emit.module_source(obj, ("fast",), True, 0)
+    obj = pb.objective_loops(spec.family, spec.d)
+    src = emit.window_module_source(obj, ("fast",), f32=True)
     lib = _lib.load()
     t0 = time.perf_counter()
     h = ctypes.c_void_p()
@@ -314,12 +349,12 @@ def leg_cold(G, pb):
     class _M:  # a Module-like wrapper around the freshly compiled handle
         handle = h
     p = G.Problem(obj, spec.params, spec.lo, spec.hi, tol=spec.tol, grid=C2_GRID, k_select=C2_K,
-                  precision="fp32", module=_M, max_iters=1000)
+                  precision="fp32", module=_M, windowed=True, max_iters=1000)
     r = p.solve()
     t2 = time.perf_counter()
     p.close()
     lib.gpugo_module_destroy(h)
-    return {"workload": "ackley50, FP32-prune / FP64-certify form, compiled from source",
+    return {"workload": "ackley50, windowed FP32-prune / FP64-certify form, compiled from source",
             "compile_s": t1 - t0, "cold_time_to_certified_s": t2 - t0, "solve_s": t2 - t1,
             "certified": r["status_name"] == "certified"}
 
@@ -522,11 +557,9 @@ def main():
     legs = set(args.legs.split(","))
     G.init(0)
     spec = pb.ackley50()
-    obj = pb.objective(spec.family, spec.d)
-    head = gpu_solves(G, obj, spec, steps=args.steps, warmup=args.warmup, grid=C2_GRID, k_select=C2_K,
-                      precision="fp32")
+    head = form_solves(G, pb, args, spec, C2_GRID, C2_K, HEADLINE_FORM)
     print(f"# headline done ({args.steps} solves)", file=sys.stderr, flush=True)
-    rl = roofline(obj, head["_timed"], precision="fp32", kernel="gpugo_bound_fast")
+    rl = head["roofline"]
     rl["traffic"] = measured_traffic("gpugo_bound_fast")
     rl["traffic_source"] = TRAFFIC_FILE
     out = {
@@ -543,7 +576,8 @@ def main():
         "dtype": "f32 prune, f64 certify",
         "data": "synthetic (seeded coefficients)",
         "config": {"workload": "ackley50: ackley_q d=50, box [-32,32]^50, tol 1e-6, seed 2",
-                   "precision": "fp32 prune, fp64 certify", "grid": list(C2_GRID), "k_select": C2_K,
+                   "precision": "fp32 prune, fp64 certify", "code_lists": "windowed",
+                   "grid": list(C2_GRID), "k_select": C2_K,
                    "n_samples": 1, "rounding": "fast", "mean_value": False,
                    "note": "certifies in one step: the child holding x* = 0 has natural-extension lower "
                            "bound 0 and its midpoint is x*; see c2_s10 for the ten-sample run"},
@@ -560,13 +594,15 @@ def main():
         out["cpu_baseline"] = with_ratio(cpu_reference(spec, C2_GRID, C2_K, 1, args.cpu_budget,
                                                        gpu_iterations=head["iterations"]),
                                          head["mean_s"], head["boxes_bounded"])
-    if "c2_fp64" in legs:
-        print("# leg c2_fp64", file=sys.stderr, flush=True)
-        out["c2_fp64"] = leg_config(G, pb, args, spec, C2_GRID, C2_K, cpu=False)
-        out["c2_fp64"]["fp32_form_speedup"] = out["c2_fp64"]["fp64"]["mean_s"] / head["mean_s"]
+    if "c2_forms" in legs:
+        print("# leg c2_forms", file=sys.stderr, flush=True)
+        out["c2_forms"] = leg_config(G, pb, args, spec, C2_GRID, C2_K, forms=OTHER_FORMS, cpu=False)
+        out["c2_forms"]["headline_speedup_over"] = {
+            form_name(f): out["c2_forms"][form_name(f)]["mean_s"] / head["mean_s"] for f in OTHER_FORMS}
     if "c2_s10" in legs:
         print("# leg c2_s10", file=sys.stderr, flush=True)
-        out["c2_s10"] = leg_config(G, pb, args, spec, C2_GRID, C2_K, precisions=("fp32", "fp64"), n_samples=10)
+        out["c2_s10"] = leg_config(G, pb, args, spec, C2_GRID, C2_K, forms=(HEADLINE_FORM,) + OTHER_FORMS,
+                                   n_samples=10)
     if "c1" in legs:
         print("# leg c1", file=sys.stderr, flush=True)
         out["c1"] = leg_config(G, pb, args, pb.chain10(), (4, 10), 1)
