@@ -454,6 +454,13 @@ def overflow_kinds(results):
     return {"box_queue": sum(1 for o in over if o & 1), "parent_arena": sum(1 for o in over if o & 2)}
 
 
+# batch slots: two growth tiers (8x and 64x the per-slot queue and arena);
+# the one-at-a-time solves get capacities that no problem of the workload
+# outgrows
+GROW_TIERS = 2
+SEQ_QUEUE, SEQ_PARENTS = 1 << 23, 1 << 18
+
+
 def leg_batch(G, pb, n, seq_sample):
     """n independent problems (families sepcos / sepcos_chain / ackley_q, d
     log-uniform in [2,32]) in one batch (schedules from the tuned table)
@@ -469,7 +476,7 @@ def leg_batch(G, pb, n, seq_sample):
         grid, ks = schedule_for(s.family, s.d)
         probs.append(BatchProblem.from_spec(s, grid=grid, k_select=ks, max_iters=maxit, tag=k))
     m = get_batch_module(pb.FAMILIES, 32, ("fast",))
-    kw = dict(module=m, max_queue=qcap, max_parents=pcap, steps_per_launch=8)
+    kw = dict(module=m, max_queue=qcap, max_parents=pcap, steps_per_launch=8, grow=GROW_TIERS)
     run_batch(Batch, probs[:64], **kw)  # warm-up
     res, el = run_batch(Batch, probs, **kw)
     cert = sum(r["status_name"] == "certified" for r in res.values())
@@ -484,7 +491,8 @@ def leg_batch(G, pb, n, seq_sample):
     for p in sample:
         obj = pb.objective(p.family, p.d)
         with G.Problem(obj, p.params, p.lo, p.hi, tol=p.tol, grid=p.grid, k_select=p.k_select,
-                       max_iters=maxit, max_queue=qcap, max_parents=pcap, module=mods[(p.family, p.d)]) as q:
+                       max_iters=maxit, max_queue=SEQ_QUEUE, max_parents=SEQ_PARENTS,
+                       module=mods[(p.family, p.d)]) as q:
             try:
                 seq[p.tag] = q.solve()
             except G.GpugoError:  # capacity exceeded: no certificate
@@ -502,6 +510,7 @@ def leg_batch(G, pb, n, seq_sample):
     return {"workload": f"{n} problems (sepcos / sepcos_chain / ackley_q, d log-uniform [2,32], box [-h,h]^d, "
                         "h~U[2,32], tol in {1e-4,1e-6}), seeds [3,k]",
             "slots": n, "steps_per_launch": 8, "graph": True, "max_queue": qcap, "max_parents": pcap,
+            "growth_tiers": GROW_TIERS, "sequential_capacity": {"max_queue": SEQ_QUEUE, "max_parents": SEQ_PARENTS},
             "max_iters": maxit, "time_s": el, "problems_per_s": n / el, "certified": cert,
             "certified_per_s": cert / el,
             "status_counts": {k: sum(r["status_name"] == k for r in res.values())

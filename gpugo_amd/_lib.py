@@ -62,7 +62,7 @@ class BatchDesc(ctypes.Structure):
     _fields_ = [
         ("slots", ctypes.c_int32), ("steps_per_launch", ctypes.c_int32), ("rounding", ctypes.c_int32),
         ("use_graph", ctypes.c_int32), ("max_queue", ctypes.c_int64), ("max_parents", ctypes.c_int64),
-        ("bound_blocks", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("bound_blocks", ctypes.c_int32), ("grow", ctypes.c_int32),
     ]
 
 

@@ -107,14 +107,17 @@ class Batch:
 
     def __init__(self, slots: int = 4096, *, families: Sequence[str] = pb.FAMILIES, dmax: int = 32,
                  rounding: str = "fast", steps_per_launch: int = 4, use_graph: bool = True,
-                 max_queue: int = 0, max_parents: int = 0, bound_blocks: int = 0,
+                 max_queue: int = 0, max_parents: int = 0, bound_blocks: int = 0, grow: int = 2,
                  module: Optional[BatchModule] = None):
+        """grow: larger tiers (0..3, each 8x the per-slot capacity of the one
+        below) into which a problem that outgrows its slot moves, state and
+        all, and resumes; 0 reports such a problem as overflow."""
         self.module = module or get_batch_module(families, dmax, (rounding,))
         self.dmax = self.module.dmax
         self.desc = _lib.BatchDesc(slots=int(slots), steps_per_launch=int(steps_per_launch),
                                    rounding=_lib.ROUND[rounding], use_graph=int(bool(use_graph)),
                                    max_queue=int(max_queue), max_parents=int(max_parents),
-                                   bound_blocks=int(bound_blocks), reserved=0)
+                                   bound_blocks=int(bound_blocks), grow=int(grow))
         h = ctypes.c_void_p()
         check(_lib.load().gpugo_batch_create(self.module.handle, ctypes.byref(self.desc), ctypes.byref(h)))
         self.handle = h
@@ -125,10 +128,25 @@ class Batch:
         desc = _lib.BatchDesc(slots=int(slots), steps_per_launch=int(kw.get("steps_per_launch", 4)),
                               rounding=_lib.ROUND[kw.get("rounding", "fast")], use_graph=1,
                               max_queue=int(kw.get("max_queue", 0)), max_parents=int(kw.get("max_parents", 0)),
-                              bound_blocks=0, reserved=0)
+                              bound_blocks=0, grow=int(kw.get("grow", 2)))
         n = ctypes.c_int64(0)
         check(_lib.load().gpugo_batch_bytes(module.handle, ctypes.byref(desc), ctypes.byref(n)))
         return n.value
+
+    @staticmethod
+    def max_slots(module: BatchModule, budget_bytes: int, limit: int = 4096, **kw) -> int:
+        """The most slots (at most `limit`, at least 1) whose batch, growth
+        tiers included, fits in budget_bytes of device memory."""
+        lo, hi = 1, int(limit)
+        if Batch.device_bytes(module, hi, **kw) <= budget_bytes:
+            return hi
+        while hi - lo > 1:
+            mid = (lo + hi) // 2
+            if Batch.device_bytes(module, mid, **kw) <= budget_bytes:
+                lo = mid
+            else:
+                hi = mid
+        return lo
 
     def admit(self, prob: BatchProblem, tag: Optional[int] = None) -> int:
         """Admit into a free slot; raises GpugoError (E_CAPACITY) when full."""

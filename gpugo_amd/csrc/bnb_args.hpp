@@ -16,9 +16,15 @@ enum gpugo_status {
 };
 #endif
 
+// A batch slot that would outgrow its box queue or parent arena in the next
+// step stops in this state instead (nothing consumed, the queue compacted),
+// so the host can move it into a slot of a larger tier and resume it there.
+// Internal to batches: never reported as a result.
+#define GPUGO_ST_GROW 6
+
 // layout version of the structs below; modules report it and the library
 // refuses a module built against another layout
-#define GPUGO_MODULE_ABI_V 5
+#define GPUGO_MODULE_ABI_V 6
 
 namespace gpugo {
 

@@ -120,7 +120,12 @@ struct SelectArgs {
   int lazy;              // K = 1 over a queue that keeps consumed (NaN) and ruled-out (lb > ub)
                          // entries between compactions: wmin always gives the box, stale
                          // entries are skipped, and the chunks to rescan are recorded
-  int pad_lazy;
+  int grow;              // batch slots with a larger tier above: a step that could exceed
+                         // the queue (q_cap) or the parent arena stops with GPUGO_ST_GROW
+                         // before anything is consumed, instead of overflowing
+  long long q_cap;       // queue capacity (entries), read when grow is set
+  unsigned W;            // wid^c children per split box, read when grow is set
+  int pad_grow;
 };
 
 // Selection key (lb, parent, pos), unique per box.  Parent records are
@@ -680,6 +685,16 @@ __device__ void select_body(const SelectArgs &a) {
   unsigned n_split, n_outs;
   const unsigned split_rank = block_excl_scan((tid < K && !is_out) ? 1u : 0u, scan_sh, &n_split);
   const unsigned out_rank = block_excl_scan((tid < K && is_out) ? 1u : 0u, scan_sh, &n_outs);
+  if (a.grow && N + (long long)n_split * (long long)a.W > a.q_cap) {
+    // this step's children might not fit the queue: stop before anything is
+    // consumed, so the slot can resume in a larger tier
+    if (tid == 0) {
+      st->status = GPUGO_ST_GROW;
+      st->q_count = N;
+      st->compact_final = 0;
+    }
+    return;
+  }
   long long pbase = st->parents_used, room = a.p_cap;
   const long long obase = st->n_out;
   const bool collected = pbase + (long long)n_split > room;
@@ -694,11 +709,20 @@ __device__ void select_body(const SelectArgs &a) {
   }
   if (pbase + (long long)n_split > room) {
     if (tid == 0) {
-      st->overflow |= 2;
-      st->status = GPUGO_ST_OVERFLOW;
-      st->q_count = N;
-      st->compact_count = 0;
-      st->compact_final = 1;
+      if (a.grow) {
+        // the records kept by the collection (renumbered, order kept) are
+        // the slot's state; nothing was consumed
+        st->status = GPUGO_ST_GROW;
+        st->parents_used = pbase;
+        st->q_count = N;
+        st->compact_final = 0;
+      } else {
+        st->overflow |= 2;
+        st->status = GPUGO_ST_OVERFLOW;
+        st->q_count = N;
+        st->compact_count = 0;
+        st->compact_final = 1;
+      }
     }
     return;
   }
@@ -789,8 +813,15 @@ struct CompactArgs {
 
 // blk / nblk: this workgroup's index and the number of workgroups sharing
 // the pass (any block size up to 1024 threads)
+// Entries per thread in one pass of a compaction workgroup: its loads are in
+// flight together, and the pass costs one scan, one global atomic and two
+// barriers for CMP_U * blockDim entries.
+#define CMP_U 8
+
 __device__ __forceinline__ void compact_body(const CompactArgs &a, int blk, int nblk) {
-  __shared__ unsigned wcnt[16];
+  // kept entries per (slab u, wavefront), then their exclusive prefix in
+  // (u, wavefront) order: the output order of the pass
+  __shared__ unsigned wcnt[CMP_U * 16];
   __shared__ unsigned long long s_base;
   DevState *st = a.st;
   // the host polls the status here instead of copying the state back
@@ -811,46 +842,63 @@ __device__ __forceinline__ void compact_body(const CompactArgs &a, int blk, int 
     s_kmax = 0ull;
   }
   const int nw = (int)(blockDim.x >> 6), bsz = (int)blockDim.x;
+  const int nv = CMP_U * nw;  // <= 128 counters
   // smallest kept key (order_key(lb), (parent ^ tie_xor) << 32 | pos) and its
   // output index, for a one-box select that needs no pass over the queue
   unsigned long long mh = ~0ull, ml = ~0ull, mo = 0ull;
-  for (long long base = (long long)blk * bsz; base < n; base += (long long)nblk * bsz) {
-    long long i = base + threadIdx.x;
-    double lb = 0;
-    bool keep = false;
-    if (i < n) {
-      lb = a.lb_in[i];
-      keep = lb <= ub;
+  const long long span = (long long)bsz * CMP_U;
+  for (long long base = (long long)blk * span; base < n; base += (long long)nblk * span) {
+    double lb[CMP_U];
+#pragma unroll
+    for (int u = 0; u < CMP_U; ++u) {
+      const long long i = base + (long long)u * bsz + threadIdx.x;
+      lb[u] = i < n ? a.lb_in[i] : 0.0;
+    }
+    unsigned kept = 0u, pre[CMP_U];
+#pragma unroll
+    for (int u = 0; u < CMP_U; ++u) {
+      const long long i = base + (long long)u * bsz + threadIdx.x;
+      const bool keep = i < n && lb[u] <= ub;
       if (keep) {
-        const unsigned long long k = order_key(lb);
+        const unsigned long long k = order_key(lb[u]);
         kmin = k < kmin ? k : kmin;
         kmax = k > kmax ? k : kmax;
+        kept |= 1u << u;
+      }
+      const unsigned long long mask = __ballot(keep);
+      pre[u] = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+      if (lane == 0) wcnt[u * nw + w] = (unsigned)__popcll(mask);
+    }
+    __syncthreads();
+    if (w == 0) {
+      // exclusive prefix of the nv counters (two per lane), one atomic
+      const unsigned v0 = lane < nv ? wcnt[lane] : 0u, v1 = lane + 64 < nv ? wcnt[lane + 64] : 0u;
+      const unsigned s0 = wave_incl_scan(v0), s1 = wave_incl_scan(v1);
+      const unsigned t0 = __shfl(s0, 63, 64), t1 = __shfl(s1, 63, 64);
+      if (lane < nv) wcnt[lane] = s0 - v0;
+      if (lane + 64 < nv) wcnt[lane + 64] = t0 + s1 - v1;
+      if (lane == 0) {
+        const unsigned tot = t0 + t1;
+        s_base = tot ? atomicAdd((unsigned long long *)&st->compact_count, (unsigned long long)tot) : 0ull;
       }
     }
-    unsigned long long mask = __ballot(keep);
-    unsigned pre = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-    if (lane == 0) wcnt[w] = __popcll(mask);
     __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned t = 0;
-      for (int q = 0; q < nw; ++q) t += wcnt[q];
-      s_base = t ? atomicAdd((unsigned long long *)&st->compact_count, (unsigned long long)t) : 0ull;
-    }
-    __syncthreads();
-    unsigned off = pre;
-    for (int q = 0; q < w; ++q) off += wcnt[q];
-    if (keep) {
-      long long o = (long long)s_base + off;
-      const unsigned par = a.par_in[i], pos = a.pos_in[i];
-      a.lb_out[o] = lb;
-      a.par_out[o] = par;
-      a.pos_out[o] = pos;
-      if (a.wmin) {
-        const unsigned long long kh = order_key(lb), kl = ((unsigned long long)(par ^ a.tie_xor) << 32) | pos;
-        if (kh < mh || (kh == mh && kl < ml)) {
-          mh = kh;
-          ml = kl;
-          mo = (unsigned long long)o;
+#pragma unroll
+    for (int u = 0; u < CMP_U; ++u) {
+      if (kept & (1u << u)) {
+        const long long i = base + (long long)u * bsz + threadIdx.x;
+        const long long o = (long long)s_base + wcnt[u * nw + w] + pre[u];
+        const unsigned par = a.par_in[i], pos = a.pos_in[i];
+        a.lb_out[o] = lb[u];
+        a.par_out[o] = par;
+        a.pos_out[o] = pos;
+        if (a.wmin) {
+          const unsigned long long kh = order_key(lb[u]), kl = ((unsigned long long)(par ^ a.tie_xor) << 32) | pos;
+          if (kh < mh || (kh == mh && kl < ml)) {
+            mh = kh;
+            ml = kl;
+            mo = (unsigned long long)o;
+          }
         }
       }
     }

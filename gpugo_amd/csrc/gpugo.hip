@@ -635,7 +635,8 @@ struct BatchArgs {
   double *params, *dom_lo, *dom_hi;
   long long *wave_off;
   long long pcap, qcap;
-  int dmax, pmax, slots, pad0;
+  int dmax, pmax, slots;
+  int grow;  // 1: a larger tier exists; slots that would outgrow this one stop with GPUGO_ST_GROW
 };
 
 __global__ __launch_bounds__(SEL_T) void kb_select(BatchArgs b) {
@@ -671,7 +672,10 @@ __global__ __launch_bounds__(SEL_T) void kb_select(BatchArgs b) {
   sa.nwmin = 0;
   sa.wmin = nullptr;
   sa.lazy = 0;
-  sa.pad_lazy = 0;
+  sa.grow = b.grow;
+  sa.q_cap = b.qcap;
+  sa.W = I.W;
+  sa.pad_grow = 0;
   select_body(sa);
 }
 
@@ -835,6 +839,50 @@ __global__ __launch_bounds__(64) void kb_admit(BatchArgs b, const char *recs, lo
   }
 }
 
+// Move slot s of batch `src` (stopped in GPUGO_ST_GROW) into slot t of batch
+// `dst`, a tier with larger queues and arenas over the same module: the
+// description, coefficients, initial box, the parent records in use, the
+// live (compacted) queue buffer and the state, which resumes as running.
+// Grid-stride over all copied elements; the state and description are
+// written by thread 0 of workgroup 0.
+__global__ __launch_bounds__(256) void kb_move(BatchArgs src, BatchArgs dst, int s, int t) {
+  const DevState S = src.st[s];
+  const long long n_rec = S.parents_used, n_q = S.compact_count;
+  const int D = src.dmax;
+  const int buf = (int)(S.compactions & 1);
+  const long long qs = ((long long)s * 2 + buf) * src.qcap, qt = ((long long)t * 2 + buf) * dst.qcap;
+  const long long rs = (long long)s * src.pcap, rt = (long long)t * dst.pcap;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (long long i = i0; i < n_rec * D; i += stride) {
+    dst.plo[rt * D + i] = src.plo[rs * D + i];
+    dst.phi[rt * D + i] = src.phi[rs * D + i];
+    dst.pw[rt * D + i] = src.pw[rs * D + i];
+  }
+  for (long long i = i0; i < n_rec; i += stride) dst.pwin[rt + i] = src.pwin[rs + i];
+  for (long long i = i0; i < n_q; i += stride) {
+    dst.q_lb[qt + i] = src.q_lb[qs + i];
+    dst.q_par[qt + i] = src.q_par[qs + i];
+    dst.q_pos[qt + i] = src.q_pos[qs + i];
+  }
+  for (long long i = i0; i < src.pmax; i += stride)
+    dst.params[(long long)t * dst.pmax + i] = src.params[(long long)s * src.pmax + i];
+  for (long long i = i0; i < D; i += stride) {
+    dst.dom_lo[(long long)t * D + i] = src.dom_lo[(long long)s * D + i];
+    dst.dom_hi[(long long)t * D + i] = src.dom_hi[(long long)s * D + i];
+  }
+  if (i0 == 0) {
+    DevState z = S;
+    z.status = GPUGO_ST_RUNNING;
+    z.overflow = 0;
+    z.compact_final = 0;
+    z.cblocks = 0;
+    z.fresh = 0;
+    dst.st[t] = z;
+    dst.info[t] = src.info[s];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // smallest live lower bound (used when a run stops without a certificate)
 // ---------------------------------------------------------------------------
@@ -876,11 +924,14 @@ __global__ __launch_bounds__(256) void k_materialize(const double *plo, const do
 // in p->wmin for a one-box select.
 #define GPUGO_COMPACT_WG 512   // gpugo_compact, 256 threads
 #define GPUGO_FUSED_WG 128     // gpugo_compact_select, 1024 threads
+#define GPUGO_LAZY_FULL_WG 512 // the same as the full compaction of a lazy (one box per step) queue:
+                               // two workgroups per CU for the queues that grow large
 #define GPUGO_LAZY_WG 256      // gpugo_compact_select as the lazy tail, 1024 threads
 #define GPUGO_WMIN_MAX GPUGO_COMPACT_WG
 #define GPUGO_CHUNK_LOG2 13    // lazy queue: 8192 entries per chunk minimum
 #define GPUGO_LAZY_MAX_EVERY 256
-static_assert(GPUGO_LAZY_WG <= GPUGO_WMIN_MAX, "wmin holds one key per lazy-tail workgroup");
+static_assert(GPUGO_LAZY_WG <= GPUGO_WMIN_MAX && GPUGO_LAZY_FULL_WG <= GPUGO_WMIN_MAX,
+              "wmin holds one key per compaction or lazy-tail workgroup");
 
 struct Queue {
   DevBuf lb, par, pos;
@@ -896,7 +947,7 @@ struct gpugo_problem_s {
   int c = 0;
   unsigned W = 0;
   long long q_cap = 0, p_cap = 0, out_cap = 0;
-  DevBuf d_lo, d_hi, d_params, d_params32, plo, phi, pw, pwin, out_lo, out_hi, out_lb, st, scratch;
+  DevBuf d_lo, d_hi, d_params, d_params32, plo, phi, pw, pwin, out_lo, out_hi, out_lb, st, minkey;
   DevBuf bar;          // persistent step loop: grid barrier [2] and error flag
   DevBuf pext;         // windowed modules: rest sums per parent record [p_cap][ext_n]
   DevBuf wmin;         // k_select 1: per-compaction-workgroup smallest key [GPUGO_WMIN_MAX][3]
@@ -948,9 +999,79 @@ static int ipow_checked(int wid, int c, unsigned *out) {
   return 1;
 }
 
-extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *desc, gpugo_problem_t *out) {
-  if (!desc || !out) return set_err(GPUGO_E_INVALID, "null pointer");
-  *out = nullptr;
+// Shape of a problem's solve state: split window and children per box, queue
+// and parent-arena capacities (desc's, else the defaults given), output
+// arena, per-record payload of windowed modules and the lazy-queue schedule.
+static int problem_shape(gpugo_problem_s *p, const gpugo_problem_desc *desc, long long q_default,
+                         long long p_default) {
+  const int d = desc->d;
+  p->c = desc->cycle < d ? desc->cycle : d;
+  if (!ipow_checked(desc->wid, p->c, &p->W)) return set_err(GPUGO_E_INVALID, "wid^cycle must be < 2^32");
+  long long qc = desc->max_queue > 0 ? desc->max_queue : q_default;
+  if (qc > 0xfffffff0LL) qc = 0xfffffff0LL;
+  // windowed modules: (1 + n_samples) * nrest rest-sum intervals per record
+  p->ext_n = p->m->nrest > 0 ? 2 * p->m->nrest * (1 + desc->n_samples) : 0;
+  p->q_cap = qc;
+  p->p_cap = desc->max_parents > 0 ? desc->max_parents : p_default;
+  // output boxes (x-width rule) up to 2^16 or 2^24 coordinates
+  p->out_cap = std::max(64LL, std::min(1LL << 16, (1LL << 24) / d));
+  if (p->desc.max_iters <= 0) p->desc.max_iters = 1LL << 40;
+  p->lazy = false;
+  p->compact_every = 1;
+  if (desc->k_select == 1 && !(desc->flags & (GPUGO_FLAG_PERSIST | GPUGO_FLAG_EAGER_COMPACT))) {
+    // stale entries between compactions stay under an eighth of the queue:
+    // (compact_every - 1) steps append at most W entries each
+    const long long every = std::min((long long)GPUGO_LAZY_MAX_EVERY, qc / 8 / (long long)p->W);
+    if (every >= 2) {
+      p->lazy = true;
+      p->compact_every = (int)every;
+    }
+  }
+  return GPUGO_OK;
+}
+
+// The device buffers of a problem, in one order for both owners:
+// alloc(buf, bytes, name) either allocates (gpugo_problem_create) or places
+// the buffer in the static arena (gpugo_solve_static).
+template <class Alloc>
+static int problem_buffers(gpugo_problem_s *p, Alloc &&alloc) {
+  const long long d = p->desc.d, np = p->desc.n_params > 0 ? p->desc.n_params : 1;
+  const long long pc = p->p_cap, qc = p->q_cap;
+  int rc;
+#define PBUF(buf, bytes) \
+  if ((rc = alloc(buf, (size_t)(bytes), #buf)) != GPUGO_OK) return rc
+  PBUF(p->st, sizeof(DevState));
+  PBUF(p->d_lo, sizeof(double) * d);
+  PBUF(p->d_hi, sizeof(double) * d);
+  PBUF(p->d_params, sizeof(double) * np);
+  PBUF(p->d_params32, 2 * sizeof(float) * np);
+  PBUF(p->plo, sizeof(double) * d * pc);
+  PBUF(p->phi, sizeof(double) * d * pc);
+  PBUF(p->pw, sizeof(double) * d * pc);
+  PBUF(p->pwin, sizeof(int) * pc);
+  PBUF(p->out_lo, sizeof(double) * d * p->out_cap);
+  PBUF(p->out_hi, sizeof(double) * d * p->out_cap);
+  PBUF(p->out_lb, sizeof(double) * p->out_cap);
+  PBUF(p->minkey, 64);
+  PBUF(p->bar, 64);
+  if (p->ext_n > 0) PBUF(p->pext, sizeof(double) * p->ext_n * pc);
+  PBUF(p->wmin, sizeof(unsigned long long) * 3 * GPUGO_WMIN_MAX);
+  if (p->lazy) {
+    const long long nch = (qc + (1LL << GPUGO_CHUNK_LOG2) - 1) >> GPUGO_CHUNK_LOG2;
+    PBUF(p->cmin, sizeof(unsigned long long) * 3 * nch);
+  }
+  for (int b = 0; b < 2; ++b) {
+    PBUF(p->q[b].lb, sizeof(double) * qc);
+    PBUF(p->q[b].par, sizeof(unsigned) * qc);
+    PBUF(p->q[b].pos, sizeof(unsigned) * qc);
+  }
+#undef PBUF
+  return GPUGO_OK;
+}
+
+// Every check of a problem description against its module (shared by
+// gpugo_problem_create and gpugo_solve_static).
+static int problem_check(gpugo_module_t m, const gpugo_problem_desc *desc) {
   const int d = desc->d;
   if (d < 1) return set_err(GPUGO_E_INVALID, "d must be >= 1");
   if (d > GPUGO_DMAX_SELECT) return set_err(GPUGO_E_INVALID, "d must be <= %d", GPUGO_DMAX_SELECT);
@@ -997,6 +1118,15 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
   if (std::isnan(desc->tol_f) || std::isnan(desc->tol_x) || std::isnan(desc->init_ub))
     return set_err(GPUGO_E_DOMAIN, "NaN tolerance or init_ub");
   if (!(desc->tol_f > 0) && !(desc->tol_x > 0)) return set_err(GPUGO_E_INVALID, "set tol_f or tol_x");
+  return GPUGO_OK;
+}
+
+extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *desc, gpugo_problem_t *out) {
+  if (!desc || !out) return set_err(GPUGO_E_INVALID, "null pointer");
+  *out = nullptr;
+  int rc = problem_check(m, desc);
+  if (rc) return rc;
+  const int d = desc->d;
   auto *p = new gpugo_problem_s;
   p->m = m;
   p->desc = *desc;
@@ -1006,65 +1136,23 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
   p->desc.lo = p->lo.data();
   p->desc.hi = p->hi.data();
   p->desc.params = p->params.data();
-  p->c = desc->cycle < d ? desc->cycle : d;
-  if (!ipow_checked(desc->wid, p->c, &p->W)) {
-    delete p;
-    return set_err(GPUGO_E_INVALID, "wid^cycle must be < 2^32");
-  }
-  long long qc = desc->max_queue > 0 ? desc->max_queue : (1LL << 24);
-  if (qc > 0xfffffff0LL) qc = 0xfffffff0LL;
-  // windowed modules: (1 + n_samples) * nrest rest-sum intervals per record
-  p->ext_n = m->nrest > 0 ? 2 * m->nrest * (1 + desc->n_samples) : 0;
-  // default arenas: parent records up to 2^20 or 8 GiB, output boxes up to
-  // 2^16 or 2^24 coordinates
-  const long long rec_bytes = 24LL * d + 4 + 8LL * p->ext_n;
-  long long pc = desc->max_parents > 0 ? desc->max_parents : std::min(1LL << 20, (8LL << 30) / rec_bytes);
-  p->q_cap = qc;
-  p->p_cap = pc;
-  p->out_cap = std::max(64LL, std::min(1LL << 16, (1LL << 24) / d));
-  if (p->desc.max_iters <= 0) p->desc.max_iters = 1LL << 40;
   auto fail = [&](int code) {
     delete p;
     return code;
   };
+  // default arenas: parent records up to 2^20 or 8 GiB
+  const long long rec_bytes = 24LL * d + 4 + 8LL * (m->nrest > 0 ? 2 * m->nrest * (1 + desc->n_samples) : 0);
+  rc = problem_shape(p, desc, 1LL << 24, std::min(1LL << 20, (8LL << 30) / rec_bytes));
+  if (rc) return fail(rc);
   hipError_t e;
-#define PALLOC(buf, bytes)                                                                     \
-  if ((e = (buf).alloc(bytes)) != hipSuccess)                                                  \
-    return fail(set_err(e == hipErrorOutOfMemory ? GPUGO_E_NOMEM : GPUGO_E_HIP, "alloc %s: %s", \
-                        #buf, hipGetErrorString(e)))
-  PALLOC(p->d_lo, sizeof(double) * d);
-  PALLOC(p->d_hi, sizeof(double) * d);
-  PALLOC(p->d_params, sizeof(double) * (desc->n_params > 0 ? desc->n_params : 1));
-  PALLOC(p->d_params32, 2 * sizeof(float) * (desc->n_params > 0 ? desc->n_params : 1));
-  PALLOC(p->plo, sizeof(double) * d * pc);
-  PALLOC(p->phi, sizeof(double) * d * pc);
-  PALLOC(p->pw, sizeof(double) * d * pc);
-  PALLOC(p->pwin, sizeof(int) * pc);
-  PALLOC(p->out_lo, sizeof(double) * d * p->out_cap);
-  PALLOC(p->out_hi, sizeof(double) * d * p->out_cap);
-  PALLOC(p->out_lb, sizeof(double) * p->out_cap);
-  PALLOC(p->st, sizeof(DevState));
-  PALLOC(p->scratch, 64);
-  PALLOC(p->bar, 64);
-  if (p->ext_n > 0) PALLOC(p->pext, sizeof(double) * p->ext_n * pc);
-  PALLOC(p->wmin, sizeof(unsigned long long) * 3 * GPUGO_WMIN_MAX);
-  if (desc->k_select == 1 && !(desc->flags & (GPUGO_FLAG_PERSIST | GPUGO_FLAG_EAGER_COMPACT))) {
-    // stale entries between compactions stay under an eighth of the queue:
-    // (compact_every - 1) steps append at most W entries each
-    const long long every = std::min((long long)GPUGO_LAZY_MAX_EVERY, qc / 8 / (long long)p->W);
-    if (every >= 2) {
-      p->lazy = true;
-      p->compact_every = (int)every;
-      const long long nch = (qc + (1LL << GPUGO_CHUNK_LOG2) - 1) >> GPUGO_CHUNK_LOG2;
-      PALLOC(p->cmin, sizeof(unsigned long long) * 3 * nch);
-    }
-  }
-  for (int b = 0; b < 2; ++b) {
-    PALLOC(p->q[b].lb, sizeof(double) * qc);
-    PALLOC(p->q[b].par, sizeof(unsigned) * qc);
-    PALLOC(p->q[b].pos, sizeof(unsigned) * qc);
-  }
-#undef PALLOC
+  rc = problem_buffers(p, [](DevBuf &buf, size_t bytes, const char *name) {
+    const hipError_t err = buf.alloc(bytes);
+    if (err != hipSuccess)
+      return set_err(err == hipErrorOutOfMemory ? GPUGO_E_NOMEM : GPUGO_E_HIP, "alloc %s: %s", name,
+                     hipGetErrorString(err));
+    return (int)GPUGO_OK;
+  });
+  if (rc) return fail(rc);
   if ((e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) != hipSuccess)
     return fail(set_err(GPUGO_E_HIP, "stream: %s", hipGetErrorString(e)));
   if ((e = hipMemcpy(p->d_lo.p, p->lo.data(), sizeof(double) * d, hipMemcpyHostToDevice)) != hipSuccess ||
@@ -1221,7 +1309,10 @@ static SelectArgs select_args(gpugo_problem_t p, const Queue &q, int nwg) {
   sa.wmin = p->desc.k_select == 1 ? p->wmin.as<unsigned long long>() : nullptr;
   sa.nwmin = sa.wmin ? nwg : 0;
   sa.lazy = p->lazy ? 1 : 0;
-  sa.pad_lazy = 0;
+  sa.grow = 0;
+  sa.q_cap = p->q_cap;
+  sa.W = p->W;
+  sa.pad_grow = 0;
   return sa;
 }
 
@@ -1350,12 +1441,12 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
   }
   p->since_compact = 0;
   if (more && p->m->compact_select_fn) {
-    // 128 workgroups of 1024 threads compact; the last one selects from B
-    SelectArgs sa = select_args(p, B, GPUGO_FUSED_WG);
+    // workgroups of 1024 threads compact; the last one selects from B
+    const int nwg = p->lazy ? GPUGO_LAZY_FULL_WG : GPUGO_FUSED_WG;
+    SelectArgs sa = select_args(p, B, nwg);
     ca.poll = nullptr;
     void *fargs[] = {&ca, &sa, &poll};
-    HIPCHECK(hipModuleLaunchKernel(p->m->compact_select_fn, GPUGO_FUSED_WG, 1, 1, SEL_T, 1, 1, 0, p->stream, fargs,
-                                   nullptr));
+    HIPCHECK(hipModuleLaunchKernel(p->m->compact_select_fn, nwg, 1, 1, SEL_T, 1, 1, 0, p->stream, fargs, nullptr));
     p->selected_ahead = true;
   } else if (p->m->compact_fn) {
     void *cargs[] = {&ca};
@@ -1385,7 +1476,7 @@ static int fill_result(gpugo_problem_t p, gpugo_result *res, double ms, bool max
     // recompute the smallest live lower bound exactly
     const int lbuf = live_buffer(p);
     long long n = s.compact_count;
-    unsigned long long *d_key = p->scratch.as<unsigned long long>();
+    unsigned long long *d_key = p->minkey.as<unsigned long long>();
     hipLaunchKernelGGL(k_minlb, dim3(1), dim3(1024), 0, p->stream, p->st.as<DevState>(), p->q[lbuf].plb(), n, d_key);
     HIPCHECK(hipGetLastError());
     unsigned long long k;
@@ -1475,7 +1566,10 @@ static int persist_steps(gpugo_problem_t p, int64_t iters) {
   sa.nwmin = 0;
   sa.wmin = nullptr;
   sa.lazy = 0;
-  sa.pad_lazy = 0;
+  sa.grow = 0;
+  sa.q_cap = p->q_cap;
+  sa.W = p->W;
+  sa.pad_grow = 0;
   for (int b = 0; b < 2; ++b) {
     pa.q_lb[b] = p->q[b].plb();
     pa.q_par[b] = p->q[b].ppar();
@@ -1646,6 +1740,7 @@ static_assert(sizeof(AdmitHead) % 8 == 0, "admission records must keep doubles a
 
 struct BatchHostSlot {
   int running = 0;
+  int growing = 0;  // stopped in GPUGO_ST_GROW, waiting for a slot in the next tier
   int64_t tag = 0;
   long long admit_poll = 0;  // polls with a smaller sequence number predate the admission
   std::chrono::steady_clock::time_point t0;
@@ -1679,7 +1774,34 @@ struct gpugo_batch_s {
   long long polls = 0;
   std::vector<std::pair<int64_t, gpugo_result>> done;
   size_t done_head = 0;
+  // growth tiers: tier k (k >= 1) has GPUGO_GROW_FACTOR^k times the per-slot
+  // queue and arena of the batch the caller created (tier 0) and
+  // slots >> (GPUGO_GROW_SLOT_SHIFT * k) slots (at least 1).  Tier 0 owns the
+  // others; finished problems of every tier are reported by tier 0.
+  int level = 0;
+  gpugo_batch_s *root = nullptr;
+  std::vector<gpugo_batch_s *> tiers;  // tier 0 only: tiers 1..grow
+  std::vector<int> grow_wait;          // slots stopped in GPUGO_ST_GROW, not yet moved
+  hipEvent_t ev_ready = nullptr;       // recorded on this tier's stream before it is the source of moves
+  hipEvent_t ev_moved = nullptr;       // recorded on this tier's stream after moves into it
+  void *kp_bb[1] = {nullptr};          // kernel argument array of the split+bound launches
 };
+
+#define GPUGO_GROW_MAX 3
+#define GPUGO_GROW_FACTOR 8
+#define GPUGO_GROW_SLOT_SHIFT 5
+
+// description of growth tier k of a batch with description d
+static gpugo_batch_desc tier_desc(const gpugo_batch_desc &d, long long q0, long long p0, int k) {
+  gpugo_batch_desc t = d;
+  long long f = 1;
+  for (int i = 0; i < k; ++i) f *= GPUGO_GROW_FACTOR;
+  t.slots = std::max(1, d.slots >> (GPUGO_GROW_SLOT_SHIFT * k));
+  t.max_queue = std::min(q0 * f, 0xfffffff0LL);
+  t.max_parents = std::min(p0 * f, 0xfffffff0LL);
+  t.grow = 0;
+  return t;
+}
 
 static int batch_sizes(gpugo_module_t m, const gpugo_batch_desc *desc, long long *qcap, long long *pcap,
                        long long *rec_bytes, int64_t *bytes) {
@@ -1694,12 +1816,19 @@ static int batch_sizes(gpugo_module_t m, const gpugo_batch_desc *desc, long long
     return set_err(GPUGO_E_INVALID, "module has no %s variant", desc->rounding ? "fast" : "exact");
   if (desc->max_queue < 0 || desc->max_parents < 0 || desc->bound_blocks < 0)
     return set_err(GPUGO_E_INVALID, "negative capacity");
+  if (desc->grow < 0 || desc->grow > GPUGO_GROW_MAX)
+    return set_err(GPUGO_E_INVALID, "grow must be in [0, %d]", GPUGO_GROW_MAX);
   const long long q = desc->max_queue ? desc->max_queue : 65536, p = desc->max_parents ? desc->max_parents : 16384;
   if (q > 0xfffffff0LL || p > 0xfffffff0LL) return set_err(GPUGO_E_INVALID, "per-slot capacity too large");
-  const long long S = desc->slots, D = m->dim, PM = m->pmax;
+  const long long D = m->dim, PM = m->pmax;
   const long long rb = (long long)sizeof(AdmitHead) + 8 * (PM + 3 * D);
-  long long total = S * (long long)(sizeof(SlotInfo) + sizeof(DevState)) + S * p * (3 * D * 8 + 4) + S * PM * 8 +
-                    S * 2 * D * 8 + S * 2 * q * 16 + (S + 1) * 8 + S * rb;
+  long long total = 0;
+  for (int k = 0; k <= desc->grow; ++k) {
+    const gpugo_batch_desc t = tier_desc(*desc, q, p, k);
+    const long long S = t.slots, tq = t.max_queue, tp = t.max_parents;
+    total += S * (long long)(sizeof(SlotInfo) + sizeof(DevState)) + S * tp * (3 * D * 8 + 4) + S * PM * 8 +
+             S * 2 * D * 8 + S * 2 * tq * 16 + (S + 1) * 8 + S * rb;
+  }
   *qcap = q;
   *pcap = p;
   *rec_bytes = rb;
@@ -1727,8 +1856,10 @@ static int batch_enqueue_step(gpugo_batch_t b) {
   HIPCHECK(hipGetLastError());
   hipLaunchKernelGGL(kb_plan, dim3(1), dim3(1024), 0, b->stream, b->ba);
   HIPCHECK(hipGetLastError());
-  void *args[] = {&b->bb};
-  HIPCHECK(hipModuleLaunchKernel(b->bound, (unsigned)b->bound_grid, 1, 1, 256, 1, 1, 0, b->stream, args, nullptr));
+  // the argument array belongs to the batch: a captured launch may keep it
+  // for the life of the graph
+  HIPCHECK(hipModuleLaunchKernel(b->bound, (unsigned)b->bound_grid, 1, 1, 256, 1, 1, 0, b->stream, b->kp_bb,
+                                 nullptr));
   hipLaunchKernelGGL(kb_compact, dim3((unsigned)b->slots * KB_CB), dim3(256), 0, b->stream, b->ba);
   HIPCHECK(hipGetLastError());
   return GPUGO_OK;
@@ -1736,7 +1867,11 @@ static int batch_enqueue_step(gpugo_batch_t b) {
 
 extern "C" void gpugo_batch_destroy(gpugo_batch_t b) {
   if (!b) return;
+  for (auto *t : b->tiers) gpugo_batch_destroy(t);
+  b->tiers.clear();
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->ev_ready) (void)hipEventDestroy(b->ev_ready);
+  if (b->ev_moved) (void)hipEventDestroy(b->ev_moved);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (int k = 0; k < 2; ++k) {
@@ -1749,7 +1884,10 @@ extern "C" void gpugo_batch_destroy(gpugo_batch_t b) {
   delete b;
 }
 
-extern "C" int gpugo_batch_create(gpugo_module_t m, const gpugo_batch_desc *desc, gpugo_batch_t *out) {
+// One tier of a batch: its slots, pools, stream, polling buffers and (with
+// use_graph) the captured launch.  grow_next: a larger tier exists above it.
+static int batch_create_level(gpugo_module_t m, const gpugo_batch_desc *desc, int level, int grow_next,
+                              gpugo_batch_t *out) {
   if (!out) return set_err(GPUGO_E_INVALID, "null pointer");
   *out = nullptr;
   long long qcap, pcap, rb;
@@ -1759,6 +1897,7 @@ extern "C" int gpugo_batch_create(gpugo_module_t m, const gpugo_batch_desc *desc
   auto *b = new gpugo_batch_s;
   b->m = m;
   b->desc = *desc;
+  b->level = level;
   b->dmax = m->dim;
   b->pmax = m->pmax;
   b->slots = desc->slots;
@@ -1793,6 +1932,9 @@ extern "C" int gpugo_batch_create(gpugo_module_t m, const gpugo_batch_desc *desc
 #undef BALLOC
   if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess)
     return fail(set_err(GPUGO_E_HIP, "stream: %s", hipGetErrorString(e)));
+  if ((e = hipEventCreateWithFlags(&b->ev_ready, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&b->ev_moved, hipEventDisableTiming)) != hipSuccess)
+    return fail(set_err(GPUGO_E_HIP, "event: %s", hipGetErrorString(e)));
   for (int k = 0; k < 2; ++k) {
     if ((e = hipHostMalloc((void **)&b->stage[k], (size_t)(rb * S), hipHostMallocDefault)) != hipSuccess ||
         (e = hipHostMalloc((void **)&b->hpoll[k], sizeof(DevState) * S, hipHostMallocDefault)) != hipSuccess)
@@ -1832,7 +1974,7 @@ extern "C" int gpugo_batch_create(gpugo_module_t m, const gpugo_batch_desc *desc
   a.dmax = b->dmax;
   a.pmax = b->pmax;
   a.slots = b->slots;
-  a.pad0 = 0;
+  a.grow = grow_next;
   BatchBoundArgs &g = b->bb;
   memset(&g, 0, sizeof g);
   g.info = a.info;
@@ -1863,6 +2005,7 @@ extern "C" int gpugo_batch_create(gpugo_module_t m, const gpugo_batch_desc *desc
   b->hs.resize(S);
   b->free_list.reserve(S);
   for (long long s = S - 1; s >= 0; --s) b->free_list.push_back((int)s);
+  b->kp_bb[0] = &b->bb;
   if (desc->use_graph) {
     if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
       return fail(set_err(GPUGO_E_HIP, "graph capture: %s", hipGetErrorString(e)));
@@ -1882,6 +2025,42 @@ extern "C" int gpugo_batch_create(gpugo_module_t m, const gpugo_batch_desc *desc
   }
   *out = b;
   return GPUGO_OK;
+}
+
+extern "C" int gpugo_batch_create(gpugo_module_t m, const gpugo_batch_desc *desc, gpugo_batch_t *out) {
+  if (!out || !desc) return set_err(GPUGO_E_INVALID, "null pointer");
+  *out = nullptr;
+  gpugo_batch_t b = nullptr;
+  int rc = batch_create_level(m, desc, 0, desc->grow > 0, &b);
+  if (rc) return rc;
+  b->root = b;
+  const long long q0 = b->qcap, p0 = b->pcap;
+  for (int k = 1; k <= desc->grow; ++k) {
+    const gpugo_batch_desc td = tier_desc(*desc, q0, p0, k);
+    gpugo_batch_t t = nullptr;
+    rc = batch_create_level(m, &td, k, k < desc->grow, &t);
+    if (rc) {
+      gpugo_batch_destroy(b);
+      return rc;
+    }
+    t->root = b;
+    b->tiers.push_back(t);
+  }
+  *out = b;
+  return GPUGO_OK;
+}
+
+// the tier above b (nullptr for the last)
+static gpugo_batch_t next_tier(gpugo_batch_t b) {
+  gpugo_batch_t r = b->root;
+  return b->level < (int)r->tiers.size() ? r->tiers[b->level] : nullptr;
+}
+
+// problems admitted into the batch and not yet reported, over all tiers
+static long long total_running(gpugo_batch_t r) {
+  long long n = r->running;
+  for (auto *t : r->tiers) n += t->running;
+  return n;
 }
 
 extern "C" int gpugo_batch_admit(gpugo_batch_t b, int32_t family, const gpugo_problem_desc *desc, int64_t tag,
@@ -2029,9 +2208,17 @@ static int batch_collect_one(gpugo_batch_t b) {
   for (int s = 0; s < b->slots; ++s) {
     BatchHostSlot &H = b->hs[s];
     if (!H.running || H.admit_poll > seq || ps[s].status == GPUGO_ST_RUNNING) continue;
+    if (ps[s].status == GPUGO_ST_GROW) {
+      // frozen until it moves into the next tier (its later steps are no-ops)
+      if (!H.growing) {
+        H.growing = 1;
+        b->grow_wait.push_back(s);
+      }
+      continue;
+    }
     gpugo_result r;
     batch_result(ps[s], &r, std::chrono::duration<double, std::milli>(now - H.t0).count());
-    b->done.emplace_back(H.tag, r);
+    b->root->done.emplace_back(H.tag, r);
     H.running = 0;
     b->free_list.push_back(s);
     --b->running;
@@ -2041,44 +2228,114 @@ static int batch_collect_one(gpugo_batch_t b) {
   return GPUGO_OK;
 }
 
+// Move the slots of every tier that stopped in GPUGO_ST_GROW into free
+// slots of the tier above.  The copy runs on the destination tier's stream,
+// so no launch of that tier overlaps it (its launches read every slot's
+// state), after the source tier's work so far (the frozen state is final);
+// the source tier waits for the copy before its next work, which may write
+// the freed slot (an admission, or a move into it from the tier below).  A
+// slot that finds no free slot above stays frozen and moves on a later call.
+static int batch_process_moves(gpugo_batch_t r) {
+  std::vector<gpugo_batch_t> lv{r};
+  lv.insert(lv.end(), r->tiers.begin(), r->tiers.end());
+  for (size_t k = 0; k + 1 < lv.size(); ++k) {
+    gpugo_batch_t X = lv[k], Y = lv[k + 1];
+    bool moved = false;
+    while (!X->grow_wait.empty() && !Y->free_list.empty()) {
+      const int s = X->grow_wait.front(), t = Y->free_list.back();
+      X->grow_wait.erase(X->grow_wait.begin());
+      Y->free_list.pop_back();
+      if (!moved) {
+        HIPCHECK(hipEventRecord(X->ev_ready, X->stream));
+        HIPCHECK(hipStreamWaitEvent(Y->stream, X->ev_ready, 0));
+      }
+      hipLaunchKernelGGL(kb_move, dim3(1024), dim3(256), 0, Y->stream, X->ba, Y->ba, s, t);
+      HIPCHECK(hipGetLastError());
+      BatchHostSlot &HX = X->hs[s], &HY = Y->hs[t];
+      HY.running = 1;
+      HY.growing = 0;
+      HY.tag = HX.tag;
+      HY.t0 = HX.t0;
+      HY.admit_poll = Y->polls;  // polls taken before the move show the slot's old state
+      ++Y->running;
+      HX.running = 0;
+      HX.growing = 0;
+      X->free_list.push_back(s);
+      --X->running;
+      moved = true;
+    }
+    if (moved) {
+      HIPCHECK(hipEventRecord(Y->ev_moved, Y->stream));
+      HIPCHECK(hipStreamWaitEvent(X->stream, Y->ev_moved, 0));
+    }
+  }
+  return GPUGO_OK;
+}
+
+// One launch of tier b (its graph, or steps_per_launch steps), then a copy of
+// all slot states to pinned memory; at most two copies are in flight, the
+// older one is collected first.
+static int batch_launch_one(gpugo_batch_t b) {
+  int rc;
+  if (b->gexec) {
+    HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
+  } else {
+    for (int k = 0; k < b->desc.steps_per_launch; ++k) {
+      rc = batch_enqueue_step(b);
+      if (rc) return rc;
+    }
+  }
+  if (b->pend_n == 2) {
+    rc = batch_collect_one(b);
+    if (rc) return rc;
+  }
+  const int k = (b->pend_first + b->pend_n) & 1;
+  HIPCHECK(hipMemcpyAsync(b->hpoll[k], b->st.p, sizeof(DevState) * b->slots, hipMemcpyDeviceToHost, b->stream));
+  HIPCHECK(hipEventRecord(b->pev[k], b->stream));
+  b->poll_seq[k] = b->polls++;
+  ++b->pend_n;
+  if (b->pend_n == 2) {
+    rc = batch_collect_one(b);
+    if (rc) return rc;
+  }
+  return GPUGO_OK;
+}
+
 extern "C" int gpugo_batch_run(gpugo_batch_t b, int64_t launches) {
   if (!b) return set_err(GPUGO_E_INVALID, "null batch");
+  if (b->level != 0) return set_err(GPUGO_E_INVALID, "not a batch handle");
   if (launches < 0) return set_err(GPUGO_E_INVALID, "negative launches");
   int rc = batch_flush(b);
   if (rc) return rc;
-  for (int64_t i = 0; i < launches && b->running > 0; ++i) {
-    if (b->gexec) {
-      HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
-    } else {
-      for (int k = 0; k < b->desc.steps_per_launch; ++k) {
-        rc = batch_enqueue_step(b);
+  for (int64_t i = 0; i < launches && total_running(b) > 0; ++i) {
+    // every tier with problems in it advances by one launch on its own stream
+    if (b->running > 0) {
+      rc = batch_launch_one(b);
+      if (rc) return rc;
+    }
+    for (auto *t : b->tiers) {
+      if (t->running > 0) {
+        rc = batch_launch_one(t);
         if (rc) return rc;
       }
     }
-    if (b->pend_n == 2) {
-      rc = batch_collect_one(b);
-      if (rc) return rc;
-    }
-    const int k = (b->pend_first + b->pend_n) & 1;
-    HIPCHECK(hipMemcpyAsync(b->hpoll[k], b->st.p, sizeof(DevState) * b->slots, hipMemcpyDeviceToHost, b->stream));
-    HIPCHECK(hipEventRecord(b->pev[k], b->stream));
-    b->poll_seq[k] = b->polls++;
-    ++b->pend_n;
-    if (b->pend_n == 2) {
-      rc = batch_collect_one(b);
-      if (rc) return rc;
-    }
+    rc = batch_process_moves(b);
+    if (rc) return rc;
   }
   return GPUGO_OK;
 }
 
 extern "C" int gpugo_batch_sync(gpugo_batch_t b) {
   if (!b) return set_err(GPUGO_E_INVALID, "null batch");
-  while (b->pend_n > 0) {
-    int rc = batch_collect_one(b);
-    if (rc) return rc;
+  std::vector<gpugo_batch_t> lv{b};
+  lv.insert(lv.end(), b->tiers.begin(), b->tiers.end());
+  for (auto *x : lv) {
+    while (x->pend_n > 0) {
+      int rc = batch_collect_one(x);
+      if (rc) return rc;
+    }
+    HIPCHECK(hipStreamSynchronize(x->stream));
   }
-  HIPCHECK(hipStreamSynchronize(b->stream));
   return GPUGO_OK;
 }
 
@@ -2102,7 +2359,7 @@ extern "C" int gpugo_batch_results(gpugo_batch_t b, int64_t max, int64_t *tags, 
 
 extern "C" int gpugo_batch_counts(gpugo_batch_t b, int64_t *running, int64_t *free_slots, int64_t *pending) {
   if (!b) return set_err(GPUGO_E_INVALID, "null batch");
-  if (running) *running = b->running;
+  if (running) *running = total_running(b);
   if (free_slots) *free_slots = (int64_t)b->free_list.size();
   if (pending) *pending = (int64_t)(b->done.size() - b->done_head);
   return GPUGO_OK;

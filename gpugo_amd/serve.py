@@ -2,8 +2,10 @@
 
 Requests (optimisation problems of any family, dimension and tolerance)
 arrive over time.  Each dimension class has a Batch of device-resident
-slots (queues, parent records and incumbents stay in HBM); the slot count
-of each class is set from the device memory it is given.  A request is
+slots (queues, parent records and incumbents stay in HBM) with growth tiers
+(a problem that outgrows its slot moves into a larger one and resumes); the
+slot count of each class is the most whose batch, tiers included, fits the
+device memory it is given.  A request is
 admitted into a free slot of the smallest class that holds its dimension
 as soon as one is free, so its work overlaps on the GPU with whatever else
 is running; every batch advances by one launch per loop turn on its own
@@ -45,7 +47,7 @@ class Server:
 
     def __init__(self, slots: Optional[Dict[int, int]] = None, *, dmax_classes: Sequence[int] = DMAX_CLASSES,
                  mem_fraction: float = 0.5, max_queue: int = 1 << 18, max_parents: int = 8192,
-                 max_iters: int = 2000, steps_per_launch: int = 4, rounding: str = "fast"):
+                 max_iters: int = 2000, steps_per_launch: int = 4, rounding: str = "fast", grow: int = 2):
         _lib.init()
         self.classes = sorted(dmax_classes)
         self.max_queue = max_queue
@@ -57,15 +59,13 @@ class Server:
             # HBM evenly over the classes
             free = ctypes_mem_free()
             share = free * mem_fraction / len(self.classes)
-            slots = {}
-            for dm in self.classes:
-                per = Batch.device_bytes(mods[dm], 1, max_queue=max_queue, max_parents=max_parents)
-                slots[dm] = int(max(1, min(4096, share // per)))
+            slots = {dm: Batch.max_slots(mods[dm], int(share), max_queue=max_queue, max_parents=max_parents,
+                                         grow=grow) for dm in self.classes}
         self.slots = dict(slots)
         for dm in self.classes:
             self.batches[dm] = Batch(slots=self.slots[dm], module=mods[dm], rounding=rounding,
                                      steps_per_launch=steps_per_launch, max_queue=max_queue,
-                                     max_parents=max_parents)
+                                     max_parents=max_parents, grow=grow)
         self.waiting: List[Request] = []
         self.inflight: Dict[int, Request] = {}
 

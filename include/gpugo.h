@@ -291,7 +291,19 @@ typedef struct {
   int64_t max_queue;         /* live boxes per slot (0: 65536)               */
   int64_t max_parents;       /* parent records per slot (0: 16384)           */
   int32_t bound_blocks;      /* persistent split+bound grid (0: 8 per CU)    */
-  int32_t reserved;
+  int32_t grow;              /* growth tiers, 0..3.  0: a slot whose problem
+                                outgrows its queue or parent arena stops with
+                                GPUGO_ST_OVERFLOW.  n > 0: the batch also owns
+                                n larger tiers; tier k has 8^k times the
+                                per-slot max_queue and max_parents and
+                                slots >> (5 k) slots (at least 1).  A problem
+                                whose next step might not fit its slot stops
+                                before that step, moves (queue, parent
+                                records, incumbent, counters) into a free slot
+                                of the next tier and resumes there, with the
+                                same result as in a slot large enough from the
+                                start; only the last tier reports overflow.
+                                gpugo_batch_bytes counts all tiers.          */
 } gpugo_batch_desc;
 
 typedef struct gpugo_batch_s *gpugo_batch_t;

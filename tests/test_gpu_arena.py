@@ -10,7 +10,9 @@ order, so the (lb, parent, pos) order of the queue is unchanged.  Bar:
     size;
   - in FAST rounding a small arena and the default arena give identical
     results and identical live boxes, alone and in a batch;
-  - an arena too small for the records in use is reported as overflow.
+  - an arena too small for the records in use is reported as overflow
+    (batch slots without growth tiers), or the slot moves into a larger
+    tier and its result equals the single solve (with tiers).
 The cases are chosen from the oracle's own counts: the queue never holds
 more boxes than 7/8 of the arena minus one step's records (so no overflow),
 while the records written in total exceed the arena (so collection runs).
@@ -85,6 +87,14 @@ def test_batch_small_arena_equals_single(gpu):
         for key in KEYS:
             assert res[p.tag][key] == ref[key], (p.tag, key)
         assert res[p.tag]["status_name"] == "certified"
-    with Batch(slots=1, module=bmod, rounding="fast", max_parents=8) as b:
+    # without growth tiers the slot reports the overflow; with them (the
+    # default) it moves into a larger tier and certifies as alone
+    with Batch(slots=1, module=bmod, rounding="fast", max_parents=8, grow=0) as b:
         r = b.solve_all(probs[:1])[0]
     assert r["status_name"] == "overflow"
+    with Batch(slots=1, module=bmod, rounding="fast", max_parents=8) as b:
+        r = b.solve_all(probs[:1])[0]
+    s, g, k = cases[0]
+    with gpu_problem(gpu, s, tol=1e-12, grid=g, k_select=k, rounding="fast") as q:
+        ref = q.solve()
+    assert r["status_name"] == "certified" and all(r[key] == ref[key] for key in KEYS)

@@ -143,3 +143,61 @@ def test_mean_value_batch_equals_single(gpu):
             s = q.solve()
         for key in KEYS:
             assert res[p.tag][key] == s[key], (p.tag, key)
+
+
+def test_growth_equals_single(gpu, bmod):
+    """Slots far too small for these problems (a 4096-entry queue and 64
+    parent records) with two growth tiers above them: problems that outgrow
+    a slot move into the next tier and resume, and every result equals the
+    single-problem solve with capacities large enough from the start.  The
+    same batch without tiers reports overflow, so problems did move."""
+    probs = mixed_problems(16, seed=11)
+    kw = dict(module=bmod, rounding="fast", steps_per_launch=4, max_queue=4096, max_parents=64)
+    with Batch(slots=6, grow=2, **kw) as b:
+        res = b.solve_all(probs)
+    with Batch(slots=6, grow=0, **kw) as b:
+        flat = b.solve_all(probs)
+    assert any(r["status_name"] == "overflow" for r in flat.values())
+    for p in probs:
+        s = single(gpu, p)
+        r = res[p.tag]
+        assert r["status_name"] != "overflow", p.tag
+        for key in KEYS:
+            assert r[key] == s[key], (p.tag, p.family, p.d, key, r[key], s[key])
+
+
+def test_c3_workload_equals_single(gpu):
+    """The 4096-problem workload's first 64 problems as configured
+    (problems.batch_problem, seeds [3, k]: the three families, d
+    log-uniform in [2, 32], box [-h, h]^d with h ~ U[2, 32], tol in
+    {1e-4, 1e-6}) with their tuned schedules on the dmax-32 batch module, the
+    bench's per-slot capacities (queue 2^18, 8192 parent records) and two
+    growth tiers: every result equals gpugo_solve with capacities large
+    enough from the start, nothing overflows, and certified enclosures are
+    no wider than tol and contain f* where it is known in closed form."""
+    from gpugo_amd.schedule import schedule_for
+    m = get_batch_module(pb.FAMILIES, 32, ("fast",))
+    probs = []
+    for k in range(64):
+        s = pb.batch_problem(k)
+        grid, ks = schedule_for(s.family, s.d)
+        probs.append(BatchProblem.from_spec(s, grid=grid, k_select=ks, max_iters=2000, tag=k))
+        probs[-1].extra["f_star"] = s.f_star
+    with Batch(slots=24, module=m, rounding="fast", steps_per_launch=8, max_queue=1 << 18, max_parents=8192,
+               grow=2) as b:
+        res = b.solve_all(probs)
+    assert sorted(res) == list(range(64))
+    for p in probs:
+        obj = pb.objective(p.family, p.d)
+        with gpu.Problem(obj, p.params, p.lo, p.hi, tol=p.tol, grid=p.grid, k_select=p.k_select,
+                         rounding="fast", max_iters=p.max_iters, max_queue=1 << 23, max_parents=1 << 18) as q:
+            s = q.solve()
+        r = res[p.tag]
+        assert r["status_name"] != "overflow", p.tag
+        for key in KEYS:
+            assert r[key] == s[key], (p.tag, p.family, p.d, key, r[key], s[key])
+        if r["status_name"] == "certified":
+            assert r["ub"] - r["lb"] <= p.tol
+            fs = p.extra["f_star"]
+            if fs is not None:
+                assert r["lb"] <= fs <= r["ub"], p.tag

@@ -19,8 +19,9 @@ reference evaluates every coordinate for every child
   - two solves are identical (deterministic);
   - with a parent arena small enough that records are collected (their rest
     sums move with them) the solve equals the default arena's;
-  - d = 50 for every windowable reference family, d = 1000 for Levy,
-    Rosenbrock, Fu and Rastrigin and d = 2000, 5000 and 10,000 for Levy (the
+  - d = 50 and d = 1000 for every windowable reference family (Ackley,
+    Belegundu, Breiman, Fu, Levy, Rastrigin, Rosenbrock, Salomon) and
+    d = 2000, 5000 and 10,000 for Levy (the
     reference's levy1000.py ... levy10000.py boxes and initial upper bounds,
     10 diagonal samples, 3^12 children per step, one box per step, a queue of
     up to 2^29 boxes, 2^31 at d = 10,000): certified enclosures of the
@@ -255,22 +256,23 @@ def test_window_lazy_queue_matches_eager(gpu, name, d, S, kw):
 # face of 2^d children and none of them can be ruled out.  The reference's
 # own wid 4 (planes -1, -0.25, 0.5, 1.25, 2) keeps x* inside one child at
 # every level; Breiman runs with 4^9 children per step.
-D50_GRID = {"breiman": (4, 9)}
+REF_GRID = {"breiman": (4, 9)}
 
 
 @pytest.mark.parametrize("name", WIN)
 def test_window_d50_certifies(gpu, name):
     d = 50
-    r, s = ref_solve(gpu, name, d, D50_GRID.get(name, (3, 12)), 1, 10, wmod(gpu, F.objective_loops(name, d)), True,
+    r, s = ref_solve(gpu, name, d, REF_GRID.get(name, (3, 12)), 1, 10, wmod(gpu, F.objective_loops(name, d)), True,
                      max_iters=20000, max_queue=1 << 24)
     assert r["status_name"] == "certified", (name, r)
     assert r["lb"] <= s.f_star <= r["ub"] and r["ub"] - r["lb"] <= 1e-6, (name, r)
 
 
 @pytest.mark.parametrize("name,d", [("levy", 1000), ("rosenbrock", 1000), ("fu", 1000), ("rastrigin", 1000),
+                                    ("ackley", 1000), ("belegundu", 1000), ("breiman", 1000), ("salomon", 1000),
                                     ("levy", 2000), ("levy", 5000), ("levy", 10000)])
 def test_window_large_d_certifies(gpu, name, d):
-    r, s = ref_solve(gpu, name, d, (3, 12), 1, 10, wmod(gpu, F.objective_loops(name, d)), True,
+    r, s = ref_solve(gpu, name, d, REF_GRID.get(name, (3, 12)), 1, 10, wmod(gpu, F.objective_loops(name, d)), True,
                      max_iters=20000, max_queue=1 << (31 if d > 5000 else 29))
     assert r["status_name"] == "certified", (name, r)
     assert r["lb"] <= s.f_star <= r["ub"] and r["ub"] - r["lb"] <= 1e-6, (name, r)
