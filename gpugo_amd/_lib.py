@@ -37,6 +37,7 @@ EXPORTS = [
     "gpugo_problem_timing", "gpugo_batch_bytes", "gpugo_mem_info", "gpugo_batch_create",
     "gpugo_batch_admit", "gpugo_batch_run", "gpugo_batch_sync", "gpugo_batch_results",
     "gpugo_batch_counts", "gpugo_batch_destroy", "gpugo_eval_boxes_f32", "gpugo_window_ranges",
+    "gpugo_solve_static",
 ]
 
 
@@ -133,6 +134,7 @@ def load():
     lib.gpugo_batch_destroy.argtypes = [vp]
     lib.gpugo_batch_destroy.restype = None
     lib.gpugo_window_ranges.argtypes = [ctypes.c_int64, I32, I32]
+    lib.gpugo_solve_static.argtypes = [vp, ctypes.POINTER(ProblemDesc), vp, ctypes.POINTER(Result)]
     _lib = lib
     return lib
 

@@ -22,6 +22,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -65,6 +66,8 @@ extern "C" int gpugo_device_count(void) {
   return n;
 }
 
+static int static_arena(int dev, unsigned char **out);
+
 extern "C" int gpugo_init(int device) {
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
@@ -72,7 +75,8 @@ extern "C" int gpugo_init(int device) {
   if (device < 0 || device >= n) return set_err(GPUGO_E_INVALID, "device %d out of range [0,%d)", device, n);
   HIPCHECK(hipSetDevice(device));
   HIPCHECK(hipFree(nullptr));
-  return GPUGO_OK;
+  unsigned char *arena = nullptr;
+  return static_arena(device, &arena);
 }
 
 // ---------------------------------------------------------------------------
@@ -131,7 +135,8 @@ static int check_intervals(const double *lo, const double *hi, int64_t n, const 
 
 struct DevBuf {
   void *p = nullptr;
-  ~DevBuf() { if (p) (void)hipFree(p); }
+  bool own = true;  // false: points into the static arena (never freed)
+  ~DevBuf() { if (p && own) (void)hipFree(p); }
   hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 8); }
   template <typename T> T *as() const { return (T *)p; }
 };
@@ -1615,6 +1620,24 @@ static int run_steps(gpugo_problem_t p, int64_t iters, gpugo_result *res, bool s
   if (p->hs.status == GPUGO_ST_RUNNING && (p->desc.flags & GPUGO_FLAG_PERSIST) && iters > 0) {
     int rc = persist_steps(p, iters);
     if (rc) return rc;
+  } else if (p->hs.status == GPUGO_ST_RUNNING && !p->hpoll) {
+    // no pinned status word (static-arena solves): batches of steps, then a
+    // copy of the status to host memory and a stream synchronisation
+    const int64_t B = 8;
+    int64_t done = 0;
+    while (done < iters) {
+      const int64_t batch = B < iters - done ? B : iters - done;
+      for (int64_t i = 0; i < batch; ++i) {
+        int rc = enqueue_step(p, nullptr, done + i + 1 < iters);
+        if (rc) return rc;
+      }
+      done += batch;
+      int status = GPUGO_ST_RUNNING;
+      HIPCHECK(hipMemcpyAsync(&status, &p->st.as<DevState>()->status, sizeof status, hipMemcpyDeviceToHost,
+                              p->stream));
+      HIPCHECK(hipStreamSynchronize(p->stream));
+      if (status != GPUGO_ST_RUNNING) break;
+    }
   } else if (p->hs.status == GPUGO_ST_RUNNING) {
     // Steps are enqueued in batches; the compaction of a batch's last step
     // writes the status into a pinned slot (no copy kernel).  The host waits
@@ -1676,6 +1699,93 @@ extern "C" int gpugo_solve(gpugo_problem_t p, gpugo_result *res) {
   long long left = p->desc.max_iters - p->hs.iterations;
   if (left < 0) left = 0;
   return run_steps(p, left, res, true);
+}
+
+// ---------------------------------------------------------------------------
+// static arena: a solve that allocates nothing
+// ---------------------------------------------------------------------------
+// Device memory of gpugo_solve_static: one array per GPU, held from the
+// library's load until the process ends.  Host code never uses its name as
+// an address: the device address comes from hipGetSymbolAddress, once per
+// device (gpugo_init, else the first static solve on that device).
+#define GPUGO_STATIC_BYTES GPUGO_STATIC_ARENA_BYTES
+#define GPUGO_STATIC_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[GPUGO_STATIC_BYTES];
+static void *g_scratch_dev[GPUGO_STATIC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int static_arena(int dev, unsigned char **out) {
+  if (dev < 0 || dev >= GPUGO_STATIC_MAX_DEVICES) return set_err(GPUGO_E_INVALID, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_err(GPUGO_E_HIP, "static arena address: %s", hipGetErrorString(e));
+    g_scratch_dev[dev] = ptr;
+  }
+  *out = (unsigned char *)g_scratch_dev[dev];
+  return GPUGO_OK;
+}
+
+// outward FP32 intervals of the FP64 coefficients (FP32 pruning pass)
+__global__ __launch_bounds__(256) void k_params32(const double *p, float *out, long long n) {
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+    out[2 * i] = to_dn<float>(p[i]);
+    out[2 * i + 1] = to_up<float>(p[i]);
+  }
+}
+
+extern "C" int gpugo_solve_static(gpugo_module_t m, const gpugo_problem_desc *desc, void *stream,
+                                  gpugo_result *res) {
+  if (!desc || !res) return set_err(GPUGO_E_INVALID, "null pointer");
+  int rc = problem_check(m, desc);
+  if (rc) return rc;
+  if (desc->flags & GPUGO_FLAG_PERSIST)
+    return set_err(GPUGO_E_INVALID, "the persistent step loop needs gpugo_problem_create");
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  unsigned char *base = nullptr;
+  rc = static_arena(dev, &base);
+  if (rc) return rc;
+  gpugo_problem_s p;  // its buffers point into the arena; nothing to free
+  p.m = m;
+  p.desc = *desc;
+  const int d = desc->d;
+  // defaults sized for the arena: 2^20 queue entries, 2^15 parent records
+  rc = problem_shape(&p, desc, 1LL << 20, 1LL << 15);
+  if (rc) return rc;
+  size_t off = 0;
+  rc = problem_buffers(&p, [&](DevBuf &buf, size_t bytes, const char *) {
+    buf.p = nullptr;
+    buf.own = false;
+    if (off + bytes <= GPUGO_STATIC_BYTES) buf.p = base + off;
+    off = (off + bytes + 255) & ~(size_t)255;
+    return (int)GPUGO_OK;
+  });
+  if (rc) return rc;
+  if (off > GPUGO_STATIC_BYTES)
+    return set_err(GPUGO_E_CAPACITY,
+                   "the static arena holds %llu bytes; this problem needs %llu (queue %lld entries, %lld parent "
+                   "records): lower max_queue / max_parents",
+                   (unsigned long long)GPUGO_STATIC_BYTES, (unsigned long long)off, p.q_cap, p.p_cap);
+  p.stream = (hipStream_t)stream;
+  const long long np = desc->n_params;
+  HIPCHECK(hipMemcpyAsync(p.d_lo.p, desc->lo, sizeof(double) * d, hipMemcpyHostToDevice, p.stream));
+  HIPCHECK(hipMemcpyAsync(p.d_hi.p, desc->hi, sizeof(double) * d, hipMemcpyHostToDevice, p.stream));
+  if (np > 0) {
+    HIPCHECK(hipMemcpyAsync(p.d_params.p, desc->params, sizeof(double) * np, hipMemcpyHostToDevice, p.stream));
+    hipLaunchKernelGGL(k_params32, dim3(1), dim3(256), 0, p.stream, p.d_params.as<double>(),
+                       (float *)p.d_params32.p, np);
+    HIPCHECK(hipGetLastError());
+  }
+  memset(&p.hs, 0, sizeof p.hs);
+  rc = run_steps(&p, p.desc.max_iters, res, true);
+  // the arena is reused by the next call: nothing may still run on it
+  const hipError_t e = hipStreamSynchronize(p.stream);
+  if (!rc && e != hipSuccess) rc = set_err(GPUGO_E_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
+  p.stream = nullptr;
+  return rc;
 }
 
 extern "C" int gpugo_boxes_copy(gpugo_problem_t p, int64_t max, double *lo, double *hi, double *lb, int64_t *n) {

@@ -261,6 +261,22 @@ int  gpugo_problem_timing(gpugo_problem_t p, int enable);
 int  gpugo_step(gpugo_problem_t p, int64_t iters, gpugo_result *res);
 /* Run to termination or desc.max_iters. */
 int  gpugo_solve(gpugo_problem_t p, gpugo_result *res);
+/* One solve that allocates nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  The problem's
+ * device state (queues, parent arena, counters, incumbent) lives in a static
+ * arena of the library, GPUGO_STATIC_ARENA_BYTES per GPU, held from the
+ * library's load until the process ends; the caller passes the description
+ * (host arrays, read during the call) and the result struct.  Capacities are
+ * desc.max_queue / desc.max_parents (0: 2^20 queue entries, 2^15 parent
+ * records); a problem whose buffers exceed the arena is refused with
+ * GPUGO_E_CAPACITY before any GPU work.  Work runs on `stream` (a
+ * hipStream_t; NULL: the default stream) and the call returns after it has
+ * finished.  Every call on one GPU uses that GPU's one arena: calls that
+ * overlap on a GPU (other host threads) race.  No persistent-loop flag. */
+#define GPUGO_STATIC_ARENA_BYTES (128ull << 20)
+int  gpugo_solve_static(gpugo_module_t m, const gpugo_problem_desc *desc, void *stream,
+                        gpugo_result *res);
 /* Copy live boxes (or, when the x-width rule finished, the output boxes) to
  * the host: lo/hi are max x d row-major, lb has max entries.  *n receives the
  * number of boxes available (which may exceed max). */
