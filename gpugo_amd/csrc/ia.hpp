@@ -34,12 +34,13 @@
 //          decimal scaling itself rounds.  Magnitudes below 1e-292 widen to
 //          0 (lower bound of a positive value) or to -/+1e-292.
 //
-// exp/sin/cos use the device library's round-to-nearest functions and are
-// widened outward by W_TRANS ulps on the bit pattern, following the shape of
-// IA.py:363-570 (nearest value, then integer steps); the bound of the
-// device functions is checked against mpmath in the tests.  sin/cos find
-// interior extrema from rigorous bounds of x/pi, not from IA.py:429-444's
-// nudged floor division.
+// exp/sin/cos endpoints come from this file's own FP64 evaluation (tr64
+// below: Cody-Waite reduction in interval arithmetic, Taylor polynomials
+// with an a-priori error bound added outward), so they are rigorous by
+// construction; FP32 endpoints are the FP64 enclosures rounded outward.
+// IA.py:363-570 instead widens library values by fixed ulp counts.  sin/cos
+// find interior extrema from rigorous bounds of x/pi, not from
+// IA.py:429-444's nudged floor division.
 #pragma once
 
 #ifndef GPUGO_IA_HPP
@@ -63,7 +64,6 @@ template <> struct fp<double> {
   static constexpr double TWO_PI_LO = 0x1.921fb54442d18p+2;
   static constexpr double TRIG_BIG = 0x1p40;
   static constexpr double TRIG_NARROW = 0x1p-40;
-  static constexpr int W_TRANS = 2;
   __device__ static __forceinline__ I bits(double x) { return __double_as_longlong(x); }
   __device__ static __forceinline__ double from(I b) { return __longlong_as_double(b); }
   __device__ static __forceinline__ double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -71,9 +71,6 @@ template <> struct fp<double> {
   __device__ static __forceinline__ double sqrt_(double a) { return __builtin_sqrt(a); }
   __device__ static __forceinline__ double floor_(double a) { return __builtin_floor(a); }
   __device__ static __forceinline__ double ceil_(double a) { return __builtin_ceil(a); }
-  __device__ static __forceinline__ double exp_(double a) { return ::exp(a); }
-  __device__ static __forceinline__ double sin_(double a) { return ::sin(a); }
-  __device__ static __forceinline__ double cos_(double a) { return ::cos(a); }
 };
 
 template <> struct fp<float> {
@@ -88,7 +85,6 @@ template <> struct fp<float> {
   static constexpr float TWO_PI_LO = 0x1.921fb4p+2f;
   static constexpr float TRIG_BIG = 0x1p20f;
   static constexpr float TRIG_NARROW = 0x1p-16f;
-  static constexpr int W_TRANS = 3;
   __device__ static __forceinline__ I bits(float x) { return __float_as_int(x); }
   __device__ static __forceinline__ float from(I b) { return __int_as_float(b); }
   __device__ static __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -96,9 +92,6 @@ template <> struct fp<float> {
   __device__ static __forceinline__ float sqrt_(float a) { return __builtin_sqrtf(a); }
   __device__ static __forceinline__ float floor_(float a) { return __builtin_floorf(a); }
   __device__ static __forceinline__ float ceil_(float a) { return __builtin_ceilf(a); }
-  __device__ static __forceinline__ float exp_(float a) { return ::expf(a); }
-  __device__ static __forceinline__ float sin_(float a) { return ::sinf(a); }
-  __device__ static __forceinline__ float cos_(float a) { return ::cosf(a); }
 };
 
 // exact neighbours on the bit pattern; +inf and NaN stay, -0 treated as +0
@@ -371,9 +364,181 @@ template <> __device__ __forceinline__ float to_up<float>(double x) {
   return ((double)f < x) ? succ<float>(f) : f;
 }
 
+// ---------------------------------------------------------------------------
+// Point enclosures of exp, sin and cos in FP64, rigorous by construction (no
+// property of a device math library is assumed).
+//
+// Argument reduction (Cody & Waite, "Software Manual for the Elementary
+// Functions", 1980): x = k c + r with c = ln 2 (exp) or pi/2 (sin, cos)
+// split as c1 + c2 (+ c3); c1 and c2 carry at most 33 significant bits, so
+// k c1 and k c2 are exact for |k| < 2^20, and the last part is bracketed by
+// two doubles.  r is formed as an interval with the directed operations of
+// the policy D, so it encloses x - k c for any k (outside |k| < 2^20 it is
+// only wider).  For |k| < 2^20 (always for exp) the reduction is instead
+// three (two) fma steps in round-to-nearest whose rounding errors (at most u
+// times each intermediate) and the last part's bracket are bounded and added
+// to the error term; beyond, it is an interval in EXACT directed rounding.
+// k = rint(x / c) keeps |r| <= ln(2)/2 + 2^-40 (exp,
+// |x| < 746) and |r| <= pi/4 + 2^-11 (sin, cos, |x| <= 2^40).
+//
+// Kernels: Taylor polynomials evaluated by Horner's rule (one fma per term)
+// in round-to-nearest at r0 = lo(r).  Their error is bounded a priori and
+// added outward:
+//   truncation (Lagrange remainder): sin to r^21: |r|^23/23! < 2e-25;
+//     cos to r^22: |r|^24/24! < 1e-26; exp to r^13: e^|r| |r|^14/14! < 1e-17;
+//   rounding (Higham, Accuracy and Stability of Numerical Algorithms, 2nd
+//     ed., 2002, section 5.1: Horner's rule on a degree-n polynomial with
+//     one rounding per step returns a value within gamma_n sum_i |a_i| |x|^i
+//     of the exact one, gamma_n = n u / (1 - n u), u = 2^-53), with one more
+//     u for the rounding of each coefficient and of z = r^2 and the final
+//     step: sin: the correction r z Q(z) has |z Q| < 0.12, so the error is
+//     below (0.12 (gamma_9 + 4u) + u) |r| < 3u |r|; cos: |z C| < 0.34, error
+//     below 0.34 (gamma_10 + 4u) + u < 6u; exp: sum |a_i||r|^i < e^0.35 < 1.42,
+//     error below 1.42 (gamma_13 + 2u) < 21u.
+//   The bounds added are EPS_SIN |r|, EPS_COS and EPS_EXP, each at least
+//   three times these, plus the reduction width w (|sin'|, |cos'| <= 1,
+//   (e^r)' < 1.5).
+// ---------------------------------------------------------------------------
+template <int D> struct tr64 {
+  typedef rnd<double, D> RD;
+  static constexpr double PIO2_1 = 0x1.921fb544p+0;
+  static constexpr double PIO2_2 = 0x1.0b4611a6p-34;
+  static constexpr double PIO2_3_LO = 0x1.3198a2e037073p-69;
+  static constexpr double PIO2_3_HI = 0x1.3198a2e037074p-69;
+  static constexpr double TWO_OVER_PI = 0x1.45f306dc9c883p-1;
+  static constexpr double LN2_1 = 0x1.62e42feep-1;
+  static constexpr double LN2_2_LO = 0x1.a39ef35793c76p-33;
+  static constexpr double LN2_2_HI = 0x1.a39ef35793c77p-33;
+  static constexpr double LOG2E = 0x1.71547652b82fep+0;
+  static constexpr double EPS_SIN = 1e-15, EPS_COS = 2e-15, EPS_EXP = 8e-15;
+
+  // [a - k b] for an interval a and k b's enclosure [dn(k bl), up(k bh)]
+  __device__ static __forceinline__ iv<double> sub_k(iv<double> a, double k, double bl, double bh) {
+    const double pl = k >= 0.0 ? RD::mul_dn(k, bl) : RD::mul_dn(k, bh);
+    const double ph = k >= 0.0 ? RD::mul_up(k, bh) : RD::mul_up(k, bl);
+    return mk<double>(RD::sub_dn(a.lo, ph), RD::sub_up(a.hi, pl));
+  }
+  __device__ static __forceinline__ double sin_poly(double r) {
+    const double z = r * r;
+    double q = 1.957294106339126123085e-20;
+    q = __builtin_fma(q, z, -8.220635246624329716956e-18);
+    q = __builtin_fma(q, z, 2.811457254345520763199e-15);
+    q = __builtin_fma(q, z, -7.647163731819816475901e-13);
+    q = __builtin_fma(q, z, 1.605904383682161459939e-10);
+    q = __builtin_fma(q, z, -2.505210838544171877505e-8);
+    q = __builtin_fma(q, z, 2.755731922398589065256e-6);
+    q = __builtin_fma(q, z, -1.984126984126984126984e-4);
+    q = __builtin_fma(q, z, 8.333333333333333333333e-3);
+    q = __builtin_fma(q, z, -1.666666666666666666667e-1);
+    return __builtin_fma(r * z, q, r);
+  }
+  __device__ static __forceinline__ double cos_poly(double r) {
+    const double z = r * r;
+    double c = -8.896791392450573286749e-22;
+    c = __builtin_fma(c, z, 4.110317623312164858478e-19);
+    c = __builtin_fma(c, z, -1.561920696858622646222e-16);
+    c = __builtin_fma(c, z, 4.779477332387385297438e-14);
+    c = __builtin_fma(c, z, -1.147074559772972471385e-11);
+    c = __builtin_fma(c, z, 2.087675698786809897921e-9);
+    c = __builtin_fma(c, z, -2.755731922398589065256e-7);
+    c = __builtin_fma(c, z, 2.480158730158730158730e-5);
+    c = __builtin_fma(c, z, -1.388888888888888888889e-3);
+    c = __builtin_fma(c, z, 4.166666666666666666667e-2);
+    c = __builtin_fma(c, z, -0.5);
+    return __builtin_fma(z, c, 1.0);
+  }
+  __device__ static __forceinline__ double exp_poly(double r) {
+    double p = 1.605904383682161459939e-10;
+    p = __builtin_fma(p, r, 2.087675698786809897921e-9);
+    p = __builtin_fma(p, r, 2.505210838544171877505e-8);
+    p = __builtin_fma(p, r, 2.755731922398589065256e-7);
+    p = __builtin_fma(p, r, 2.755731922398589065256e-6);
+    p = __builtin_fma(p, r, 2.480158730158730158730e-5);
+    p = __builtin_fma(p, r, 1.984126984126984126984e-4);
+    p = __builtin_fma(p, r, 1.388888888888888888889e-3);
+    p = __builtin_fma(p, r, 8.333333333333333333333e-3);
+    p = __builtin_fma(p, r, 4.166666666666666666667e-2);
+    p = __builtin_fma(p, r, 1.666666666666666666667e-1);
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    return __builtin_fma(p, r, 1.0);
+  }
+
+  typedef rnd<double, ROUND_FAST> RF;  // one outward step after a nearest result
+  static constexpr double U = 0x1p-53;
+
+  // sin(x) (shift 0) or cos(x) (shift 1) for |x| <= 2^40
+  __device__ static iv<double> sincos(double x, int shift) {
+    const double k = __builtin_rint(x * TWO_OVER_PI);
+    const double ak = __builtin_fabs(k);
+    double r0, ered;
+    if (ak < 0x1p20) {
+      // three fma steps.  t1 = x - k c1 is exact: k c1 is a double (33-bit
+      // c1, |k| < 2^20) and, for k != 0, k c1 / 2 <= |x| <= 2 k c1 (Sterbenz);
+      // |t2 - (t1 - k c2)| <= u |t2| and |r - (t2 - k c3)| <= u |r| (one
+      // rounding each), and c3, the low end of its bracket, is within 2^-121
+      // of pi/2 - c1 - c2; so |r - (x - k pi/2)| <= u (|t2| + |r|) + |k| 2^-121
+      const double c3 = PIO2_3_LO;
+      const double t1 = __builtin_fma(-k, PIO2_1, x);
+      const double t2 = __builtin_fma(-k, PIO2_2, t1);
+      r0 = __builtin_fma(-k, c3, t2);
+      ered = __builtin_fma(U, __builtin_fabs(t2) + __builtin_fabs(r0), ak * 0x1p-120);
+    } else {
+      // the same reduction as an interval in directed rounding
+      iv<double> r = sub_k(point<double>(x), k, PIO2_1, PIO2_1);
+      r = sub_k(r, k, PIO2_2, PIO2_2);
+      r = sub_k(r, k, PIO2_3_LO, PIO2_3_HI);
+      r0 = r.lo;
+      ered = RD::sub_up(r.hi, r.lo);
+    }
+    const int q = ((int)((long long)k & 3) + shift) & 3;
+    // q = 0: sin r, 1: cos r, 2: -sin r, 3: -cos r; |sin'|, |cos'| <= 1
+    const bool use_sin = (q & 1) == 0;
+    const double v = use_sin ? sin_poly(r0) : cos_poly(r0);
+    const double e = (use_sin ? __builtin_fma(__builtin_fabs(r0), EPS_SIN, 0x1p-1074) : EPS_COS) + ered * (1.0 + 4 * U);
+    double lo = RF::sub_dn(v, e), hi = RF::add_up(v, e);
+    if (q & 2) {
+      const double t = lo;
+      lo = -hi;
+      hi = -t;
+    }
+    return mk<double>(lo, hi);
+  }
+
+  __device__ static iv<double> exp(double x) {
+    if (!(x == x)) return mk<double>(x, x);
+    if (x > 709.8) return mk<double>(fp<double>::MAXV, __builtin_huge_val());  // e^x > MAXV
+    if (x < -745.2) return mk<double>(0.0, 0x1p-1074);                          // 0 < e^x < 2^-1074
+    const double k = __builtin_rint(x * LOG2E);  // |k| <= 1076
+    // two fma steps: |t - (x - k c1)| <= u |t|, |r - (t - k c2)| <= u |r| with
+    // c2 the lower end of its bracket (one ulp, 2^-85, below its upper end)
+    const double t = __builtin_fma(-k, LN2_1, x);
+    const double r = __builtin_fma(-k, LN2_2_LO, t);
+    const double ered = __builtin_fma(U, __builtin_fabs(t) + __builtin_fabs(r), __builtin_fabs(k) * 0x1p-84);
+    const double v = exp_poly(r);
+    // (e^r)' < 1.5 on |r| <= ln(2)/2 + 2^-40
+    const double e = EPS_EXP + 1.5 * ered * (1.0 + 4 * U);
+    double lo = RF::sub_dn(v, e), hi = RF::add_up(v, e);
+    // times 2^k: exact in the normal range; outward by one step otherwise
+    const int ki = (int)k;
+    double l2 = __builtin_ldexp(lo, ki), h2 = __builtin_ldexp(hi, ki);
+    if (ki < -1020) {
+      l2 = __builtin_fmax(pred<double>(l2), 0.0);
+      h2 = succ<double>(h2);
+    }
+    if (__builtin_isinf(l2)) l2 = fp<double>::MAXV;
+    return mk<double>(l2, h2);
+  }
+};
+
 template <typename T, int R> struct IA {
   typedef rnd<T, R> RR;
   typedef iv<T> V;
+  // reduction and error-bound arithmetic of the transcendental enclosures:
+  // EXACT directed operations in every policy (the reduction's products and
+  // differences are exact or nearly so; a FAST step would widen them by an
+  // ulp of x)
+  typedef tr64<ROUND_EXACT> TR;
 
   __device__ static __forceinline__ V add(V a, V b) { return mk<T>(RR::add_dn(a.lo, b.lo), RR::add_up(a.hi, b.hi)); }
   __device__ static __forceinline__ V sub(V a, V b) { return mk<T>(RR::sub_dn(a.lo, b.hi), RR::sub_up(a.hi, b.lo)); }
@@ -476,21 +641,11 @@ template <typename T, int R> struct IA {
     return mk<T>(l, RR::sqrt_up(a.hi));
   }
 
-  __device__ static __forceinline__ T widen_dn(T x) {
-#pragma unroll
-    for (int i = 0; i < fp<T>::W_TRANS; ++i) x = pred<T>(x);
-    return x;
-  }
-  __device__ static __forceinline__ T widen_up(T x) {
-#pragma unroll
-    for (int i = 0; i < fp<T>::W_TRANS; ++i) x = succ<T>(x);
-    return x;
-  }
-
   // out of line: keeps unrolled code lists (and hipRTC compile time) small
   __device__ __attribute__((noinline)) static V exp(V a) {
-    T l = max_<T>(out_dn<RR>(widen_dn(fp<T>::exp_(a.lo))), T(0));
-    return mk<T>(l, out_up<RR>(widen_up(fp<T>::exp_(a.hi))));
+    const iv<double> el = TR::exp((double)a.lo), eh = TR::exp((double)a.hi);
+    T l = max_<T>(out_dn<RR>(to_dn<T>(el.lo)), T(0));
+    return mk<T>(l, out_up<RR>(to_up<T>(eh.hi)));
   }
 
   // shift 0: cos (extrema at m*pi), shift 1: sin (extrema at pi/2 + m*pi);
@@ -503,16 +658,15 @@ template <typename T, int R> struct IA {
     if (w <= fp<T>::TRIG_NARROW) {
       // narrow argument (sample points): |f(x) - f(a)| <= |x - a| <= w since
       // |sin'|, |cos'| <= 1, so one evaluation and a widening by w suffice
-      T v = SHIFT ? fp<T>::sin_(a.lo) : fp<T>::cos_(a.lo);
-      T l = max_<T>(RR::sub_dn(widen_dn(v), w), -one);
-      T h = min_<T>(RR::add_up(widen_up(v), w), one);
+      const iv<double> v = TR::sincos((double)a.lo, 1 - SHIFT);
+      T l = max_<T>(RR::sub_dn(to_dn<T>(v.lo), w), -one);
+      T h = min_<T>(RR::add_up(to_up<T>(v.hi), w), one);
       return mk<T>(l, h);
     }
     if (w >= fp<T>::TWO_PI_LO) return mk<T>(-one, one);
-    T va = SHIFT ? fp<T>::sin_(a.lo) : fp<T>::cos_(a.lo);
-    T vb = SHIFT ? fp<T>::sin_(a.hi) : fp<T>::cos_(a.hi);
-    T l = max_<T>(widen_dn(min_<T>(va, vb)), -one);
-    T h = min_<T>(widen_up(max_<T>(va, vb)), one);
+    const iv<double> va = TR::sincos((double)a.lo, 1 - SHIFT), vb = TR::sincos((double)a.hi, 1 - SHIFT);
+    T l = max_<T>(min_<T>(to_dn<T>(va.lo), to_dn<T>(vb.lo)), -one);
+    T h = min_<T>(max_<T>(to_up<T>(va.hi), to_up<T>(vb.hi)), one);
     T ta = a.lo >= T(0) ? RR::mul_dn(a.lo, fp<T>::INV_PI_LO) : RR::mul_dn(a.lo, fp<T>::INV_PI_HI);
     T tb = a.hi >= T(0) ? RR::mul_up(a.hi, fp<T>::INV_PI_HI) : RR::mul_up(a.hi, fp<T>::INV_PI_LO);
     if (SHIFT) { ta = RR::sub_dn(ta, T(0.5)); tb = RR::sub_up(tb, T(0.5)); }

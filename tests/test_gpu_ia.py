@@ -89,16 +89,81 @@ def test_transcendentals_contain_truth(gpu, oracle, fn, rounding):
         # narrow sin/cos arguments (<= 2^-40) may add their width (Lipschitz path)
         w = hi_in[i] - lo_in[i]
         extra = 2 * w if (fn != "exp" and w <= 2.0 ** -40) else 0.0
-        if tlo not in (0, -1) and got[0][i] > 0 or (tlo < 0 and tlo != -1):
-            assert ulps_between(got[0][i], float(tlo)) <= 4 or float(tlo) - got[0][i] <= extra
-        if thi != 1 and math.isfinite(got[1][i]):
-            assert ulps_between(float(thi), got[1][i]) <= 4 or got[1][i] - float(thi) <= extra
-        # FAST rounds the argument width up, so at the narrow-path threshold the
-        # two sides may take different (both rigorous) paths
-        at_threshold = fn != "exp" and 2.0 ** -42 <= w <= 2.0 ** -38
-        if math.isfinite(orc[1][i]) and math.isfinite(got[1][i]) and orc[0][i] > 1e-300 and not at_threshold:
-            assert abs(ulps_between(orc[0][i], got[0][i])) <= 3
-            assert abs(ulps_between(orc[1][i], got[1][i])) <= 3
+        # width: the a-priori bounds of the FP64 evaluation (sin: 1e-15 |r|,
+        # cos: 2e-15, exp: 8e-15 before the 2^k scaling) plus outward steps
+        slack = (TRANS_ABS[fn] * max(abs(float(tlo)), abs(float(thi)), 1e-300 if fn == "exp" else 1.0)
+                 + extra)
+        if math.isfinite(got[0][i]):
+            assert float(tlo) - got[0][i] <= slack or ulps_between(got[0][i], float(tlo)) <= 4, (i, fn)
+        if math.isfinite(got[1][i]):
+            assert got[1][i] - float(thi) <= slack or ulps_between(float(thi), got[1][i]) <= 4, (i, fn)
+        # the oracle's glibc-based enclosure also contains the range: the two intersect
+        assert max(got[0][i], orc[0][i]) <= min(got[1][i], orc[1][i])
+
+
+# absolute error bound of a transcendental endpoint relative to the result's
+# magnitude (exp: relative), with margin for the final outward steps
+TRANS_ABS = {"exp": 1.2e-14, "sin": 5e-15, "cos": 5e-15}
+
+
+def mp_trig(fn, x):
+    return mp.sin(mp.mpf(x)) if fn == "sin" else mp.cos(mp.mpf(x))
+
+
+@pytest.mark.parametrize("rounding", ["exact", "fast"])
+@pytest.mark.parametrize("fn", ["sin", "cos"])
+def test_trig_adversarial(gpu, fn, rounding):
+    """Arguments next to k pi/2 (where the reduced argument cancels) for k up
+    to 2^39, where k pi/2 is exact-reducible (|k| < 2^20) and beyond; tiny,
+    subnormal and signed-zero arguments.  Point intervals and intervals of
+    width 1e-12 around them: each enclosure contains the 300-bit range."""
+    mp.mp.prec = 300
+    ks = [1, 2, 3, 4, 7, 100, 1023, 1025, 2 ** 19 - 1, 2 ** 19 + 1, 2 ** 20 - 3, 2 ** 20 + 5, 2 ** 25 + 1,
+          2 ** 30 - 7, 2 ** 35 + 3, 2 ** 39 - 1]
+    xs = []
+    for k in ks:
+        c = float(mp.mpf(k) * mp.pi / 2)
+        for s in (1, -1):
+            x = s * c
+            xs += [x, math.nextafter(x, math.inf), math.nextafter(x, -math.inf)]
+    xs += [0.0, -0.0, 5e-324, -5e-324, 1e-310, 1e-300, -1e-200, 1e-8, 0.785398, -0.7854, 2.0 ** -27]
+    xs = np.array(xs)
+    for width in (0.0, 1e-12):
+        lo, hi = xs - width, xs + width
+        got = gpu.ia_eval(fn, lo, hi, rounding=rounding)
+        for i in range(len(xs)):
+            if width == 0.0:
+                t = mp_trig(fn, lo[i])
+                tlo, thi = t, t
+            else:
+                tlo, thi = trig_range(lo[i], hi[i], fn)
+            assert mp.mpf(got[0][i]) <= tlo and thi <= mp.mpf(got[1][i]), (fn, i, lo[i], hi[i])
+            if width == 0.0 and abs(lo[i]) < 2.0 ** 20:
+                # exact reduction: the a-priori bound holds
+                assert got[1][i] - got[0][i] <= 2 * TRANS_ABS[fn] * max(abs(float(t)), 1e-300) + 1e-300 + \
+                    4 * abs(math.ulp(float(t))), (fn, i, lo[i], got[0][i], got[1][i])
+
+
+@pytest.mark.parametrize("rounding", ["exact", "fast"])
+def test_exp_adversarial(gpu, rounding):
+    """exp near overflow (ln MAXV = 709.78...), near and below the subnormal
+    range (-708 ... -745.2), at tiny and zero arguments, and at k ln 2 where
+    the reduced argument cancels: enclosures contain the 300-bit value."""
+    mp.mp.prec = 300
+    xs = [709.78, 709.782712893383, 709.782712893384, 709.79, 709.8, 710.0, 800.0, -700.0, -708.0, -708.4,
+          -708.5, -720.0, -740.0, -744.0, -745.0, -745.13, -745.1332191019411, -745.2, -746.0, -1000.0,
+          0.0, -0.0, 5e-324, -5e-324, 1e-300, -1e-300, 1e-17, -1e-17, 0.3465, -0.3466]
+    for k in (1, 2, 10, 100, 512, 1000, 1023, -1, -10, -1000, -1074):
+        c = float(mp.mpf(k) * mp.log(2))
+        xs += [c, math.nextafter(c, math.inf), math.nextafter(c, -math.inf)]
+    xs = np.array(xs)
+    got = gpu.ia_eval("exp", xs, xs, rounding=rounding)
+    for i, x in enumerate(xs):
+        t = mp.exp(mp.mpf(x))
+        assert got[0][i] >= 0 and mp.mpf(got[0][i]) <= t, (i, x, got[0][i])
+        assert t <= mp.mpf(got[1][i]), (i, x, got[1][i])
+        if -700 < x < 709:
+            assert got[1][i] - got[0][i] <= 2 * TRANS_ABS["exp"] * float(t) + 4 * math.ulp(float(t)), (i, x)
 
 
 @pytest.mark.parametrize("op", ["add", "mul", "div", "sqr", "sqrt", "scale", "exp", "cos"])
