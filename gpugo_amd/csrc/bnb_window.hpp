@@ -44,9 +44,9 @@ __device__ __forceinline__ bool f32_prunes_w(const BoundArgs &a, const ChildBox 
 #endif
 }
 
-// Rest sums of the parent records split in this step: one workgroup per
-// record, threads over the terms, a butterfly in each wavefront and a
-// fixed-order sum over the wavefronts (deterministic).  Per record
+// Rest sums (and products) of the parent records split in this step: one
+// workgroup per record, threads over the terms, a butterfly in each
+// wavefront and a fixed-order fold over the wavefronts (deterministic).  Per record
 // (1 + S) * NREST intervals: [0, NREST) over the box, then NREST at each of
 // its S diagonal sample points (DiagPoint t = 0 .. S-1).
 template <int R>
@@ -70,12 +70,13 @@ __device__ __forceinline__ void prep_impl(const BoundArgs &a) {
   iv<double> v = k == 0 ? obj_rest<R>(s, x, a.params, a.d, win, a.c, tid, (int)blockDim.x)
                         : obj_rest<R>(s, DiagPoint<RowBox>{x, k - 1, S}, a.params, a.d, win, a.c, tid,
                                       (int)blockDim.x);
-  v = group_sum<R, 64>(v);
+  const bool prod = rest_is_prod(s);
+  v = prod ? group_prod<R, 64>(v) : group_sum<R, 64>(v);
   if (lane == 0) part[w] = v;
   __syncthreads();
   if (tid == 0) {
     iv<double> t = part[0];
-    for (int i = 1; i < nw; ++i) t = A::add(t, part[i]);
+    for (int i = 1; i < nw; ++i) t = prod ? A::mul(t, part[i]) : A::add(t, part[i]);
     out[k * GPUGO_NREST + s] = t;
   }
 }
@@ -295,7 +296,7 @@ __device__ __forceinline__ void eval32_w_impl(const EvalArgs &a) {
   const RowBox x{a.lo + i * GPUGO_D, a.hi + i * GPUGO_D};
   const F32Box<RowBox> xf{x};
   iv<double> rest[GPUGO_NREST];
-  for (int s = 0; s < GPUGO_NREST; ++s) rest[s] = mk<double>(0.0, 0.0);
+  for (int s = 0; s < GPUGO_NREST; ++s) rest[s] = rest_is_prod(s) ? mk<double>(1.0, 1.0) : mk<double>(0.0, 0.0);
   const iv<float> f = obj_value_w32<R>(xf, (const iv<float> *)a.ub, a.d, rest, 0, a.d);
   a.flo[i] = (double)f.lo;
   a.fhi[i] = (double)f.hi;

@@ -22,7 +22,7 @@ import os
 import numpy as np
 from typing import Dict, List, Optional, Tuple
 
-from .expr import Expr, Objective, wrap, Const, has_loop_index
+from .expr import Expr, Objective, wrap, Const, has_loop_index, TEMPLATE_REDUCTIONS
 
 _CSRC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc")
 
@@ -216,22 +216,41 @@ class _FuncG(_Func):
     def __init__(self, ty: str = "double"):
         super().__init__(ty)
         self.in_loop = False
+        self.jvar = "j"          # C name of the innermost template index
+        self.loop_keys = set()   # memo keys of values that depend on it
+
+    def lower(self, e: Expr) -> Tuple[str, str]:
+        r = super().lower(e)
+        if self.in_loop and has_loop_index(e):
+            self.loop_keys.add(e.key())
+        return r
 
     def _lower(self, e: Expr) -> Tuple[str, str]:
         op = e.op
+        jv = self.jvar
         if op in ("var", "param") and isinstance(e.val, tuple):
             if not self.in_loop:
                 raise ValueError("loop index used outside a template loop")
             _, a, b = e.val
-            idx = f"({a} * j + {b})" if b >= 0 else f"({a} * j - {-b})"
+            idx = f"({a} * {jv} + {b})" if b >= 0 else f"({a} * {jv} - {-b})"
             if op == "var":
                 return ("i", f"x[{idx}]")
             # FP32 code lists take the coefficients as outward FP32 intervals
             return ("i" if self.ty == "float" else "s", f"P[{idx}]")
-        if op == "tsum":
+        if op == "jnum":
+            if not self.in_loop:
+                raise ValueError("loop index used outside a template loop")
+            b = e.val
+            return ("s", f"(({self.ty})({jv} + {b}))" if b >= 0 else f"(({self.ty})({jv} - {-b}))")
+        if op in TEMPLATE_REDUCTIONS:
             return self.loop_sum(e)
+        if op == "tprodx":
+            if not self.in_loop or self.jvar != "j":
+                raise ValueError("TProdEx belongs in a gradient template (one level deep)")
+            return self.loop_prod_except(e)
         if op == "twhen":
-            cond = {"j+1<d": "j + 1 < d", "j>0": "j > 0", "j==0": "j == 0", "j+1==d": "j + 1 == d"}[e.val]
+            cond = {"j+1<d": f"{jv} + 1 < d", "j>0": f"{jv} > 0", "j==0": f"{jv} == 0",
+                    "j+1==d": f"{jv} + 1 == d"}[e.val]
             name = f"t{self.n}"
             self.n += 1
             self.lines.append(f"  iv<{self.ty}> {name} = mk<{self.ty}>(0, 0);")
@@ -272,12 +291,57 @@ class _FuncG(_Func):
     def loop_sum(self, e: Expr) -> Tuple[str, str]:
         acc = f"t{self.n}"
         self.n += 1
-        off = e.val
-        lim = "d" if off == 0 else (f"d + {off}" if off > 0 else f"d - {-off}")
-        self.lines.append(f"  iv<{self.ty}> {acc} = mk<{self.ty}>(0, 0);")
-        self.loop(e.args[0], lim, lambda code: [f"    {acc} = A::add({acc}, {code});"])
-        self.lines.append(f"  {acc} = group_sum<R, G>({acc});")
+        lim = _limit(e.val)
+        prod = e.op == "tprod"
+        self.lines.append(f"  iv<{self.ty}> {acc} = mk<{self.ty}>({int(prod)}, {int(prod)});")
+        fn = "mul" if prod else "add"
+        self.loop(e.args[0], lim, lambda code: [f"    {acc} = A::{fn}({acc}, {code});"])
+        self.lines.append(f"  {acc} = group_{'prod' if prod else 'sum'}<R, G>({acc});")
         return ("i", acc)
+
+    def loop_prod_except(self, e: Expr) -> Tuple[str, str]:
+        """Product over i != j of the body at J = i: a sequential loop inside
+        the component loop (O(d) per component; the windowed lists use the
+        parent's rest product instead)."""
+        acc = f"t{self.n}"
+        self.n += 1
+        self.lines.append(f"  iv<{self.ty}> {acc} = mk<{self.ty}>(1, 1);")
+        saved = dict(self.memo)
+        self.memo = {k: v for k, v in saved.items() if k not in self.loop_keys}
+        self.jvar = "i"
+        self.lines.append(f"  for (int i = 0; i < {_limit(e.val)}; ++i) {{")
+        self.lines.append("    if (i == j) continue;")
+        code = self.as_iv(self.lower(e.args[0]))
+        self.lines.append(f"    {acc} = A::mul({acc}, {code});")
+        self.lines.append("  }")
+        self.jvar = "j"
+        self.memo = saved
+        return ("i", acc)
+
+
+def _limit(off: int) -> str:
+    """C expression of a template's trip count d + off."""
+    return "d" if off == 0 else (f"d + {off}" if off > 0 else f"d - {-off}")
+
+
+# Product over a group of G lanes (template products), the butterfly of
+# group_sum (bnb_box.hpp) with A::mul; emitted by the code lists that need it.
+_GROUP_PROD = """#ifndef GPUGO_GROUP_PROD
+#define GPUGO_GROUP_PROD 1
+namespace gpugo {
+template <int R, int G>
+__device__ __forceinline__ iv<double> group_prod(iv<double> v) {
+  typedef IA<double, R> A;
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) {
+    const double lo = __shfl_xor(v.lo, o, 64), hi = __shfl_xor(v.hi, o, 64);
+    v = A::mul(v, mk<double>(lo, hi));
+  }
+  return v;
+}
+}  // namespace gpugo
+#endif
+"""
 
 
 def split_lists(obj: Objective) -> str:
@@ -286,7 +350,8 @@ def split_lists(obj: Objective) -> str:
     t = obj.templates
     if not t or "value" not in t or "grad" not in t:
         raise ValueError(f"objective {obj.name} has no loop templates")
-    out = ["namespace gpugo {"]
+    has_prod = any(e.op == "tprod" for e in _tsums(t["value"], t["grad"]))
+    out = ([_GROUP_PROD] if has_prod else []) + ["namespace gpugo {"]
     f = _FuncG()
     v = f.lower(wrap(t["value"]))
     out.append("template <int R, int G, class XS>\n__device__ __forceinline__ iv<double> obj_value_g("
@@ -337,16 +402,42 @@ def _walk(e: Expr, fn, seen=None):
         _walk(a, fn, seen)
 
 
+def _reduction_of(e: Expr) -> Expr:
+    """The template reduction a node stands for: itself for a sum or a
+    product, the full product for a product-except-j."""
+    return Expr("tprod", e.args, e.val) if e.op == "tprodx" else e
+
+
 def _tsums(*exprs) -> List[Expr]:
-    """Distinct template sums of the expressions, in order of appearance."""
+    """Distinct template reductions (sums and products; a product-except-j
+    counts as its full product) of the expressions, in order of appearance."""
     out, keys = [], set()
 
     def fn(e):
-        if e.op == "tsum" and e.key() not in keys:
-            keys.add(e.key())
-            out.append(e)
+        if e.op in TEMPLATE_REDUCTIONS or e.op == "tprodx":
+            r = _reduction_of(e)
+            if r.key() not in keys:
+                keys.add(r.key())
+                out.append(r)
     for e in exprs:
         _walk(wrap(e), fn)
+    return out
+
+
+def _global_reductions(e: Expr) -> List[Expr]:
+    """Template sums and products that are not under a product-except-j
+    (a gradient that uses one needs every component re-tested)."""
+    out = []
+
+    def walk(x):
+        if x.op == "tprodx":
+            return
+        if x.op in TEMPLATE_REDUCTIONS:
+            out.append(x)
+            return
+        for a in x.args:
+            walk(a)
+    walk(wrap(e))
     return out
 
 
@@ -356,22 +447,32 @@ def _x_offsets(body: Expr, what: str) -> Tuple[int, int]:
     offs = []
 
     def fn(e):
+        if e.op == "tprodx":
+            return
         if e.op == "var":
             if not isinstance(e.val, tuple):
                 raise ValueError(f"{what}: fixed coordinate X[{e.val}] inside a loop body is not windowable")
             if e.val[1] != 1:
                 raise ValueError(f"{what}: coordinate index {e.val[1]}*j{e.val[2]:+d} is not windowable")
             offs.append(e.val[2])
-    _walk(body, fn)
+        for a in e.args:
+            fn(a)
+    fn(wrap(body))
     return (min(offs), max(offs)) if offs else (0, 0)
+
+
+def _has_op(e: Expr, op: str) -> bool:
+    found = []
+    _walk(wrap(e), lambda x: found.append(1) if x.op == op else None)
+    return bool(found)
 
 
 def _check_local_body(body: Expr, what: str):
     """A loop body may not use coordinate-dependent values computed outside
     its own term (they would differ between a parent and its children)."""
     def fn(e):
-        if e.op == "tsum":
-            raise ValueError(f"{what}: nested template sums are not windowable")
+        if e.op in TEMPLATE_REDUCTIONS or e.op == "tprodx":
+            raise ValueError(f"{what}: nested template reductions are not windowable")
         if not has_loop_index(e) and e.depends_on_x():
             raise ValueError(f"{what}: loop-invariant coordinate term {e!r} is not windowable")
     _walk(body, fn)
@@ -397,9 +498,24 @@ class _FuncW(_FuncG):
         else:
             self.lines.append(f"  iv<double> {acc} = rest[{s}];")
             term = f"wterm_{s}"
+        fn = "mul" if e.op == "tprod" else "add"
         self.lines.append(f"  {{ int rlo[2], rhi[2]; const int nr = wranges_{s}(win, c, d, rlo, rhi);")
         self.lines.append("    for (int r = 0; r < nr; ++r)")
-        self.lines.append(f"      for (int j = rlo[r]; j < rhi[r]; ++j) {acc} = A::add({acc}, {term}<R>(x, P, d, j)); }}")
+        self.lines.append(f"      for (int j = rlo[r]; j < rhi[r]; ++j) {acc} = A::{fn}({acc}, {term}<R>(x, P, d, j)); }}")
+        return ("i", acc)
+
+    def loop_prod_except(self, e: Expr) -> Tuple[str, str]:
+        """Inside the component loop j (j in the window): the parent's rest
+        product (which has no window term) times the window's terms i != j
+        over the child."""
+        s = self.slots[_reduction_of(e).key()]
+        acc = f"t{self.n}"
+        self.n += 1
+        self.lines.append(f"  iv<double> {acc} = rest[{s}];")
+        self.lines.append(f"  {{ int rlo[2], rhi[2]; const int nr = wranges_{s}(win, c, d, rlo, rhi);")
+        self.lines.append("    for (int r = 0; r < nr; ++r)")
+        self.lines.append("      for (int i = rlo[r]; i < rhi[r]; ++i)")
+        self.lines.append(f"        if (i != j) {acc} = A::mul({acc}, wterm_{s}<R>(x, P, d, i)); }}")
         return ("i", acc)
 
 
@@ -460,9 +576,13 @@ def window_lists(obj: Objective) -> str:
     slots = {e.key(): i for i, e in enumerate(sums)}
     S = len(sums)
     signs = param_signs(obj)
-    out = [f"#define GPUGO_D {obj.d}", f"#define GPUGO_NPARAMS {obj.n_params}",
+    out = [_GROUP_PROD, f"#define GPUGO_D {obj.d}", f"#define GPUGO_NPARAMS {obj.n_params}",
            "#define GPUGO_PSIGN " + (", ".join(str(v) for v in signs) if signs else "0"),
            f"#define GPUGO_NREST {max(S, 1)}", "namespace gpugo {"]
+    prod_mask = sum(1 << s for s, e in enumerate(sums) if e.op == "tprod")
+    out.append("// slot s of the rest reductions is a product (identity 1, combined\n"
+               "// with A::mul) when bit s is set, else a sum (identity 0, A::add)")
+    out.append(f"__device__ __forceinline__ bool rest_is_prod(int s) {{ return ({prod_mask}u >> s) & 1u; }}")
     for s, e in enumerate(sums):
         body = e.args[0]
         _check_local_body(body, f"template sum {s}")
@@ -476,7 +596,7 @@ def window_lists(obj: Objective) -> str:
         out += f.lines
         out.append(f"  return {code};")
         out.append("}")
-        nt = "d" if e.val == 0 else (f"d + {e.val}" if e.val > 0 else f"d - {-e.val}")
+        nt = _limit(e.val)
         out.append(f"__device__ __forceinline__ int wranges_{s}(int win, int c, int d, int *lo, int *hi) {{\n"
                    f"  return win_ranges(win, c, d, {klo}, {khi}, {nt}, lo, hi);\n}}")
     # rest sums of a parent record: slot s over an accessor (the box, or
@@ -488,10 +608,13 @@ def window_lists(obj: Objective) -> str:
     out.append("  int rlo[2], rhi[2];")
     out.append("  switch (s) {")
     for s_, e in enumerate(sums):
-        nt = "d" if e.val == 0 else (f"d + {e.val}" if e.val > 0 else f"d - {-e.val}")
+        nt = _limit(e.val)
+        prod = e.op == "tprod"
         out.append(f"  case {s_}: {{ const int nr = wranges_{s_}(win, c, d, rlo, rhi);")
+        if prod:
+            out.append("    acc = mk<double>(1.0, 1.0);")
         out.append(f"    for (int j = t; j < {nt}; j += nt)")
-        out.append(f"      if (!in_ranges(j, nr, rlo, rhi)) acc = A::add(acc, wterm_{s_}<R>(x, P, d, j));")
+        out.append(f"      if (!in_ranges(j, nr, rlo, rhi)) acc = A::{'mul' if prod else 'add'}(acc, wterm_{s_}<R>(x, P, d, j));")
         out.append("    break; }")
     out.append("  default: break;")
     out.append("  }")
@@ -509,7 +632,15 @@ def window_lists(obj: Objective) -> str:
     out.append("}")
     # monotonicity test
     f = _FuncW(slots)
-    global_grad = bool(_tsums(grad))
+    global_grad = bool(_global_reductions(grad))
+    if _has_op(grad, "tprodx"):
+        # the rest product leaves out exactly the window's terms, so the
+        # components tested must be the window's coordinates
+        if global_grad or _x_offsets(grad, "gradient template") != (0, 0):
+            raise ValueError("a product-except-j gradient is windowable only with X(J) alone outside it")
+        for e in _tsums(grad):
+            if e.op == "tprod" and _x_offsets(e.args[0], "template product") != (0, 0):
+                raise ValueError("a windowed template product may read X(J) only")
     f.hoist(grad)
     if global_grad:
         f.lines.append("  for (int j = 0; j < d; ++j) {")
@@ -799,12 +930,12 @@ def _count(exprs, tsum_leaf: bool = False) -> int:
         if k in seen:
             return
         seen.add(k)
-        if tsum_leaf and e.op == "tsum":
+        if tsum_leaf and (e.op in TEMPLATE_REDUCTIONS or e.op == "tprodx"):
             return
         for a in e.args:
             walk(a)
         op = e.op
-        if op in ("var", "const", "param", "iconst", "guard", "twhen", "dim") or e.is_scalar():
+        if op in ("var", "const", "param", "iconst", "guard", "twhen", "dim", "jnum") or e.is_scalar():
             return
         if op == "gsum":
             total += _FLOP["add"] * (len(e.args) - 1)
@@ -852,15 +983,19 @@ def window_flop_counts(obj: Objective, c: int) -> dict:
         total = 0
         for ts in _tsums(e):
             klo, khi = _x_offsets(ts.args[0], "template sum")
-            total += min(c + khi - klo, obj.d + ts.val) * (_count([ts.args[0]]) + _FLOP["add"])
+            fold = _FLOP["mul" if ts.op == "tprod" else "add"]
+            total += min(c + khi - klo, obj.d + ts.val) * (_count([ts.args[0]]) + fold)
         return total
     v = _count([value], tsum_leaf=True) + sums_cost(value)
-    if _tsums(grad):
+    if _global_reductions(grad):
         ncomp, gsum = obj.d, sums_cost(grad)
     else:
         gklo, gkhi = _x_offsets(grad, "gradient template")
         ncomp, gsum = min(c + gkhi - gklo, obj.d), 0
-    g = gsum + ncomp * _count([grad], tsum_leaf=True)
+    # a product-except-j: the window's other terms, c - 1 products each
+    px = sum(min(c, obj.d) * (_count([ts.args[0]]) + _FLOP["mul"])
+             for ts in _tsums(grad) if _has_op(grad, "tprodx") and ts.op == "tprod")
+    g = gsum + ncomp * (_count([grad], tsum_leaf=True) + px)
     return {"value": v, "grad": g, "sample": v}
 
 

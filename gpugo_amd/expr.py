@@ -81,7 +81,7 @@ class Expr:
     # exact scalars -------------------------------------------------------
     def is_scalar(self) -> bool:
         """True if the value is a single double known exactly (no rounding)."""
-        if self.op in ("const", "param", "dim"):
+        if self.op in ("const", "param", "dim", "jnum"):
             return True
         if self.op == "neg":
             return self.args[0].is_scalar()
@@ -102,6 +102,8 @@ class Expr:
             return 1 if self.val > 0 else (-1 if self.val < 0 else 0)
         if self.op == "dim":
             return 1
+        if self.op == "jnum":
+            return 1 if self.val > 0 else 0
         if self.op == "param":
             return 0 if self.val2 is None else self.val2
         if self.op == "neg":
@@ -174,6 +176,28 @@ def TSum(body, off: int = 0) -> Expr:
     return Expr("tsum", (wrap(body),), int(off))
 
 
+def TProd(body, off: int = 0) -> Expr:
+    """Template product: product over j in [0, d + off) of `body` with
+    J = j (left fold in increasing j); the product form of TSum (Griewank,
+    Styblinski-Tang, Zabinsky)."""
+    return Expr("tprod", (wrap(body),), int(off))
+
+
+def TProdEx(body, off: int = 0) -> Expr:
+    """Inside a gradient template: the product over i in [0, d + off),
+    i != J, of `body` with J = i (the factor a product's j-th partial
+    derivative leaves).  `body` binds its own J; the excluded index is the
+    enclosing template's J."""
+    return Expr("tprodx", (wrap(body),), int(off))
+
+
+def JNum(b: int = 0) -> Expr:
+    """Inside a template: the loop index j + b as an exact scalar."""
+    return Expr("jnum", (), int(b))
+
+
+TEMPLATE_REDUCTIONS = ("tsum", "tprod")
+
 TWHEN_CONDS = ("j+1<d", "j>0", "j==0", "j+1==d")
 
 
@@ -190,7 +214,9 @@ def has_loop_index(e: Expr) -> bool:
     (a TSum binds J inside itself)."""
     if e.op in ("var", "param") and isinstance(e.val, tuple):
         return True
-    if e.op == "tsum":
+    if e.op in ("jnum", "tprodx"):
+        return True
+    if e.op in TEMPLATE_REDUCTIONS:
         return False
     return any(has_loop_index(a) for a in e.args)
 

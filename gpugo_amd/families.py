@@ -19,7 +19,7 @@ from typing import Callable, Dict, List
 
 import numpy as np
 
-from .expr import E, PI, Const, J, Objective, TSum, TWhen, X, cos, exp, fsum, sin, sqr, sqrt
+from .expr import E, PI, Const, J, JNum, Objective, TProd, TProdEx, TSum, TWhen, X, cos, exp, fsum, sin, sqr, sqrt
 from .problems import ProblemSpec
 
 
@@ -177,8 +177,31 @@ def _t_rosenbrock(d):
             + TWhen("j>0", (X(J) - sqr(X(J - 1))) * 200.0)}
 
 
+# Product families: the product is a template product; its j-th partial
+# derivative keeps the product of the other factors (TProdEx), which the
+# windowed lists form from the parent's rest product and the window's other
+# terms.
+def _t_griewank(d):
+    rj = sqrt(JNum(1))          # sqrt(j + 1): coordinates numbered from 1
+    return {"value": (TSum(sqr(X(J))) / 4000.0 + 1.0) - TProd(cos(X(J) / rj)),
+            "grad": X(J) / 2000.0 + (sin(X(J) / rj) / rj) * TProdEx(cos(X(J) / rj))}
+
+
+def _t_styblinski(d):
+    return {"value": TSum(sqr(X(J))) / (2.0 * d) - TProd(cos(X(J))) * (4.0 * d),
+            "grad": X(J) / float(d) + (sin(X(J)) * TProdEx(cos(X(J)))) * (4.0 * d)}
+
+
+def _t_zabinsky(d):
+    z = PI / 6.0
+    u, v = X(J) - z, (X(J) - z) * 5.0
+    return {"value": -(TProd(sin(u)) * 2.5 + TProd(sin(v))),
+            "grad": -((cos(u) * TProdEx(sin(u))) * 2.5 + (cos(v) * 5.0) * TProdEx(sin(v)))}
+
+
 TEMPLATES = {"ackley": _t_ackley, "belegundu": _t_belegundu, "breiman": _t_breiman, "fu": _t_fu,
-             "levy": _t_levy, "rastrigin": _t_rastrigin, "salomon": _t_salomon, "rosenbrock": _t_rosenbrock}
+             "levy": _t_levy, "rastrigin": _t_rastrigin, "salomon": _t_salomon, "rosenbrock": _t_rosenbrock,
+             "griewank": _t_griewank, "styblinski": _t_styblinski, "zabinsky": _t_zabinsky}
 
 
 @dataclass
@@ -231,7 +254,7 @@ def objective_loops(name: str, d: int) -> Objective:
     lists; for the windowed large-d modules, whose size does not grow with d).
     The unrolled value is not built: its `value` is the template."""
     if name not in TEMPLATES:
-        raise ValueError(f"{name} has no loop templates (product families are not windowable)")
+        raise ValueError(f"{name} has no loop templates")
     tpl = TEMPLATES[name](d)
     return Objective(d, 0, tpl["value"], None, name=f"ref_{name}_loops_d{d}", templates=tpl)
 

@@ -50,7 +50,7 @@ def test_split_certifies(gpu, oracle, spec, grid, k, mv, rounding):
 
 
 def test_split_needs_templates(gpu):
-    from gpugo_amd import families
-    obj = families.objective("griewank", 2)
+    from gpugo_amd.expr import Objective, X, sqr
+    obj = Objective(2, 0, sqr(X(0)) + sqr(X(1)), name="no_templates")
     with pytest.raises(ValueError):
         gpu.get_module(obj, roundings=("fast",), split=4)

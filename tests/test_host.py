@@ -44,6 +44,17 @@ def ir_eval(e, x, P, memo=None, j=None):
         r = mp.mpf(0)
         for jj in range(len(x) + e.val):
             r += ir_eval(e.args[0], x, P, {}, jj)
+    elif op == "tprod":
+        r = mp.mpf(1)
+        for jj in range(len(x) + e.val):
+            r *= ir_eval(e.args[0], x, P, {}, jj)
+    elif op == "tprodx":
+        r = mp.mpf(1)
+        for jj in range(len(x) + e.val):
+            if jj != j:
+                r *= ir_eval(e.args[0], x, P, {}, jj)
+    elif op == "jnum":
+        r = mp.mpf(j + e.val)
     elif op == "twhen":
         ok = {"j+1<d": j + 1 < len(x), "j>0": j > 0, "j==0": j == 0, "j+1==d": j + 1 == len(x)}[e.val]
         r = ir_eval(e.args[0], x, P, {}, j) if ok else mp.mpf(0)
@@ -240,8 +251,22 @@ def test_window_module_emits():
     assert "win_ranges(win, c, d, -1, 1, d, glo, ghi)" in src  # local gradient: window components
     src = emit.window_module_source(F.objective_loops("ackley", 1000), ("fast",))
     assert "#define GPUGO_NREST 2" in src and "for (int j = 0; j < d; ++j) {" in src
+    # product families: template products (rest product per parent, window
+    # terms times it), the product-except-j gradient over the window only
+    src = emit.window_module_source(F.objective_loops("griewank", 1000), ("fast",))
+    assert "#define GPUGO_NREST 2" in src and "rest_is_prod(int s) { return (2u >> s) & 1u; }" in src
+    assert "A::mul(t" in src and "if (i != j)" in src
+    kkt = src[src.index("bool obj_kkt_w("):]
+    kkt = kkt[:kkt.index("\n}\n")]
+    assert "win_ranges(win, c, d, 0, 0, d, glo, ghi)" in kkt and "for (int j = 0; j < d; ++j) {" not in kkt
+    src = emit.window_module_source(F.objective_loops("zabinsky", 100), ("fast",))
+    assert "rest_is_prod(int s) { return (3u >> s) & 1u; }" in src
+    # a product-except-j next to a global gradient term is refused
+    from gpugo_amd.expr import J, Objective, TProd, TProdEx, TSum, X, cos, sqr
+    bad = Objective(8, 0, TProd(cos(X(J))), None, name="bad",
+                    templates={"value": TProd(cos(X(J))), "grad": TSum(sqr(X(J))) * TProdEx(cos(X(J)))})
     with pytest.raises(ValueError):
-        F.objective_loops("griewank", 10)
+        emit.window_module_source(bad, ("fast",))
     # the 15-significant-digit IA: its kernels are instantiated on request only
     src = emit.window_module_source(F.objective_loops("levy", 50), ("digits15",))
     assert "#define GPUGO_WITH_DIGITS15 1" in src and "#define GPUGO_WITH_FAST 0" in src
