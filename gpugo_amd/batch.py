@@ -106,12 +106,18 @@ class Batch:
     """Device-resident slots with continuous admission."""
 
     def __init__(self, slots: int = 4096, *, families: Sequence[str] = pb.FAMILIES, dmax: int = 32,
-                 rounding: str = "fast", steps_per_launch: int = 4, use_graph: bool = True,
-                 max_queue: int = 0, max_parents: int = 0, bound_blocks: int = 0, grow: int = 2,
+                 rounding: str = "fast", steps_per_launch: Optional[int] = None, use_graph: bool = True,
+                 max_queue: int = 0, max_parents: int = 0, bound_blocks: Optional[int] = None, grow: int = 2,
                  module: Optional[BatchModule] = None):
         """grow: larger tiers (0..3, each 8x the per-slot capacity of the one
         below) into which a problem that outgrows its slot moves, state and
-        all, and resumes; 0 reports such a problem as overflow."""
+        all, and resumes; 0 reports such a problem as overflow.
+        steps_per_launch / bound_blocks: None takes the tuned launch shape
+        of the schedule table (gpugo_amd.schedule.launch_for)."""
+        from .schedule import launch_for
+        bb0, spl0 = launch_for()
+        steps_per_launch = spl0 if steps_per_launch is None else steps_per_launch
+        bound_blocks = bb0 if bound_blocks is None else bound_blocks
         self.module = module or get_batch_module(families, dmax, (rounding,))
         self.dmax = self.module.dmax
         self.desc = _lib.BatchDesc(slots=int(slots), steps_per_launch=int(steps_per_launch),
@@ -125,7 +131,7 @@ class Batch:
 
     @staticmethod
     def device_bytes(module: BatchModule, slots: int, **kw) -> int:
-        desc = _lib.BatchDesc(slots=int(slots), steps_per_launch=int(kw.get("steps_per_launch", 4)),
+        desc = _lib.BatchDesc(slots=int(slots), steps_per_launch=int(kw.get("steps_per_launch") or 4),
                               rounding=_lib.ROUND[kw.get("rounding", "fast")], use_graph=1,
                               max_queue=int(kw.get("max_queue", 0)), max_parents=int(kw.get("max_parents", 0)),
                               bound_blocks=0, grow=int(kw.get("grow", 2)))

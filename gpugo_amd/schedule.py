@@ -13,13 +13,19 @@ least wall time, wins.
 
 Odd wid keeps a box centred on the centre of the initial box at every
 level (with wid = 2 the centre lies on a split plane of 2^d boxes).
+
+Two launch knobs are tuned too: for batches, the split+bound grid (bound
+workgroups) and the steps captured per graph launch, on the whole sample
+mix at once (`tune_launch`, criterion certified problems per second); for
+single solves, the lanes that share one child (`tune_lanes`: 1 = one lane
+per child, G > 1 = the term-parallel kernels with G lanes per child).
 """
 from __future__ import annotations
 
 import json
 import os
 import time
-from typing import Dict, Iterable, List, Sequence, Tuple
+from typing import Dict, Iterable, List, Optional, Sequence, Tuple
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE_PATH = os.path.join(_HERE, "schedules.json")
@@ -28,6 +34,11 @@ TABLE_PATH = os.path.join(_HERE, "schedules.json")
 D_EDGES = (4, 8, 16, 32, 64, 128)
 DEFAULT = (3, 4, 32)
 CANDIDATES = [(3, 4, 32), (3, 6, 8), (3, 3, 64), (3, 5, 16), (2, 6, 32), (5, 3, 16)]
+# batch launch shapes: (bound workgroups, steps per launch); 0 workgroups =
+# 8 per CU
+DEFAULT_LAUNCH = (0, 4)
+LAUNCH_CANDIDATES = [(0, 4), (1024, 4), (4096, 4), (0, 2), (0, 8), (4096, 8)]
+LANE_CANDIDATES = (1, 4, 8)
 
 _table = None
 
@@ -40,14 +51,26 @@ def d_class(d: int) -> int:
 
 
 def load_table(path: str = TABLE_PATH) -> Dict[str, list]:
-    global _table
+    global _table, _launch
     if _table is None:
+        data = {}
         if os.path.exists(path):
             with open(path) as fh:
-                _table = json.load(fh).get("schedules", {})
-        else:
-            _table = {}
+                data = json.load(fh)
+        _table = data.get("schedules", {})
+        _launch = data.get("launch")
     return _table
+
+
+_launch = None
+
+
+def launch_for() -> Tuple[int, int]:
+    """(bound workgroups, steps per launch) of the batch form (tuned, else
+    the default)."""
+    load_table()
+    bb, spl = _launch if _launch else DEFAULT_LAUNCH
+    return int(bb), int(spl)
 
 
 def schedule_for(family: str, d: int) -> Tuple[Tuple[int, int], int]:
@@ -107,10 +130,93 @@ def tune(classes: Dict[str, List], candidates: Iterable[Sequence[int]] = CANDIDA
     return out
 
 
-def save_table(table: Dict[str, list], meta: dict, path: str = TABLE_PATH):
+def _run_mix(probs, slots: int, module, time_limit: float, **batch_kw) -> Tuple[dict, float]:
+    """Admit the problems into one batch (as slots free up) and run it to
+    the end or the time limit; (results by tag, wall seconds)."""
+    from .batch import Batch
+    res = {}
+    with Batch(slots=slots, module=module, **batch_kw) as b:
+        pending = list(probs)
+        t0 = time.perf_counter()
+        while True:
+            while pending and b.counts()[0] < slots:
+                b.admit(pending.pop(0))
+            b.run(1)
+            for tag, r in b.results():
+                res[tag] = r
+            if (not pending and b.counts()[0] == 0) or time.perf_counter() - t0 > time_limit:
+                break
+        b.sync()
+        for tag, r in b.results():
+            res[tag] = r
+        el = time.perf_counter() - t0
+    return res, el
+
+
+def tune_launch(specs: List, candidates: Iterable[Sequence[int]] = LAUNCH_CANDIDATES, *, module=None,
+                slots: int = 256, max_queue: int = 1 << 18, max_parents: int = 8192, max_iters: int = 2000,
+                time_limit: float = 8.0, log=print) -> Tuple[list, list]:
+    """Measure each (bound workgroups, steps per launch) on the whole mix of
+    sample problems (each on its class's schedule) and return the one with
+    the most certified problems per second, and the measurements."""
+    from .batch import BatchProblem
+    probs = []
+    for i, s in enumerate(specs):
+        grid, k = schedule_for(s.family, s.d)
+        probs.append(BatchProblem.from_spec(s, grid=grid, k_select=k, max_iters=max_iters, tag=i))
+    _run_mix(probs[:8], min(slots, 8), module, time_limit, max_queue=max_queue, max_parents=max_parents)  # warm-up
+    best, rows = None, []
+    for cand in candidates:
+        bb, spl = (int(v) for v in cand)
+        res, el = _run_mix(probs, slots, module, time_limit, max_queue=max_queue, max_parents=max_parents,
+                           bound_blocks=bb, steps_per_launch=spl)
+        cert = sum(r["status_name"] == "certified" for r in res.values())
+        rate = cert / el if el > 0 else 0.0
+        rows.append({"bound_blocks": bb, "steps_per_launch": spl, "certified": cert, "n": len(probs),
+                     "seconds": el, "certified_per_s": rate})
+        log(f"tune launch {(bb, spl)}: certified {cert}/{len(probs)} in {el:.3f} s ({rate:.1f}/s)")
+        if best is None or rate > best[0]:
+            best = (rate, [bb, spl])
+    return best[1], rows
+
+
+def tune_lanes(obj, lo, hi, lanes: Iterable[int] = LANE_CANDIDATES, *, roundings=("fast",), repeats: int = 3,
+               log=print, **solve_kw) -> Tuple[int, list]:
+    """Solve one problem with each lane count (1: one lane per child, else
+    the term-parallel kernels with that many lanes per child; the objective
+    needs loop templates for those) and return the fastest lane count that
+    certifies (median of `repeats` timed solves after one untimed), and the
+    measurements."""
+    import statistics
+    from .solver import get_module, minimize
+    rows, best = [], None
+    for g in lanes:
+        g = int(g)
+        split = g if g > 1 else 0
+        mod = get_module(obj, roundings=roundings, split=split)
+        kw = dict(solve_kw, module=mod, split=split, rounding=roundings[0])
+        r = minimize(obj, lo, hi, **kw)
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            r = minimize(obj, lo, hi, **kw)
+            ts.append(time.perf_counter() - t0)
+        med = statistics.median(ts)
+        ok = r["status_name"] == "certified"
+        rows.append({"lanes": g, "certified": ok, "seconds": med, "lb": r["lb"], "ub": r["ub"]})
+        log(f"tune lanes {g}: {r['status_name']} in {med * 1e3:.2f} ms")
+        if ok and (best is None or med < best[0]):
+            best = (med, g)
+    return (best[1] if best else None), rows
+
+
+def save_table(table: Dict[str, list], meta: dict, path: str = TABLE_PATH, launch: Optional[list] = None):
     global _table
+    data = {"schedules": table, "meta": meta}
+    if launch is not None:
+        data["launch"] = list(launch)
     with open(path, "w") as fh:
-        json.dump({"schedules": table, "meta": meta}, fh, indent=1, sort_keys=True)
+        json.dump(data, fh, indent=1, sort_keys=True)
     _table = None
 
 
@@ -126,6 +232,7 @@ def main():
     ap.add_argument("--pool", type=int, default=4096)
     ap.add_argument("--dmax", type=int, default=32)
     ap.add_argument("--out", default=TABLE_PATH)
+    ap.add_argument("--launch-sample", type=int, default=256, help="problems of the launch-shape measurement")
     a = ap.parse_args()
     init(0)
     classes = defaultdict(list)
@@ -142,7 +249,15 @@ def main():
     table = tune(classes, module=m)
     save_table(table, {"per_class": a.per_class, "dmax": a.dmax, "candidates": CANDIDATES,
                        "criterion": "most certified, then least wall time, one batch per class"}, a.out)
-    print(json.dumps(table, indent=1))
+    # launch shape on a mix drawn like the workload (table schedules in use)
+    mix = [s for s in (pb.batch_problem(2 * 10 ** 6 + i) for i in range(4 * a.launch_sample))
+           if s.d <= a.dmax][:a.launch_sample]
+    launch, rows = tune_launch(mix, module=m)
+    save_table(table, {"per_class": a.per_class, "dmax": a.dmax, "candidates": CANDIDATES,
+                       "criterion": "most certified, then least wall time, one batch per class",
+                       "launch_candidates": LAUNCH_CANDIDATES, "launch_rows": rows,
+                       "launch_criterion": "certified problems per second on the whole mix"}, a.out, launch)
+    print(json.dumps({"schedules": table, "launch": launch}, indent=1))
 
 
 if __name__ == "__main__":

@@ -47,7 +47,8 @@ class Server:
 
     def __init__(self, slots: Optional[Dict[int, int]] = None, *, dmax_classes: Sequence[int] = DMAX_CLASSES,
                  mem_fraction: float = 0.5, max_queue: int = 1 << 18, max_parents: int = 8192,
-                 max_iters: int = 2000, steps_per_launch: int = 4, rounding: str = "fast", grow: int = 2):
+                 max_iters: int = 2000, steps_per_launch: Optional[int] = None, rounding: str = "fast",
+                 grow: int = 2):
         _lib.init()
         self.classes = sorted(dmax_classes)
         self.max_queue = max_queue
