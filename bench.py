@@ -76,6 +76,14 @@ def parse():
     ap.add_argument("--batch-n", type=int, default=4096)
     ap.add_argument("--seq-sample", type=int, default=256,
                     help="problems of the batch solved one at a time for the sequential baseline")
+    ap.add_argument("--large-d-cases", default="", metavar="NAME:D,...",
+                    help="objectives of the large_d leg (default: Levy, Rosenbrock, Fu, Rastrigin at d=1000, "
+                         "Levy at d=2000)")
+    ap.add_argument("--large-d-reps", type=int, default=2, help="timed solves per large_d case")
+    ap.add_argument("--no-headline", action="store_true",
+                    help="run only the legs named in --legs (profiling one workload at a time); value is null")
+    ap.add_argument("--batch-no-graph", action="store_true",
+                    help="batch leg with stream launches instead of the captured graph (kernel traces)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (scales HBM byte counters)")
     return ap.parse_args()
@@ -461,7 +469,7 @@ GROW_TIERS = 2
 SEQ_QUEUE, SEQ_PARENTS = 1 << 23, 1 << 18
 
 
-def leg_batch(G, pb, n, seq_sample):
+def leg_batch(G, pb, n, seq_sample, use_graph=True):
     """n independent problems (families sepcos / sepcos_chain / ackley_q, d
     log-uniform in [2,32]) in one batch (schedules from the tuned table)
     against the same problems solved one at a time with the single-problem
@@ -476,7 +484,7 @@ def leg_batch(G, pb, n, seq_sample):
         grid, ks = schedule_for(s.family, s.d)
         probs.append(BatchProblem.from_spec(s, grid=grid, k_select=ks, max_iters=maxit, tag=k))
     m = get_batch_module(pb.FAMILIES, 32, ("fast",))
-    kw = dict(module=m, max_queue=qcap, max_parents=pcap, steps_per_launch=8, grow=GROW_TIERS)
+    kw = dict(module=m, max_queue=qcap, max_parents=pcap, steps_per_launch=8, grow=GROW_TIERS, use_graph=use_graph)
     run_batch(Batch, probs[:64], **kw)  # warm-up
     res, el = run_batch(Batch, probs, **kw)
     cert = sum(r["status_name"] == "certified" for r in res.values())
@@ -509,7 +517,7 @@ def leg_batch(G, pb, n, seq_sample):
     widths = [r["width"] for r in res.values() if r["status_name"] == "certified"]
     return {"workload": f"{n} problems (sepcos / sepcos_chain / ackley_q, d log-uniform [2,32], box [-h,h]^d, "
                         "h~U[2,32], tol in {1e-4,1e-6}), seeds [3,k]",
-            "slots": n, "steps_per_launch": 8, "graph": True, "max_queue": qcap, "max_parents": pcap,
+            "slots": n, "steps_per_launch": 8, "graph": use_graph, "max_queue": qcap, "max_parents": pcap,
             "growth_tiers": GROW_TIERS, "sequential_capacity": {"max_queue": SEQ_QUEUE, "max_parents": SEQ_PARENTS},
             "max_iters": maxit, "time_s": el, "problems_per_s": n / el, "certified": cert,
             "certified_per_s": cert / el,
@@ -566,6 +574,11 @@ def main():
     legs = set(args.legs.split(","))
     G.init(0)
     spec = pb.ackley50()
+    if args.no_headline:
+        out = {"metric": METRIC, "value": None, "unit": "s", "n_gpus": 1, "legs_only": sorted(legs)}
+        run_legs(G, pb, args, legs, spec, out, None)
+        print(json.dumps(out), flush=True)
+        return
     head = form_solves(G, pb, args, spec, C2_GRID, C2_K, HEADLINE_FORM)
     print(f"# headline done ({args.steps} solves)", file=sys.stderr, flush=True)
     rl = head["roofline"]
@@ -603,11 +616,17 @@ def main():
         out["cpu_baseline"] = with_ratio(cpu_reference(spec, C2_GRID, C2_K, 1, args.cpu_budget,
                                                        gpu_iterations=head["iterations"]),
                                          head["mean_s"], head["boxes_bounded"])
+    run_legs(G, pb, args, legs, spec, out, head)
+    print(json.dumps(out), flush=True)
+
+
+def run_legs(G, pb, args, legs, spec, out, head):
     if "c2_forms" in legs:
         print("# leg c2_forms", file=sys.stderr, flush=True)
         out["c2_forms"] = leg_config(G, pb, args, spec, C2_GRID, C2_K, forms=OTHER_FORMS, cpu=False)
-        out["c2_forms"]["headline_speedup_over"] = {
-            form_name(f): out["c2_forms"][form_name(f)]["mean_s"] / head["mean_s"] for f in OTHER_FORMS}
+        if head is not None:
+            out["c2_forms"]["headline_speedup_over"] = {
+                form_name(f): out["c2_forms"][form_name(f)]["mean_s"] / head["mean_s"] for f in OTHER_FORMS}
     if "c2_s10" in legs:
         print("# leg c2_s10", file=sys.stderr, flush=True)
         out["c2_s10"] = leg_config(G, pb, args, spec, C2_GRID, C2_K, forms=(HEADLINE_FORM,) + OTHER_FORMS,
@@ -626,17 +645,19 @@ def main():
         out["cold"] = leg_cold(G, pb)
     if "large_d" in legs:
         print("# leg large_d", file=sys.stderr, flush=True)
-        out["large_d"] = leg_large_d(G, no_cpu=args.no_cpu)
+        cases = LARGE_D
+        if args.large_d_cases:
+            cases = tuple((c.split(":")[0], int(c.split(":")[1])) for c in args.large_d_cases.split(","))
+        out["large_d"] = leg_large_d(G, cases, reps=args.large_d_reps, no_cpu=args.no_cpu)
     if "xl_d" in legs:
         print("# leg xl_d", file=sys.stderr, flush=True)
         out["xl_d"] = leg_large_d(G, XL_D, reps=1, max_queue=1 << 31, warm=False, no_cpu=True)
     if "batch" in legs:
         print("# leg batch", file=sys.stderr, flush=True)
-        out["batch"] = leg_batch(G, pb, args.batch_n, args.seq_sample)
+        out["batch"] = leg_batch(G, pb, args.batch_n, args.seq_sample, use_graph=not args.batch_no_graph)
     if "served" in legs:
         print("# leg served", file=sys.stderr, flush=True)
         out["served"] = leg_served(pb)
-    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

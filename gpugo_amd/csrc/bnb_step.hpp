@@ -382,7 +382,26 @@ __device__ long long collect_parents(const SelectArgs &a, long long N, long long
   return kept;
 }
 
-__device__ void select_body(const SelectArgs &a) {
+// Priority band of a batch slot: the compaction copies every kept entry
+// with lb < g into a side list (at most cap entries, count in n; more than
+// cap leaves it unusable), and the select takes the K smallest from it
+// instead of passing over the whole queue whenever it holds at least K
+// entries: every entry outside it has lb >= g, above all of them.  The
+// select of a step resets n and sets g for its compaction from the
+// histogram of a full pass (about BAND_TARGET entries below g).
+struct BandState {
+  double g;                 // threshold (-inf: no band)
+  unsigned long long n;     // entries appended by the last compaction
+};
+struct BandView {
+  const double *lb;
+  const unsigned *par, *pos, *idx;  // idx: the entry's index in the compacted queue
+  BandState *st;
+  long long cap;
+};
+#define BAND_TARGET 2048
+
+__device__ void select_body(const SelectArgs &a, const BandView *band = nullptr) {
   __shared__ unsigned hist[SEL_BINS];
   __shared__ u128 skey[SEL_KMAX];
   __shared__ unsigned sidx[SEL_KMAX];
@@ -393,6 +412,7 @@ __device__ void select_body(const SelectArgs &a) {
   __shared__ int s_pbits;
   __shared__ unsigned s_krem, s_region, s_b, s_cb, s_ns, s_nc;
   __shared__ unsigned long long s_outmin;
+  __shared__ double s_gnext;
 
   DevState *st = a.st;
   const int tid = threadIdx.x;
@@ -400,6 +420,7 @@ __device__ void select_body(const SelectArgs &a) {
     if (tid == 0) st->compact_final = 0;
     return;
   }
+  if (tid == 0) s_gnext = -__builtin_huge_val();
   if (st->fresh) {
     // a batch slot's first step splits the root record the host wrote
     __syncthreads();
@@ -500,11 +521,27 @@ __device__ void select_body(const SelectArgs &a) {
       }
     }
   } else {
+    // Source of the selection: the slot's band when it is complete and holds
+    // at least K entries, else the whole queue.
+    const long long bn = band ? (long long)band->st->n : -1;
+    const double bg = band ? band->st->g : -__builtin_huge_val();
+    const bool use_band = bn >= (long long)K && bn <= band->cap && bg > -__builtin_huge_val();
+    SelectArgs sv = a;
+    long long Nsel = N;
+    double lmax = key_value(st->lbmax_key);
+    if (use_band) {
+      sv.q_lb = const_cast<double *>(band->lb);
+      sv.q_par = const_cast<unsigned *>(band->par);
+      sv.q_pos = const_cast<unsigned *>(band->pos);
+      Nsel = bn;
+      if (bg < lmax) lmax = bg;  // every band entry is below g
+    }
+    if (tid == 0) s_gnext = bg;
     // Fast path: one histogram pass over the live range [lbmin, lbmax]
     // (tracked by the compaction).  The bin map is monotone in lb, so every
     // key in a bin below the boundary bin is smaller than every key in it.
     bool fast_done = false;
-    const double lmin = key_value(st->lbmin_key), lmax = key_value(st->lbmax_key);
+    const double lmin = key_value(st->lbmin_key);
     const double span = lmax - lmin;
     if (lmin > -__builtin_huge_val() && lmax < __builtin_huge_val() && span > 0) {
       const double scale = (double)SEL_BINS / span;
@@ -512,16 +549,16 @@ __device__ void select_body(const SelectArgs &a) {
       __syncthreads();
       // SEL_U loads in flight per thread: one workgroup's pass over the
       // queue is latency-bound otherwise
-      for (long long i0 = tid; i0 < N; i0 += (long long)SEL_T * SEL_U) {
+      for (long long i0 = tid; i0 < Nsel; i0 += (long long)SEL_T * SEL_U) {
         double v[SEL_U];
 #pragma unroll
         for (int u = 0; u < SEL_U; ++u) {
           const long long i = i0 + (long long)u * SEL_T;
-          v[u] = i < N ? a.q_lb[i] : 0.0;
+          v[u] = i < Nsel ? sv.q_lb[i] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < SEL_U; ++u)
-          if (i0 + (long long)u * SEL_T < N) atomicAdd(&hist[value_bin(v[u], lmin, scale)], 1u);
+          if (i0 + (long long)u * SEL_T < Nsel) atomicAdd(&hist[value_bin(v[u], lmin, scale)], 1u);
       }
       __syncthreads();
       unsigned h[4], ts = 0;
@@ -532,6 +569,7 @@ __device__ void select_body(const SelectArgs &a) {
       }
       unsigned tot;
       unsigned cum = block_excl_scan(ts, scan_sh, &tot);
+      const unsigned gt = (unsigned)BAND_TARGET > (unsigned)K ? (unsigned)BAND_TARGET : (unsigned)K;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if (cum < (unsigned)K && (unsigned)K <= cum + h[q]) {
@@ -539,8 +577,12 @@ __device__ void select_body(const SelectArgs &a) {
           s_cb = cum;
           s_region = h[q];
         }
+        if (band && !use_band && cum < gt && gt <= cum + h[q])
+          s_gnext = (4 * tid + q + 1 == SEL_BINS) ? __builtin_huge_val()
+                                                   : lmin + (double)(4 * tid + q + 1) * (span / (double)SEL_BINS);
         cum += h[q];
       }
+      if (band && !use_band && tid == 0 && tot < gt) s_gnext = __builtin_huge_val();
       __syncthreads();
       if (s_region <= SEL_FINAL || s_region == (unsigned)K - s_cb) {
         if (tid == 0) {
@@ -550,28 +592,28 @@ __device__ void select_body(const SelectArgs &a) {
         }
         __syncthreads();
         const unsigned bb = s_b;
-        for (long long i0 = tid; i0 < N; i0 += (long long)SEL_T * SEL_U) {
+        for (long long i0 = tid; i0 < Nsel; i0 += (long long)SEL_T * SEL_U) {
           double v[SEL_U];
 #pragma unroll
           for (int u = 0; u < SEL_U; ++u) {
             const long long i = i0 + (long long)u * SEL_T;
-            v[u] = i < N ? a.q_lb[i] : 0.0;
+            v[u] = i < Nsel ? sv.q_lb[i] : 0.0;
           }
 #pragma unroll
           for (int u = 0; u < SEL_U; ++u) {
             const long long i = i0 + (long long)u * SEL_T;
-            if (i >= N) continue;
+            if (i >= Nsel) continue;
             const unsigned bin = value_bin(v[u], lmin, scale);
             if (bin < bb) {
               unsigned slot = atomicAdd(&s_ns, 1u);
               if (slot < SEL_KMAX) {
-                skey[slot] = qkey(a, i);
+                skey[slot] = qkey(sv, i);
                 sidx[slot] = (unsigned)i;
               }
             } else if (bin == bb) {
               unsigned slot = atomicAdd(&s_nc, 1u);
               if (slot < SEL_FINAL) {
-                ckey[slot] = qkey(a, i);
+                ckey[slot] = qkey(sv, i);
                 cidx[slot] = (unsigned)i;
               }
             }
@@ -581,7 +623,7 @@ __device__ void select_body(const SelectArgs &a) {
         fast_done = true;
       }
     }
-    if (!fast_done) radix_gather(a, N, K, hist, skey, sidx, ckey, cidx, scan_sh, s_pfx, s_pbits, s_krem, s_region,
+    if (!fast_done) radix_gather(sv, Nsel, K, hist, skey, sidx, ckey, cidx, scan_sh, s_pfx, s_pbits, s_krem, s_region,
                                  s_b, s_cb, s_ns, s_nc);
     // the s_krem smallest candidates fill the remaining slots (rank by
     // counting: keys are unique)
@@ -596,6 +638,11 @@ __device__ void select_body(const SelectArgs &a) {
       }
     }
     __syncthreads();
+    if (use_band) {
+      // band positions -> queue indices
+      if (tid < K) sidx[tid] = band->idx[sidx[tid]];
+      __syncthreads();
+    }
   }
   // deterministic order of the selection: rank by counting into ckey/cidx
   {
@@ -788,6 +835,11 @@ __device__ void select_body(const SelectArgs &a) {
       // records, which are part of the keys, so every chunk minimum is stale
       st->dirty_lo = collected ? 0 : (N < st->dirty_lo ? N : st->dirty_lo);
       st->consumed = idx;
+    }
+    if (band) {
+      // this step's compaction rebuilds the band below the (new) threshold
+      band->st->n = 0;
+      band->st->g = K == 1 && a.wmin ? -__builtin_huge_val() : s_gnext;
     }
   }
 }

@@ -642,7 +642,29 @@ struct BatchArgs {
   long long pcap, qcap;
   int dmax, pmax, slots;
   int grow;  // 1: a larger tier exists; slots that would outgrow this one stop with GPUGO_ST_GROW
+  int cb;    // compaction workgroups per slot
+  int pad_cb;
+  // priority bands: [slots][band_cap] entries and [slots] states
+  double *band_lb;
+  unsigned *band_par, *band_pos, *band_idx;
+  BandState *band;
+  long long band_cap;
 };
+
+// entries of a slot's priority band
+#define GPUGO_BAND_CAP 16384
+
+__device__ __forceinline__ BandView slot_band(const BatchArgs &b, int slot) {
+  BandView v;
+  const long long o = (long long)slot * b.band_cap;
+  v.lb = b.band_lb + o;
+  v.par = b.band_par + o;
+  v.pos = b.band_pos + o;
+  v.idx = b.band_idx + o;
+  v.st = b.band + slot;
+  v.cap = b.band_cap;
+  return v;
+}
 
 __global__ __launch_bounds__(SEL_T) void kb_select(BatchArgs b) {
   const int slot = blockIdx.x;
@@ -681,17 +703,19 @@ __global__ __launch_bounds__(SEL_T) void kb_select(BatchArgs b) {
   sa.q_cap = b.qcap;
   sa.W = I.W;
   sa.pad_grow = 0;
-  select_body(sa);
+  const BandView bv = slot_band(b, slot);
+  select_body(sa, &bv);
 }
 
-// The slot's queue is compacted by KB_CB workgroups (grid-stride over the
-// queue, one atomic per workgroup and chunk for the output position); the
-// last workgroup of the slot to finish switches the live buffer.
-#define KB_CB 8
+// The slot's queue is compacted by b.cb workgroups (grid-stride over the
+// queue, CMP_U entries per thread and one atomic per workgroup pass for the
+// output position); the last workgroup of the slot to finish switches the
+// live buffer.  Larger tiers (longer queues) use more workgroups per slot.
 __global__ __launch_bounds__(256) void kb_compact(BatchArgs b) {
-  __shared__ unsigned wcnt[4];
+  __shared__ unsigned wcnt[CMP_U * 4];
   __shared__ unsigned long long s_kmin, s_kmax, s_base;
-  const int slot = blockIdx.x / KB_CB, sub = blockIdx.x % KB_CB;
+  const int cb = b.cb;
+  const int slot = blockIdx.x / cb, sub = blockIdx.x % cb;
   const SlotInfo I = b.info[slot];
   if (!I.active) return;
   DevState *st = b.st + slot;
@@ -707,36 +731,72 @@ __global__ __launch_bounds__(256) void kb_compact(BatchArgs b) {
     s_kmin = ~0ull;
     s_kmax = 0ull;
   }
-  for (long long base = (long long)sub * 256; base < n; base += (long long)KB_CB * 256) {
-    const long long i = base + threadIdx.x;
-    double lb = 0;
-    bool keep = false;
-    if (i < n) {
-      lb = b.q_lb[in + i];
-      keep = lb <= ub;
+  // kept entries below the band threshold also go to the slot's band
+  const BandView bv = slot_band(b, slot);
+  const double bg = bv.st->g;
+  const long long span = 256LL * CMP_U;
+  for (long long base = (long long)sub * span; base < n; base += (long long)cb * span) {
+    double lb[CMP_U];
+#pragma unroll
+    for (int u = 0; u < CMP_U; ++u) {
+      const long long i = base + u * 256 + threadIdx.x;
+      lb[u] = i < n ? b.q_lb[in + i] : 0.0;
+    }
+    unsigned kept = 0u, pre[CMP_U];
+#pragma unroll
+    for (int u = 0; u < CMP_U; ++u) {
+      const long long i = base + u * 256 + threadIdx.x;
+      const bool keep = i < n && lb[u] <= ub;
       if (keep) {
-        const unsigned long long k = order_key(lb);
+        const unsigned long long k = order_key(lb[u]);
         kmin = k < kmin ? k : kmin;
         kmax = k > kmax ? k : kmax;
+        kept |= 1u << u;
       }
+      const unsigned long long mask = __ballot(keep);
+      pre[u] = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+      if (lane == 0) wcnt[u * 4 + w] = (unsigned)__popcll(mask);
     }
-    const unsigned long long mask = __ballot(keep);
-    const unsigned pre =
-        __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-    if (lane == 0) wcnt[w] = (unsigned)__popcll(mask);
     __syncthreads();
     if (threadIdx.x == 0) {
-      const unsigned t = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+      // exclusive prefix over (u, wavefront): the output order of the pass
+      unsigned t = 0;
+      for (int q = 0; q < CMP_U * 4; ++q) {
+        const unsigned c = wcnt[q];
+        wcnt[q] = t;
+        t += c;
+      }
       s_base = t ? atomicAdd((unsigned long long *)&st->compact_count, (unsigned long long)t) : 0ull;
     }
     __syncthreads();
-    unsigned off = pre;
-    for (int q = 0; q < w; ++q) off += wcnt[q];
-    if (keep) {
-      const long long o = out + (long long)s_base + off;
-      b.q_lb[o] = lb;
-      b.q_par[o] = b.q_par[in + i];
-      b.q_pos[o] = b.q_pos[in + i];
+#pragma unroll
+    for (int u = 0; u < CMP_U; ++u) {
+      const bool k = (kept >> u) & 1u;
+      const bool inband = k && lb[u] < bg;
+      const unsigned long long bmask = __ballot(inband);
+      unsigned long long bbase = 0;
+      if (bmask && lane == 0)
+        bbase = atomicAdd((unsigned long long *)&bv.st->n, (unsigned long long)__popcll(bmask));
+      bbase = (unsigned long long)__shfl((long long)bbase, 0, 64);
+      if (k) {
+        const long long i = base + u * 256 + threadIdx.x;
+        const long long orel = (long long)s_base + wcnt[u * 4 + w] + pre[u];
+        const unsigned par = b.q_par[in + i], pos = b.q_pos[in + i];
+        b.q_lb[out + orel] = lb[u];
+        b.q_par[out + orel] = par;
+        b.q_pos[out + orel] = pos;
+        if (inband) {
+          const unsigned long long bo =
+              bbase + __builtin_amdgcn_mbcnt_hi((unsigned)(bmask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bmask, 0u));
+          if (bo < (unsigned long long)bv.cap) {
+            const long long e = (long long)slot * b.band_cap + (long long)bo;
+            b.band_lb[e] = lb[u];
+            b.band_par[e] = par;
+            b.band_pos[e] = pos;
+            b.band_idx[e] = (unsigned)orel;
+          }
+        }
+      }
     }
     __syncthreads();
   }
@@ -757,7 +817,7 @@ __global__ __launch_bounds__(256) void kb_compact(BatchArgs b) {
       atomicMax(&st->lbmax_key, s_kmax);
     }
     __threadfence();
-    if (atomicAdd(&st->cblocks, 1) == KB_CB - 1) {
+    if (atomicAdd(&st->cblocks, 1) == cb - 1) {
       // every workgroup of this slot has read `cur` and written its part
       __threadfence();
       if (n > st->queue_peak) st->queue_peak = n;
@@ -841,6 +901,8 @@ __global__ __launch_bounds__(64) void kb_admit(BatchArgs b, const char *recs, lo
     b.pwin[root] = 0;
     b.st[s] = h->st;
     b.info[s] = h->info;
+    b.band[s].g = -__builtin_huge_val();
+    b.band[s].n = 0;
   }
 }
 
@@ -885,6 +947,8 @@ __global__ __launch_bounds__(256) void kb_move(BatchArgs src, BatchArgs dst, int
     z.fresh = 0;
     dst.st[t] = z;
     dst.info[t] = src.info[s];
+    dst.band[t].g = -__builtin_huge_val();
+    dst.band[t].n = 0;
   }
 }
 
@@ -952,6 +1016,7 @@ struct gpugo_problem_s {
   int c = 0;
   unsigned W = 0;
   long long q_cap = 0, p_cap = 0, out_cap = 0;
+  long long q_req = 0;  // queue capacity asked for (q_cap adds a lazy queue's stale budget)
   DevBuf d_lo, d_hi, d_params, d_params32, plo, phi, pw, pwin, out_lo, out_hi, out_lb, st, minkey;
   DevBuf bar;          // persistent step loop: grid barrier [2] and error flag
   DevBuf pext;         // windowed modules: rest sums per parent record [p_cap][ext_n]
@@ -1017,6 +1082,7 @@ static int problem_shape(gpugo_problem_s *p, const gpugo_problem_desc *desc, lon
   // windowed modules: (1 + n_samples) * nrest rest-sum intervals per record
   p->ext_n = p->m->nrest > 0 ? 2 * p->m->nrest * (1 + desc->n_samples) : 0;
   p->q_cap = qc;
+  p->q_req = qc;
   p->p_cap = desc->max_parents > 0 ? desc->max_parents : p_default;
   // output boxes (x-width rule) up to 2^16 or 2^24 coordinates
   p->out_cap = std::max(64LL, std::min(1LL << 16, (1LL << 24) / d));
@@ -1024,12 +1090,16 @@ static int problem_shape(gpugo_problem_s *p, const gpugo_problem_desc *desc, lon
   p->lazy = false;
   p->compact_every = 1;
   if (desc->k_select == 1 && !(desc->flags & (GPUGO_FLAG_PERSIST | GPUGO_FLAG_EAGER_COMPACT))) {
-    // stale entries between compactions stay under an eighth of the queue:
-    // (compact_every - 1) steps append at most W entries each
+    // A compaction leaves the live entries, which an eagerly compacted
+    // queue holds at the same step (<= qc, or that queue had overflowed);
+    // the compact_every steps up to the next one append at most W each.
+    // Allocating that much beyond qc means a lazy queue never overflows
+    // where an eager one would not (the eighth of qc bounds the extra).
     const long long every = std::min((long long)GPUGO_LAZY_MAX_EVERY, qc / 8 / (long long)p->W);
-    if (every >= 2) {
+    if (every >= 2 && qc + every * (long long)p->W <= 0xfffffff0LL) {
       p->lazy = true;
       p->compact_every = (int)every;
+      p->q_cap = qc + every * (long long)p->W;
     }
   }
   return GPUGO_OK;
@@ -1684,7 +1754,7 @@ static int run_steps(gpugo_problem_t p, int64_t iters, gpugo_result *res, bool s
   if (rc) return rc;
   res->bound_ms = p->bound_ms;
   res->bound_launches = p->bound_launches;
-  if (p->hs.overflow & 1) return set_err(GPUGO_E_CAPACITY, "box queue capacity (%lld) exceeded", p->q_cap);
+  if (p->hs.overflow & 1) return set_err(GPUGO_E_CAPACITY, "box queue capacity (%lld) exceeded", p->q_req);
   if (p->hs.overflow & 2) return set_err(GPUGO_E_CAPACITY, "parent arena capacity (%lld) exceeded", p->p_cap);
   return GPUGO_OK;
 }
@@ -1862,6 +1932,7 @@ struct gpugo_batch_s {
   int dmax = 0, pmax = 0, slots = 0, bound_grid = 0;
   long long qcap = 0, pcap = 0, rec_bytes = 0;
   DevBuf info, st, plo, phi, pw, pwin, params, dom_lo, dom_hi, q_lb, q_par, q_pos, wave_off, recs;
+  DevBuf band_lb, band_par, band_pos, band_idx, band;
   hipStream_t stream = nullptr;
   BatchArgs ba;
   BatchBoundArgs bb;
@@ -1899,7 +1970,7 @@ struct gpugo_batch_s {
 
 #define GPUGO_GROW_MAX 3
 #define GPUGO_GROW_FACTOR 8
-#define GPUGO_GROW_SLOT_SHIFT 5
+#define GPUGO_GROW_SLOT_SHIFT 4
 
 // description of growth tier k of a batch with description d
 static gpugo_batch_desc tier_desc(const gpugo_batch_desc &d, long long q0, long long p0, int k) {
@@ -1937,7 +2008,8 @@ static int batch_sizes(gpugo_module_t m, const gpugo_batch_desc *desc, long long
     const gpugo_batch_desc t = tier_desc(*desc, q, p, k);
     const long long S = t.slots, tq = t.max_queue, tp = t.max_parents;
     total += S * (long long)(sizeof(SlotInfo) + sizeof(DevState)) + S * tp * (3 * D * 8 + 4) + S * PM * 8 +
-             S * 2 * D * 8 + S * 2 * tq * 16 + (S + 1) * 8 + S * rb;
+             S * 2 * D * 8 + S * 2 * tq * 16 + (S + 1) * 8 + S * rb +
+             S * (GPUGO_BAND_CAP * 20LL + (long long)sizeof(BandState));
   }
   *qcap = q;
   *pcap = p;
@@ -1970,7 +2042,7 @@ static int batch_enqueue_step(gpugo_batch_t b) {
   // for the life of the graph
   HIPCHECK(hipModuleLaunchKernel(b->bound, (unsigned)b->bound_grid, 1, 1, 256, 1, 1, 0, b->stream, b->kp_bb,
                                  nullptr));
-  hipLaunchKernelGGL(kb_compact, dim3((unsigned)b->slots * KB_CB), dim3(256), 0, b->stream, b->ba);
+  hipLaunchKernelGGL(kb_compact, dim3((unsigned)b->slots * b->ba.cb), dim3(256), 0, b->stream, b->ba);
   HIPCHECK(hipGetLastError());
   return GPUGO_OK;
 }
@@ -2039,6 +2111,11 @@ static int batch_create_level(gpugo_module_t m, const gpugo_batch_desc *desc, in
   BALLOC(b->q_pos, sizeof(unsigned) * S * 2 * qcap);
   BALLOC(b->wave_off, sizeof(long long) * (S + 1));
   BALLOC(b->recs, (size_t)(rb * S));
+  BALLOC(b->band_lb, sizeof(double) * S * GPUGO_BAND_CAP);
+  BALLOC(b->band_par, sizeof(unsigned) * S * GPUGO_BAND_CAP);
+  BALLOC(b->band_pos, sizeof(unsigned) * S * GPUGO_BAND_CAP);
+  BALLOC(b->band_idx, sizeof(unsigned) * S * GPUGO_BAND_CAP);
+  BALLOC(b->band, sizeof(BandState) * S);
 #undef BALLOC
   if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess)
     return fail(set_err(GPUGO_E_HIP, "stream: %s", hipGetErrorString(e)));
@@ -2064,6 +2141,13 @@ static int batch_create_level(gpugo_module_t m, const gpugo_batch_desc *desc, in
     for (auto &z : zs) z.status = GPUGO_ST_EXHAUSTED;
     if ((e = hipMemcpy(b->st.p, zs.data(), sizeof(DevState) * S, hipMemcpyHostToDevice)) != hipSuccess)
       return fail(set_err(GPUGO_E_HIP, "upload: %s", hipGetErrorString(e)));
+    std::vector<BandState> zb(S);
+    for (auto &z : zb) {
+      z.g = -__builtin_huge_val();
+      z.n = 0;
+    }
+    if ((e = hipMemcpy(b->band.p, zb.data(), sizeof(BandState) * S, hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(set_err(GPUGO_E_HIP, "upload: %s", hipGetErrorString(e)));
   }
   BatchArgs &a = b->ba;
   a.info = b->info.as<SlotInfo>();
@@ -2085,6 +2169,15 @@ static int batch_create_level(gpugo_module_t m, const gpugo_batch_desc *desc, in
   a.pmax = b->pmax;
   a.slots = b->slots;
   a.grow = grow_next;
+  // compaction workgroups per slot: 8, and 8 times more per growth tier
+  a.cb = 8 << (3 * std::min(level, 2));
+  a.pad_cb = 0;
+  a.band_lb = b->band_lb.as<double>();
+  a.band_par = b->band_par.as<unsigned>();
+  a.band_pos = b->band_pos.as<unsigned>();
+  a.band_idx = b->band_idx.as<unsigned>();
+  a.band = b->band.as<BandState>();
+  a.band_cap = GPUGO_BAND_CAP;
   BatchBoundArgs &g = b->bb;
   memset(&g, 0, sizeof g);
   g.info = a.info;

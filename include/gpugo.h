@@ -312,7 +312,7 @@ typedef struct {
                                 GPUGO_ST_OVERFLOW.  n > 0: the batch also owns
                                 n larger tiers; tier k has 8^k times the
                                 per-slot max_queue and max_parents and
-                                slots >> (5 k) slots (at least 1).  A problem
+                                slots >> (4 k) slots (at least 1).  A problem
                                 whose next step might not fit its slot stops
                                 before that step, moves (queue, parent
                                 records, incumbent, counters) into a free slot

@@ -152,6 +152,30 @@ def test_reference_grid_matches_oracle_first_steps(gpu, oracle):
     assert abs(r["queue_size"] - o["queue_size"]) <= max(2, 0.001 * o["queue_size"])
 
 
+def test_reference_grid_certifies(gpu):
+    """Config 1 on its configured grid to the end (4^10 children of one box
+    per step, the reference's schedule, natural extension): certified, the
+    enclosure intersects the tuned-schedule solve's (4^5, K = 32, mean-value
+    bound, 4 lanes per child), and its lower bound is below the value of an
+    L-BFGS witness evaluated in 256-bit arithmetic."""
+    from scipy.optimize import minimize as sp_minimize
+    from tests.test_oracle_objectives import f_true
+    spec = pb.chain10()
+    print("# solve chain10 grid 4^10 K=1", flush=True)
+    with gpu_problem(gpu, spec, tol=1e-6, grid=(4, 10), k_select=1, max_iters=20000) as p:
+        r = p.solve()
+    with gpu_problem(gpu, spec, tol=1e-6, grid=(4, 5), k_select=32, mean_value=True, split=4) as p:
+        t = p.solve()
+    assert r["status_name"] == "certified" and r["width"] <= 1e-6, r
+    assert t["status_name"] == "certified"
+    assert max(r["lb"], t["lb"]) <= min(r["ub"], t["ub"])
+    rng = np.random.default_rng(0)
+    f = lambda x: float(f_true(spec.family, spec.params, x))
+    best = min((sp_minimize(f, rng.uniform(spec.lo, spec.hi), method="L-BFGS-B",
+                            bounds=list(zip(spec.lo, spec.hi))) for _ in range(10)), key=lambda z: z.fun)
+    assert r["lb"] <= float(f_true(spec.family, spec.params, best.x)) + 1e-12
+
+
 @pytest.mark.parametrize("spec,grid,k,kw", [
     (pb.rosenbrock(4), (2, 4), 16, {"rounding": "exact"}),
     (pb.sepcos2(), (2, 2), 16, {}),

@@ -19,21 +19,23 @@ reference evaluates every coordinate for every child
   - two solves are identical (deterministic);
   - with a parent arena small enough that records are collected (their rest
     sums move with them) the solve equals the default arena's;
-  - d = 50 and d = 1000 for every windowable reference family (Ackley,
-    Belegundu, Breiman, Fu, Levy, Rastrigin, Rosenbrock, Salomon) and
-    d = 2000, 5000 and 10,000 for Levy (the
+  - d = 50 for all eleven reference families (the product objectives
+    Griewank, Styblinski, Zabinsky through template products), d = 1000 for
+    Ackley, Belegundu, Breiman, Fu, Levy, Rastrigin, Rosenbrock, Salomon and
+    the product objectives, and d = 2000, 5000 and 10,000 for Levy (the
     reference's levy1000.py ... levy10000.py boxes and initial upper bounds,
     10 diagonal samples, 3^12 children per step, one box per step, a queue of
     up to 2^29 boxes, 2^31 at d = 10,000): certified enclosures of the
     closed-form f*;
   - one box per step, the default queue (compacted every few steps, chunk
-    minima between) gives the same solve as compaction after every step.
+    minima between) gives the same solve as compaction after every step,
+    also with the capacity set to the eager solve's queue peak.
 """
 import mpmath as mp
 import numpy as np
 import pytest
 
-from gpugo_amd import families as F, problems as pb
+from gpugo_amd import GpugoError, families as F, problems as pb
 from tests.test_families import f_ref
 
 pytestmark = pytest.mark.gpu
@@ -249,6 +251,40 @@ def test_window_lazy_queue_matches_eager(gpu, name, d, S, kw):
     for key in RES_KEYS + ("width", "pruned_value", "pruned_kkt"):
         assert a[key] == b[key], (key, a[key], b[key])
     assert a["queue_peak"] >= b["queue_peak"]
+
+
+# (objective, d, samples, grid, tol): one-box-per-step solves; those whose
+# eager queue peak leaves room for the lazy schedule (peak / 8 >= 2 wid^cycle)
+# are compared at that capacity
+CAP_CASES = [("rastrigin", 50, 1, (2, 4), 1e-6), ("rastrigin", 50, 1, (3, 4), 1e-6), ("levy", 50, 1, (3, 4), 1e-6),
+             ("ackley", 50, 1, (2, 4), 1e-6)]
+
+
+def test_lazy_queue_at_capacity_matches_eager(gpu):
+    """Capacity set to the eagerly compacted solve's queue peak (the most
+    entries it ever held, so it just fits): the lazy queue, which keeps
+    consumed and ruled-out entries until its next compaction, fits too and
+    gives the same solve; one entry less overflows the eager queue."""
+    exercised = 0
+    for name, d, S, grid, tol in CAP_CASES:
+        m = wmod(gpu, F.objective_loops(name, d))
+        e, _ = ref_solve(gpu, name, d, grid, 1, S, m, True, tol=tol, max_iters=20000, eager_compact=True)
+        assert e["status_name"] == "certified", e
+        cap = int(e["queue_peak"])
+        print(f"# eager queue peak {cap}, wid^cycle {grid[0] ** grid[1]}", flush=True)
+        if cap // 8 // (grid[0] ** grid[1]) < 2:
+            continue
+        exercised += 1
+        a, _ = ref_solve(gpu, name, d, grid, 1, S, m, True, tol=tol, max_iters=20000, max_queue=cap)
+        b, _ = ref_solve(gpu, name, d, grid, 1, S, m, True, tol=tol, max_iters=20000, max_queue=cap,
+                         eager_compact=True)
+        for key in RES_KEYS + ("width", "pruned_value", "pruned_kkt"):
+            assert a[key] == b[key] == e[key], (name, grid, key, a[key], b[key], e[key])
+        assert a["queue_peak"] >= cap         # stale entries are held beside the live ones
+        with pytest.raises(GpugoError, match="CAPACITY"):
+            ref_solve(gpu, name, d, grid, 1, S, m, True, tol=tol, max_iters=20000, max_queue=cap - 1,
+                      eager_compact=True)
+    assert exercised >= 2, "too few cases had a queue large enough for the lazy schedule"
 
 
 # Breiman's box is [-1, 2] (3-Breiman/cosine50.py:857) and its minimiser is
