@@ -77,9 +77,9 @@ def parse():
     ap.add_argument("--seq-sample", type=int, default=256,
                     help="problems of the batch solved one at a time for the sequential baseline")
     ap.add_argument("--large-d-cases", default="", metavar="NAME:D,...",
-                    help="objectives of the large_d leg (default: Levy, Rosenbrock, Fu, Rastrigin at d=1000, "
+                    help="objectives of the large_d leg (default: the eleven reference objectives at d=1000, "
                          "Levy at d=2000)")
-    ap.add_argument("--large-d-reps", type=int, default=2, help="timed solves per large_d case")
+    ap.add_argument("--large-d-reps", type=int, default=1, help="timed solves per large_d case (after one untimed)")
     ap.add_argument("--no-headline", action="store_true",
                     help="run only the legs named in --legs (profiling one workload at a time); value is null")
     ap.add_argument("--batch-no-graph", action="store_true",
@@ -367,9 +367,13 @@ def leg_cold(G, pb):
             "certified": r["status_name"] == "certified"}
 
 
-LARGE_D = (("levy", 1000), ("rosenbrock", 1000), ("fu", 1000), ("rastrigin", 1000), ("levy", 2000))
+LARGE_D = tuple((n, 1000) for n in ("levy", "rosenbrock", "fu", "rastrigin", "ackley", "belegundu", "breiman",
+                                     "salomon", "griewank", "styblinski", "zabinsky")) + (("levy", 2000),)
 # the reference's levy10000.py dimension: one solve, a queue of up to 2^31 boxes
 XL_D = (("levy", 10000),)
+# Breiman's minimiser x* = 0 lies on the wid-3 split planes of its box
+# [-1, 2] (every level), so it runs on the reference's wid 4
+LARGE_D_GRID = {"breiman": (4, 9)}
 
 
 def leg_large_d(G, cases=LARGE_D, reps=2, cpu_seconds=2.0, max_queue=1 << 28, warm=True, no_cpu=False):
@@ -383,12 +387,13 @@ def leg_large_d(G, cases=LARGE_D, reps=2, cpu_seconds=2.0, max_queue=1 << 28, wa
     the time it would need for the GPU's boxes."""
     from gpugo_amd import emit, families as F, problems as pb
     from oracle import oracle as O
-    grid, k, S = (3, 12), 1, 10
+    k, S = 1, 10
     dims = "/".join(sorted({str(d) for _, d in cases}, key=int))
-    out = {"workload": f"reference objectives d={dims}, reference boxes and initial upper bounds, grid 3^12, "
-                       f"one box per step, {S} diagonal samples, tol 1e-6, windowed code lists, "
+    out = {"workload": f"reference objectives d={dims}, reference boxes and initial upper bounds, grid 3^12 "
+                       f"(Breiman 4^9), one box per step, {S} diagonal samples, tol 1e-6, windowed code lists, "
                        f"queue up to 2^{max_queue.bit_length() - 1} boxes"}
     for name, d in cases:
+        grid = LARGE_D_GRID.get(name, (3, 12))
         s = F.spec(name, d)
         obj = F.objective_loops(name, d)
         m = G.get_module(obj, roundings=("exact", "fast"), windowed=True)

@@ -81,10 +81,53 @@ __device__ __forceinline__ void prep_impl(const BoundArgs &a) {
   }
 }
 
+// Wavefronts where at most this many children reach an FP64 test evaluate
+// them one at a time with all 64 lanes (obj_value_wp / obj_kkt_wp): one
+// child's FP64 tests on one lane would otherwise be the launch's critical
+// path (a lone lane runs its whole dependent chain with nothing to overlap).
+#ifndef GPUGO_COOP_MAX
+#define GPUGO_COOP_MAX 8
+#endif
+
+// The FP64 tests of child (p, pos) by the whole wavefront: value (unless
+// lb is given), then, if lb <= ub0, the monotonicity test and the S
+// diagonal samples.  Every lane returns the same results.
+template <int R>
+__device__ __forceinline__ void coop_tests(const BoundArgs &a, long long p, unsigned pos, double ub0, bool value,
+                                           int lane, double &lb, bool &keep, bool &pv, bool &pk, double &su) {
+  const int win = a.pwin[p];
+  const ChildBox x = child_box(a.plo + p * GPUGO_D, a.phi + p * GPUGO_D, a.pw + p * GPUGO_D, win, GPUGO_D, a.c,
+                               (unsigned)a.wid, a.inv_wid, a.half_inv_wid, pos);
+  const int S = a.n_samples;
+  const iv<double> *rest = (const iv<double> *)(a.prest) + p * (long long)((1 + S) * GPUGO_NREST);
+  if (value) {
+    lb = obj_value_wp<R, 64>(x, a.params, a.d, rest, win, a.c, lane).lo;
+    if (!(lb == lb)) lb = -__builtin_huge_val();
+  }
+  keep = false;
+  pv = false;
+  pk = false;
+  su = __builtin_huge_val();
+  if (lb <= ub0) {
+    keep = !obj_kkt_wp<R, 64>(x, a.params, a.dom_lo, a.dom_hi, a.d, rest, win, a.c, lane);
+    pk = !keep;
+    for (int ts = 0; ts < S; ++ts) {
+      const DiagPoint<ChildBox> xs{x, ts, S};
+      const double u = obj_value_wp<R, 64>(xs, a.params, a.d, rest + (1 + ts) * GPUGO_NREST, win, a.c, lane).hi;
+      su = u < su ? u : su;
+    }
+  } else {
+    pv = true;
+  }
+}
+
 // Split+bound with windowed code lists: value test, then for survivors the
 // monotonicity test and the diagonal samples; survivors are appended to the
 // queue (one atomic per workgroup), the incumbent is lowered by one atomic
-// per workgroup (same protocol as bound_impl in bnb_kernel.hpp).
+// per workgroup (same protocol as bound_impl in bnb_kernel.hpp).  A
+// wavefront with few children left for an FP64 stage (after the FP32 test,
+// or after the FP64 value test) runs that stage cooperatively, one child at
+// a time (coop_tests).
 template <int R>
 __device__ __forceinline__ void bound_w_impl(const BoundArgs &a) {
   __shared__ unsigned s_wcnt[4];
@@ -111,6 +154,7 @@ __device__ __forceinline__ void bound_w_impl(const BoundArgs &a) {
     unsigned pos = 0;
     double lb = __builtin_huge_val(), su = __builtin_huge_val();
     bool keep = false, pv = false, pk = false;
+    bool need = false;  // the child reaches the FP64 value test
     if (t < n_child) {
       pl = (long long)__builtin_fma((double)t, invW, halfInvW);
       pos = (unsigned)(t - pl * (long long)a.W);
@@ -118,27 +162,76 @@ __device__ __forceinline__ void bound_w_impl(const BoundArgs &a) {
       const int win = a.pwin[p];
       const ChildBox x = child_box(a.plo + p * GPUGO_D, a.phi + p * GPUGO_D, a.pw + p * GPUGO_D, win, GPUGO_D, a.c,
                                    (unsigned)a.wid, a.inv_wid, a.half_inv_wid, pos);
-      const int S = a.n_samples;
-      const iv<double> *rest = (const iv<double> *)(a.prest) + p * (long long)((1 + S) * GPUGO_NREST);
-      if (f32_prunes_w<R>(a, x, rest, win, ub0)) {
-        pv = true;
-      } else {
+      const iv<double> *rest = (const iv<double> *)(a.prest) + p * (long long)((1 + a.n_samples) * GPUGO_NREST);
+      if (f32_prunes_w<R>(a, x, rest, win, ub0)) pv = true;
+      else need = true;
+    }
+    // the wavefront's children in each FP64 stage: few -> all lanes on each
+    const unsigned long long need_m = __ballot(need);
+    bool stage2 = false;  // value computed on the child's own lane; tests next
+    if (need_m != 0ull && __popcll(need_m) <= GPUGO_COOP_MAX) {
+      unsigned long long m = need_m;
+      while (m != 0ull) {
+        const int src = __builtin_ctzll(m);
+        m &= m - 1ull;
+        const long long ps = first + (long long)__shfl((int)pl, src, 64);
+        const unsigned poss = (unsigned)__shfl((int)pos, src, 64);
+        double l2 = 0.0, s2;
+        bool k2, v2, q2;
+        coop_tests<R>(a, ps, poss, ub0, true, lane, l2, k2, v2, q2, s2);
+        if (lane == src) {
+          lb = l2;
+          keep = k2;
+          pv = v2;
+          pk = q2;
+          su = s2;
+        }
+      }
+    } else if (need) {
+      const long long p = first + pl;
+      const int win = a.pwin[p];
+      const ChildBox x = child_box(a.plo + p * GPUGO_D, a.phi + p * GPUGO_D, a.pw + p * GPUGO_D, win, GPUGO_D, a.c,
+                                   (unsigned)a.wid, a.inv_wid, a.half_inv_wid, pos);
+      const iv<double> *rest = (const iv<double> *)(a.prest) + p * (long long)((1 + a.n_samples) * GPUGO_NREST);
       const iv<double> f = obj_value_w<R>(x, a.params, a.d, rest, win, a.c);
       lb = f.lo;
       if (!(lb == lb)) lb = -__builtin_huge_val();
-      if (lb <= ub0) {
-        keep = !obj_kkt_w<R>(x, a.params, a.dom_lo, a.dom_hi, a.d, rest, win, a.c);
-        pk = !keep;
-        // S points on the child's diagonal: window coordinates from the
-        // child, the others from the parent's same sample point (its rest)
-        for (int ts = 0; ts < S; ++ts) {
-          const DiagPoint<ChildBox> xs{x, ts, S};
-          const double u = obj_value_w<R>(xs, a.params, a.d, rest + (1 + ts) * GPUGO_NREST, win, a.c).hi;
-          su = u < su ? u : su;
+      if (lb <= ub0) stage2 = true;
+      else pv = true;
+    }
+    const unsigned long long s2_m = __ballot(stage2);
+    if (s2_m != 0ull && __popcll(s2_m) <= GPUGO_COOP_MAX) {
+      unsigned long long m = s2_m;
+      while (m != 0ull) {
+        const int src = __builtin_ctzll(m);
+        m &= m - 1ull;
+        const long long ps = first + (long long)__shfl((int)pl, src, 64);
+        const unsigned poss = (unsigned)__shfl((int)pos, src, 64);
+        double l2 = __shfl(lb, src, 64), s2;
+        bool k2, v2, q2;
+        coop_tests<R>(a, ps, poss, ub0, false, lane, l2, k2, v2, q2, s2);
+        if (lane == src) {
+          keep = k2;
+          pv = v2;
+          pk = q2;
+          su = s2;
         }
-      } else {
-        pv = true;
       }
+    } else if (stage2) {
+      const long long p = first + pl;
+      const int win = a.pwin[p];
+      const ChildBox x = child_box(a.plo + p * GPUGO_D, a.phi + p * GPUGO_D, a.pw + p * GPUGO_D, win, GPUGO_D, a.c,
+                                   (unsigned)a.wid, a.inv_wid, a.half_inv_wid, pos);
+      const int S = a.n_samples;
+      const iv<double> *rest = (const iv<double> *)(a.prest) + p * (long long)((1 + S) * GPUGO_NREST);
+      keep = !obj_kkt_w<R>(x, a.params, a.dom_lo, a.dom_hi, a.d, rest, win, a.c);
+      pk = !keep;
+      // S points on the child's diagonal: window coordinates from the
+      // child, the others from the parent's same sample point (its rest)
+      for (int ts = 0; ts < S; ++ts) {
+        const DiagPoint<ChildBox> xs{x, ts, S};
+        const double u = obj_value_w<R>(xs, a.params, a.d, rest + (1 + ts) * GPUGO_NREST, win, a.c).hi;
+        su = u < su ? u : su;
       }
     }
     const unsigned long long mask = __ballot(keep);

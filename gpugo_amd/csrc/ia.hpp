@@ -35,9 +35,10 @@
 //          0 (lower bound of a positive value) or to -/+1e-292.
 //
 // exp/sin/cos endpoints come from this file's own FP64 evaluation (tr64
-// below: Cody-Waite reduction in interval arithmetic, Taylor polynomials
-// with an a-priori error bound added outward), so they are rigorous by
-// construction; FP32 endpoints are the FP64 enclosures rounded outward.
+// below: Cody-Waite reduction with its rounding error bounded, Taylor
+// polynomials with an a-priori error bound added outward), so they are
+// rigorous by construction; FP32 endpoints are FP64 enclosures from shorter
+// polynomials (error far below an FP32 ulp) rounded outward.
 // IA.py:363-570 instead widens library values by fixed ulp counts.  sin/cos
 // find interior extrema from rigorous bounds of x/pi, not from
 // IA.py:429-444's nudged floor division.
@@ -464,11 +465,46 @@ template <int D> struct tr64 {
     return __builtin_fma(p, r, 1.0);
   }
 
+  // Kernels of the FP32 interval functions, whose endpoints are rounded
+  // outward to FP32 (unit roundoff 2^-24): Taylor to r^9 (sin), r^10 (cos),
+  // r^7 (exp).  Truncation: |r|^11/11! < 2.3e-9 |r|, |r|^12/12! < 1.2e-10,
+  // e^|r| |r|^8/8! < 7.2e-9; the rounding of these short Horner loops is
+  // below 30u.  The bounds added, EPS_*_LP, are at least three times these.
+  static constexpr double EPS_SIN_LP = 7e-9, EPS_COS_LP = 4e-10, EPS_EXP_LP = 2.2e-8;
+  __device__ static __forceinline__ double sin_poly_lp(double r) {
+    const double z = r * r;
+    double q = 2.755731922398589065256e-6;
+    q = __builtin_fma(q, z, -1.984126984126984126984e-4);
+    q = __builtin_fma(q, z, 8.333333333333333333333e-3);
+    q = __builtin_fma(q, z, -1.666666666666666666667e-1);
+    return __builtin_fma(r * z, q, r);
+  }
+  __device__ static __forceinline__ double cos_poly_lp(double r) {
+    const double z = r * r;
+    double c = -2.755731922398589065256e-7;
+    c = __builtin_fma(c, z, 2.480158730158730158730e-5);
+    c = __builtin_fma(c, z, -1.388888888888888888889e-3);
+    c = __builtin_fma(c, z, 4.166666666666666666667e-2);
+    c = __builtin_fma(c, z, -0.5);
+    return __builtin_fma(z, c, 1.0);
+  }
+  __device__ static __forceinline__ double exp_poly_lp(double r) {
+    double p = 1.984126984126984126984e-4;
+    p = __builtin_fma(p, r, 1.388888888888888888889e-3);
+    p = __builtin_fma(p, r, 8.333333333333333333333e-3);
+    p = __builtin_fma(p, r, 4.166666666666666666667e-2);
+    p = __builtin_fma(p, r, 1.666666666666666666667e-1);
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    return __builtin_fma(p, r, 1.0);
+  }
+
   typedef rnd<double, ROUND_FAST> RF;  // one outward step after a nearest result
   static constexpr double U = 0x1p-53;
 
-  // sin(x) (shift 0) or cos(x) (shift 1) for |x| <= 2^40
-  __device__ static iv<double> sincos(double x, int shift) {
+  // sin(x) (shift 0) or cos(x) (shift 1) for |x| <= 2^40; LP: the short
+  // kernels (for endpoints that are rounded to FP32)
+  template <bool LP = false> __device__ static iv<double> sincos(double x, int shift) {
     const double k = __builtin_rint(x * TWO_OVER_PI);
     const double ak = __builtin_fabs(k);
     double r0, ered;
@@ -494,8 +530,9 @@ template <int D> struct tr64 {
     const int q = ((int)((long long)k & 3) + shift) & 3;
     // q = 0: sin r, 1: cos r, 2: -sin r, 3: -cos r; |sin'|, |cos'| <= 1
     const bool use_sin = (q & 1) == 0;
-    const double v = use_sin ? sin_poly(r0) : cos_poly(r0);
-    const double e = (use_sin ? __builtin_fma(__builtin_fabs(r0), EPS_SIN, 0x1p-1074) : EPS_COS) + ered * (1.0 + 4 * U);
+    const double v = use_sin ? (LP ? sin_poly_lp(r0) : sin_poly(r0)) : (LP ? cos_poly_lp(r0) : cos_poly(r0));
+    const double es = LP ? EPS_SIN_LP : EPS_SIN, ec = LP ? EPS_COS_LP : EPS_COS;
+    const double e = (use_sin ? __builtin_fma(__builtin_fabs(r0), es, 0x1p-1074) : ec) + ered * (1.0 + 4 * U);
     double lo = RF::sub_dn(v, e), hi = RF::add_up(v, e);
     if (q & 2) {
       const double t = lo;
@@ -505,7 +542,7 @@ template <int D> struct tr64 {
     return mk<double>(lo, hi);
   }
 
-  __device__ static iv<double> exp(double x) {
+  template <bool LP = false> __device__ static iv<double> exp(double x) {
     if (!(x == x)) return mk<double>(x, x);
     if (x > 709.8) return mk<double>(fp<double>::MAXV, __builtin_huge_val());  // e^x > MAXV
     if (x < -745.2) return mk<double>(0.0, 0x1p-1074);                          // 0 < e^x < 2^-1074
@@ -515,9 +552,9 @@ template <int D> struct tr64 {
     const double t = __builtin_fma(-k, LN2_1, x);
     const double r = __builtin_fma(-k, LN2_2_LO, t);
     const double ered = __builtin_fma(U, __builtin_fabs(t) + __builtin_fabs(r), __builtin_fabs(k) * 0x1p-84);
-    const double v = exp_poly(r);
+    const double v = LP ? exp_poly_lp(r) : exp_poly(r);
     // (e^r)' < 1.5 on |r| <= ln(2)/2 + 2^-40
-    const double e = EPS_EXP + 1.5 * ered * (1.0 + 4 * U);
+    const double e = (LP ? EPS_EXP_LP : EPS_EXP) + 1.5 * ered * (1.0 + 4 * U);
     double lo = RF::sub_dn(v, e), hi = RF::add_up(v, e);
     // times 2^k: exact in the normal range; outward by one step otherwise
     const int ki = (int)k;
@@ -539,6 +576,8 @@ template <typename T, int R> struct IA {
   // differences are exact or nearly so; a FAST step would widen them by an
   // ulp of x)
   typedef tr64<ROUND_EXACT> TR;
+  // FP32 intervals take the short kernels: their error is far below an FP32 ulp
+  static constexpr bool LP = sizeof(T) == 4;
 
   __device__ static __forceinline__ V add(V a, V b) { return mk<T>(RR::add_dn(a.lo, b.lo), RR::add_up(a.hi, b.hi)); }
   __device__ static __forceinline__ V sub(V a, V b) { return mk<T>(RR::sub_dn(a.lo, b.hi), RR::sub_up(a.hi, b.lo)); }
@@ -643,7 +682,7 @@ template <typename T, int R> struct IA {
 
   // out of line: keeps unrolled code lists (and hipRTC compile time) small
   __device__ __attribute__((noinline)) static V exp(V a) {
-    const iv<double> el = TR::exp((double)a.lo), eh = TR::exp((double)a.hi);
+    const iv<double> el = TR::template exp<LP>((double)a.lo), eh = TR::template exp<LP>((double)a.hi);
     T l = max_<T>(out_dn<RR>(to_dn<T>(el.lo)), T(0));
     return mk<T>(l, out_up<RR>(to_up<T>(eh.hi)));
   }
@@ -655,16 +694,18 @@ template <typename T, int R> struct IA {
     if (!(fp<T>::abs_(a.lo) <= fp<T>::TRIG_BIG && fp<T>::abs_(a.hi) <= fp<T>::TRIG_BIG))
       return mk<T>(-one, one);
     const T w = RR::sub_up(a.hi, a.lo);
-    if (w <= fp<T>::TRIG_NARROW) {
-      // narrow argument (sample points): |f(x) - f(a)| <= |x - a| <= w since
-      // |sin'|, |cos'| <= 1, so one evaluation and a widening by w suffice
-      const iv<double> v = TR::sincos((double)a.lo, 1 - SHIFT);
+    if (w <= fp<T>::TRIG_NARROW * max_<T>(one, fp<T>::abs_(a.lo))) {
+      // narrow argument (sample points, a few ulps of a large argument):
+      // |f(x) - f(a)| <= |x - a| <= w since |sin'|, |cos'| <= 1, so one
+      // evaluation and a widening by w suffice
+      const iv<double> v = TR::template sincos<LP>((double)a.lo, 1 - SHIFT);
       T l = max_<T>(RR::sub_dn(to_dn<T>(v.lo), w), -one);
       T h = min_<T>(RR::add_up(to_up<T>(v.hi), w), one);
       return mk<T>(l, h);
     }
     if (w >= fp<T>::TWO_PI_LO) return mk<T>(-one, one);
-    const iv<double> va = TR::sincos((double)a.lo, 1 - SHIFT), vb = TR::sincos((double)a.hi, 1 - SHIFT);
+    const iv<double> va = TR::template sincos<LP>((double)a.lo, 1 - SHIFT),
+                     vb = TR::template sincos<LP>((double)a.hi, 1 - SHIFT);
     T l = max_<T>(min_<T>(to_dn<T>(va.lo), to_dn<T>(vb.lo)), -one);
     T h = min_<T>(max_<T>(to_up<T>(va.hi), to_up<T>(vb.hi)), one);
     T ta = a.lo >= T(0) ? RR::mul_dn(a.lo, fp<T>::INV_PI_LO) : RR::mul_dn(a.lo, fp<T>::INV_PI_HI);

@@ -519,6 +519,28 @@ class _FuncW(_FuncG):
         return ("i", acc)
 
 
+class _FuncWP(_FuncW):
+    """Cooperative windowed lowering: the G lanes of a group share one child;
+    a template reduction splits its window terms over the lanes (term q of
+    the window ranges on lane q mod G), folds them with a butterfly and
+    then with the parent's rest reduction."""
+
+    def loop_sum(self, e: Expr) -> Tuple[str, str]:
+        s = self.slots[e.key()]
+        acc = f"t{self.n}"
+        self.n += 1
+        prod = e.op == "tprod"
+        fn = "mul" if prod else "add"
+        self.lines.append(f"  iv<double> {acc} = mk<double>({int(prod)}, {int(prod)});")
+        self.lines.append(f"  {{ int rlo[2], rhi[2]; const int nr = wranges_{s}(win, c, d, rlo, rhi);")
+        self.lines.append("    const int n0 = nr > 0 ? rhi[0] - rlo[0] : 0, nq = n0 + (nr > 1 ? rhi[1] - rlo[1] : 0);")
+        self.lines.append("    for (int q = lg; q < nq; q += G) {")
+        self.lines.append("      const int j = q < n0 ? rlo[0] + q : rlo[1] + (q - n0);")
+        self.lines.append(f"      {acc} = A::{fn}({acc}, wterm_{s}<R>(x, P, d, j)); }} }}")
+        self.lines.append(f"  {acc} = A::{fn}(rest[{s}], group_{'prod' if prod else 'sum'}<R, G>({acc}));")
+        return ("i", acc)
+
+
 def window_lists_f32(obj: Objective) -> str:
     """FP32 windowed value code list of the FP32-prune form, after
     window_lists (it uses its wranges_s):
@@ -665,8 +687,55 @@ def window_lists(obj: Objective) -> str:
     out += f.lines
     out.append("  return false;")
     out.append("}")
+    out += _window_lists_coop(value, grad, slots, global_grad)
     out.append("}  // namespace gpugo")
     return "\n".join(out) + "\n"
+
+
+def _window_lists_coop(value: Expr, grad: Expr, slots: Dict[tuple, int], global_grad: bool) -> List[str]:
+    """Cooperative value and monotonicity lists: the G lanes of a group
+    (lane lg of G) evaluate one child together; every lane returns the same
+    value / flag.  Used for the few children of a wavefront that reach the
+    FP64 tests, whose one-lane evaluation would be the step's critical path.
+      obj_value_wp<R, G>(x, P, d, rest, win, c, lg)
+      obj_kkt_wp<R, G>(x, P, dlo, dhi, d, rest, win, c, lg)"""
+    out = []
+    f = _FuncWP(slots)
+    v = f.lower(value)
+    out.append("template <int R, int G, class XS>\n__device__ __forceinline__ iv<double> obj_value_wp("
+               "const XS &x, const double *__restrict__ P, int d, const iv<double> *__restrict__ rest, "
+               "int win, int c, int lg) {")
+    out.append("  typedef IA<double, R> A;")
+    out += f.lines
+    out.append(f"  return {f.as_iv(v)};")
+    out.append("}")
+    f = _FuncWP(slots)
+    f.hoist(grad)
+    f.lines.append("  bool prune = false;")
+    if global_grad:
+        f.lines.append("  for (int j = lg; j < d; j += G) {")
+    else:
+        gklo, gkhi = _x_offsets(grad, "gradient template")
+        f.lines.append(f"  int glo[2], ghi[2]; const int ng = win_ranges(win, c, d, {gklo}, {gkhi}, d, glo, ghi);")
+        f.lines.append("  const int g0 = ng > 0 ? ghi[0] - glo[0] : 0, gq = g0 + (ng > 1 ? ghi[1] - glo[1] : 0);")
+        f.lines.append("  for (int q = lg; q < gq; q += G) {")
+        f.lines.append("    const int j = q < g0 ? glo[0] + q : glo[1] + (q - g0);")
+    saved = dict(f.memo)
+    f.in_loop = True
+    code = f.as_iv(f.lower(grad))
+    f.in_loop = False
+    f.memo = saved
+    f.lines += [f"    const iv<double> gj = {code};",
+                "    prune |= (gj.lo > 0.0 && x[j].lo != dlo[j]) || (gj.hi < 0.0 && x[j].hi != dhi[j]);",
+                "  }"]
+    out.append("template <int R, int G, class XS>\n__device__ __forceinline__ bool obj_kkt_wp("
+               "const XS &x, const double *__restrict__ P, const double *__restrict__ dlo, "
+               "const double *__restrict__ dhi, int d, const iv<double> *__restrict__ rest, int win, int c, int lg) {")
+    out.append("  typedef IA<double, R> A;")
+    out += f.lines
+    out.append("  return group_any<G>(prune);")
+    out.append("}")
+    return out
 
 
 def window_module_source(obj: Objective, roundings=("fast",), f32: bool = False) -> str:
@@ -687,6 +756,9 @@ def window_module_source(obj: Objective, roundings=("fast",), f32: bool = False)
         f"#define GPUGO_WITH_DIGITS15 {int('digits15' in roundings)}",
         f"#define GPUGO_WITH_F32 {int(bool(f32))}",
         "#define GPUGO_WINDOWED 1",
+        # a child's coordinates are read only inside its window (c of them):
+        # let the compiler keep the parent loads instead of redoing them
+        "#define GPUGO_ACCESSOR_D 0",
         _read("ia.hpp"), _read("bnb_args.hpp"), _read("bnb_box.hpp"),
         window_lists(obj),
     ]

@@ -86,9 +86,10 @@ def test_transcendentals_contain_truth(gpu, oracle, fn, rounding):
         else:
             tlo, thi = trig_range(lo_in[i], hi_in[i], fn)
         assert mp.mpf(got[0][i]) <= tlo and mp.mpf(got[1][i]) >= thi, (i, lo_in[i], hi_in[i])
-        # narrow sin/cos arguments (<= 2^-40) may add their width (Lipschitz path)
+        # narrow sin/cos arguments (width <= 2^-40 max(1, |lo|)) may add their
+        # width (one evaluation widened by it: |sin'|, |cos'| <= 1)
         w = hi_in[i] - lo_in[i]
-        extra = 2 * w if (fn != "exp" and w <= 2.0 ** -40) else 0.0
+        extra = 2 * w if (fn != "exp" and w <= 2.0 ** -40 * max(1.0, abs(lo_in[i]))) else 0.0
         # width: the a-priori bounds of the FP64 evaluation (sin: 1e-15 |r|,
         # cos: 2e-15, exp: 8e-15 before the 2^k scaling) plus outward steps
         slack = (TRANS_ABS[fn] * max(abs(float(tlo)), abs(float(thi)), 1e-300 if fn == "exp" else 1.0)
@@ -166,11 +167,11 @@ def test_exp_adversarial(gpu, rounding):
             assert got[1][i] - got[0][i] <= 2 * TRANS_ABS["exp"] * float(t) + 4 * math.ulp(float(t)), (i, x)
 
 
-@pytest.mark.parametrize("op", ["add", "mul", "div", "sqr", "sqrt", "scale", "exp", "cos"])
+@pytest.mark.parametrize("op", ["add", "mul", "div", "sqr", "sqrt", "scale", "exp", "cos", "sin"])
 def test_fp32_contains(gpu, oracle, op):
     rng = np.random.default_rng(9)
     alo, ahi, blo, bhi, s = inputs(op, rng)
-    if op in ("exp", "cos"):
+    if op in ("exp", "cos", "sin"):
         c = rng.uniform(-20, 20, alo.size)
         alo, ahi = c - 0.01, c + 0.02
     ok = (np.abs(alo) < 1e30) & (np.abs(ahi) < 1e30) & (np.abs(alo) > 1e-30) & (np.abs(blo) > 1e-30) \
@@ -178,15 +179,71 @@ def test_fp32_contains(gpu, oracle, op):
     alo, ahi, blo, bhi = alo[ok], ahi[ok], blo[ok], bhi[ok]
     for rounding in ("exact", "fast"):
         got = gpu.ia_eval(op, alo, ahi, blo, bhi, s, prec="fp32", rounding=rounding)
-        if op in ("exp", "cos"):
+        if op in ("exp", "cos", "sin"):
             for i in range(alo.size):
                 tlo, thi = ((mp.exp(mp.mpf(alo[i])), mp.exp(mp.mpf(ahi[i]))) if op == "exp"
-                            else trig_range(alo[i], ahi[i], "cos"))
+                            else trig_range(alo[i], ahi[i], op))
                 assert mp.mpf(got[0][i]) <= tlo and mp.mpf(got[1][i]) >= thi
         else:
             want = oracle.ia_eval(OPC[op], alo, ahi, blo, bhi, s)
             fin = np.isfinite(want[0]) & np.isfinite(want[1])
             assert np.all(got[0][fin] <= want[0][fin]) and np.all(got[1][fin] >= want[1][fin])
+
+
+@pytest.mark.parametrize("fn", ["sin", "cos", "exp"])
+@pytest.mark.parametrize("rounding", ["exact", "fast"])
+def test_fp32_transcendentals_adversarial(gpu, fn, rounding):
+    """FP32 intervals (their endpoints come from the short FP64 kernels,
+    rounded outward): FP32 arguments next to k pi/2 for k up to 2^18 (the
+    FP32 trig range is |x| <= 2^20), next to k ln 2, near the FP32 exp
+    overflow (88.72) and underflow (-87.3 normal, -103.97 subnormal) edges,
+    tiny and zero arguments; points and intervals a few ulps wide (the narrow
+    path: one evaluation widened by the width) and 1e-3 wide.  Each
+    enclosure contains the 300-bit range, and a point's is at most six FP32
+    ulps of the value wide (plus 8 * 2^-126: FAST steps a zero bound by about the
+    smallest normal): the rounding of the FP64 enclosure to FP32 and, in FAST
+    rounding, two outward steps per end."""
+    mp.mp.prec = 300
+    f32 = lambda v: float(np.float32(v))
+    xs = []
+    if fn == "exp":
+        for c in [88.72, 88.7228, 88.73, 89.0, -87.3, -87.34, -100.0, -103.9, -103.98, -104.0, -110.0, 0.0,
+                  1e-30, -1e-30, 0.34657, -0.34657, 20.5, -33.25]:
+            xs.append(f32(c))
+        for k in (1, 3, 10, 50, 100, 127, -1, -10, -100, -126, -149):
+            c = f32(mp.mpf(k) * mp.log(2))
+            xs += [c, float(np.nextafter(np.float32(c), np.float32(np.inf))),
+                   float(np.nextafter(np.float32(c), np.float32(-np.inf)))]
+    else:
+        for k in (1, 2, 3, 7, 100, 1023, 2 ** 12 + 1, 2 ** 15 - 1, 2 ** 18 + 3):
+            for sgn in (1, -1):
+                c = np.float32(sgn * float(mp.mpf(k) * mp.pi / 2))
+                xs += [float(c), float(np.nextafter(c, np.float32(np.inf))), float(np.nextafter(c, np.float32(-np.inf)))]
+        xs += [0.0, -0.0, 1e-40, -1e-38, 1e-8, f32(0.785398), f32(-0.7854), 1000.25, -65536.5]
+    xs = np.array(xs)
+    tf = {"sin": mp.sin, "cos": mp.cos, "exp": mp.exp}[fn]
+    for width in (0.0, 4.0, 1e-3):
+        if width == 4.0:   # a few FP32 ulps of the argument
+            lo = np.array([float(np.nextafter(np.nextafter(np.float32(x), np.float32(-np.inf)), np.float32(-np.inf)))
+                           for x in xs])
+            hi = np.array([float(np.nextafter(np.nextafter(np.float32(x), np.float32(np.inf)), np.float32(np.inf)))
+                           for x in xs])
+        else:
+            lo, hi = np.array([f32(x - width) for x in xs]), np.array([f32(x + width) for x in xs])
+            lo, hi = np.minimum(lo, xs), np.maximum(hi, xs)
+        got = gpu.ia_eval(fn, lo, hi, prec="fp32", rounding=rounding)
+        for i in range(len(xs)):
+            if fn == "exp":
+                tlo, thi = mp.exp(mp.mpf(lo[i])), mp.exp(mp.mpf(hi[i]))
+            elif lo[i] == hi[i]:
+                tlo = thi = tf(mp.mpf(lo[i]))
+            else:
+                tlo, thi = trig_range(lo[i], hi[i], fn)
+            assert mp.mpf(got[0][i]) <= tlo and thi <= mp.mpf(got[1][i]), (fn, width, i, lo[i], hi[i], got[0][i], got[1][i])
+            if width == 0.0 and not (fn == "exp" and (lo[i] > 88.7 or lo[i] < -87.3)):
+                t = float(tlo)
+                ulp = float(np.spacing(np.float32(abs(t)))) if t != 0 else 0.0
+                assert got[1][i] - got[0][i] <= 6 * ulp + 8 * 2.0 ** -126, (fn, i, lo[i], got[0][i], got[1][i], t)
 
 
 def test_domain_errors(gpu):

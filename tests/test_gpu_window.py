@@ -306,6 +306,7 @@ def test_window_d50_certifies(gpu, name):
 
 @pytest.mark.parametrize("name,d", [("levy", 1000), ("rosenbrock", 1000), ("fu", 1000), ("rastrigin", 1000),
                                     ("ackley", 1000), ("belegundu", 1000), ("breiman", 1000), ("salomon", 1000),
+                                    ("griewank", 1000), ("styblinski", 1000), ("zabinsky", 1000),
                                     ("levy", 2000), ("levy", 5000), ("levy", 10000)])
 def test_window_large_d_certifies(gpu, name, d):
     r, s = ref_solve(gpu, name, d, REF_GRID.get(name, (3, 12)), 1, 10, wmod(gpu, F.objective_loops(name, d)), True,

@@ -99,3 +99,41 @@ def test_error_bounds_cover_the_analysis():
     # truncation terms are far below the bounds
     assert r_sin ** 23 / math.factorial(23) < 1e-24 and r_sin ** 24 / math.factorial(24) < 1e-25
     assert math.exp(r_exp) * r_exp ** 14 / math.factorial(14) < 1e-17
+
+
+@pytest.mark.parametrize("fn,expect", [
+    ("sin_poly_lp", [F((-1) ** i, math.factorial(2 * i + 1)) for i in range(4, 0, -1)]),
+    ("cos_poly_lp", [F((-1) ** i, math.factorial(2 * i)) for i in range(5, 0, -1)]),
+    ("exp_poly_lp", [F(1, math.factorial(i)) for i in range(7, 0, -1)]),
+])
+def test_short_kernel_coefficients_are_nearest(fn, expect):
+    """The kernels of the FP32 interval functions (endpoints rounded to FP32
+    afterwards): the same Taylor series, cut shorter."""
+    got = coeffs(fn)[:len(expect)]
+    assert len(got) == len(expect)
+    for g, e in zip(got, expect):
+        assert g == float(e), (fn, g, e)
+
+
+def test_short_kernel_bounds_cover_the_analysis():
+    u = 2.0 ** -53
+    gamma = lambda n: n * u / (1 - n * u)
+    r_sin = math.pi / 4 + 2.0 ** -11
+    z = r_sin * r_sin
+    # sin to r^9: truncation |r|^11/11! relative to |r|, plus rounding
+    trunc = r_sin ** 10 / math.factorial(11)
+    zq = sum(z ** i / math.factorial(2 * i + 1) for i in range(1, 5))
+    assert 3 * (trunc + zq * (gamma(3) + 4 * u) + u) <= const("EPS_SIN_LP")
+    # cos to r^10
+    trunc = r_sin ** 12 / math.factorial(12)
+    cq = sum(z ** i / math.factorial(2 * i) for i in range(1, 6))
+    assert 3 * (trunc + cq * (gamma(4) + 4 * u) + u) <= const("EPS_COS_LP")
+    # exp to r^7 (bound on e^r, which lies in [0.70, 1.42])
+    r_exp = math.log(2) / 2 + 2.0 ** -40
+    trunc = math.exp(r_exp) * r_exp ** 8 / math.factorial(8)
+    sa = sum(r_exp ** i / math.factorial(i) for i in range(8))
+    assert 3 * (trunc + sa * (gamma(7) + 2 * u)) <= const("EPS_EXP_LP")
+    # far below an FP32 ulp (2^-24 relative), so FP32 endpoints lose at most one ulp
+    assert const("EPS_SIN_LP") < 2.0 ** -24 / 8 and const("EPS_EXP_LP") < 0.7 * 2.0 ** -24
+    # teeth: one term fewer would not be covered
+    assert 3 * r_sin ** 8 / math.factorial(9) > const("EPS_SIN_LP")
