@@ -136,7 +136,7 @@ def cpu_model():
 # GPU solves
 # ---------------------------------------------------------------------------
 def roofline(obj, results, *, precision="fp64", mean_value=False, n_samples=1, kernel="gpugo_bound_fast",
-             windowed_c=0):
+             windowed_c=0, wid=3, tables=True):
     """Algorithmic FLOP of the split+bound launches over their HIP-event
     time (per-op weights add/sub/scalar 2, sqr 2, pown(k) 2 ceil(log2 k),
     interval product 8, division 10, sqrt 2, exp/sin/cos 40, counted by
@@ -147,9 +147,13 @@ def roofline(obj, results, *, precision="fp64", mean_value=False, n_samples=1, k
     interval products and additions).  In the FP32-prune form the FP32
     value list is charged for every child and the FP64 lists for the value
     survivors only.  frac is therefore an upper bound of the VALU work
-    actually done over the peak."""
+    actually done over the peak.  Windowed modules read their window terms
+    from the step's term tables: a child is charged the table folds and the
+    non-sum part only (the tables are filled once per split record by the
+    prep kernel, whose FLOP are reported as flop_table_per_record)."""
     from gpugo_amd import emit
-    fl = emit.window_flop_counts(obj, windowed_c) if windowed_c else emit.flop_counts(obj)
+    fl = (emit.window_flop_counts(obj, windowed_c, tables=tables, wid=wid, n_samples=n_samples) if windowed_c
+          else emit.flop_counts(obj))
     boxes = sum(x["boxes_bounded"] for x in results)
     pv = sum(x["pruned_value"] for x in results)
     bound_ms = sum(x["bound_ms"] for x in results)
@@ -167,6 +171,7 @@ def roofline(obj, results, *, precision="fp64", mean_value=False, n_samples=1, k
             "flop_per_launch": flops / max(launches, 1), "flop_per_child": flops / max(boxes, 1),
             "children_per_launch": boxes / max(launches, 1),
             "avg_launch_ms": bound_ms / max(launches, 1), "launches": launches,
+            **({"term_tables": bool(tables), "flop_table_per_record": fl.get("table", 0)} if windowed_c else {}),
             "timing": "HIP events from each split+bound dispatch, second pass of the same solves"}
 
 
@@ -310,7 +315,7 @@ def form_solves(G, pb, args, spec, grid, k, form, *, n_samples=1, mean_value=Fal
     kname = "gpugo_bound_fast" + ("_mv" if mean_value else "") + ("_g" if split else "")
     g["roofline"] = roofline(obj, g["_timed"], precision=prec, mean_value=mean_value, n_samples=n_samples,
                              kernel=kname + (" (windowed)" if windowed else ""),
-                             windowed_c=min(grid[1], spec.d) if windowed else 0)
+                             windowed_c=min(grid[1], spec.d) if windowed else 0, wid=grid[0])
     return g
 
 
@@ -412,7 +417,7 @@ def leg_large_d(G, cases=LARGE_D, reps=2, cpu_seconds=2.0, max_queue=1 << 28, wa
         p.close()
         r = res[-1]
         t = statistics.median(times)
-        rl = roofline(obj, res, n_samples=S, kernel="gpugo_bound_fast (windowed)", windowed_c=grid[1])
+        rl = roofline(obj, res, n_samples=S, kernel="gpugo_bound_fast (windowed)", windowed_c=grid[1], wid=grid[0])
         leg = {"time_to_certified_s": t, "certified": all(x["status_name"] == "certified" for x in res),
                "lb": r["lb"], "ub": r["ub"], "width": r["width"], "f_star": s.f_star,
                "contains_f_star": r["lb"] <= s.f_star <= r["ub"], "iterations": r["iterations"],

@@ -27,6 +27,7 @@ FLAG_MEAN_VALUE = 1
 FLAG_SPLIT = 4
 FLAG_PERSIST = 8
 FLAG_EAGER_COMPACT = 16
+FLAG_NO_TABLES = 32
 STATUS = {0: "running", 1: "certified", 2: "x_converged", 3: "max_iters", 4: "exhausted", 5: "overflow"}
 
 EXPORTS = [
@@ -45,6 +46,11 @@ class GpugoError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"gpugo error {_ERR_NAMES.get(code, code)}: {msg}")
         self.code = code
+        self.msg = msg
+
+    def __reduce__(self):
+        # picklable, so an error raised in a worker process reaches its parent
+        return (GpugoError, (self.code, self.msg))
 
 
 class ProblemDesc(ctypes.Structure):

@@ -24,7 +24,7 @@ enum gpugo_status {
 
 // layout version of the structs below; modules report it and the library
 // refuses a module built against another layout
-#define GPUGO_MODULE_ABI_V 6
+#define GPUGO_MODULE_ABI_V 7
 
 namespace gpugo {
 
@@ -80,6 +80,12 @@ struct BoundArgs {
   const void *params32;          // coefficients as outward FP32 intervals (iv<float>)
   const double *prest;           // windowed modules: rest sums per parent record, [p][2][nrest] intervals
   int nrest, pad1;
+  // windowed modules: term tables of the records split in this step (record
+  // first_parent + b at [b][tab_n] / [b][tab_n32]), written by the step's
+  // prep kernel; tab_n == 0: no tables, terms evaluated per child
+  void *ptab;                    // iv<double>
+  void *ptab32;                  // iv<float> (FP32 pruning pass)
+  int tab_n, tab_n32;
 };
 
 // BoundArgs.flags bit of the FP32 pruning pass (problem precision F32)

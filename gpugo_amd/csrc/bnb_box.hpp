@@ -127,6 +127,53 @@ struct RowBox {
   __device__ __forceinline__ iv<double> operator[](int j) const { return mk<double>(lo[j], hi[j]); }
 };
 
+// Term tables of the windowed modules.  A term whose coordinates are
+// base, base + 1, ..., base + W - 1 takes, over a child, one value per
+// combination of the digits of those coordinates that lie in the split
+// window (the others are the parent's).  Entry index = sum of digit_k *
+// wid^k, with digit 0 for a coordinate outside the window or outside [0, d).
+template <int W>
+__device__ __forceinline__ unsigned tab_idx(const ChildBox &x, int base) {
+  unsigned idx = 0, m = 1;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const int i = base + k;
+    int kk = i - x.win;
+    if (kk < 0) kk += x.d;
+    if (i >= 0 && i < x.d && kk < x.c) idx += (unsigned)((x.packed >> (kk * x.bpd)) & x.mask) * m;
+    m *= x.wid;
+  }
+  return idx;
+}
+
+// The box of one table entry as a coordinate accessor: coordinate j of the
+// stencil (base <= j) in the window takes digit (combo / wid^(j - base)) mod
+// wid, with ChildBox's arithmetic (so a term over it is bit-identical to
+// the same term over every child with those digits); the others are the
+// parent's.
+struct TabBox {
+  const double *__restrict__ plo;
+  const double *__restrict__ phi;
+  const double *__restrict__ pw;
+  int win, d, c, base;
+  unsigned wid, combo;
+  __device__ __forceinline__ iv<double> operator[](int j) const {
+    const double L = plo[j], U = phi[j];
+    int k = j - win;
+    if (k < 0) k += d;
+    if (k < c) {
+      unsigned r = combo;
+      for (int t = base; t < j; ++t) r /= wid;
+      const unsigned dg = r % wid;
+      const double w = pw[j];
+      const double lo = __builtin_fma(w, (double)dg, L);
+      const double hi = (dg + 1u == wid) ? U : __builtin_fma(w, (double)(dg + 1u), L);
+      return mk<double>(lo, hi);
+    }
+    return mk<double>(L, U);
+  }
+};
+
 __device__ __forceinline__ ChildBox child_box(const double *plo, const double *phi, const double *pw, int win, int d,
                                               int c, unsigned wid, double inv_wid, double half_inv_wid, unsigned pos) {
   ChildBox x;

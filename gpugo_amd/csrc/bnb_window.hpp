@@ -44,6 +44,143 @@ __device__ __forceinline__ bool f32_prunes_w(const BoundArgs &a, const ChildBox 
 #endif
 }
 
+#if GPUGO_TAB
+// Entries [blk * 256, blk * 256 + 256) of the term tables of record
+// first_parent + b (layout tab_layout): each thread evaluates one term, or
+// one gradient component, over the TabBox of its digit combination, at the
+// record's box or at one of its diagonal sample points, in FP64 and, for
+// the FP32 pass, in FP32.  Entries of terms the window does not reach, or
+// of digit combinations no child has, are not written: no child reads them.
+template <int R>
+__device__ __forceinline__ void tab_build(const BoundArgs &a, int b, int blk) {
+  if (a.tab_n == 0) return;
+  const int e = blk * (int)blockDim.x + (int)threadIdx.x;
+  const int S = a.n_samples, d = a.d, c = a.c;
+  const unsigned wid = (unsigned)a.wid;
+  const TabLayout L = tab_layout(c, wid, S);
+  if (e >= L.total) return;
+  const long long p = a.st->first_parent + b;
+  const int win = a.pwin[p];
+  const int NT = L.off[GPUGO_NREST];
+  int s, q, pt, w, klo, nr, rlo[2], rhi[2];
+  unsigned combo;
+  if (e < L.goff) {
+    pt = e / NT;
+    const int e2 = e - pt * NT;
+    s = 0;
+    while (s + 1 < GPUGO_NREST && e2 >= L.off[s + 1]) ++s;
+    q = (e2 - L.off[s]) / (int)L.E[s];
+    combo = (unsigned)(e2 - L.off[s]) - (unsigned)q * L.E[s];
+    w = tab_w(s);
+    klo = tab_klo(s);
+    nr = wranges_any(s, win, c, d, rlo, rhi);
+  } else {
+    pt = 0;
+    s = -1;
+    const int e2 = e - L.goff;
+    q = e2 / (int)L.gE;
+    combo = (unsigned)e2 - (unsigned)q * L.gE;
+    w = GPUGO_TAB_GW;
+    klo = GPUGO_TAB_GKLO;
+    nr = win_ranges(win, c, d, GPUGO_TAB_GKLO, GPUGO_TAB_GKLO + GPUGO_TAB_GW - 1, d, rlo, rhi);
+  }
+  // term (component) j: the q-th of the window ranges, in the order the
+  // code lists walk them
+  const int n0 = nr > 0 ? rhi[0] - rlo[0] : 0, nq = n0 + (nr > 1 ? rhi[1] - rlo[1] : 0);
+  if (q >= nq) return;
+  const int j = q < n0 ? rlo[0] + q : rlo[1] + (q - n0);
+  unsigned r = combo;
+  for (int k = 0; k < w; ++k) {
+    const int i = j + klo + k;
+    int kk = i - win;
+    if (kk < 0) kk += d;
+    if (!(i >= 0 && i < d && kk < c) && (r % wid) != 0u) return;
+    r /= wid;
+  }
+  const TabBox tb{a.plo + p * GPUGO_D, a.phi + p * GPUGO_D, a.pw + p * GPUGO_D, win, d, c, j + klo, wid, combo};
+  iv<double> *T = (iv<double> *)a.ptab + (long long)b * a.tab_n;
+  if (s < 0) {
+#if GPUGO_TAB_GRAD
+    T[e] = wgrad_t<R>(tb, a.params, d, j);
+#endif
+    return;
+  }
+  if (pt == 0) {
+    T[e] = wterm_any<R>(s, tb, a.params, d, j);
+  } else {
+    const DiagPoint<TabBox> xs{tb, pt - 1, S};
+    T[e] = wterm_any<R>(s, xs, a.params, d, j);
+  }
+#if GPUGO_WITH_F32
+  if (a.tab_n32 > 0) {
+    iv<float> *T32 = (iv<float> *)a.ptab32 + (long long)b * a.tab_n32;
+    const iv<float> *P32 = (const iv<float> *)a.params32;
+    if (pt == 0) {
+      T32[e] = wterm32_any<R>(s, F32Box<TabBox>{tb}, P32, d, j);
+    } else {
+      const DiagPoint<TabBox> xs{tb, pt - 1, S};
+      T32[e] = wterm32_any<R>(s, F32Box<DiagPoint<TabBox>>{xs}, P32, d, j);
+    }
+  }
+#endif
+}
+#endif
+
+#if !GPUGO_TAB
+// objectives whose stencils are too wide for term tables
+struct TabLayout {};
+__device__ __forceinline__ TabLayout tab_layout(int, unsigned, int) { return TabLayout{}; }
+#endif
+
+// The tests of one child (record first_parent + b) from the term tables
+// when the launch has them (tab_n > 0), else from the terms themselves;
+// both give the same bits.  pt: 0 the child, 1 + t its diagonal sample t;
+// rest_pt: the record's rest sums at that point.
+template <int R>
+__device__ __forceinline__ iv<double> value_wx(const BoundArgs &a, const ChildBox &x, long long b, int pt,
+                                               const iv<double> *rest_pt, int win, const TabLayout &L) {
+#if GPUGO_TAB
+  if (a.tab_n > 0) {
+    const iv<double> *T = (const iv<double> *)a.ptab + b * a.tab_n + (long long)pt * L.off[GPUGO_NREST];
+    if (pt == 0) return obj_value_wt<R>(x, x, T, L, a.params, a.d, rest_pt, win, a.c);
+    const DiagPoint<ChildBox> xs{x, pt - 1, a.n_samples};
+    return obj_value_wt<R>(xs, x, T, L, a.params, a.d, rest_pt, win, a.c);
+  }
+#endif
+  if (pt == 0) return obj_value_w<R>(x, a.params, a.d, rest_pt, win, a.c);
+  const DiagPoint<ChildBox> xs{x, pt - 1, a.n_samples};
+  return obj_value_w<R>(xs, a.params, a.d, rest_pt, win, a.c);
+}
+
+template <int R>
+__device__ __forceinline__ bool kkt_wx(const BoundArgs &a, const ChildBox &x, long long b, const iv<double> *rest,
+                                       int win, const TabLayout &L) {
+#if GPUGO_TAB && GPUGO_TAB_GRAD
+  if (a.tab_n > 0)
+    return obj_kkt_wt<R>(x, (const iv<double> *)a.ptab + b * a.tab_n + L.goff, L, a.dom_lo, a.dom_hi, a.d, win, a.c);
+#elif GPUGO_TAB
+  if (a.tab_n > 0)
+    return obj_kkt_wr<R>(x, (const iv<double> *)a.ptab + b * a.tab_n, L, a.params, a.dom_lo, a.dom_hi, a.d, rest,
+                         win, a.c);
+#endif
+  return obj_kkt_w<R>(x, a.params, a.dom_lo, a.dom_hi, a.d, rest, win, a.c);
+}
+
+template <int R>
+__device__ __forceinline__ bool f32_prunes_wx(const BoundArgs &a, const ChildBox &x, long long b,
+                                              const iv<double> *rest, int win, double ub0, const TabLayout &L) {
+#if GPUGO_WITH_F32 && GPUGO_TAB
+  if (a.tab_n32 > 0) {
+    if (!(a.flags & GPUGO_FLAG_F32_PRUNE) || !(ub0 < __builtin_huge_val())) return false;
+    const F32Box<ChildBox> xf{x};
+    const iv<float> f = obj_value_wt32<R>(xf, x, (const iv<float> *)a.ptab32 + b * a.tab_n32, L,
+                                          (const iv<float> *)a.params32, a.d, rest, win, a.c);
+    return (double)f.lo > ub0;
+  }
+#endif
+  return f32_prunes_w<R>(a, x, rest, win, ub0);
+}
+
 // Rest sums (and products) of the parent records split in this step: one
 // workgroup per record, threads over the terms, a butterfly in each
 // wavefront and a fixed-order fold over the wavefronts (deterministic).  Per record
@@ -55,13 +192,20 @@ __device__ __forceinline__ void prep_impl(const BoundArgs &a) {
   __shared__ iv<double> part[16];
   const DevState *st = a.st;
   if (st->status != 0) return;
-  // grid (pairs, parents): workgroup (q, b) sums template s = q mod NREST at
-  // point k = q / NREST (0: the parent box, k > 0: diagonal sample k - 1)
+  // grid (pairs + table blocks, parents): workgroup (q, b) with q < pairs
+  // sums template s = q mod NREST at point k = q / NREST (0: the parent
+  // box, k > 0: diagonal sample k - 1); the workgroups after them fill the
+  // record's term tables
   const int b = blockIdx.y;
   if (b >= st->n_split) return;
   const int S = a.n_samples;
   const int q = blockIdx.x, k = q / GPUGO_NREST, s = q - k * GPUGO_NREST;
-  if (k > S) return;
+  if (k > S) {
+#if GPUGO_TAB
+    tab_build<R>(a, b, q - (1 + S) * GPUGO_NREST);
+#endif
+    return;
+  }
   const long long p = st->first_parent + b;
   const int win = a.pwin[p];
   const RowBox x{a.plo + p * GPUGO_D, a.phi + p * GPUGO_D};
@@ -142,6 +286,7 @@ __device__ __forceinline__ void bound_w_impl(const BoundArgs &a) {
   const double ub0 = st->ub_snap;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const double invW = 1.0 / (double)a.W, halfInvW = 0.5 / (double)a.W;
+  const TabLayout TL = tab_layout(a.c, (unsigned)a.wid, a.n_samples);
   if (threadIdx.x == 0) {
     s_ubmin = order_key(__builtin_huge_val());
     s_pv = 0;
@@ -163,7 +308,7 @@ __device__ __forceinline__ void bound_w_impl(const BoundArgs &a) {
       const ChildBox x = child_box(a.plo + p * GPUGO_D, a.phi + p * GPUGO_D, a.pw + p * GPUGO_D, win, GPUGO_D, a.c,
                                    (unsigned)a.wid, a.inv_wid, a.half_inv_wid, pos);
       const iv<double> *rest = (const iv<double> *)(a.prest) + p * (long long)((1 + a.n_samples) * GPUGO_NREST);
-      if (f32_prunes_w<R>(a, x, rest, win, ub0)) pv = true;
+      if (f32_prunes_wx<R>(a, x, pl, rest, win, ub0, TL)) pv = true;
       else need = true;
     }
     // the wavefront's children in each FP64 stage: few -> all lanes on each
@@ -193,7 +338,7 @@ __device__ __forceinline__ void bound_w_impl(const BoundArgs &a) {
       const ChildBox x = child_box(a.plo + p * GPUGO_D, a.phi + p * GPUGO_D, a.pw + p * GPUGO_D, win, GPUGO_D, a.c,
                                    (unsigned)a.wid, a.inv_wid, a.half_inv_wid, pos);
       const iv<double> *rest = (const iv<double> *)(a.prest) + p * (long long)((1 + a.n_samples) * GPUGO_NREST);
-      const iv<double> f = obj_value_w<R>(x, a.params, a.d, rest, win, a.c);
+      const iv<double> f = value_wx<R>(a, x, pl, 0, rest, win, TL);
       lb = f.lo;
       if (!(lb == lb)) lb = -__builtin_huge_val();
       if (lb <= ub0) stage2 = true;
@@ -224,13 +369,12 @@ __device__ __forceinline__ void bound_w_impl(const BoundArgs &a) {
                                    (unsigned)a.wid, a.inv_wid, a.half_inv_wid, pos);
       const int S = a.n_samples;
       const iv<double> *rest = (const iv<double> *)(a.prest) + p * (long long)((1 + S) * GPUGO_NREST);
-      keep = !obj_kkt_w<R>(x, a.params, a.dom_lo, a.dom_hi, a.d, rest, win, a.c);
+      keep = !kkt_wx<R>(a, x, pl, rest, win, TL);
       pk = !keep;
       // S points on the child's diagonal: window coordinates from the
       // child, the others from the parent's same sample point (its rest)
       for (int ts = 0; ts < S; ++ts) {
-        const DiagPoint<ChildBox> xs{x, ts, S};
-        const double u = obj_value_w<R>(xs, a.params, a.d, rest + (1 + ts) * GPUGO_NREST, win, a.c).hi;
+        const double u = value_wx<R>(a, x, pl, 1 + ts, rest + (1 + ts) * GPUGO_NREST, win, TL).hi;
         su = u < su ? u : su;
       }
     }
@@ -339,6 +483,7 @@ __device__ __forceinline__ void sample32_w_impl(const BoundArgs &a) {
   const long long first = st->first_parent;
   const double invW = 1.0 / (double)a.W, halfInvW = 0.5 / (double)a.W;
   const int lane = threadIdx.x & 63, S = a.n_samples;
+  const TabLayout TL = tab_layout(a.c, (unsigned)a.wid, S);
   if (threadIdx.x == 0) s_ubmin = order_key(__builtin_huge_val());
   __syncthreads();
   double m = __builtin_huge_val();
@@ -352,10 +497,23 @@ __device__ __forceinline__ void sample32_w_impl(const BoundArgs &a) {
                                  (unsigned)a.wid, a.inv_wid, a.half_inv_wid, pos);
     const iv<double> *rest = (const iv<double> *)(a.prest) + p * (long long)((1 + S) * GPUGO_NREST);
     for (int ts = 0; ts < S; ++ts) {
-      const DiagPoint<ChildBox> xs{x, ts, S};
-      const F32Box<DiagPoint<ChildBox>> xf{xs};
-      const double u =
-          (double)obj_value_w32<R>(xf, (const iv<float> *)a.params32, a.d, rest + (1 + ts) * GPUGO_NREST, win, a.c).hi;
+      double u;
+#if GPUGO_TAB
+      if (a.tab_n32 > 0) {
+        const DiagPoint<ChildBox> xs{x, ts, S};
+        const F32Box<DiagPoint<ChildBox>> xf{xs};
+        u = (double)obj_value_wt32<R>(xf, x,
+                                      (const iv<float> *)a.ptab32 + pl * a.tab_n32 + (1 + ts) * TL.off[GPUGO_NREST],
+                                      TL, (const iv<float> *)a.params32, a.d, rest + (1 + ts) * GPUGO_NREST, win, a.c)
+                .hi;
+      } else
+#endif
+      {
+        const DiagPoint<ChildBox> xs{x, ts, S};
+        const F32Box<DiagPoint<ChildBox>> xf{xs};
+        u = (double)obj_value_w32<R>(xf, (const iv<float> *)a.params32, a.d, rest + (1 + ts) * GPUGO_NREST, win, a.c)
+                .hi;
+      }
       m = u < m ? u : m;
     }
   }
@@ -461,4 +619,14 @@ extern "C" __global__ void gpugo_info(int *out) {
   out[4] = GPUGO_WITH_F32;
   out[5] = 0;
   out[6] = GPUGO_NREST;
+  // term tables: out[7] bit 0 tables, bit 1 gradient table; out[8 + s] the
+  // stencil width of slot s; out[16] the gradient's (the host sizes the
+  // tables with tab_layout's formula)
+#if GPUGO_TAB
+  out[7] = 1 | (GPUGO_TAB_GRAD ? 2 : 0);
+  for (int s = 0; s < GPUGO_NREST && s < 8; ++s) out[8 + s] = gpugo::tab_w(s);
+  out[16] = GPUGO_TAB_GW;
+#else
+  out[7] = 0;
+#endif
 }

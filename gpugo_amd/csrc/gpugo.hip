@@ -221,6 +221,9 @@ struct gpugo_module_s {
   // windowed modules (large d): rest-sum kernel per rounding, rest slots
   hipFunction_t prep[3] = {nullptr, nullptr, nullptr};
   int nrest = 0;
+  // term tables (windowed modules): stencil width of each rest slot and of
+  // the gradient template; tab bit 0: tables, bit 1: gradient table
+  int tab = 0, tab_w[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tab_gw = 1;
   int dim = 0, nparams = 0, abi = 0;
   bool f32 = false;                // has the FP32 value code list (precision F32)
   std::vector<signed char> psign;  // declared coefficient signs
@@ -279,11 +282,12 @@ static int module_finish(gpugo_module_t m) {
     return batch_module_finish(m, info);
   }
   DevBuf d;
-  HIPCHECK(d.alloc(8 * sizeof(int)));
-  HIPCHECK(hipMemset(d.p, 0, 8 * sizeof(int)));
+  int h[32];
+  memset(h, 0, sizeof h);
+  HIPCHECK(d.alloc(sizeof h));
+  HIPCHECK(hipMemset(d.p, 0, sizeof h));
   void *args[] = {&d.p};
   HIPCHECK(hipModuleLaunchKernel(info, 1, 1, 1, 1, 1, 1, 0, nullptr, args, nullptr));
-  int h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   HIPCHECK(hipMemcpy(h, d.p, sizeof h, hipMemcpyDeviceToHost));
   m->dim = h[0];
   m->nparams = h[1];
@@ -291,6 +295,11 @@ static int module_finish(gpugo_module_t m) {
   m->f32 = h[4] != 0;
   m->split_g = h[5];
   m->nrest = h[6];
+  if (m->nrest > 0 && m->nrest <= 8 && (h[7] & 1)) {
+    m->tab = h[7] & 3;
+    for (int s2 = 0; s2 < m->nrest; ++s2) m->tab_w[s2] = h[8 + s2];
+    m->tab_gw = h[16];
+  }
   if (hipModuleGetFunction(&m->select_fn, m->mod, "gpugo_select") != hipSuccess ||
       hipModuleGetFunction(&m->compact_fn, m->mod, "gpugo_compact") != hipSuccess ||
       hipModuleGetFunction(&m->root_fn, m->mod, "gpugo_root") != hipSuccess ||
@@ -999,6 +1008,7 @@ __global__ __launch_bounds__(256) void k_materialize(const double *plo, const do
 #define GPUGO_WMIN_MAX GPUGO_COMPACT_WG
 #define GPUGO_CHUNK_LOG2 13    // lazy queue: 8192 entries per chunk minimum
 #define GPUGO_LAZY_MAX_EVERY 256
+#define GPUGO_TAB_MAX (1 << 16) // term-table entries per parent record, at most
 static_assert(GPUGO_LAZY_WG <= GPUGO_WMIN_MAX && GPUGO_LAZY_FULL_WG <= GPUGO_WMIN_MAX,
               "wmin holds one key per compaction or lazy-tail workgroup");
 
@@ -1020,6 +1030,8 @@ struct gpugo_problem_s {
   DevBuf d_lo, d_hi, d_params, d_params32, plo, phi, pw, pwin, out_lo, out_hi, out_lb, st, minkey;
   DevBuf bar;          // persistent step loop: grid barrier [2] and error flag
   DevBuf pext;         // windowed modules: rest sums per parent record [p_cap][ext_n]
+  DevBuf ptab, ptab32; // windowed modules: term tables of a step's records [k_select][tab_n] (FP64 / FP32)
+  int tab_n = 0, tab_n32 = 0;
   DevBuf wmin;         // k_select 1: per-compaction-workgroup smallest key [GPUGO_WMIN_MAX][3]
   DevBuf cmin;         // lazy queue: smallest key per chunk of 2^GPUGO_CHUNK_LOG2 entries, [3] each
   // One box per step: the queue is compacted every compact_every steps; the
@@ -1083,6 +1095,27 @@ static int problem_shape(gpugo_problem_s *p, const gpugo_problem_desc *desc, lon
   p->ext_n = p->m->nrest > 0 ? 2 * p->m->nrest * (1 + desc->n_samples) : 0;
   p->q_cap = qc;
   p->q_req = qc;
+  // term tables of a windowed module (the module's tab_layout): per point
+  // (the record, then its n_samples diagonal points) NT entries, slot s
+  // (c + 2(w_s - 1)) terms x wid^w_s digit combinations; then the gradient
+  // table.  Left out when large (wide grids) or switched off.
+  p->tab_n = p->tab_n32 = 0;
+  if (p->m->tab && !(desc->flags & GPUGO_FLAG_NO_TABLES)) {
+    auto ipow = [](long long b, int e) {
+      long long r = 1;
+      for (int i = 0; i < e; ++i) r *= b;
+      return r;
+    };
+    long long nt = 0;
+    for (int s2 = 0; s2 < p->m->nrest; ++s2)
+      nt += (long long)(p->c + 2 * (p->m->tab_w[s2] - 1)) * ipow(desc->wid, p->m->tab_w[s2]);
+    long long tot = (1LL + desc->n_samples) * nt;
+    if (p->m->tab & 2) tot += (long long)(p->c + 2 * (p->m->tab_gw - 1)) * ipow(desc->wid, p->m->tab_gw);
+    if (tot <= GPUGO_TAB_MAX) {
+      p->tab_n = (int)tot;
+      p->tab_n32 = desc->precision == GPUGO_PREC_F32 ? (int)((1LL + desc->n_samples) * nt) : 0;
+    }
+  }
   p->p_cap = desc->max_parents > 0 ? desc->max_parents : p_default;
   // output boxes (x-width rule) up to 2^16 or 2^24 coordinates
   p->out_cap = std::max(64LL, std::min(1LL << 16, (1LL << 24) / d));
@@ -1130,6 +1163,8 @@ static int problem_buffers(gpugo_problem_s *p, Alloc &&alloc) {
   PBUF(p->minkey, 64);
   PBUF(p->bar, 64);
   if (p->ext_n > 0) PBUF(p->pext, sizeof(double) * p->ext_n * pc);
+  if (p->tab_n > 0) PBUF(p->ptab, 16 * (size_t)p->tab_n * p->desc.k_select);
+  if (p->tab_n32 > 0) PBUF(p->ptab32, 8 * (size_t)p->tab_n32 * p->desc.k_select);
   PBUF(p->wmin, sizeof(unsigned long long) * 3 * GPUGO_WMIN_MAX);
   if (p->lazy) {
     const long long nch = (qc + (1LL << GPUGO_CHUNK_LOG2) - 1) >> GPUGO_CHUNK_LOG2;
@@ -1167,7 +1202,8 @@ static int problem_check(gpugo_module_t m, const gpugo_problem_desc *desc) {
     return set_err(GPUGO_E_INVALID, "unknown precision %d", desc->precision);
   if (desc->precision == GPUGO_PREC_F32 && !m->f32)
     return set_err(GPUGO_E_INVALID, "module has no FP32 code list (emit it with f32=True)");
-  if (desc->flags & ~(GPUGO_FLAG_MEAN_VALUE | GPUGO_FLAG_SPLIT | GPUGO_FLAG_PERSIST | GPUGO_FLAG_EAGER_COMPACT))
+  if (desc->flags & ~(GPUGO_FLAG_MEAN_VALUE | GPUGO_FLAG_SPLIT | GPUGO_FLAG_PERSIST | GPUGO_FLAG_EAGER_COMPACT |
+                      GPUGO_FLAG_NO_TABLES))
     return set_err(GPUGO_E_INVALID, "unknown flags 0x%x", desc->flags);
   if ((desc->flags & GPUGO_FLAG_SPLIT) && m->split_g < 2)
     return set_err(GPUGO_E_INVALID, "module has no term-parallel kernels (emit it with split=G)");
@@ -1438,14 +1474,20 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
   ba.params32 = p->d_params32.p;
   ba.prest = p->pext.as<double>();
   ba.nrest = p->m->nrest;
+  ba.ptab = p->ptab.p;
+  ba.ptab32 = p->ptab32.p;
+  ba.tab_n = p->tab_n;
+  ba.tab_n32 = p->tab_n32;
   if (p->m->nrest > 0) {
     // rest sums of the parent records this step splits: one workgroup per
-    // (point, template sum) pair and parent
+    // (point, template sum) pair and parent, then the workgroups that fill
+    // each record's term tables (256 entries each)
     const unsigned nb = p->steps_enqueued == 0 ? 1u : (unsigned)p->desc.k_select;
     const unsigned npairs = (unsigned)((1 + p->desc.n_samples) * p->m->nrest);
+    const unsigned ntab = (unsigned)((p->tab_n + 255) / 256);
     void *args[] = {&ba};
-    HIPCHECK(hipModuleLaunchKernel(p->m->prep[p->desc.rounding], npairs, nb, 1, 256, 1, 1, 0, p->stream, args,
-                                   nullptr));
+    HIPCHECK(hipModuleLaunchKernel(p->m->prep[p->desc.rounding], npairs + ntab, nb, 1, 256, 1, 1, 0, p->stream,
+                                   args, nullptr));
   }
   if (p->steps_enqueued == 0 && p->desc.precision == GPUGO_PREC_F32 && p->desc.rounding < 2 &&
       p->m->sample32[p->desc.rounding]) {

@@ -541,6 +541,204 @@ class _FuncWP(_FuncW):
         return ("i", acc)
 
 
+class _FuncWT(_FuncW):
+    """Windowed lowering over term tables: term j of template sum s over a
+    child depends only on the digits of the window coordinates in its
+    stencil [klo, khi], so the step's prep kernel evaluates it once per
+    (term, digit combination) and a child adds table entries instead of
+    evaluating terms.  Same operations on the same intervals, in the same
+    order, as _FuncW: the result is bit-identical."""
+
+    def __init__(self, slots: Dict[tuple, int], stencils: List[Tuple[int, int]], ty: str = "double"):
+        super().__init__(slots, ty)
+        self.stencils = stencils
+
+    def loop_sum(self, e: Expr) -> Tuple[str, str]:
+        s = self.slots[e.key()]
+        klo, khi = self.stencils[s]
+        acc = f"t{self.n}"
+        self.n += 1
+        ty = self.ty
+        if ty == "float":
+            self.lines.append(f"  iv<float> {acc} = mk<float>(to_dn<float>(rest[{s}].lo), to_up<float>(rest[{s}].hi));")
+        else:
+            self.lines.append(f"  iv<double> {acc} = rest[{s}];")
+        fn = "mul" if e.op == "tprod" else "add"
+        self.lines.append(f"  {{ int rlo[2], rhi[2]; const int nr = wranges_{s}(win, c, d, rlo, rhi);")
+        self.lines.append(f"    const iv<{ty}> *__restrict__ Ts = T + L.off[{s}]; const unsigned Es = L.E[{s}];")
+        self.lines.append("    unsigned q = 0;")
+        self.lines.append("    for (int r = 0; r < nr; ++r)")
+        self.lines.append(f"      for (int j = rlo[r]; j < rhi[r]; ++j, ++q) {acc} = A::{fn}({acc}, "
+                          f"Ts[q * Es + tab_idx<{khi - klo + 1}>(xd, j + ({klo}))]); }}")
+        return ("i", acc)
+
+    def loop_prod_except(self, e: Expr) -> Tuple[str, str]:
+        """Inside the component loop j: the parent's rest product times the
+        window's factors i != j, read from the table of the full product."""
+        s = self.slots[_reduction_of(e).key()]
+        klo, khi = self.stencils[s]
+        acc = f"t{self.n}"
+        self.n += 1
+        self.lines.append(f"  iv<double> {acc} = rest[{s}];")
+        self.lines.append(f"  {{ int rlo[2], rhi[2]; const int nr = wranges_{s}(win, c, d, rlo, rhi);")
+        self.lines.append(f"    const iv<double> *__restrict__ Ts = T + L.off[{s}]; const unsigned Es = L.E[{s}];")
+        self.lines.append("    unsigned q = 0;")
+        self.lines.append("    for (int r = 0; r < nr; ++r)")
+        self.lines.append("      for (int i = rlo[r]; i < rhi[r]; ++i, ++q)")
+        self.lines.append(f"        if (i != j) {acc} = A::mul({acc}, Ts[q * Es + tab_idx<{khi - klo + 1}>(xd, i + ({klo}))]); }}")
+        return ("i", acc)
+
+
+# widest stencil (coordinates per term) a term table covers: wid^w entries per term
+TAB_MAX_W = 3
+
+
+def _window_tables(value: Expr, grad: Expr, sums: List[Expr], slots: Dict[tuple, int], global_grad: bool,
+                   grad_tab: bool) -> List[str]:
+    """Term tables of the windowed lists (FP64 part):
+      TabLayout, tab_layout(c, wid, S)   entries per parent record: (1 + S)
+          points (the box, then its S diagonal sample points) of NT entries
+          (slot s: (c + 2(w_s - 1)) terms x wid^w_s digit combinations at
+          off[s]), then the gradient table (box only) when the gradient
+          template is local
+      wterm_any<R>(s, x, P, d, j)         term j of slot s
+      wgrad_t<R>(x, P, d, j)              gradient component j (local templates)
+      obj_value_wt<R>(x, xd, T, L, P, d, rest, win, c)   value at a point of
+          a child (x: the child or one of its diagonal points, for the
+          coordinates read outside the template sums; xd: the child, for the
+          digits) from that point's table
+      obj_kkt_wt<R>(x, TG, L, P, dlo, dhi, d, win, c)  monotonicity test from
+          the gradient table"""
+    stencils = [_x_offsets(e.args[0], f"template sum {s}") for s, e in enumerate(sums)]
+    widths = [khi - klo + 1 for klo, khi in stencils]
+    if grad_tab:
+        gklo, gkhi = _x_offsets(grad, "gradient template")
+        gw = gkhi - gklo + 1
+    else:
+        gklo, gkhi, gw = 0, 0, 1
+    out = ["// term tables (see _window_tables in gpugo_amd/emit.py for the layout)",
+           f"#define GPUGO_TAB_GRAD {int(grad_tab)}",
+           f"#define GPUGO_TAB_GKLO {gklo}",
+           f"#define GPUGO_TAB_GW {gw}",
+           "struct TabLayout {",
+           f"  int off[{len(sums) + 1}];  // per point: slot s at off[s]; off[NREST] = NT entries",
+           f"  unsigned E[{len(sums)}];    // wid^w_s digit combinations per term",
+           "  int goff;                   // gradient table: (1 + S) * NT",
+           "  unsigned gE;                // wid^w_g",
+           "  int total;                  // entries per parent record",
+           "};",
+           "__host__ __device__ __forceinline__ TabLayout tab_layout(int c, unsigned wid, int S) {",
+           "  TabLayout L;",
+           "  int o = 0;"]
+    for s, w in enumerate(widths):
+        out.append(f"  L.off[{s}] = o; L.E[{s}] = " + " * ".join(["wid"] * w) + f"; o += (c + {2 * (w - 1)}) * (int)L.E[{s}];")
+    out += [f"  L.off[{len(sums)}] = o;",
+            "  L.goff = (1 + S) * o;",
+            "  L.gE = " + " * ".join(["wid"] * gw) + ";",
+            f"  L.total = L.goff + (GPUGO_TAB_GRAD ? (c + {2 * (gw - 1)}) * (int)L.gE : 0);",
+            "  return L;",
+            "}"]
+    # term of any slot (the prep kernel's table build)
+    out.append("template <int R, class XS>\n__device__ __forceinline__ iv<double> wterm_any("
+               "int s, const XS &x, const double *__restrict__ P, int d, int j) {")
+    out.append("  switch (s) {")
+    for s in range(len(sums)):
+        out.append(f"  case {s}: return wterm_{s}<R>(x, P, d, j);")
+    out.append("  default: return mk<double>(0.0, 0.0);")
+    out.append("  }")
+    out.append("}")
+    out.append("__device__ __forceinline__ int wranges_any(int s, int win, int c, int d, int *lo, int *hi) {")
+    out.append("  switch (s) {")
+    for s in range(len(sums)):
+        out.append(f"  case {s}: return wranges_{s}(win, c, d, lo, hi);")
+    out.append("  default: return 0;")
+    out.append("  }")
+    out.append("}")
+    out.append("__device__ __forceinline__ int tab_klo(int s) {")
+    out.append("  switch (s) {")
+    for s, (klo, _) in enumerate(stencils):
+        out.append(f"  case {s}: return {klo};")
+    out.append("  default: return 0;")
+    out.append("  }")
+    out.append("}")
+    out.append("__device__ __forceinline__ int tab_w(int s) {")
+    out.append("  switch (s) {")
+    for s, w in enumerate(widths):
+        out.append(f"  case {s}: return {w};")
+    out.append("  default: return 1;")
+    out.append("  }")
+    out.append("}")
+    # value over a child from a point's table
+    f = _FuncWT(slots, stencils)
+    v = f.lower(value)
+    out.append("template <int R, class XS>\n__device__ __forceinline__ iv<double> obj_value_wt("
+               "const XS &x, const ChildBox &xd, const iv<double> *__restrict__ T, const TabLayout &L, "
+               "const double *__restrict__ P, int d, const iv<double> *__restrict__ rest, int win, int c) {")
+    out.append("  typedef IA<double, R> A;")
+    out += f.lines
+    out.append(f"  return {f.as_iv(v)};")
+    out.append("}")
+    if not grad_tab:
+        # monotonicity test with the gradient's reductions from the tables
+        # and each component evaluated over the child
+        f = _FuncWT(slots, stencils)
+        _kkt_lines(f, grad, global_grad)
+        out.append("template <int R>\n__device__ __forceinline__ bool obj_kkt_wr("
+                   "const ChildBox &x, const iv<double> *__restrict__ T, const TabLayout &L, "
+                   "const double *__restrict__ P, const double *__restrict__ dlo, const double *__restrict__ dhi, "
+                   "int d, const iv<double> *__restrict__ rest, int win, int c) {")
+        out.append("  typedef IA<double, R> A;")
+        out.append("  const ChildBox &xd = x;")
+        out += f.lines
+        out.append("  return false;")
+        out.append("}")
+    if grad_tab:
+        f = _FuncG()
+        f.in_loop = True
+        code = f.as_iv(f.lower(grad))
+        out.append("template <int R, class XS>\n__device__ __forceinline__ iv<double> wgrad_t("
+                   "const XS &x, const double *__restrict__ P, int d, int j) {")
+        out.append("  typedef IA<double, R> A;")
+        out += f.lines
+        out.append(f"  return {code};")
+        out.append("}")
+        out.append("template <int R>\n__device__ __forceinline__ bool obj_kkt_wt("
+                   "const ChildBox &x, const iv<double> *__restrict__ TG, const TabLayout &L, "
+                   "const double *__restrict__ dlo, const double *__restrict__ dhi, int d, int win, int c) {")
+        out.append(f"  int glo[2], ghi[2]; const int ng = win_ranges(win, c, d, {gklo}, {gkhi}, d, glo, ghi);")
+        out.append("  unsigned q = 0;")
+        out.append("  for (int r = 0; r < ng; ++r)")
+        out.append("  for (int j = glo[r]; j < ghi[r]; ++j, ++q) {")
+        out.append(f"    const iv<double> gj = TG[q * L.gE + tab_idx<{gw}>(x, j + ({gklo}))];")
+        out.append("    if (gj.lo > 0.0 && x[j].lo != dlo[j]) return true;")
+        out.append("    if (gj.hi < 0.0 && x[j].hi != dhi[j]) return true;")
+        out.append("  }")
+        out.append("  return false;")
+        out.append("}")
+    return out
+
+
+def _tab_ok(sums: List[Expr]) -> bool:
+    """Every template reduction has a stencil narrow enough for a table
+    (and the module info has room for the slots' widths)."""
+    if len(sums) > 8:
+        return False
+    for s, e in enumerate(sums):
+        klo, khi = _x_offsets(e.args[0], f"template sum {s}")
+        if khi - klo + 1 > TAB_MAX_W:
+            return False
+    return True
+
+
+def _grad_tab_ok(grad: Expr, global_grad: bool) -> bool:
+    """The gradient template is local (no reduction, no product-except-j)
+    with a narrow stencil: one table entry per (component, digits)."""
+    if global_grad or _has_op(grad, "tprodx") or _tsums(grad):
+        return False
+    klo, khi = _x_offsets(grad, "gradient template")
+    return khi - klo + 1 <= TAB_MAX_W
+
+
 def window_lists_f32(obj: Objective) -> str:
     """FP32 windowed value code list of the FP32-prune form, after
     window_lists (it uses its wranges_s):
@@ -573,6 +771,26 @@ def window_lists_f32(obj: Objective) -> str:
     out += f.lines
     out.append(f"  return {f.as_iv(v)};")
     out.append("}")
+    if _tab_ok(sums):
+        # FP32 term tables (same layout as the FP64 ones, FP32 entries)
+        stencils = [_x_offsets(e.args[0], f"template sum {s}") for s, e in enumerate(sums)]
+        out.append("template <int R, class XS>\n__device__ __forceinline__ iv<float> wterm32_any("
+                   "int s, const XS &x, const iv<float> *__restrict__ P, int d, int j) {")
+        out.append("  switch (s) {")
+        for s in range(len(sums)):
+            out.append(f"  case {s}: return wterm32_{s}<R>(x, P, d, j);")
+        out.append("  default: return mk<float>(0.0f, 0.0f);")
+        out.append("  }")
+        out.append("}")
+        f = _FuncWT(slots, stencils, "float")
+        v = f.lower(value)
+        out.append("template <int R, class XS>\n__device__ __forceinline__ iv<float> obj_value_wt32("
+                   "const XS &x, const ChildBox &xd, const iv<float> *__restrict__ T, const TabLayout &L, "
+                   "const iv<float> *__restrict__ P, int d, const iv<double> *__restrict__ rest, int win, int c) {")
+        out.append("  typedef IA<float, R> A;")
+        out += f.lines
+        out.append(f"  return {f.as_iv(v)};")
+        out.append("}")
     out.append("}  // namespace gpugo")
     return "\n".join(out) + "\n"
 
@@ -663,6 +881,27 @@ def window_lists(obj: Objective) -> str:
         for e in _tsums(grad):
             if e.op == "tprod" and _x_offsets(e.args[0], "template product") != (0, 0):
                 raise ValueError("a windowed template product may read X(J) only")
+    _kkt_lines(f, grad, global_grad)
+    out.append("template <int R, class XS>\n__device__ __forceinline__ bool obj_kkt_w("
+               "const XS &x, const double *__restrict__ P, const double *__restrict__ dlo, "
+               "const double *__restrict__ dhi, int d, const iv<double> *__restrict__ rest, int win, int c) {")
+    out.append("  typedef IA<double, R> A;")
+    out += f.lines
+    out.append("  return false;")
+    out.append("}")
+    out += _window_lists_coop(value, grad, slots, global_grad)
+    tab = _tab_ok(sums)
+    out.append(f"#define GPUGO_TAB {int(tab)}")
+    if tab:
+        out += _window_tables(value, grad, sums, slots, global_grad, _grad_tab_ok(grad, global_grad))
+    out.append("}  // namespace gpugo")
+    return "\n".join(out) + "\n"
+
+
+def _kkt_lines(f: "_FuncW", grad: Expr, global_grad: bool):
+    """The windowed monotonicity test into f's lines: the gradient's
+    reductions (hoisted), then each component the window can change; true
+    at the first component that rules the child out."""
     f.hoist(grad)
     if global_grad:
         f.lines.append("  for (int j = 0; j < d; ++j) {")
@@ -680,16 +919,6 @@ def window_lists(obj: Objective) -> str:
                 "    if (gj.lo > 0.0 && x[j].lo != dlo[j]) return true;",
                 "    if (gj.hi < 0.0 && x[j].hi != dhi[j]) return true;",
                 "  }"]
-    out.append("template <int R, class XS>\n__device__ __forceinline__ bool obj_kkt_w("
-               "const XS &x, const double *__restrict__ P, const double *__restrict__ dlo, "
-               "const double *__restrict__ dhi, int d, const iv<double> *__restrict__ rest, int win, int c) {")
-    out.append("  typedef IA<double, R> A;")
-    out += f.lines
-    out.append("  return false;")
-    out.append("}")
-    out += _window_lists_coop(value, grad, slots, global_grad)
-    out.append("}  // namespace gpugo")
-    return "\n".join(out) + "\n"
 
 
 def _window_lists_coop(value: Expr, grad: Expr, slots: Dict[tuple, int], global_grad: bool) -> List[str]:
@@ -1040,23 +1269,32 @@ def flop_counts(obj: Objective) -> dict:
     return {"value": v, "grad": g, "sample": v}
 
 
-def window_flop_counts(obj: Objective, c: int) -> dict:
+def window_flop_counts(obj: Objective, c: int, tables: bool = False, wid: int = 3, n_samples: int = 1) -> dict:
     """Algorithmic FLOP per child of the windowed code lists at a nominal
     window (c coordinates away from the ends): value = the non-sum part plus,
     per template sum, its terms that touch the window (c + stencil span) and
     their additions; grad = the monotonicity test over the components the
     window changes (c + span, or all d for a gradient that uses a template
     sum); sample = the value code list at one point.  The per-parent rest
-    sums (O(d) once per parent record) are not included."""
+    sums (O(d) once per parent record) are not included.
+    tables: the step reads window terms from its term tables, so a child
+    pays only the folds of the table entries (and, for a local gradient
+    template, no arithmetic at all in the monotonicity test); `table` is the
+    FLOP of filling one record's tables (wid per split coordinate,
+    n_samples diagonal points), paid once per split record by the prep
+    kernel."""
     t = obj.templates
     value, grad = wrap(t["value"]), wrap(t["grad"])
+    sums = _tsums(value, grad)
+    tab = tables and _tab_ok(sums)
+    gtab = tab and _grad_tab_ok(grad, bool(_global_reductions(grad)))
 
     def sums_cost(e):
         total = 0
         for ts in _tsums(e):
             klo, khi = _x_offsets(ts.args[0], "template sum")
             fold = _FLOP["mul" if ts.op == "tprod" else "add"]
-            total += min(c + khi - klo, obj.d + ts.val) * (_count([ts.args[0]]) + fold)
+            total += min(c + khi - klo, obj.d + ts.val) * ((0 if tab else _count([ts.args[0]])) + fold)
         return total
     v = _count([value], tsum_leaf=True) + sums_cost(value)
     if _global_reductions(grad):
@@ -1065,10 +1303,18 @@ def window_flop_counts(obj: Objective, c: int) -> dict:
         gklo, gkhi = _x_offsets(grad, "gradient template")
         ncomp, gsum = min(c + gkhi - gklo, obj.d), 0
     # a product-except-j: the window's other terms, c - 1 products each
-    px = sum(min(c, obj.d) * (_count([ts.args[0]]) + _FLOP["mul"])
+    px = sum(min(c, obj.d) * ((0 if tab else _count([ts.args[0]])) + _FLOP["mul"])
              for ts in _tsums(grad) if _has_op(grad, "tprodx") and ts.op == "tprod")
-    g = gsum + ncomp * (_count([grad], tsum_leaf=True) + px)
-    return {"value": v, "grad": g, "sample": v}
+    g = 0 if gtab else gsum + ncomp * (_count([grad], tsum_leaf=True) + px)
+    table = 0
+    if tab:
+        for ts in sums:
+            klo, khi = _x_offsets(ts.args[0], "template sum")
+            table += (1 + n_samples) * min(c + khi - klo, obj.d + ts.val) * wid ** (khi - klo + 1) * _count([ts.args[0]])
+        if gtab:
+            gklo, gkhi = _x_offsets(grad, "gradient template")
+            table += min(c + gkhi - gklo, obj.d) * wid ** (gkhi - gklo + 1) * _count([grad])
+    return {"value": v, "grad": g, "sample": v, "table": table}
 
 
 def _read(name: str) -> str:

@@ -193,6 +193,14 @@ int gpugo_window_ranges(int64_t n, const int32_t *args, int32_t *out);
                                     then counts the stale entries too, and the
                                     queue can fill up to M * wid^cycle entries
                                     sooner.                                    */
+#define GPUGO_FLAG_NO_TABLES  32 /* windowed (large-d) modules: evaluate every
+                                    window term per child instead of reading
+                                    the step's term tables (each term over a
+                                    child depends only on the digits of its
+                                    window coordinates, so by default it is
+                                    evaluated once per digit combination and
+                                    looked up).  Same bits either way; for
+                                    measurement and parity tests.              */
 
 /* ---- problems -------------------------------------------------------------
  * One box-constrained minimisation.  Replaces the module-level constants and

@@ -232,6 +232,28 @@ def test_window_ranges_exact():
         assert set().union(*got) == w, (row, o, sorted(w))
 
 
+def test_window_terms_fit_table_rows():
+    """Term tables keep (c + 2(w - 1)) rows per template sum of stencil
+    width w (and per gradient template): the window's ranges never hold
+    more terms than that, for every window of small d (wrapping ones
+    included), stencils up to the widest a table covers."""
+    from gpugo_amd.emit import TAB_MAX_W
+    from gpugo_amd.solver import window_ranges
+    rows = []
+    for d in range(1, 15):
+        for c in range(1, d + 1):
+            for win in range(d):
+                for w in range(1, TAB_MAX_W + 1):
+                    for klo in range(-(w - 1), 1):
+                        for nt in {d, max(d - 1, 0), d + 1}:
+                            rows.append((win, c, d, klo, klo + w - 1, nt))
+    out = window_ranges(rows)
+    for row, o in zip(rows, out):
+        c, w = row[1], row[4] - row[3] + 1
+        nq = sum(o[2 + 2 * r] - o[1 + 2 * r] for r in range(o[0]))
+        assert nq <= c + 2 * (w - 1), (row, o)
+
+
 def test_window_ranges_rejects_bad_rows():
     from gpugo_amd.solver import window_ranges
     with pytest.raises(_lib.GpugoError):
