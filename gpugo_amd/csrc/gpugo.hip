@@ -1048,6 +1048,11 @@ struct gpugo_problem_s {
   bool started = false;
   bool selected_ahead = false;  // the last compaction also ran the next step's select
   long long steps_enqueued = 0;
+  // Upper bound of the queue entries (stale ones included) before the next
+  // step's appends: each step appends at most (boxes split) * W.  Sizes the
+  // compaction grids, so a small queue does not pay for 512 workgroups.
+  long long q_hi = 0;
+  int last_nwg = GPUGO_COMPACT_WG;  // workgroups of the last compaction (its wmin entries)
   DevState hs;  // last state read back
   // state polling: pinned copies of DevState and an event per slot
   int *hpoll = nullptr;   // [2] pinned, written by the compaction of a batch's last step
@@ -1365,6 +1370,8 @@ static int root_init(gpugo_problem_t p) {
   s.lbmin_key = ~0ull;
   s.lbmax_key = 0ull;
   p->hs = s;
+  p->q_hi = 0;
+  p->last_nwg = GPUGO_COMPACT_WG;
   RootArgs ra;
   ra.st = p->st.as<DevState>();
   ra.plo = p->plo.as<double>();
@@ -1431,15 +1438,17 @@ static SelectArgs select_args(gpugo_problem_t p, const Queue &q, int nwg) {
 // modules' rest sums, split+bound, compact (or, for a lazy queue, refresh
 // the chunk minima).  poll: pinned status word written after the step (or
 // null).  more: another step follows in this call, so the compaction may
-// run its select.  A call's last step always compacts.
-static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
+// run its select.  A call's last step always compacts.  compact_only: the
+// problem has stopped and only the stop step's final compaction is left
+// (the select, rest sums and split+bound of a stopped problem do nothing).
+static int enqueue_step(gpugo_problem_t p, int *poll, bool more, bool compact_only = false) {
   const int d = p->desc.d;
   // every compaction moves the queue to the other buffer, a lazy tail does not
   const int cur = (int)((p->steps_enqueued - p->lazy_steps) & 1);
   Queue &A = p->q[cur], &B = p->q[cur ^ 1];
   DevState *st = p->st.as<DevState>();
-  if (p->steps_enqueued > 0 && !p->selected_ahead) {
-    SelectArgs sa = select_args(p, A, GPUGO_COMPACT_WG);  // after a plain compaction
+  if (!compact_only && p->steps_enqueued > 0 && !p->selected_ahead) {
+    SelectArgs sa = select_args(p, A, p->last_nwg);  // after a plain compaction
     if (p->m->select_fn) {
       void *sargs[] = {&sa};
       HIPCHECK(hipModuleLaunchKernel(p->m->select_fn, 1, 1, 1, SEL_T, 1, 1, 0, p->stream, sargs, nullptr));
@@ -1478,7 +1487,7 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
   ba.ptab32 = p->ptab32.p;
   ba.tab_n = p->tab_n;
   ba.tab_n32 = p->tab_n32;
-  if (p->m->nrest > 0) {
+  if (p->m->nrest > 0 && !compact_only) {
     // rest sums of the parent records this step splits: one workgroup per
     // (point, template sum) pair and parent, then the workgroups that fill
     // each record's term tables (256 entries each)
@@ -1489,7 +1498,7 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
     HIPCHECK(hipModuleLaunchKernel(p->m->prep[p->desc.rounding], npairs + ntab, nb, 1, 256, 1, 1, 0, p->stream,
                                    args, nullptr));
   }
-  if (p->steps_enqueued == 0 && p->desc.precision == GPUGO_PREC_F32 && p->desc.rounding < 2 &&
+  if (!compact_only && p->steps_enqueued == 0 && p->desc.precision == GPUGO_PREC_F32 && p->desc.rounding < 2 &&
       p->m->sample32[p->desc.rounding]) {
     // first step of an FP32-prune solve: sample every child in FP32 first,
     // so the FP32 value test has an incumbent in this step too
@@ -1499,7 +1508,7 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
     HIPCHECK(hipModuleLaunchKernel(p->m->sample32[p->desc.rounding], (unsigned)blocks, 1, 1, 256, 1, 1, 0, p->stream,
                                    args, nullptr));
   }
-  {
+  if (!compact_only) {
     const bool split = (p->desc.flags & GPUGO_FLAG_SPLIT) != 0;
     long long maxk = p->steps_enqueued == 0 ? 1 : p->desc.k_select;
     long long nthreads = maxk * (long long)p->W * (split ? p->m->split_g : 1);
@@ -1541,14 +1550,27 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
   ca.pad_c = 0;
   const bool lazy_tail = p->lazy && more && p->m->compact_select_fn && p->steps_enqueued > 0 &&
                          p->since_compact + 1 < p->compact_every;
+  // entries the compaction may read: the bound before this step plus its appends
+  {
+    const long long maxk = p->steps_enqueued == 0 ? 1 : p->desc.k_select;
+    const long long n_hi = p->q_hi + maxk * (long long)p->W;
+    p->q_hi = n_hi < p->q_cap ? n_hi : p->q_cap;
+  }
+  // workgroups for up to q_hi entries: a full pass covers SEL_T * 8 entries
+  // per workgroup (CMP_U = 8), a lazy tail one chunk per workgroup
+  auto grid_for = [&](long long per_wg, int max_wg) {
+    const long long n = (p->q_hi + per_wg - 1) / per_wg;
+    return (int)(n < 1 ? 1 : (n > max_wg ? max_wg : n));
+  };
   if (lazy_tail) {
-    // the queue stays in A; 256 workgroups refresh the changed chunk
-    // minima, the last one selects from A
+    // the queue stays in A; workgroups refresh the changed chunk minima,
+    // the last one selects from A
+    const int nwg = grid_for(1LL << GPUGO_CHUNK_LOG2, GPUGO_LAZY_WG);
     ca.cmin = p->cmin.as<unsigned long long>();
-    SelectArgs sa = select_args(p, A, GPUGO_LAZY_WG);
+    SelectArgs sa = select_args(p, A, nwg);
     ca.poll = nullptr;
     void *fargs[] = {&ca, &sa, &poll};
-    HIPCHECK(hipModuleLaunchKernel(p->m->compact_select_fn, GPUGO_LAZY_WG, 1, 1, SEL_T, 1, 1, 0, p->stream, fargs,
+    HIPCHECK(hipModuleLaunchKernel(p->m->compact_select_fn, nwg, 1, 1, SEL_T, 1, 1, 0, p->stream, fargs,
                                    nullptr));
     p->selected_ahead = true;
     ++p->lazy_steps;
@@ -1559,18 +1581,22 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more) {
   p->since_compact = 0;
   if (more && p->m->compact_select_fn) {
     // workgroups of 1024 threads compact; the last one selects from B
-    const int nwg = p->lazy ? GPUGO_LAZY_FULL_WG : GPUGO_FUSED_WG;
+    const int nwg = grid_for(8LL * SEL_T, p->lazy ? GPUGO_LAZY_FULL_WG : GPUGO_FUSED_WG);
     SelectArgs sa = select_args(p, B, nwg);
     ca.poll = nullptr;
     void *fargs[] = {&ca, &sa, &poll};
     HIPCHECK(hipModuleLaunchKernel(p->m->compact_select_fn, nwg, 1, 1, SEL_T, 1, 1, 0, p->stream, fargs, nullptr));
     p->selected_ahead = true;
-  } else if (p->m->compact_fn) {
-    void *cargs[] = {&ca};
-    HIPCHECK(hipModuleLaunchKernel(p->m->compact_fn, GPUGO_COMPACT_WG, 1, 1, 256, 1, 1, 0, p->stream, cargs, nullptr));
   } else {
-    hipLaunchKernelGGL(k_compact, dim3(GPUGO_COMPACT_WG), dim3(256), 0, p->stream, ca);
-    HIPCHECK(hipGetLastError());
+    const int nwg = grid_for(8LL * 256, GPUGO_COMPACT_WG);
+    p->last_nwg = nwg;
+    if (p->m->compact_fn) {
+      void *cargs[] = {&ca};
+      HIPCHECK(hipModuleLaunchKernel(p->m->compact_fn, nwg, 1, 1, 256, 1, 1, 0, p->stream, cargs, nullptr));
+    } else {
+      hipLaunchKernelGGL(k_compact, dim3(nwg), dim3(256), 0, p->stream, ca);
+      HIPCHECK(hipGetLastError());
+    }
   }
   ++p->steps_enqueued;
   return GPUGO_OK;
@@ -1756,11 +1782,15 @@ static int run_steps(gpugo_problem_t p, int64_t iters, gpugo_result *res, bool s
     // only for the previous batch, so it stays one batch ahead of the device; steps
     // enqueued after a stop are no-ops on the device.  Exactly `iters`
     // steps are enqueued at most.
+    // The first step of a solve runs alone and the host waits for it: a
+    // problem that certifies in its first step then enqueues no steps that
+    // would find it stopped (one host round trip for the others).
     const int64_t B = 2;
     int64_t done = 0;
     int slot = 0, pending = 0;
     while (done < iters) {
-      int64_t batch = B < iters - done ? B : iters - done;
+      const bool first = p->steps_enqueued == 0;
+      int64_t batch = first ? 1 : (B < iters - done ? B : iters - done);
       p->hpoll[slot] = GPUGO_ST_RUNNING;
       for (int64_t i = 0; i < batch; ++i) {
         int rc = enqueue_step(p, i + 1 == batch ? p->dpoll + slot : nullptr, done + i + 1 < iters);
@@ -1769,11 +1799,19 @@ static int run_steps(gpugo_problem_t p, int64_t iters, gpugo_result *res, bool s
       done += batch;
       HIPCHECK(hipEventRecord(p->poll_ev[slot], p->stream));
       ++pending;
-      if (pending == 2) {
-        const int old = slot ^ 1;
+      if (pending == 2 || first) {
+        const int old = first ? slot : slot ^ 1;
         HIPCHECK(hipEventSynchronize(p->poll_ev[old]));
         --pending;
-        if (__atomic_load_n(&p->hpoll[old], __ATOMIC_ACQUIRE) != GPUGO_ST_RUNNING) break;
+        if (__atomic_load_n(&p->hpoll[old], __ATOMIC_ACQUIRE) != GPUGO_ST_RUNNING) {
+          // no later step is in flight to run the stop step's final
+          // compaction: enqueue it alone
+          if (pending == 0) {
+            int rc = enqueue_step(p, nullptr, false, true);
+            if (rc) return rc;
+          }
+          break;
+        }
       }
       slot ^= 1;
     }

@@ -224,17 +224,21 @@ def test_lazy_queue_boxes_between_calls(gpu):
 
 def test_unknown_flag_rejected(gpu):
     """A description with a flag bit the header does not define is refused
-    with GPUGO_E_INVALID and a message; the eager-compaction bit is known."""
+    with GPUGO_E_INVALID and a message; the eager-compaction and no-tables
+    bits are known."""
     import ctypes
     from gpugo_amd import _lib
     spec = pb.sepcos2()
     with gpu_problem(gpu, spec, tol=1e-6, grid=(2, 2), k_select=1, eager_compact=True) as p:
         lib = _lib.load()
         d2 = _lib.ProblemDesc.from_buffer_copy(p.desc)
-        d2.flags = 32
+        d2.flags = 64
         h = ctypes.c_void_p()
         assert lib.gpugo_problem_create(p.module.handle, ctypes.byref(d2), ctypes.byref(h)) == _lib.E_INVALID
         assert b"unknown flags" in lib.gpugo_last_error()
+        d2.flags = _lib.FLAG_NO_TABLES
+        assert lib.gpugo_problem_create(p.module.handle, ctypes.byref(d2), ctypes.byref(h)) == _lib.OK
+        lib.gpugo_problem_destroy(h)
         assert p.solve()["status_name"] == "certified"
 
 
