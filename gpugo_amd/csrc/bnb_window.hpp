@@ -57,13 +57,44 @@ __device__ __forceinline__ void tab_build(const BoundArgs &a, int b, int blk) {
   const int e = blk * (int)blockDim.x + (int)threadIdx.x;
   const int S = a.n_samples, d = a.d, c = a.c;
   const unsigned wid = (unsigned)a.wid;
-  const TabLayout L = tab_layout(c, wid, S);
+  const TabLayout L = tab_layout(c, wid, S, d);
   if (e >= L.total) return;
   const long long p = a.st->first_parent + b;
   const int win = a.pwin[p];
   const int NT = L.off[GPUGO_NREST];
   int s, q, pt, w, klo, nr, rlo[2], rhi[2];
   unsigned combo;
+#if GPUGO_TAB_PARTS
+  if (e >= L.poff) {
+    // parts table: part k of component j over the record's box (r < d), or
+    // over the TabBox of a window component's digit combination
+    iv<double> *T = (iv<double> *)a.ptab + (long long)b * a.tab_n;
+    const int k = (e - L.poff) / L.pstride, r = (e - L.poff) - k * L.pstride;
+    if (r < d) {
+      const RowBox xb{a.plo + p * GPUGO_D, a.phi + p * GPUGO_D};
+      T[e] = wgpart<R>(k, xb, a.params, d, r);
+      return;
+    }
+    q = (r - d) / (int)L.gE;
+    combo = (unsigned)(r - d) - (unsigned)q * L.gE;
+    nr = win_ranges(win, c, d, GPUGO_TAB_GKLO, GPUGO_TAB_GKLO + GPUGO_TAB_GW - 1, d, rlo, rhi);
+    const int n0 = nr > 0 ? rhi[0] - rlo[0] : 0, nq = n0 + (nr > 1 ? rhi[1] - rlo[1] : 0);
+    if (q >= nq) return;
+    const int j = q < n0 ? rlo[0] + q : rlo[1] + (q - n0);
+    unsigned rr = combo;
+    for (int t = 0; t < GPUGO_TAB_GW; ++t) {
+      const int i = j + GPUGO_TAB_GKLO + t;
+      int kk = i - win;
+      if (kk < 0) kk += d;
+      if (!(i >= 0 && i < d && kk < c) && (rr % wid) != 0u) return;
+      rr /= wid;
+    }
+    const TabBox tb{a.plo + p * GPUGO_D, a.phi + p * GPUGO_D, a.pw + p * GPUGO_D, win, d, c, j + GPUGO_TAB_GKLO, wid,
+                    combo};
+    T[e] = wgpart<R>(k, tb, a.params, d, j);
+    return;
+  }
+#endif
   if (e < L.goff) {
     pt = e / NT;
     const int e2 = e - pt * NT;
@@ -129,7 +160,7 @@ __device__ __forceinline__ void tab_build(const BoundArgs &a, int b, int blk) {
 #if !GPUGO_TAB
 // objectives whose stencils are too wide for term tables
 struct TabLayout {};
-__device__ __forceinline__ TabLayout tab_layout(int, unsigned, int) { return TabLayout{}; }
+__device__ __forceinline__ TabLayout tab_layout(int, unsigned, int, int) { return TabLayout{}; }
 #endif
 
 // The tests of one child (record first_parent + b) from the term tables
@@ -158,6 +189,10 @@ __device__ __forceinline__ bool kkt_wx(const BoundArgs &a, const ChildBox &x, lo
 #if GPUGO_TAB && GPUGO_TAB_GRAD
   if (a.tab_n > 0)
     return obj_kkt_wt<R>(x, (const iv<double> *)a.ptab + b * a.tab_n + L.goff, L, a.dom_lo, a.dom_hi, a.d, win, a.c);
+#elif GPUGO_TAB && GPUGO_TAB_PARTS
+  if (a.tab_n > 0)
+    return obj_kkt_wl<R>(x, (const iv<double> *)a.ptab + b * a.tab_n, L, a.params, a.dom_lo, a.dom_hi, a.d, rest,
+                         win, a.c);
 #elif GPUGO_TAB
   if (a.tab_n > 0)
     return obj_kkt_wr<R>(x, (const iv<double> *)a.ptab + b * a.tab_n, L, a.params, a.dom_lo, a.dom_hi, a.d, rest,
@@ -286,7 +321,7 @@ __device__ __forceinline__ void bound_w_impl(const BoundArgs &a) {
   const double ub0 = st->ub_snap;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const double invW = 1.0 / (double)a.W, halfInvW = 0.5 / (double)a.W;
-  const TabLayout TL = tab_layout(a.c, (unsigned)a.wid, a.n_samples);
+  const TabLayout TL = tab_layout(a.c, (unsigned)a.wid, a.n_samples, a.d);
   if (threadIdx.x == 0) {
     s_ubmin = order_key(__builtin_huge_val());
     s_pv = 0;
@@ -483,7 +518,7 @@ __device__ __forceinline__ void sample32_w_impl(const BoundArgs &a) {
   const long long first = st->first_parent;
   const double invW = 1.0 / (double)a.W, halfInvW = 0.5 / (double)a.W;
   const int lane = threadIdx.x & 63, S = a.n_samples;
-  const TabLayout TL = tab_layout(a.c, (unsigned)a.wid, S);
+  const TabLayout TL = tab_layout(a.c, (unsigned)a.wid, S, a.d);
   if (threadIdx.x == 0) s_ubmin = order_key(__builtin_huge_val());
   __syncthreads();
   double m = __builtin_huge_val();
@@ -619,13 +654,15 @@ extern "C" __global__ void gpugo_info(int *out) {
   out[4] = GPUGO_WITH_F32;
   out[5] = 0;
   out[6] = GPUGO_NREST;
-  // term tables: out[7] bit 0 tables, bit 1 gradient table; out[8 + s] the
-  // stencil width of slot s; out[16] the gradient's (the host sizes the
-  // tables with tab_layout's formula)
+  // term tables: out[7] bit 0 tables, bit 1 gradient table, bit 2 parts
+  // table; out[8 + s] the stencil width of slot s; out[16] the gradient's;
+  // out[17] the gradient parts (the host sizes the tables with tab_layout's
+  // formula)
 #if GPUGO_TAB
-  out[7] = 1 | (GPUGO_TAB_GRAD ? 2 : 0);
+  out[7] = 1 | (GPUGO_TAB_GRAD ? 2 : 0) | (GPUGO_TAB_PARTS ? 4 : 0);
   for (int s = 0; s < GPUGO_NREST && s < 8; ++s) out[8 + s] = gpugo::tab_w(s);
   out[16] = GPUGO_TAB_GW;
+  out[17] = GPUGO_TAB_PARTS;
 #else
   out[7] = 0;
 #endif

@@ -223,7 +223,7 @@ struct gpugo_module_s {
   int nrest = 0;
   // term tables (windowed modules): stencil width of each rest slot and of
   // the gradient template; tab bit 0: tables, bit 1: gradient table
-  int tab = 0, tab_w[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tab_gw = 1;
+  int tab = 0, tab_w[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tab_gw = 1, tab_parts = 0;
   int dim = 0, nparams = 0, abi = 0;
   bool f32 = false;                // has the FP32 value code list (precision F32)
   std::vector<signed char> psign;  // declared coefficient signs
@@ -296,9 +296,10 @@ static int module_finish(gpugo_module_t m) {
   m->split_g = h[5];
   m->nrest = h[6];
   if (m->nrest > 0 && m->nrest <= 8 && (h[7] & 1)) {
-    m->tab = h[7] & 3;
+    m->tab = h[7] & 7;
     for (int s2 = 0; s2 < m->nrest; ++s2) m->tab_w[s2] = h[8 + s2];
     m->tab_gw = h[16];
+    m->tab_parts = (h[7] & 4) ? h[17] : 0;
   }
   if (hipModuleGetFunction(&m->select_fn, m->mod, "gpugo_select") != hipSuccess ||
       hipModuleGetFunction(&m->compact_fn, m->mod, "gpugo_compact") != hipSuccess ||
@@ -1008,7 +1009,7 @@ __global__ __launch_bounds__(256) void k_materialize(const double *plo, const do
 #define GPUGO_WMIN_MAX GPUGO_COMPACT_WG
 #define GPUGO_CHUNK_LOG2 13    // lazy queue: 8192 entries per chunk minimum
 #define GPUGO_LAZY_MAX_EVERY 256
-#define GPUGO_TAB_MAX (1 << 16) // term-table entries per parent record, at most
+#define GPUGO_TAB_MAX (1 << 20) // term-table entries per parent record, at most
 static_assert(GPUGO_LAZY_WG <= GPUGO_WMIN_MAX && GPUGO_LAZY_FULL_WG <= GPUGO_WMIN_MAX,
               "wmin holds one key per compaction or lazy-tail workgroup");
 
@@ -1103,7 +1104,8 @@ static int problem_shape(gpugo_problem_s *p, const gpugo_problem_desc *desc, lon
   // term tables of a windowed module (the module's tab_layout): per point
   // (the record, then its n_samples diagonal points) NT entries, slot s
   // (c + 2(w_s - 1)) terms x wid^w_s digit combinations; then the gradient
-  // table.  Left out when large (wide grids) or switched off.
+  // table or the gradient parts table.  Left out when large (wide grids) or
+  // switched off.
   p->tab_n = p->tab_n32 = 0;
   if (p->m->tab && !(desc->flags & GPUGO_FLAG_NO_TABLES)) {
     auto ipow = [](long long b, int e) {
@@ -1115,7 +1117,10 @@ static int problem_shape(gpugo_problem_s *p, const gpugo_problem_desc *desc, lon
     for (int s2 = 0; s2 < p->m->nrest; ++s2)
       nt += (long long)(p->c + 2 * (p->m->tab_w[s2] - 1)) * ipow(desc->wid, p->m->tab_w[s2]);
     long long tot = (1LL + desc->n_samples) * nt;
-    if (p->m->tab & 2) tot += (long long)(p->c + 2 * (p->m->tab_gw - 1)) * ipow(desc->wid, p->m->tab_gw);
+    const long long gtab = (long long)(p->c + 2 * (p->m->tab_gw - 1)) * ipow(desc->wid, p->m->tab_gw);
+    if (p->m->tab & 2) tot += gtab;
+    // parts table: per gradient part, d entries then the window components'
+    tot += (long long)p->m->tab_parts * (desc->d + gtab);
     if (tot <= GPUGO_TAB_MAX) {
       p->tab_n = (int)tot;
       p->tab_n32 = desc->precision == GPUGO_PREC_F32 ? (int)((1LL + desc->n_samples) * nt) : 0;

@@ -553,6 +553,12 @@ class _FuncWT(_FuncW):
         super().__init__(slots, ty)
         self.stencils = stencils
 
+    def _lower(self, e: Expr) -> Tuple[str, str]:
+        if e.op == "tpart":
+            # a gradient part read from the parts table (obj_kkt_wl)
+            return ("i", f"gp{e.val}")
+        return super()._lower(e)
+
     def loop_sum(self, e: Expr) -> Tuple[str, str]:
         s = self.slots[e.key()]
         klo, khi = self.stencils[s]
@@ -593,8 +599,72 @@ class _FuncWT(_FuncW):
 TAB_MAX_W = 3
 
 
+def _grad_parts(grad: Expr) -> List[Expr]:
+    """Maximal subexpressions of a gradient template that read the
+    coordinates only through X(J + k): no template reduction and no
+    loop-invariant coordinate value (those are the gradient's hoisted
+    globals).  Over a child, part p of component j depends only on the digits
+    of the window coordinates in its stencil, like a template term."""
+    parts, keys = [], {}
+
+    def local(e):
+        if e.op in TEMPLATE_REDUCTIONS or e.op == "tprodx":
+            return False
+        if not has_loop_index(e) and e.depends_on_x():
+            return False
+        return all(local(a) for a in e.args)
+
+    def walk(e):
+        if e.op in TEMPLATE_REDUCTIONS or e.op == "tprodx":
+            return  # a reduction binds its own J: hoisted, not a part
+        if e.depends_on_x() and has_loop_index(e) and local(e):
+            if e.key() not in keys:
+                keys[e.key()] = len(parts)
+                parts.append(e)
+            return
+        for a in e.args:
+            walk(a)
+    walk(wrap(grad))
+    return parts
+
+
+def _replace(e: Expr, mapping: Dict[tuple, Expr], memo=None) -> Expr:
+    """e with the subexpressions whose keys are in `mapping` replaced
+    (not inside template reductions, which bind their own J)."""
+    if memo is None:
+        memo = {}
+    if e.op in TEMPLATE_REDUCTIONS or e.op == "tprodx":
+        return e
+    k = e.key()
+    if k in mapping:
+        return mapping[k]
+    if k in memo:
+        return memo[k]
+    r = Expr(e.op, tuple(_replace(a, mapping, memo) for a in e.args), e.val, e.val2) if e.args else e
+    memo[k] = r
+    return r
+
+
+def _parts_ok(grad: Expr, global_grad: bool) -> bool:
+    """A gradient that uses template reductions (so every component is
+    tested) and whose coordinate-dependent local work can be tabled per
+    component: the parts table of obj_kkt_wl."""
+    if not global_grad or _has_op(grad, "tprodx"):
+        return False
+    parts = _grad_parts(grad)
+    if not parts or len(parts) > 8:
+        return False
+    # a table read only pays for a part that does interval work beyond a
+    # scalar operation on a coordinate (measured: Salomon, whose only part is
+    # X(J), was ~10% slower with the table)
+    if max(_count([prt]) for prt in parts) < _FLOP["mul"]:
+        return False
+    klo, khi = _x_offsets(grad, "gradient template")
+    return khi - klo + 1 <= TAB_MAX_W
+
+
 def _window_tables(value: Expr, grad: Expr, sums: List[Expr], slots: Dict[tuple, int], global_grad: bool,
-                   grad_tab: bool) -> List[str]:
+                   grad_tab: bool, parts_tab: bool) -> List[str]:
     """Term tables of the windowed lists (FP64 part):
       TabLayout, tab_layout(c, wid, S)   entries per parent record: (1 + S)
           points (the box, then its S diagonal sample points) of NT entries
@@ -608,26 +678,35 @@ def _window_tables(value: Expr, grad: Expr, sums: List[Expr], slots: Dict[tuple,
           coordinates read outside the template sums; xd: the child, for the
           digits) from that point's table
       obj_kkt_wt<R>(x, TG, L, P, dlo, dhi, d, win, c)  monotonicity test from
-          the gradient table"""
+          the gradient table
+      obj_kkt_wl<R>(x, T, L, P, dlo, dhi, d, rest, win, c)  monotonicity test
+          of a gradient with template reductions (every component tested):
+          the reductions from the point-0 tables, the components' local parts
+          (_grad_parts) from the parts table: per part, d entries over the
+          record's box, then (c + 2(w - 1)) x wid^w entries of the components
+          whose stencil meets the window"""
     stencils = [_x_offsets(e.args[0], f"template sum {s}") for s, e in enumerate(sums)]
     widths = [khi - klo + 1 for klo, khi in stencils]
-    if grad_tab:
+    if grad_tab or parts_tab:
         gklo, gkhi = _x_offsets(grad, "gradient template")
         gw = gkhi - gklo + 1
     else:
         gklo, gkhi, gw = 0, 0, 1
+    parts = _grad_parts(grad) if parts_tab else []
     out = ["// term tables (see _window_tables in gpugo_amd/emit.py for the layout)",
            f"#define GPUGO_TAB_GRAD {int(grad_tab)}",
+           f"#define GPUGO_TAB_PARTS {len(parts)}",
            f"#define GPUGO_TAB_GKLO {gklo}",
            f"#define GPUGO_TAB_GW {gw}",
            "struct TabLayout {",
            f"  int off[{len(sums) + 1}];  // per point: slot s at off[s]; off[NREST] = NT entries",
            f"  unsigned E[{len(sums)}];    // wid^w_s digit combinations per term",
            "  int goff;                   // gradient table: (1 + S) * NT",
-           "  unsigned gE;                // wid^w_g",
+           "  unsigned gE;                // wid^w_g (gradient and parts tables)",
+           "  int poff, pstride;          // parts table: part k at poff + k * pstride (d + (c + 2(w_g - 1)) * gE)",
            "  int total;                  // entries per parent record",
            "};",
-           "__host__ __device__ __forceinline__ TabLayout tab_layout(int c, unsigned wid, int S) {",
+           "__host__ __device__ __forceinline__ TabLayout tab_layout(int c, unsigned wid, int S, int d) {",
            "  TabLayout L;",
            "  int o = 0;"]
     for s, w in enumerate(widths):
@@ -635,7 +714,9 @@ def _window_tables(value: Expr, grad: Expr, sums: List[Expr], slots: Dict[tuple,
     out += [f"  L.off[{len(sums)}] = o;",
             "  L.goff = (1 + S) * o;",
             "  L.gE = " + " * ".join(["wid"] * gw) + ";",
-            f"  L.total = L.goff + (GPUGO_TAB_GRAD ? (c + {2 * (gw - 1)}) * (int)L.gE : 0);",
+            f"  L.poff = L.goff + (GPUGO_TAB_GRAD ? (c + {2 * (gw - 1)}) * (int)L.gE : 0);",
+            f"  L.pstride = d + (c + {2 * (gw - 1)}) * (int)L.gE;",
+            "  L.total = L.poff + GPUGO_TAB_PARTS * L.pstride;",
             "  return L;",
             "}"]
     # term of any slot (the prep kernel's table build)
@@ -678,6 +759,55 @@ def _window_tables(value: Expr, grad: Expr, sums: List[Expr], slots: Dict[tuple,
     out += f.lines
     out.append(f"  return {f.as_iv(v)};")
     out.append("}")
+    if parts:
+        # local parts of component j (the prep kernel's parts table)
+        out.append("template <int R, class XS>\n__device__ __forceinline__ iv<double> wgpart("
+                   "int k, const XS &x, const double *__restrict__ P, int d, int j) {")
+        out.append("  typedef IA<double, R> A;")
+        out.append("  switch (k) {")
+        for k, prt in enumerate(parts):
+            f = _FuncG()
+            f.in_loop = True
+            code = f.as_iv(f.lower(prt))
+            out.append(f"  case {k}: {{")
+            out += ["  " + ln for ln in f.lines]
+            out.append(f"    return {code};")
+            out.append("  }")
+        out.append("  default: return mk<double>(0.0, 0.0);")
+        out.append("  }")
+        out.append("}")
+        skel = _replace(wrap(grad), {prt.key(): Expr("tpart", (), k) for k, prt in enumerate(parts)})
+        f = _FuncWT(slots, stencils)
+        f.hoist(skel)
+        f.lines.append(f"  int glo[2], ghi[2]; const int ng = win_ranges(win, c, d, {gklo}, {gkhi}, d, glo, ghi);")
+        f.lines.append("  const int g0 = ng > 0 ? ghi[0] - glo[0] : 0;")
+        f.lines.append("  const iv<double> *__restrict__ TP = T + L.poff;")
+        f.lines.append("  for (int j = 0; j < d; ++j) {")
+        f.lines.append("    int q = -1;")
+        f.lines.append("    if (ng > 0 && j >= glo[0] && j < ghi[0]) q = j - glo[0];")
+        f.lines.append("    else if (ng > 1 && j >= glo[1] && j < ghi[1]) q = g0 + (j - glo[1]);")
+        f.lines.append(f"    const iv<double> *__restrict__ tp = q < 0 ? TP + j : TP + d + q * (int)L.gE + "
+                       f"(int)tab_idx<{gw}>(xd, j + ({gklo}));")
+        for k in range(len(parts)):
+            f.lines.append(f"    const iv<double> gp{k} = tp[{k} * L.pstride];")
+        saved = dict(f.memo)
+        f.in_loop = True
+        code = f.as_iv(f.lower(skel))
+        f.in_loop = False
+        f.memo = saved
+        f.lines += [f"    const iv<double> gj = {code};",
+                    "    if (gj.lo > 0.0 && x[j].lo != dlo[j]) return true;",
+                    "    if (gj.hi < 0.0 && x[j].hi != dhi[j]) return true;",
+                    "  }"]
+        out.append("template <int R>\n__device__ __forceinline__ bool obj_kkt_wl("
+                   "const ChildBox &x, const iv<double> *__restrict__ T, const TabLayout &L, "
+                   "const double *__restrict__ P, const double *__restrict__ dlo, const double *__restrict__ dhi, "
+                   "int d, const iv<double> *__restrict__ rest, int win, int c) {")
+        out.append("  typedef IA<double, R> A;")
+        out.append("  const ChildBox &xd = x;")
+        out += f.lines
+        out.append("  return false;")
+        out.append("}")
     if not grad_tab:
         # monotonicity test with the gradient's reductions from the tables
         # and each component evaluated over the child
@@ -893,7 +1023,8 @@ def window_lists(obj: Objective) -> str:
     tab = _tab_ok(sums)
     out.append(f"#define GPUGO_TAB {int(tab)}")
     if tab:
-        out += _window_tables(value, grad, sums, slots, global_grad, _grad_tab_ok(grad, global_grad))
+        out += _window_tables(value, grad, sums, slots, global_grad, _grad_tab_ok(grad, global_grad),
+                              _parts_ok(grad, global_grad))
     out.append("}  // namespace gpugo")
     return "\n".join(out) + "\n"
 

@@ -214,7 +214,7 @@ def has_loop_index(e: Expr) -> bool:
     (a TSum binds J inside itself)."""
     if e.op in ("var", "param") and isinstance(e.val, tuple):
         return True
-    if e.op in ("jnum", "tprodx"):
+    if e.op in ("jnum", "tprodx", "tpart"):
         return True
     if e.op in TEMPLATE_REDUCTIONS:
         return False

@@ -5,7 +5,10 @@ coordinates in its stencil (the other coordinates are the parent record's),
 so the step's prep kernel evaluates each term once per digit combination (at
 the record's box and at its diagonal sample points, FP64 and FP32, and the
 gradient components of local gradient templates) and the split+bound and
-sample kernels add table entries instead of evaluating terms per child.  The
+sample kernels add table entries instead of evaluating terms per child; a
+gradient with template sums (all d components tested per child) reads each
+component's local parts (the subexpressions in X(J + k) alone) from a parts
+table instead.  The
 table entry is the same interval operations on the same intervals, added in
 the same order, so the bar is bit-identity with the per-child evaluation
 (GPUGO_FLAG_NO_TABLES):
@@ -18,7 +21,9 @@ the same order, so the bar is bit-identity with the per-child evaluation
   - the FP32-prune form (FP32 tables, the FP32 sample pass) on the d = 50
     ackley_q headline problem and on Levy and Rastrigin at d = 50;
   - the 15-significant-digit IA (Levy d = 50);
-  - Levy and Rastrigin at d = 1000.
+  - Levy, Rastrigin and Ackley at d = 1000 (Ackley's gradient uses template
+    sums, so every component is tested: its local parts come from the parts
+    table).
 Solves also certify and contain f* where it is known.
 """
 import numpy as np
@@ -127,7 +132,7 @@ def test_tables_equal_terms_digits15(gpu):
     assert r["status_name"] == "certified" and r["lb"] <= s.f_star <= r["ub"], r
 
 
-@pytest.mark.parametrize("name", ["levy", "rastrigin"])
+@pytest.mark.parametrize("name", ["levy", "rastrigin", "ackley"])
 def test_tables_equal_terms_d1000(gpu, name):
     d = 1000
     s = F.spec(name, d)
