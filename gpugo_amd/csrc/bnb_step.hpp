@@ -1199,6 +1199,14 @@ __device__ __forceinline__ void root_body(const RootArgs &a) {
 
 }  // namespace gpugo
 
+// queues up to this many entries get the stop's final compaction from the
+// workgroup that stopped the problem; larger ones in a launch of their own
+#ifndef GPUGO_FINAL_INLINE
+#define GPUGO_FINAL_INLINE 65536
+#endif
+// bit of the polled status word: the stop's final compaction has not run
+#define GPUGO_POLL_FINAL 0x100
+
 namespace gpugo {
 // Compaction of this step fused with the select of the next one: every
 // workgroup compacts its share; the last to finish (ticket in st->cblocks,
@@ -1228,8 +1236,40 @@ __device__ __forceinline__ void compact_select_body(const CompactArgs &ca, const
   if (!s_last) return;
   select_body(sa);
   __syncthreads();
+  // A stop with a small queue: this workgroup runs the stop's final
+  // compaction itself (from the buffer the select read into the other one),
+  // so the host needs no launch after the stop.
+  __shared__ int s_fin;
+  if (threadIdx.x == 0) {
+    const DevState *st = ca.st;
+    s_fin = st->status != GPUGO_ST_RUNNING && st->compact_final && st->q_count <= GPUGO_FINAL_INLINE;
+  }
+  __syncthreads();
+  if (s_fin) {
+    CompactArgs fa = ca;
+    if (!ca.cmin) {
+      // a full pass wrote the out buffer and the select read it
+      fa.lb_in = ca.lb_out;
+      fa.par_in = ca.par_out;
+      fa.pos_in = ca.pos_out;
+      // (both buffers are the problem's own writable queue arrays)
+      fa.lb_out = const_cast<double *>(ca.lb_in);
+      fa.par_out = const_cast<unsigned *>(ca.par_in);
+      fa.pos_out = const_cast<unsigned *>(ca.pos_in);
+    }
+    fa.wmin = nullptr;
+    fa.cmin = nullptr;
+    fa.poll = nullptr;
+    compact_body(fa, 0, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) ca.st->compact_final = 0;
+    __syncthreads();
+  }
+  // the host reads the status, with GPUGO_POLL_FINAL if the stop's final
+  // compaction is still to run
   if (poll && threadIdx.x == 0)
-    __hip_atomic_store(poll, ca.st->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(poll, ca.st->status | (ca.st->compact_final ? GPUGO_POLL_FINAL : 0), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
 }
 }  // namespace gpugo
 

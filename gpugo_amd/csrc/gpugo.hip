@@ -1808,10 +1808,11 @@ static int run_steps(gpugo_problem_t p, int64_t iters, gpugo_result *res, bool s
         const int old = first ? slot : slot ^ 1;
         HIPCHECK(hipEventSynchronize(p->poll_ev[old]));
         --pending;
-        if (__atomic_load_n(&p->hpoll[old], __ATOMIC_ACQUIRE) != GPUGO_ST_RUNNING) {
-          // no later step is in flight to run the stop step's final
-          // compaction: enqueue it alone
-          if (pending == 0) {
+        const int pv = __atomic_load_n(&p->hpoll[old], __ATOMIC_ACQUIRE);
+        if (pv != GPUGO_ST_RUNNING) {
+          // the stop's final compaction still to run (a large queue) and no
+          // later step in flight to run it: enqueue it alone
+          if (pending == 0 && (pv & GPUGO_POLL_FINAL)) {
             int rc = enqueue_step(p, nullptr, false, true);
             if (rc) return rc;
           }

@@ -570,12 +570,28 @@ class _FuncWT(_FuncW):
         else:
             self.lines.append(f"  iv<double> {acc} = rest[{s}];")
         fn = "mul" if e.op == "tprod" else "add"
+        w = khi - klo + 1
+        # the window's terms as one run q = 0 .. nq-1 (term j of range 0,
+        # then of range 1); table entries are loaded TAB_LOADS at a time and
+        # folded in q order (the same order as term by term)
         self.lines.append(f"  {{ int rlo[2], rhi[2]; const int nr = wranges_{s}(win, c, d, rlo, rhi);")
         self.lines.append(f"    const iv<{ty}> *__restrict__ Ts = T + L.off[{s}]; const unsigned Es = L.E[{s}];")
-        self.lines.append("    unsigned q = 0;")
-        self.lines.append("    for (int r = 0; r < nr; ++r)")
-        self.lines.append(f"      for (int j = rlo[r]; j < rhi[r]; ++j, ++q) {acc} = A::{fn}({acc}, "
-                          f"Ts[q * Es + tab_idx<{khi - klo + 1}>(xd, j + ({klo}))]); }}")
+        self.lines.append("    const int n0 = nr > 0 ? rhi[0] - rlo[0] : 0, nq = n0 + (nr > 1 ? rhi[1] - rlo[1] : 0);")
+        self.lines.append("    int q = 0;")
+        self.lines.append(f"    for (; q + {TAB_LOADS} <= nq; q += {TAB_LOADS}) {{")
+        self.lines.append(f"      iv<{ty}> te[{TAB_LOADS}];")
+        self.lines.append("#pragma unroll")
+        self.lines.append(f"      for (int u = 0; u < {TAB_LOADS}; ++u) {{")
+        self.lines.append("        const int qq = q + u, j = qq < n0 ? rlo[0] + qq : rlo[1] + (qq - n0);")
+        self.lines.append(f"        te[u] = Ts[qq * Es + tab_idx<{w}>(xd, j + ({klo}))];")
+        self.lines.append("      }")
+        self.lines.append("#pragma unroll")
+        self.lines.append(f"      for (int u = 0; u < {TAB_LOADS}; ++u) {acc} = A::{fn}({acc}, te[u]);")
+        self.lines.append("    }")
+        self.lines.append("    for (; q < nq; ++q) {")
+        self.lines.append("      const int j = q < n0 ? rlo[0] + q : rlo[1] + (q - n0);")
+        self.lines.append(f"      {acc} = A::{fn}({acc}, Ts[q * Es + tab_idx<{w}>(xd, j + ({klo}))]);")
+        self.lines.append("    } }")
         return ("i", acc)
 
     def loop_prod_except(self, e: Expr) -> Tuple[str, str]:
@@ -597,6 +613,8 @@ class _FuncWT(_FuncW):
 
 # widest stencil (coordinates per term) a term table covers: wid^w entries per term
 TAB_MAX_W = 3
+# table entries a child loads before folding them (loads in flight together)
+TAB_LOADS = 4
 
 
 def _grad_parts(grad: Expr) -> List[Expr]:
