@@ -1367,10 +1367,10 @@ _FLOP = {"add": 2, "sub": 2, "neg": 0, "scale": 2, "addc": 2, "rsubc": 2, "divc"
          "mul": 8, "div": 10, "sqrt": 2, "exp": 40, "sin": 40, "cos": 40}
 
 
-def _count(exprs, tsum_leaf: bool = False) -> int:
+def _count(exprs, tsum_leaf: bool = False, only=None) -> int:
     """FLOP of the expressions (each distinct subexpression once); with
     tsum_leaf, template sums count as leaves (their terms are counted by the
-    caller)."""
+    caller); `only` (a predicate) counts only the nodes it accepts."""
     seen = set()
     total = 0
 
@@ -1384,8 +1384,10 @@ def _count(exprs, tsum_leaf: bool = False) -> int:
             return
         for a in e.args:
             walk(a)
+        if only is not None and not only(e):
+            return
         op = e.op
-        if op in ("var", "const", "param", "iconst", "guard", "twhen", "dim", "jnum") or e.is_scalar():
+        if op in ("var", "const", "param", "iconst", "guard", "twhen", "dim", "jnum", "tpart") or e.is_scalar():
             return
         if op == "gsum":
             total += _FLOP["add"] * (len(e.args) - 1)
@@ -1454,8 +1456,19 @@ def window_flop_counts(obj: Objective, c: int, tables: bool = False, wid: int = 
     # a product-except-j: the window's other terms, c - 1 products each
     px = sum(min(c, obj.d) * ((0 if tab else _count([ts.args[0]])) + _FLOP["mul"])
              for ts in _tsums(grad) if _has_op(grad, "tprodx") and ts.op == "tprod")
-    g = 0 if gtab else gsum + ncomp * (_count([grad], tsum_leaf=True) + px)
+    ptab = tab and not gtab and _parts_ok(grad, bool(_global_reductions(grad)))
+    parts = _grad_parts(grad) if ptab else []
+    # the parts come from the parts table: a component costs its skeleton
+    skel = _replace(grad, {prt.key(): Expr("tpart", (), k) for k, prt in enumerate(parts)}) if ptab else grad
+    # loop-invariant subexpressions (the gradient's globals) are evaluated
+    # once per child, the rest once per component
+    comp = _count([skel], tsum_leaf=True, only=has_loop_index)
+    hoisted = _count([skel], tsum_leaf=True, only=lambda e: not has_loop_index(e))
+    g = 0 if gtab else gsum + hoisted + ncomp * (comp + px)
     table = 0
+    if ptab:
+        gklo, gkhi = _x_offsets(grad, "gradient template")
+        table += (obj.d + min(c + gkhi - gklo, obj.d) * wid ** (gkhi - gklo + 1)) * _count(parts)
     if tab:
         for ts in sums:
             klo, khi = _x_offsets(ts.args[0], "template sum")
