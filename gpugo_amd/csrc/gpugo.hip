@@ -1010,6 +1010,7 @@ __global__ __launch_bounds__(256) void k_materialize(const double *plo, const do
 #define GPUGO_CHUNK_LOG2 13    // lazy queue: 8192 entries per chunk minimum
 #define GPUGO_LAZY_MAX_EVERY 256
 #define GPUGO_TAB_MAX (1 << 20) // term-table entries per parent record, at most
+#define GPUGO_POLL_PENDING (-1) // polled status word before the device writes it
 static_assert(GPUGO_LAZY_WG <= GPUGO_WMIN_MAX && GPUGO_LAZY_FULL_WG <= GPUGO_WMIN_MAX,
               "wmin holds one key per compaction or lazy-tail workgroup");
 
@@ -1751,6 +1752,38 @@ static int persist_steps(gpugo_problem_t p, int64_t iters) {
   return GPUGO_OK;
 }
 
+// The status word of a batch of steps: the host spins on the pinned word
+// (the batch's last compaction stores it with a system-scope store) instead
+// of blocking in hipEventSynchronize, whose wake-up left the GPU idle for
+// tens of microseconds between batches.  Every few hundred spins it asks the
+// batch's event; if that completed without a word (not expected), the
+// status is read from the device state.
+static int wait_poll(gpugo_problem_t p, int slot, int *out) {
+  for (unsigned it = 1;; ++it) {
+    const int v = __atomic_load_n(&p->hpoll[slot], __ATOMIC_ACQUIRE);
+    if (v != GPUGO_POLL_PENDING) {
+      *out = v;
+      return GPUGO_OK;
+    }
+    if ((it & 511u) == 0u) {
+      const hipError_t e = hipEventQuery(p->poll_ev[slot]);
+      if (e == hipSuccess) {
+        const int w = __atomic_load_n(&p->hpoll[slot], __ATOMIC_ACQUIRE);
+        if (w != GPUGO_POLL_PENDING) {
+          *out = w;
+          return GPUGO_OK;
+        }
+        int st = 0;
+        HIPCHECK(hipMemcpy(&st, &p->st.as<DevState>()->status, sizeof st, hipMemcpyDeviceToHost));
+        *out = st;
+        return GPUGO_OK;
+      }
+      if (e != hipErrorNotReady) return set_err(GPUGO_E_HIP, "event query: %s", hipGetErrorString(e));
+    }
+    __builtin_ia32_pause();
+  }
+}
+
 static int run_steps(gpugo_problem_t p, int64_t iters, gpugo_result *res, bool solve) {
   if (!p || !res) return set_err(GPUGO_E_INVALID, "null pointer");
   auto t0 = std::chrono::steady_clock::now();
@@ -1796,7 +1829,7 @@ static int run_steps(gpugo_problem_t p, int64_t iters, gpugo_result *res, bool s
     while (done < iters) {
       const bool first = p->steps_enqueued == 0;
       int64_t batch = first ? 1 : (B < iters - done ? B : iters - done);
-      p->hpoll[slot] = GPUGO_ST_RUNNING;
+      __atomic_store_n(&p->hpoll[slot], GPUGO_POLL_PENDING, __ATOMIC_RELEASE);
       for (int64_t i = 0; i < batch; ++i) {
         int rc = enqueue_step(p, i + 1 == batch ? p->dpoll + slot : nullptr, done + i + 1 < iters);
         if (rc) return rc;
@@ -1806,9 +1839,10 @@ static int run_steps(gpugo_problem_t p, int64_t iters, gpugo_result *res, bool s
       ++pending;
       if (pending == 2 || first) {
         const int old = first ? slot : slot ^ 1;
-        HIPCHECK(hipEventSynchronize(p->poll_ev[old]));
+        int pv;
+        int rc = wait_poll(p, old, &pv);
+        if (rc) return rc;
         --pending;
-        const int pv = __atomic_load_n(&p->hpoll[old], __ATOMIC_ACQUIRE);
         if (pv != GPUGO_ST_RUNNING) {
           // the stop's final compaction still to run (a large queue) and no
           // later step in flight to run it: enqueue it alone
