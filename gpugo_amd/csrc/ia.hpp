@@ -47,6 +47,15 @@
 #ifndef GPUGO_IA_HPP
 #define GPUGO_IA_HPP
 
+// exp and sin/cos of the interval type: out of line by default (unrolled code
+// lists call them many times); a module that calls them a few times per box
+// defines GPUGO_TRANS_INLINE 1 and saves the call's register saves
+#if defined(GPUGO_TRANS_INLINE) && GPUGO_TRANS_INLINE
+#define GPUGO_TRANS_ATTR __forceinline__
+#else
+#define GPUGO_TRANS_ATTR __attribute__((noinline))
+#endif
+
 namespace gpugo {
 
 enum { ROUND_EXACT = 0, ROUND_FAST = 1, ROUND_DIGITS15 = 2 };
@@ -680,8 +689,9 @@ template <typename T, int R> struct IA {
     return mk<T>(l, RR::sqrt_up(a.hi));
   }
 
-  // out of line: keeps unrolled code lists (and hipRTC compile time) small
-  __device__ __attribute__((noinline)) static V exp(V a) {
+  // out of line: keeps unrolled code lists (and hipRTC compile time) small;
+  // inline where a code list calls them a few times (GPUGO_TRANS_INLINE)
+  __device__ GPUGO_TRANS_ATTR static V exp(V a) {
     const iv<double> el = TR::template exp<LP>((double)a.lo), eh = TR::template exp<LP>((double)a.hi);
     T l = max_<T>(out_dn<RR>(to_dn<T>(el.lo)), T(0));
     return mk<T>(l, out_up<RR>(to_up<T>(eh.hi)));
@@ -689,7 +699,7 @@ template <typename T, int R> struct IA {
 
   // shift 0: cos (extrema at m*pi), shift 1: sin (extrema at pi/2 + m*pi);
   // m even -> maximum +1, m odd -> minimum -1.
-  template <int SHIFT> __device__ __attribute__((noinline)) static V trig(V a) {
+  template <int SHIFT> __device__ GPUGO_TRANS_ATTR static V trig(V a) {
     const T one = T(1);
     if (!(fp<T>::abs_(a.lo) <= fp<T>::TRIG_BIG && fp<T>::abs_(a.hi) <= fp<T>::TRIG_BIG))
       return mk<T>(-one, one);

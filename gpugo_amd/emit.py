@@ -615,6 +615,10 @@ class _FuncWT(_FuncW):
 TAB_MAX_W = 3
 # table entries a child loads before folding them (loads in flight together)
 TAB_LOADS = 4
+# windowed modules keep exp/sin/cos out of line (inlined, measured with term
+# tables: the d = 50 headline unchanged, its ten-sample solve 10% faster, Salomon,
+# Belegundu, Griewank and Zabinsky at d = 1000 5-9% slower, compile time doubled)
+WINDOW_TRANS_INLINE = False
 
 
 def _grad_parts(grad: Expr) -> List[Expr]:
@@ -1137,6 +1141,7 @@ def window_module_source(obj: Objective, roundings=("fast",), f32: bool = False)
         # a child's coordinates are read only inside its window (c of them):
         # let the compiler keep the parent loads instead of redoing them
         "#define GPUGO_ACCESSOR_D 0",
+        f"#define GPUGO_TRANS_INLINE {int(WINDOW_TRANS_INLINE)}",
         _read("ia.hpp"), _read("bnb_args.hpp"), _read("bnb_box.hpp"),
         window_lists(obj),
     ]
