@@ -222,6 +222,63 @@ def test_lazy_queue_boxes_between_calls(gpu):
         assert calls >= 3 and ra["status_name"] == "certified"
 
 
+def _live_rows(p, cap=1 << 16):
+    lo, hi, lb, n = p.boxes(cap)
+    a = np.column_stack([lb, lo, hi])
+    return a[np.lexsort(a.T[::-1])], n
+
+
+def test_stop_final_compaction_small_queue(gpu):
+    """The stop step's final compaction, run by the workgroup whose select
+    stopped the problem (a queue of at most 65,536 entries): gpugo_solve
+    (first step alone, then steps two at a time) leaves the same live boxes,
+    queue size and enclosure as steps taken in calls of 3, whose last step
+    always compacts in a launch of its own."""
+    spec = pb.rastrigin(3)
+    kw = dict(tol=1e-6, grid=(4, 3), k_select=1, max_iters=20000)
+    with gpu_problem(gpu, spec, **kw) as a, gpu_problem(gpu, spec, **kw) as b:
+        ra = a.solve()
+        while True:
+            rb = b.step(3)
+            if rb["status_name"] != "running":
+                break
+        assert ra["status_name"] == rb["status_name"] == "certified"
+        for key in ("lb", "ub", "width", "iterations", "boxes_bounded", "queue_size"):
+            assert ra[key] == rb[key], (key, ra[key], rb[key])
+        xa, na = _live_rows(a)
+        xb, nb = _live_rows(b)
+        assert na == nb == ra["queue_size"] and 0 < na <= 65536
+        assert np.array_equal(xa, xb)
+        assert np.all(xa[:, 0] <= ra["ub"])
+
+
+def test_stop_final_compaction_large_queue(gpu):
+    """A stop whose queue holds more than 65,536 entries (live or stale):
+    the final compaction runs in a launch of its own (enqueued by the host
+    when no later step is in flight).  Levy d = 1000 (windowed, the
+    reference's box, 10 samples; the queue peaks near 5e7 entries): the
+    solve's queue size equals the live boxes the library reports, every one
+    with lb <= ub, and the eager-compaction solve's."""
+    from gpugo_amd import families as F
+    d = 1000
+    s = F.spec("levy", d)
+    obj = F.objective_loops("levy", d)
+    m = gpu.get_module(obj, roundings=("exact", "fast"), windowed=True)
+    kw = dict(tol=1e-6, grid=(3, 12), k_select=1, n_samples=10, init_ub=s.extra["init_ub"], module=m,
+              windowed=True, max_iters=20000, max_queue=1 << 28)
+    with gpu.Problem(obj, np.zeros(0), s.lo, s.hi, **kw) as a:
+        ra = a.solve()
+        lo, hi, lb, n = a.boxes(4096)
+    with gpu.Problem(obj, np.zeros(0), s.lo, s.hi, eager_compact=True, **kw) as b:
+        rb = b.solve()
+    assert ra["status_name"] == rb["status_name"] == "certified"
+    assert ra["queue_peak"] > 65536, ra["queue_peak"]
+    assert n == ra["queue_size"] == rb["queue_size"]
+    assert np.all(lb <= ra["ub"]) and np.all(lo <= hi)
+    for key in ("lb", "ub", "width", "iterations", "boxes_bounded"):
+        assert ra[key] == rb[key], (key, ra[key], rb[key])
+
+
 def test_unknown_flag_rejected(gpu):
     """A description with a flag bit the header does not define is refused
     with GPUGO_E_INVALID and a message; the eager-compaction and no-tables
