@@ -38,7 +38,7 @@ EXPORTS = [
     "gpugo_problem_timing", "gpugo_batch_bytes", "gpugo_mem_info", "gpugo_batch_create",
     "gpugo_batch_admit", "gpugo_batch_run", "gpugo_batch_sync", "gpugo_batch_results",
     "gpugo_batch_counts", "gpugo_batch_destroy", "gpugo_eval_boxes_f32", "gpugo_window_ranges",
-    "gpugo_solve_static",
+    "gpugo_solve_static", "gpugo_batch_advance",
 ]
 
 
@@ -70,6 +70,7 @@ class BatchDesc(ctypes.Structure):
         ("slots", ctypes.c_int32), ("steps_per_launch", ctypes.c_int32), ("rounding", ctypes.c_int32),
         ("use_graph", ctypes.c_int32), ("max_queue", ctypes.c_int64), ("max_parents", ctypes.c_int64),
         ("bound_blocks", ctypes.c_int32), ("grow", ctypes.c_int32),
+        ("promote_age", ctypes.c_int32), ("pad0", ctypes.c_int32),
     ]
 
 
@@ -134,6 +135,7 @@ def load():
     lib.gpugo_batch_create.argtypes = [vp, ctypes.POINTER(BatchDesc), ctypes.POINTER(vp)]
     lib.gpugo_batch_admit.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ProblemDesc), ctypes.c_int64, I32]
     lib.gpugo_batch_run.argtypes = [vp, ctypes.c_int64]
+    lib.gpugo_batch_advance.argtypes = [vp, I64]
     lib.gpugo_batch_sync.argtypes = [vp]
     lib.gpugo_batch_results.argtypes = [vp, ctypes.c_int64, I64, ctypes.POINTER(Result), I64]
     lib.gpugo_batch_counts.argtypes = [vp, I64, I64, I64]

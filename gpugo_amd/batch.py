@@ -108,12 +108,15 @@ class Batch:
     def __init__(self, slots: int = 4096, *, families: Sequence[str] = pb.FAMILIES, dmax: int = 32,
                  rounding: str = "fast", steps_per_launch: Optional[int] = None, use_graph: bool = True,
                  max_queue: int = 0, max_parents: int = 0, bound_blocks: Optional[int] = None, grow: int = 2,
-                 module: Optional[BatchModule] = None):
+                 module: Optional[BatchModule] = None, promote_age: int = 0):
         """grow: larger tiers (0..3, each 8x the per-slot capacity of the one
         below) into which a problem that outgrows its slot moves, state and
         all, and resumes; 0 reports such a problem as overflow.
         steps_per_launch / bound_blocks: None takes the tuned launch shape
-        of the schedule table (gpugo_amd.schedule.launch_for)."""
+        of the schedule table (gpugo_amd.schedule.launch_for).
+        promote_age (with grow): a tier-0 problem that has run this many
+        steps moves to tier 1 (0: never), so tier 0's launches stay short
+        for the problems admitted since; same results."""
         from .schedule import launch_for
         bb0, spl0 = launch_for()
         steps_per_launch = spl0 if steps_per_launch is None else steps_per_launch
@@ -123,7 +126,7 @@ class Batch:
         self.desc = _lib.BatchDesc(slots=int(slots), steps_per_launch=int(steps_per_launch),
                                    rounding=_lib.ROUND[rounding], use_graph=int(bool(use_graph)),
                                    max_queue=int(max_queue), max_parents=int(max_parents),
-                                   bound_blocks=int(bound_blocks), grow=int(grow))
+                                   bound_blocks=int(bound_blocks), grow=int(grow), promote_age=int(promote_age))
         h = ctypes.c_void_p()
         check(_lib.load().gpugo_batch_create(self.module.handle, ctypes.byref(self.desc), ctypes.byref(h)))
         self.handle = h
@@ -175,6 +178,14 @@ class Batch:
 
     def run(self, launches: int = 1):
         check(_lib.load().gpugo_batch_run(self.handle, int(launches)))
+
+    def advance(self) -> int:
+        """Collect what has landed and launch every tier that has running
+        slots and fewer than two launches in flight, without waiting;
+        returns the launches made."""
+        n = ctypes.c_int64(0)
+        check(_lib.load().gpugo_batch_advance(self.handle, ctypes.byref(n)))
+        return n.value
 
     def sync(self):
         check(_lib.load().gpugo_batch_sync(self.handle))

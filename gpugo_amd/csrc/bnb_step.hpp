@@ -125,7 +125,7 @@ struct SelectArgs {
                          // before anything is consumed, instead of overflowing
   long long q_cap;       // queue capacity (entries), read when grow is set
   unsigned W;            // wid^c children per split box, read when grow is set
-  int pad_grow;
+  int grow_age;          // with grow: > 0 also stops a slot that has run this many steps
 };
 
 // Selection key (lb, parent, pos), unique per box.  Parent records are
@@ -732,9 +732,11 @@ __device__ void select_body(const SelectArgs &a, const BandView *band = nullptr)
   unsigned n_split, n_outs;
   const unsigned split_rank = block_excl_scan((tid < K && !is_out) ? 1u : 0u, scan_sh, &n_split);
   const unsigned out_rank = block_excl_scan((tid < K && is_out) ? 1u : 0u, scan_sh, &n_outs);
-  if (a.grow && N + (long long)n_split * (long long)a.W > a.q_cap) {
-    // this step's children might not fit the queue: stop before anything is
-    // consumed, so the slot can resume in a larger tier
+  if (a.grow && (N + (long long)n_split * (long long)a.W > a.q_cap ||
+                 (a.grow_age > 0 && st->iterations >= (long long)a.grow_age))) {
+    // this step's children might not fit the queue (or the slot has run its
+    // steps in this tier): stop before anything is consumed, so the slot can
+    // resume in a larger tier
     if (tid == 0) {
       st->status = GPUGO_ST_GROW;
       st->q_count = N;

@@ -653,7 +653,7 @@ struct BatchArgs {
   int dmax, pmax, slots;
   int grow;  // 1: a larger tier exists; slots that would outgrow this one stop with GPUGO_ST_GROW
   int cb;    // compaction workgroups per slot
-  int pad_cb;
+  int grow_age;  // tier 0 with a tier above: slots that have run this many steps move up (0: never)
   // priority bands: [slots][band_cap] entries and [slots] states
   double *band_lb;
   unsigned *band_par, *band_pos, *band_idx;
@@ -712,7 +712,7 @@ __global__ __launch_bounds__(SEL_T) void kb_select(BatchArgs b) {
   sa.grow = b.grow;
   sa.q_cap = b.qcap;
   sa.W = I.W;
-  sa.pad_grow = 0;
+  sa.grow_age = b.grow_age;
   const BandView bv = slot_band(b, slot);
   select_body(sa, &bv);
 }
@@ -1436,7 +1436,7 @@ static SelectArgs select_args(gpugo_problem_t p, const Queue &q, int nwg) {
   sa.grow = 0;
   sa.q_cap = p->q_cap;
   sa.W = p->W;
-  sa.pad_grow = 0;
+  sa.grow_age = 0;
   return sa;
 }
 
@@ -1718,7 +1718,7 @@ static int persist_steps(gpugo_problem_t p, int64_t iters) {
   sa.grow = 0;
   sa.q_cap = p->q_cap;
   sa.W = p->W;
-  sa.pad_grow = 0;
+  sa.grow_age = 0;
   for (int b = 0; b < 2; ++b) {
     pa.q_lb[b] = p->q[b].plb();
     pa.q_par[b] = p->q[b].ppar();
@@ -2119,6 +2119,7 @@ static int batch_sizes(gpugo_module_t m, const gpugo_batch_desc *desc, long long
     return set_err(GPUGO_E_INVALID, "negative capacity");
   if (desc->grow < 0 || desc->grow > GPUGO_GROW_MAX)
     return set_err(GPUGO_E_INVALID, "grow must be in [0, %d]", GPUGO_GROW_MAX);
+  if (desc->promote_age < 0) return set_err(GPUGO_E_INVALID, "promote_age must be >= 0");
   const long long q = desc->max_queue ? desc->max_queue : 65536, p = desc->max_parents ? desc->max_parents : 16384;
   if (q > 0xfffffff0LL || p > 0xfffffff0LL) return set_err(GPUGO_E_INVALID, "per-slot capacity too large");
   const long long D = m->dim, PM = m->pmax;
@@ -2291,7 +2292,7 @@ static int batch_create_level(gpugo_module_t m, const gpugo_batch_desc *desc, in
   a.grow = grow_next;
   // compaction workgroups per slot: 8, and 8 times more per growth tier
   a.cb = 8 << (3 * std::min(level, 2));
-  a.pad_cb = 0;
+  a.grow_age = (level == 0 && grow_next) ? desc->promote_age : 0;
   a.band_lb = b->band_lb.as<double>();
   a.band_par = b->band_par.as<unsigned>();
   a.band_pos = b->band_pos.as<unsigned>();
@@ -2598,7 +2599,10 @@ static int batch_process_moves(gpugo_batch_t r) {
 // One launch of tier b (its graph, or steps_per_launch steps), then a copy of
 // all slot states to pinned memory; at most two copies are in flight, the
 // older one is collected first.
-static int batch_launch_one(gpugo_batch_t b) {
+// One launch of a tier, then a copy of its slots' states to host memory.
+// block: keep at most one launch in flight (wait for the older one); else
+// the caller has made sure that fewer than two are in flight.
+static int batch_launch_one(gpugo_batch_t b, bool block = true) {
   int rc;
   if (b->gexec) {
     HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
@@ -2617,10 +2621,48 @@ static int batch_launch_one(gpugo_batch_t b) {
   HIPCHECK(hipEventRecord(b->pev[k], b->stream));
   b->poll_seq[k] = b->polls++;
   ++b->pend_n;
-  if (b->pend_n == 2) {
+  if (block && b->pend_n == 2) {
     rc = batch_collect_one(b);
     if (rc) return rc;
   }
+  return GPUGO_OK;
+}
+
+// Collect the launches of a tier whose state copies have landed, oldest
+// first, without waiting for the others.
+static int batch_collect_ready(gpugo_batch_t b) {
+  while (b->pend_n > 0) {
+    const hipError_t e = hipEventQuery(b->pev[b->pend_first]);
+    if (e == hipErrorNotReady) return GPUGO_OK;
+    if (e != hipSuccess) return set_err(GPUGO_E_HIP, "event query: %s", hipGetErrorString(e));
+    const int rc = batch_collect_one(b);
+    if (rc) return rc;
+  }
+  return GPUGO_OK;
+}
+
+extern "C" int gpugo_batch_advance(gpugo_batch_t b, int64_t *launched) {
+  if (!b) return set_err(GPUGO_E_INVALID, "null batch");
+  if (b->level != 0) return set_err(GPUGO_E_INVALID, "not a batch handle");
+  int rc = batch_flush(b);
+  if (rc) return rc;
+  std::vector<gpugo_batch_t> lv{b};
+  lv.insert(lv.end(), b->tiers.begin(), b->tiers.end());
+  for (auto *x : lv) {
+    rc = batch_collect_ready(x);
+    if (rc) return rc;
+  }
+  int64_t n = 0;
+  for (auto *x : lv) {
+    if (x->running > 0 && x->pend_n < 2) {
+      rc = batch_launch_one(x, false);
+      if (rc) return rc;
+      ++n;
+    }
+  }
+  rc = batch_process_moves(b);
+  if (rc) return rc;
+  if (launched) *launched = n;
   return GPUGO_OK;
 }
 

@@ -8,8 +8,11 @@ slot count of each class is the most whose batch, tiers included, fits the
 device memory it is given.  A request is
 admitted into a free slot of the smallest class that holds its dimension
 as soon as one is free, so its work overlaps on the GPU with whatever else
-is running; every batch advances by one launch per loop turn on its own
-stream.  Latency is measured from arrival to the host seeing the result.
+is running; every batch advances on its own stream, and a loop turn never
+waits for a batch: it collects the launches whose results have landed and
+launches again where fewer than two launches are in flight
+(gpugo_batch_advance).  Latency is measured from arrival to the host seeing
+the result.
 
 A slot runs the per-problem loop of gpugo_solve, so each result is the
 same alone and in any mix (bit for bit, for the same schedule).
@@ -30,6 +33,10 @@ from .batch import Batch, BatchProblem, get_batch_module
 from .schedule import fits, schedule_for
 
 DMAX_CLASSES = (16, 32, 64, 100)
+# steps after which a request leaves its class's tier 0 for tier 1: the
+# long-running requests then no longer lengthen tier 0's launches, which
+# every new request waits on
+PROMOTE_AGE = 64
 
 
 @dataclass
@@ -48,7 +55,7 @@ class Server:
     def __init__(self, slots: Optional[Dict[int, int]] = None, *, dmax_classes: Sequence[int] = DMAX_CLASSES,
                  mem_fraction: float = 0.5, max_queue: int = 1 << 18, max_parents: int = 8192,
                  max_iters: int = 2000, steps_per_launch: Optional[int] = None, rounding: str = "fast",
-                 grow: int = 2):
+                 grow: int = 2, promote_age: int = PROMOTE_AGE):
         _lib.init()
         self.classes = sorted(dmax_classes)
         self.max_queue = max_queue
@@ -66,7 +73,7 @@ class Server:
         for dm in self.classes:
             self.batches[dm] = Batch(slots=self.slots[dm], module=mods[dm], rounding=rounding,
                                      steps_per_launch=steps_per_launch, max_queue=max_queue,
-                                     max_parents=max_parents, grow=grow)
+                                     max_parents=max_parents, grow=grow, promote_age=promote_age)
         self.waiting: List[Request] = []
         self.inflight: Dict[int, Request] = {}
 
@@ -121,10 +128,12 @@ class Server:
                 nxt += 1
             if self.waiting:
                 self._admit_waiting(now)
+            # every batch advances on its own: one with long steps in flight
+            # does not hold up the others (gpugo_batch_advance never waits)
             busy = False
             for b in self.batches.values():
                 if b.counts()[0] > 0:
-                    b.run(1)
+                    b.advance()
                     busy = True
             self._collect(time.perf_counter() - t0, done)
             if not busy and not self.waiting and nxt < len(order):

@@ -328,6 +328,12 @@ typedef struct {
                                 same result as in a slot large enough from the
                                 start; only the last tier reports overflow.
                                 gpugo_batch_bytes counts all tiers.          */
+  int32_t promote_age;       /* with grow > 0: a tier-0 problem that has run
+                                this many steps moves to tier 1 the same way
+                                (0: never), so tier 0's launches stay short
+                                for the problems admitted since (served
+                                form).  Same results.                        */
+  int32_t pad0;
 } gpugo_batch_desc;
 
 typedef struct gpugo_batch_s *gpugo_batch_t;
@@ -349,6 +355,13 @@ int  gpugo_batch_admit(gpugo_batch_t b, int32_t family, const gpugo_problem_desc
  * collects the slots that finished (see gpugo_batch_results).  Returns early
  * when no slot is running. */
 int  gpugo_batch_run(gpugo_batch_t b, int64_t launches);
+/* Continuous batching without waiting (no counterpart in the reference,
+ * which has no served form: for a request loop over several batches, so a
+ * batch busy with long steps does not hold up the others).  Collects the
+ * launches whose results have landed, then enqueues one more launch of every
+ * tier that has running slots and fewer than two launches in flight;
+ * *launched (or null) receives the launches enqueued.  Never blocks. */
+int  gpugo_batch_advance(gpugo_batch_t b, int64_t *launched);
 /* Wait for the launch in flight and collect what finished. */
 int  gpugo_batch_sync(gpugo_batch_t b);
 /* Move up to `max` finished results (tags[i], res[i]) out of the batch;

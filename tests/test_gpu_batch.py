@@ -166,7 +166,8 @@ def test_growth_equals_single(gpu, bmod):
             assert r[key] == s[key], (p.tag, p.family, p.d, key, r[key], s[key])
 
 
-def test_c3_workload_equals_single(gpu):
+@pytest.mark.parametrize("promote_age", [0, 16])
+def test_c3_workload_equals_single(gpu, promote_age):
     """The 4096-problem workload's first 64 problems as configured
     (problems.batch_problem, seeds [3, k]: the three families, d
     log-uniform in [2, 32], box [-h, h]^d with h ~ U[2, 32], tol in
@@ -174,7 +175,9 @@ def test_c3_workload_equals_single(gpu):
     bench's per-slot capacities (queue 2^18, 8192 parent records) and two
     growth tiers: every result equals gpugo_solve with capacities large
     enough from the start, nothing overflows, and certified enclosures are
-    no wider than tol and contain f* where it is known in closed form."""
+    no wider than tol and contain f* where it is known in closed form.
+    promote_age 16: every problem that runs 16 steps also moves to tier 1
+    (the served form's promotion), with the same results."""
     from gpugo_amd.schedule import schedule_for
     m = get_batch_module(pb.FAMILIES, 32, ("fast",))
     probs = []
@@ -184,7 +187,7 @@ def test_c3_workload_equals_single(gpu):
         probs.append(BatchProblem.from_spec(s, grid=grid, k_select=ks, max_iters=2000, tag=k))
         probs[-1].extra["f_star"] = s.f_star
     with Batch(slots=24, module=m, rounding="fast", steps_per_launch=8, max_queue=1 << 18, max_parents=8192,
-               grow=2) as b:
+               grow=2, promote_age=promote_age) as b:
         res = b.solve_all(probs)
     assert sorted(res) == list(range(64))
     for p in probs:
