@@ -61,6 +61,10 @@ ONE_CORE_BUDGET = 10.0
 # (precision, windowed code lists): the headline form and the others measured beside it
 HEADLINE_FORM = ("fp32", True)
 OTHER_FORMS = (("fp64", True), ("fp32", False), ("fp64", False))
+# the d = 10 chain problem on its 4^10 grid: the windowed forms, whose window
+# covers all ten coordinates (every term from term tables; the first is the
+# one the CPU baseline is compared with), and the unrolled form
+C1_FORMS = (("fp32", True), ("fp64", True), ("fp64", False))
 
 
 def parse():
@@ -643,7 +647,7 @@ def run_legs(G, pb, args, legs, spec, out, head):
                                    n_samples=10)
     if "c1" in legs:
         print("# leg c1", file=sys.stderr, flush=True)
-        out["c1"] = leg_config(G, pb, args, pb.chain10(), (4, 10), 1)
+        out["c1"] = leg_config(G, pb, args, pb.chain10(), (4, 10), 1, forms=C1_FORMS)
     if "c1_tuned" in legs:
         print("# leg c1_tuned", file=sys.stderr, flush=True)
         out["c1_tuned"] = leg_config(G, pb, args, pb.chain10(), (4, 5), 32, mean_value=True, split=4, cpu=False)
