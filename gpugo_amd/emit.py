@@ -571,6 +571,19 @@ class _FuncWT(_FuncW):
             self.lines.append(f"  iv<double> {acc} = rest[{s}];")
         fn = "mul" if e.op == "tprod" else "add"
         w = khi - klo + 1
+        if w == 1 and klo == 0 and e.val == 0:
+            # one coordinate per term and a term for every coordinate: the
+            # window's q-th term is window coordinate q (the ranges start at
+            # win and wrap to 0), so its entry is the q-th packed digit; same
+            # entries in the same order as the general loop below
+            self.lines.append(f"  {{ const iv<{ty}> *__restrict__ Ts = T + L.off[{s}]; const unsigned Es = L.E[{s}];")
+            self.lines.append("    unsigned long long pk = xd.packed;")
+            self.lines.append(f"#pragma unroll {TAB_LOADS}")
+            self.lines.append("    for (int q = 0; q < c; ++q) {")
+            self.lines.append(f"      {acc} = A::{fn}({acc}, Ts[q * Es + (unsigned)(pk & xd.mask)]);")
+            self.lines.append("      pk >>= xd.bpd;")
+            self.lines.append("    } }")
+            return ("i", acc)
         # the window's terms as one run q = 0 .. nq-1 (term j of range 0,
         # then of range 1); table entries are loaded TAB_LOADS at a time and
         # folded in q order (the same order as term by term)
