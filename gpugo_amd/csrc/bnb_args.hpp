@@ -24,7 +24,7 @@ enum gpugo_status {
 
 // layout version of the structs below; modules report it and the library
 // refuses a module built against another layout
-#define GPUGO_MODULE_ABI_V 7
+#define GPUGO_MODULE_ABI_V 8
 
 namespace gpugo {
 
@@ -86,6 +86,12 @@ struct BoundArgs {
   void *ptab;                    // iv<double>
   void *ptab32;                  // iv<float> (FP32 pruning pass)
   int tab_n, tab_n32;
+  // windowed modules at large d: each rest sum is split over rsplit
+  // workgroups (chunks of the terms); their partials, [b][pair][rsplit],
+  // are folded in chunk order by the step's prep fold kernel (rsplit 1: one
+  // workgroup per sum, no fold)
+  void *rpart;                   // iv<double>
+  int rsplit, pad3;
 };
 
 // BoundArgs.flags bit of the FP32 pruning pass (problem precision F32)

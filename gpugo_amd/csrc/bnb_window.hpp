@@ -234,21 +234,26 @@ __device__ __forceinline__ void prep_impl(const BoundArgs &a) {
   const int b = blockIdx.y;
   if (b >= st->n_split) return;
   const int S = a.n_samples;
-  const int q = blockIdx.x, k = q / GPUGO_NREST, s = q - k * GPUGO_NREST;
-  if (k > S) {
+  const int npairs = (1 + S) * GPUGO_NREST, rs = a.rsplit;
+  if ((int)blockIdx.x >= npairs * rs) {
 #if GPUGO_TAB
-    tab_build<R>(a, b, q - (1 + S) * GPUGO_NREST);
+    tab_build<R>(a, b, (int)blockIdx.x - npairs * rs);
 #endif
     return;
   }
+  // pair (k, s) and chunk ch of its terms: j in [ch * CH, (ch + 1) * CH)
+  const int pair = (int)blockIdx.x / rs, ch = (int)blockIdx.x - pair * rs;
+  const int k = pair / GPUGO_NREST, s = pair - k * GPUGO_NREST;
+  const int CH = (a.d + 2 + rs - 1) / rs;  // every template runs over fewer than d + 2 terms
   const long long p = st->first_parent + b;
   const int win = a.pwin[p];
   const RowBox x{a.plo + p * GPUGO_D, a.phi + p * GPUGO_D};
   iv<double> *out = (iv<double> *)(a.prest) + p * (long long)((1 + S) * GPUGO_NREST);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
-  iv<double> v = k == 0 ? obj_rest<R>(s, x, a.params, a.d, win, a.c, tid, (int)blockDim.x)
-                        : obj_rest<R>(s, DiagPoint<RowBox>{x, k - 1, S}, a.params, a.d, win, a.c, tid,
-                                      (int)blockDim.x);
+  const int j0 = ch * CH, j1 = j0 + CH;
+  iv<double> v = k == 0 ? obj_rest<R>(s, x, a.params, a.d, win, a.c, j0 + tid, (int)blockDim.x, j1)
+                        : obj_rest<R>(s, DiagPoint<RowBox>{x, k - 1, S}, a.params, a.d, win, a.c, j0 + tid,
+                                      (int)blockDim.x, j1);
   const bool prod = rest_is_prod(s);
   v = prod ? group_prod<R, 64>(v) : group_sum<R, 64>(v);
   if (lane == 0) part[w] = v;
@@ -256,8 +261,30 @@ __device__ __forceinline__ void prep_impl(const BoundArgs &a) {
   if (tid == 0) {
     iv<double> t = part[0];
     for (int i = 1; i < nw; ++i) t = prod ? A::mul(t, part[i]) : A::add(t, part[i]);
-    out[k * GPUGO_NREST + s] = t;
+    if (rs == 1) out[k * GPUGO_NREST + s] = t;
+    else ((iv<double> *)a.rpart)[((long long)b * npairs + pair) * rs + ch] = t;
   }
+}
+
+// The rest sums split over several workgroups (rsplit > 1): fold each
+// pair's partials in chunk order (deterministic), one thread per pair.
+template <int R>
+__device__ __forceinline__ void prep_fold_impl(const BoundArgs &a) {
+  typedef IA<double, R> A;
+  const DevState *st = a.st;
+  if (st->status != 0) return;
+  const int b = blockIdx.y;
+  if (b >= st->n_split) return;
+  const int S = a.n_samples, npairs = (1 + S) * GPUGO_NREST, rs = a.rsplit;
+  const int pair = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (pair >= npairs) return;
+  const int s = pair % GPUGO_NREST;
+  const bool prod = rest_is_prod(s);
+  const iv<double> *pp = (const iv<double> *)a.rpart + ((long long)b * npairs + pair) * rs;
+  iv<double> t = pp[0];
+  for (int i = 1; i < rs; ++i) t = prod ? A::mul(t, pp[i]) : A::add(t, pp[i]);
+  const long long p = st->first_parent + b;
+  ((iv<double> *)(a.prest) + p * (long long)npairs)[pair] = t;
 }
 
 // Wavefronts where at most this many children reach an FP64 test evaluate
@@ -614,6 +641,9 @@ extern "C" __global__ __launch_bounds__(256) void gpugo_bound_exact(gpugo::Bound
 extern "C" __global__ __launch_bounds__(256) void gpugo_prep_exact(gpugo::BoundArgs a) {
   gpugo::prep_impl<gpugo::ROUND_EXACT>(a);
 }
+extern "C" __global__ __launch_bounds__(64) void gpugo_prep_fold_exact(gpugo::BoundArgs a) {
+  gpugo::prep_fold_impl<gpugo::ROUND_EXACT>(a);
+}
 extern "C" __global__ __launch_bounds__(64) void gpugo_eval_exact(gpugo::EvalArgs a) {
   gpugo::eval_w_impl<gpugo::ROUND_EXACT>(a);
 }
@@ -626,6 +656,9 @@ extern "C" __global__ __launch_bounds__(256) void gpugo_bound_digits15(gpugo::Bo
 extern "C" __global__ __launch_bounds__(256) void gpugo_prep_digits15(gpugo::BoundArgs a) {
   gpugo::prep_impl<gpugo::ROUND_DIGITS15>(a);
 }
+extern "C" __global__ __launch_bounds__(64) void gpugo_prep_fold_digits15(gpugo::BoundArgs a) {
+  gpugo::prep_fold_impl<gpugo::ROUND_DIGITS15>(a);
+}
 extern "C" __global__ __launch_bounds__(64) void gpugo_eval_digits15(gpugo::EvalArgs a) {
   gpugo::eval_w_impl<gpugo::ROUND_DIGITS15>(a);
 }
@@ -636,6 +669,9 @@ extern "C" __global__ __launch_bounds__(256) void gpugo_bound_fast(gpugo::BoundA
 }
 extern "C" __global__ __launch_bounds__(256) void gpugo_prep_fast(gpugo::BoundArgs a) {
   gpugo::prep_impl<gpugo::ROUND_FAST>(a);
+}
+extern "C" __global__ __launch_bounds__(64) void gpugo_prep_fold_fast(gpugo::BoundArgs a) {
+  gpugo::prep_fold_impl<gpugo::ROUND_FAST>(a);
 }
 extern "C" __global__ __launch_bounds__(64) void gpugo_eval_fast(gpugo::EvalArgs a) {
   gpugo::eval_w_impl<gpugo::ROUND_FAST>(a);

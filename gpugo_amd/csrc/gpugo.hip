@@ -220,6 +220,7 @@ struct gpugo_module_s {
   hipFunction_t persist_mv[2] = {nullptr, nullptr};
   // windowed modules (large d): rest-sum kernel per rounding, rest slots
   hipFunction_t prep[3] = {nullptr, nullptr, nullptr};
+  hipFunction_t prep_fold[3] = {nullptr, nullptr, nullptr};  // rest sums split over workgroups
   int nrest = 0;
   // term tables (windowed modules): stencil width of each rest slot and of
   // the gradient template; tab bit 0: tables, bit 1: gradient table
@@ -350,6 +351,8 @@ static int module_finish(gpugo_module_t m) {
       // windowed module: rest sums per parent record, no mean-value variant
       snprintf(nm, sizeof nm, "gpugo_prep_%s", rname[r]);
       HIPCHECK(hipModuleGetFunction(&m->prep[r], m->mod, nm));
+      snprintf(nm, sizeof nm, "gpugo_prep_fold_%s", rname[r]);
+      HIPCHECK(hipModuleGetFunction(&m->prep_fold[r], m->mod, nm));
     } else {
       snprintf(nm, sizeof nm, "gpugo_bound_%s_mv", rname[r]);
       HIPCHECK(hipModuleGetFunction(&m->bound_mv[r], m->mod, nm));
@@ -1034,6 +1037,8 @@ struct gpugo_problem_s {
   DevBuf pext;         // windowed modules: rest sums per parent record [p_cap][ext_n]
   DevBuf ptab, ptab32; // windowed modules: term tables of a step's records [k_select][tab_n] (FP64 / FP32)
   int tab_n = 0, tab_n32 = 0;
+  DevBuf rpart;        // windowed modules at large d: rest-sum partials [k_select][pairs][rsplit]
+  int rsplit = 1;
   DevBuf wmin;         // k_select 1: per-compaction-workgroup smallest key [GPUGO_WMIN_MAX][3]
   DevBuf cmin;         // lazy queue: smallest key per chunk of 2^GPUGO_CHUNK_LOG2 entries, [3] each
   // One box per step: the queue is compacted every compact_every steps; the
@@ -1107,6 +1112,9 @@ static int problem_shape(gpugo_problem_s *p, const gpugo_problem_desc *desc, lon
   // (c + 2(w_s - 1)) terms x wid^w_s digit combinations; then the gradient
   // table or the gradient parts table.  Left out when large (wide grids) or
   // switched off.
+  // rest sums over more than 4096 terms: chunks of about 2048 terms per
+  // workgroup, folded in chunk order by the prep fold kernel
+  p->rsplit = (p->m->nrest > 0 && d + 2 > 4096) ? (d + 2 + 2047) / 2048 : 1;
   p->tab_n = p->tab_n32 = 0;
   if (p->m->tab && !(desc->flags & GPUGO_FLAG_NO_TABLES)) {
     auto ipow = [](long long b, int e) {
@@ -1176,6 +1184,8 @@ static int problem_buffers(gpugo_problem_s *p, Alloc &&alloc) {
   if (p->ext_n > 0) PBUF(p->pext, sizeof(double) * p->ext_n * pc);
   if (p->tab_n > 0) PBUF(p->ptab, 16 * (size_t)p->tab_n * p->desc.k_select);
   if (p->tab_n32 > 0) PBUF(p->ptab32, 8 * (size_t)p->tab_n32 * p->desc.k_select);
+  if (p->rsplit > 1)
+    PBUF(p->rpart, 16 * (size_t)p->rsplit * (size_t)(1 + p->desc.n_samples) * p->m->nrest * p->desc.k_select);
   PBUF(p->wmin, sizeof(unsigned long long) * 3 * GPUGO_WMIN_MAX);
   if (p->lazy) {
     const long long nch = (qc + (1LL << GPUGO_CHUNK_LOG2) - 1) >> GPUGO_CHUNK_LOG2;
@@ -1493,6 +1503,8 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more, bool compact_on
   ba.ptab32 = p->ptab32.p;
   ba.tab_n = p->tab_n;
   ba.tab_n32 = p->tab_n32;
+  ba.rpart = p->rpart.p;
+  ba.rsplit = p->rsplit;
   if (p->m->nrest > 0 && !compact_only) {
     // rest sums of the parent records this step splits: one workgroup per
     // (point, template sum) pair and parent, then the workgroups that fill
@@ -1501,8 +1513,11 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more, bool compact_on
     const unsigned npairs = (unsigned)((1 + p->desc.n_samples) * p->m->nrest);
     const unsigned ntab = (unsigned)((p->tab_n + 255) / 256);
     void *args[] = {&ba};
-    HIPCHECK(hipModuleLaunchKernel(p->m->prep[p->desc.rounding], npairs + ntab, nb, 1, 256, 1, 1, 0, p->stream,
-                                   args, nullptr));
+    HIPCHECK(hipModuleLaunchKernel(p->m->prep[p->desc.rounding], npairs * (unsigned)p->rsplit + ntab, nb, 1, 256, 1,
+                                   1, 0, p->stream, args, nullptr));
+    if (p->rsplit > 1)
+      HIPCHECK(hipModuleLaunchKernel(p->m->prep_fold[p->desc.rounding], (npairs + 63) / 64, nb, 1, 64, 1, 1, 0,
+                                     p->stream, args, nullptr));
   }
   if (!compact_only && p->steps_enqueued == 0 && p->desc.precision == GPUGO_PREC_F32 && p->desc.rounding < 2 &&
       p->m->sample32[p->desc.rounding]) {

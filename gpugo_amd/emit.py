@@ -964,9 +964,9 @@ def window_lists(obj: Objective) -> str:
     """Windowed code lists from the objective's loop templates:
       wterm_s<R>(x, P, d, j)   term j of template sum s
       wranges_s(win, c, d, lo, hi)  its terms that touch the window
-      obj_rest<R>(s, x, P, d, win, c, t, nt)  thread t of nt: partial sum
-          of sum s's other terms over x (the parent box or one of its
-          diagonal sample points)
+      obj_rest<R>(s, x, P, d, win, c, t, nt, jend)  thread t of nt: partial
+          sum of sum s's other terms j < jend over x (the parent box or one of
+          its diagonal sample points), j = t, t + nt, ...
       obj_value_w<R>(x, P, d, rest, win, c)   value over a child
       obj_kkt_w<R>(x, P, dlo, dhi, d, rest, win, c)  monotonicity test
     The monotonicity test covers the gradient components the window can
@@ -1007,7 +1007,7 @@ def window_lists(obj: Objective) -> str:
     # rest sums of a parent record: slot s over an accessor (the box, or
     # one of its diagonal sample points)
     out.append("template <int R, class XS>\n__device__ __forceinline__ iv<double> obj_rest("
-               "int s, const XS &x, const double *__restrict__ P, int d, int win, int c, int t, int nt) {")
+               "int s, const XS &x, const double *__restrict__ P, int d, int win, int c, int t, int nt, int jend) {")
     out.append("  typedef IA<double, R> A;")
     out.append("  iv<double> acc = mk<double>(0.0, 0.0);")
     out.append("  int rlo[2], rhi[2];")
@@ -1018,7 +1018,7 @@ def window_lists(obj: Objective) -> str:
         out.append(f"  case {s_}: {{ const int nr = wranges_{s_}(win, c, d, rlo, rhi);")
         if prod:
             out.append("    acc = mk<double>(1.0, 1.0);")
-        out.append(f"    for (int j = t; j < {nt}; j += nt)")
+        out.append(f"    for (int j = t, je = min({nt}, jend); j < je; j += nt)")
         out.append(f"      if (!in_ranges(j, nr, rlo, rhi)) acc = A::{'mul' if prod else 'add'}(acc, wterm_{s_}<R>(x, P, d, j));")
         out.append("    break; }")
     out.append("  default: break;")
