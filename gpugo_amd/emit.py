@@ -817,10 +817,11 @@ def _window_tables(value: Expr, grad: Expr, sums: List[Expr], slots: Dict[tuple,
         f.lines.append(f"  int glo[2], ghi[2]; const int ng = win_ranges(win, c, d, {gklo}, {gkhi}, d, glo, ghi);")
         f.lines.append("  const int g0 = ng > 0 ? ghi[0] - glo[0] : 0;")
         f.lines.append("  const iv<double> *__restrict__ TP = T + L.poff;")
-        f.lines.append("  for (int j = 0; j < d; ++j) {")
-        f.lines.append("    int q = -1;")
-        f.lines.append("    if (ng > 0 && j >= glo[0] && j < ghi[0]) q = j - glo[0];")
-        f.lines.append("    else if (ng > 1 && j >= glo[1] && j < ghi[1]) q = g0 + (j - glo[1]);")
+        # the window's components first (see _kkt_lines), then the others
+        f.lines.append("  for (int r = 0; r <= ng; ++r)")
+        f.lines.append("  for (int j = r < ng ? glo[r] : 0, je = r < ng ? ghi[r] : d; j < je; ++j) {")
+        f.lines.append("    if (r == ng && in_ranges(j, ng, glo, ghi)) continue;")
+        f.lines.append("    const int q = r == ng ? -1 : (r == 0 ? j - glo[0] : g0 + (j - glo[1]));")
         f.lines.append(f"    const iv<double> *__restrict__ tp = q < 0 ? TP + j : TP + d + q * (int)L.gE + "
                        f"(int)tab_idx<{gw}>(xd, j + ({gklo}));")
         for k in range(len(parts)):
@@ -1070,7 +1071,14 @@ def _kkt_lines(f: "_FuncW", grad: Expr, global_grad: bool):
     at the first component that rules the child out."""
     f.hoist(grad)
     if global_grad:
-        f.lines.append("  for (int j = 0; j < d; ++j) {")
+        # every component is tested, the window's first: they changed from
+        # the parent's, which did not rule the parent out, so they are the
+        # likeliest to rule the child out (the test is an OR: same result)
+        gklo, gkhi = _x_offsets(grad, "gradient template")
+        f.lines.append(f"  int glo[2], ghi[2]; const int ng = win_ranges(win, c, d, {gklo}, {gkhi}, d, glo, ghi);")
+        f.lines.append("  for (int r = 0; r <= ng; ++r)")
+        f.lines.append("  for (int j = r < ng ? glo[r] : 0, je = r < ng ? ghi[r] : d; j < je; ++j) {")
+        f.lines.append("    if (r == ng && in_ranges(j, ng, glo, ghi)) continue;")
     else:
         gklo, gkhi = _x_offsets(grad, "gradient template")
         f.lines.append(f"  int glo[2], ghi[2]; const int ng = win_ranges(win, c, d, {gklo}, {gkhi}, d, glo, ghi);")
