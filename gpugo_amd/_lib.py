@@ -28,6 +28,7 @@ FLAG_SPLIT = 4
 FLAG_PERSIST = 8
 FLAG_EAGER_COMPACT = 16
 FLAG_NO_TABLES = 32
+FLAG_NO_BAND = 64
 STATUS = {0: "running", 1: "certified", 2: "x_converged", 3: "max_iters", 4: "exhausted", 5: "overflow"}
 
 EXPORTS = [

@@ -436,6 +436,7 @@ __device__ __forceinline__ void persist_impl(const PersistArgs &p) {
     ca.cmin = nullptr;
     ca.chunk_log2 = 0;
     ca.pad_c = 0;
+    ca.band = p.sa.band;
     compact_body(ca, blk, nblk);
     if (!grid_barrier(p.bar, nblk)) {
       if (threadIdx.x == 0) atomicOr(p.err, 1);

@@ -98,6 +98,26 @@ __device__ __forceinline__ void child_coord(const double *plo, const double *phi
 #define SEL_U 8
 #define GPUGO_DMAX_SELECT (1 << 17)
 
+// Priority band of a queue (a batch slot's, or a single solve's with more
+// than one box per step): the compaction copies every kept entry with
+// lb < g into a side list (at most cap entries, count in n; more than cap
+// leaves it unusable), and the select takes the K smallest from it instead
+// of passing over the whole queue whenever it holds at least K entries:
+// every entry outside it has lb >= g, above all of them.  The select of a
+// step resets n and sets g for its compaction from the histogram of a full
+// pass (about BAND_TARGET entries below g).
+struct BandState {
+  double g;                 // threshold (-inf: no band)
+  unsigned long long n;     // entries appended by the last compaction
+};
+struct BandView {
+  const double *lb;
+  const unsigned *par, *pos, *idx;  // idx: the entry's index in the compacted queue
+  BandState *st;                    // null: no band
+  long long cap;
+};
+#define BAND_TARGET 2048
+
 typedef unsigned __int128 u128;
 
 struct SelectArgs {
@@ -126,6 +146,7 @@ struct SelectArgs {
   long long q_cap;       // queue capacity (entries), read when grow is set
   unsigned W;            // wid^c children per split box, read when grow is set
   int grow_age;          // with grow: > 0 also stops a slot that has run this many steps
+  BandView band;         // single solves with K > 1: the queue's priority band (st null: none)
 };
 
 // Selection key (lb, parent, pos), unique per box.  Parent records are
@@ -382,26 +403,9 @@ __device__ long long collect_parents(const SelectArgs &a, long long N, long long
   return kept;
 }
 
-// Priority band of a batch slot: the compaction copies every kept entry
-// with lb < g into a side list (at most cap entries, count in n; more than
-// cap leaves it unusable), and the select takes the K smallest from it
-// instead of passing over the whole queue whenever it holds at least K
-// entries: every entry outside it has lb >= g, above all of them.  The
-// select of a step resets n and sets g for its compaction from the
-// histogram of a full pass (about BAND_TARGET entries below g).
-struct BandState {
-  double g;                 // threshold (-inf: no band)
-  unsigned long long n;     // entries appended by the last compaction
-};
-struct BandView {
-  const double *lb;
-  const unsigned *par, *pos, *idx;  // idx: the entry's index in the compacted queue
-  BandState *st;
-  long long cap;
-};
-#define BAND_TARGET 2048
 
 __device__ void select_body(const SelectArgs &a, const BandView *band = nullptr) {
+  if (!band && a.band.st) band = &a.band;
   __shared__ unsigned hist[SEL_BINS];
   __shared__ u128 skey[SEL_KMAX];
   __shared__ unsigned sidx[SEL_KMAX];
@@ -863,6 +867,7 @@ struct CompactArgs {
   unsigned long long *cmin;  // or null: chunk minima of a lazy queue, [3 * chunks] key (hi, lo) and index
   int chunk_log2;            // entries per chunk: 1 << chunk_log2
   int pad_c;
+  BandView band;             // st null, or the band this pass rebuilds (kept entries with lb < band.st->g)
 };
 
 // blk / nblk: this workgroup's index and the number of workgroups sharing
@@ -900,6 +905,9 @@ __device__ __forceinline__ void compact_body(const CompactArgs &a, int blk, int 
   // smallest kept key (order_key(lb), (parent ^ tie_xor) << 32 | pos) and its
   // output index, for a one-box select that needs no pass over the queue
   unsigned long long mh = ~0ull, ml = ~0ull, mo = 0ull;
+  // kept entries below the band threshold also go to the band
+  BandState *bst = a.band.st;
+  const double bg = bst ? bst->g : -__builtin_huge_val();
   const long long span = (long long)bsz * CMP_U;
   for (long long base = (long long)blk * span; base < n; base += (long long)nblk * span) {
     double lb[CMP_U];
@@ -939,6 +947,13 @@ __device__ __forceinline__ void compact_body(const CompactArgs &a, int blk, int 
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < CMP_U; ++u) {
+      // band slots of this wavefront's entries below the threshold (one
+      // atomic per wavefront and slab; none when no band is kept)
+      const bool inband = ((kept >> u) & 1u) && lb[u] < bg;
+      const unsigned long long bmask = __ballot(inband);
+      unsigned long long bbase = 0;
+      if (bmask && lane == 0) bbase = atomicAdd(&bst->n, (unsigned long long)__popcll(bmask));
+      if (bmask) bbase = (unsigned long long)__shfl((long long)bbase, 0, 64);
       if (kept & (1u << u)) {
         const long long i = base + (long long)u * bsz + threadIdx.x;
         const long long o = (long long)s_base + wcnt[u * nw + w] + pre[u];
@@ -946,6 +961,16 @@ __device__ __forceinline__ void compact_body(const CompactArgs &a, int blk, int 
         a.lb_out[o] = lb[u];
         a.par_out[o] = par;
         a.pos_out[o] = pos;
+        if (inband) {
+          const unsigned long long bo =
+              bbase + __builtin_amdgcn_mbcnt_hi((unsigned)(bmask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bmask, 0u));
+          if (bo < (unsigned long long)a.band.cap) {
+            const_cast<double *>(a.band.lb)[bo] = lb[u];
+            const_cast<unsigned *>(a.band.par)[bo] = par;
+            const_cast<unsigned *>(a.band.pos)[bo] = pos;
+            const_cast<unsigned *>(a.band.idx)[bo] = (unsigned)o;
+          }
+        }
         if (a.wmin) {
           const unsigned long long kh = order_key(lb[u]), kl = ((unsigned long long)(par ^ a.tie_xor) << 32) | pos;
           if (kh < mh || (kh == mh && kl < ml)) {
@@ -1172,9 +1197,14 @@ struct RootArgs {
   const double *lo, *hi;
   double init_ub;
   int d, wid;
+  BandState *band;  // or null: the problem's priority band, emptied with the state
 };
 
 __device__ __forceinline__ void root_body(const RootArgs &a) {
+  if (a.band && threadIdx.x == 0) {
+    a.band->g = -__builtin_huge_val();
+    a.band->n = 0;
+  }
   for (int j = threadIdx.x; j < a.d; j += blockDim.x) {
     a.plo[j] = a.lo[j];
     a.phi[j] = a.hi[j];
@@ -1262,6 +1292,7 @@ __device__ __forceinline__ void compact_select_body(const CompactArgs &ca, const
     fa.wmin = nullptr;
     fa.cmin = nullptr;
     fa.poll = nullptr;
+    fa.band.st = nullptr;  // the problem has stopped: no select reads a band again
     compact_body(fa, 0, 1);
     __syncthreads();
     if (threadIdx.x == 0) ca.st->compact_final = 0;

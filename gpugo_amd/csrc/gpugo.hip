@@ -687,6 +687,7 @@ __global__ __launch_bounds__(SEL_T) void kb_select(BatchArgs b) {
   const long long qoff = ((long long)slot * 2 + (st->compactions & 1)) * b.qcap;
   const long long poff = (long long)slot * b.pcap;
   SelectArgs sa;
+  sa.band = BandView{};  // the slot's band is passed to select_body below
   sa.st = st;
   sa.q_lb = b.q_lb + qoff;
   sa.q_par = b.q_par + qoff;
@@ -1010,6 +1011,7 @@ __global__ __launch_bounds__(256) void k_materialize(const double *plo, const do
                                // two workgroups per CU for the queues that grow large
 #define GPUGO_LAZY_WG 256      // gpugo_compact_select as the lazy tail, 1024 threads
 #define GPUGO_WMIN_MAX GPUGO_COMPACT_WG
+#define GPUGO_SBAND_CAP 16384  // priority band of a single solve with k_select > 1 (entries)
 #define GPUGO_CHUNK_LOG2 13    // lazy queue: 8192 entries per chunk minimum
 #define GPUGO_LAZY_MAX_EVERY 256
 #define GPUGO_TAB_MAX (1 << 20) // term-table entries per parent record, at most
@@ -1041,6 +1043,11 @@ struct gpugo_problem_s {
   int rsplit = 1;
   DevBuf wmin;         // k_select 1: per-compaction-workgroup smallest key [GPUGO_WMIN_MAX][3]
   DevBuf cmin;         // lazy queue: smallest key per chunk of 2^GPUGO_CHUNK_LOG2 entries, [3] each
+  // k_select > 1 over an eagerly compacted queue: the priority band (entries
+  // below a threshold, rebuilt by each compaction) the select reads instead
+  // of the whole queue; band_cap 0: none
+  DevBuf band_lb, band_par, band_pos, band_idx, band_st;
+  long long band_cap = 0;
   // One box per step: the queue is compacted every compact_every steps; the
   // steps between leave consumed and ruled-out entries in place and refresh
   // chunk minima instead (lazy tails).
@@ -1154,6 +1161,7 @@ static int problem_shape(gpugo_problem_s *p, const gpugo_problem_desc *desc, lon
       p->q_cap = qc + every * (long long)p->W;
     }
   }
+  p->band_cap = desc->k_select > 1 && !(desc->flags & GPUGO_FLAG_NO_BAND) ? GPUGO_SBAND_CAP : 0;
   return GPUGO_OK;
 }
 
@@ -1191,6 +1199,13 @@ static int problem_buffers(gpugo_problem_s *p, Alloc &&alloc) {
     const long long nch = (qc + (1LL << GPUGO_CHUNK_LOG2) - 1) >> GPUGO_CHUNK_LOG2;
     PBUF(p->cmin, sizeof(unsigned long long) * 3 * nch);
   }
+  if (p->band_cap > 0) {
+    PBUF(p->band_lb, sizeof(double) * p->band_cap);
+    PBUF(p->band_par, sizeof(unsigned) * p->band_cap);
+    PBUF(p->band_pos, sizeof(unsigned) * p->band_cap);
+    PBUF(p->band_idx, sizeof(unsigned) * p->band_cap);
+    PBUF(p->band_st, sizeof(BandState));
+  }
   for (int b = 0; b < 2; ++b) {
     PBUF(p->q[b].lb, sizeof(double) * qc);
     PBUF(p->q[b].par, sizeof(unsigned) * qc);
@@ -1224,7 +1239,7 @@ static int problem_check(gpugo_module_t m, const gpugo_problem_desc *desc) {
   if (desc->precision == GPUGO_PREC_F32 && !m->f32)
     return set_err(GPUGO_E_INVALID, "module has no FP32 code list (emit it with f32=True)");
   if (desc->flags & ~(GPUGO_FLAG_MEAN_VALUE | GPUGO_FLAG_SPLIT | GPUGO_FLAG_PERSIST | GPUGO_FLAG_EAGER_COMPACT |
-                      GPUGO_FLAG_NO_TABLES))
+                      GPUGO_FLAG_NO_TABLES | GPUGO_FLAG_NO_BAND))
     return set_err(GPUGO_E_INVALID, "unknown flags 0x%x", desc->flags);
   if ((desc->flags & GPUGO_FLAG_SPLIT) && m->split_g < 2)
     return set_err(GPUGO_E_INVALID, "module has no term-parallel kernels (emit it with split=G)");
@@ -1399,6 +1414,7 @@ static int root_init(gpugo_problem_t p) {
   ra.init_ub = p->desc.init_ub;
   ra.d = p->desc.d;
   ra.wid = p->desc.wid;
+  ra.band = p->band_cap > 0 ? p->band_st.as<BandState>() : nullptr;
   if (p->m->root_fn) {
     void *args[] = {&ra};
     HIPCHECK(hipModuleLaunchKernel(p->m->root_fn, 1, 1, 1, 64, 1, 1, 0, p->stream, args, nullptr));
@@ -1407,6 +1423,20 @@ static int root_init(gpugo_problem_t p) {
     HIPCHECK(hipGetLastError());
   }
   return GPUGO_OK;
+}
+
+// The problem's priority band (st null when it keeps none).
+static BandView band_view(gpugo_problem_t p) {
+  BandView v{};
+  if (p->band_cap > 0) {
+    v.lb = p->band_lb.as<double>();
+    v.par = p->band_par.as<unsigned>();
+    v.pos = p->band_pos.as<unsigned>();
+    v.idx = p->band_idx.as<unsigned>();
+    v.st = p->band_st.as<BandState>();
+    v.cap = p->band_cap;
+  }
+  return v;
 }
 
 // Select of the step that reads queue buffer q (the K smallest boxes,
@@ -1447,6 +1477,7 @@ static SelectArgs select_args(gpugo_problem_t p, const Queue &q, int nwg) {
   sa.q_cap = p->q_cap;
   sa.W = p->W;
   sa.grow_age = 0;
+  sa.band = band_view(p);
   return sa;
 }
 
@@ -1569,6 +1600,7 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more, bool compact_on
   ca.cmin = nullptr;
   ca.chunk_log2 = GPUGO_CHUNK_LOG2;
   ca.pad_c = 0;
+  ca.band = band_view(p);
   const bool lazy_tail = p->lazy && more && p->m->compact_select_fn && p->steps_enqueued > 0 &&
                          p->since_compact + 1 < p->compact_every;
   // entries the compaction may read: the bound before this step plus its appends
@@ -1734,6 +1766,7 @@ static int persist_steps(gpugo_problem_t p, int64_t iters) {
   sa.q_cap = p->q_cap;
   sa.W = p->W;
   sa.grow_age = 0;
+  sa.band = band_view(p);
   for (int b = 0; b < 2; ++b) {
     pa.q_lb[b] = p->q[b].plb();
     pa.q_par[b] = p->q[b].ppar();

@@ -175,7 +175,7 @@ class Problem:
                  max_parents: int = 0, max_iters: int = 0, module: Optional[Module] = None,
                  mean_value: bool = False, precision: str = "fp64", split: int = 0,
                  persistent: bool = False, windowed: bool = False, eager_compact: bool = False,
-                 tables: bool = True):
+                 tables: bool = True, band: bool = True):
         """precision "fp32": children are first bounded in FP32 intervals
         (outward, one ulp per operation) and only those that survive get the
         FP64 value, gradient and sample; the enclosure is certified in FP64.
@@ -193,7 +193,11 @@ class Problem:
         tables (windowed): look window terms up in the step's term tables
         (one entry per term and digit combination, filled once per split
         record) instead of evaluating them per child; False evaluates them
-        per child (same results; a test and measurement switch)."""
+        per child (same results; a test and measurement switch).
+        band (k_select > 1): each compaction also keeps the entries below a
+        threshold in a side list that the next select reads instead of the
+        whole queue; False passes over the whole queue every step (same
+        results; a test and measurement switch)."""
         if precision not in _lib.PREC:
             raise ValueError(f"precision must be one of {sorted(_lib.PREC)}")
         self.module = module or get_module(obj, roundings=(rounding,), f32=precision == "fp32", split=split,
@@ -206,7 +210,7 @@ class Problem:
             precision=_lib.PREC[precision], rounding=_lib.ROUND[rounding],
             flags=((_lib.FLAG_MEAN_VALUE if mean_value else 0) | (_lib.FLAG_SPLIT if split else 0)
                    | (_lib.FLAG_PERSIST if persistent else 0) | (_lib.FLAG_EAGER_COMPACT if eager_compact else 0)
-                   | (0 if tables else _lib.FLAG_NO_TABLES)),
+                   | (0 if tables else _lib.FLAG_NO_TABLES) | (0 if band else _lib.FLAG_NO_BAND)),
             lo=ptr(self._lo), hi=ptr(self._hi),
             params=ptr(self._params), n_params=self._params.size, tol_f=float(tol), tol_x=float(tol_x),
             init_ub=float(init_ub), max_queue=int(max_queue), max_parents=int(max_parents),

@@ -201,6 +201,17 @@ int gpugo_window_ranges(int64_t n, const int32_t *args, int32_t *out);
                                     evaluated once per digit combination and
                                     looked up).  Same bits either way; for
                                     measurement and parity tests.              */
+#define GPUGO_FLAG_NO_BAND    64 /* k_select > 1: the select passes over the
+                                    whole queue every step.  By default each
+                                    compaction also copies the entries below
+                                    a threshold (about 2048 of them, set by
+                                    the last full pass) into a side list of
+                                    at most 16384, and the select takes the K
+                                    smallest from it while it holds at least
+                                    K; every other entry lies at or above the
+                                    threshold.  Same selections and results
+                                    either way; for measurement and parity
+                                    tests.                                     */
 
 /* ---- problems -------------------------------------------------------------
  * One box-constrained minimisation.  Replaces the module-level constants and

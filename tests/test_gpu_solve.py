@@ -62,10 +62,18 @@ def test_rational_path_identical_to_oracle(gpu, oracle, grid, k):
 @pytest.mark.parametrize("rounding", ["fast", "exact"])
 def test_certified_enclosure(gpu, oracle, spec, grid, k, rounding):
     if rounding == "exact" and spec.d > 10:
-        pytest.skip("exact-mode module for d=50 is exercised by test_gpu_eval")
-    with gpu_problem(gpu, spec, tol=spec.tol, grid=grid, k_select=k, rounding=rounding,
-                     max_iters=20000) as p:
-        r = p.solve()
+        # d = 50 in EXACT rounding: the windowed module of the same objective
+        # (its unrolled EXACT module is only used on explicit boxes)
+        obj = pb.objective_loops(spec.family, spec.d)
+        m = gpu.get_module(obj, roundings=("exact", "fast"), windowed=True)
+        print(f"# solve {spec.name} exact windowed grid={grid} K={k}", flush=True)
+        with gpu.Problem(obj, spec.params, spec.lo, spec.hi, tol=spec.tol, grid=grid, k_select=k,
+                         rounding="exact", module=m, windowed=True, max_iters=20000) as p:
+            r = p.solve()
+    else:
+        with gpu_problem(gpu, spec, tol=spec.tol, grid=grid, k_select=k, rounding=rounding,
+                         max_iters=20000) as p:
+            r = p.solve()
     assert r["status_name"] == "certified", r
     assert r["ub"] - r["lb"] <= spec.tol
     if spec.f_star is not None:
@@ -281,21 +289,22 @@ def test_stop_final_compaction_large_queue(gpu):
 
 def test_unknown_flag_rejected(gpu):
     """A description with a flag bit the header does not define is refused
-    with GPUGO_E_INVALID and a message; the eager-compaction and no-tables
-    bits are known."""
+    with GPUGO_E_INVALID and a message; the eager-compaction, no-tables and
+    no-band bits are known."""
     import ctypes
     from gpugo_amd import _lib
     spec = pb.sepcos2()
     with gpu_problem(gpu, spec, tol=1e-6, grid=(2, 2), k_select=1, eager_compact=True) as p:
         lib = _lib.load()
         d2 = _lib.ProblemDesc.from_buffer_copy(p.desc)
-        d2.flags = 64
+        d2.flags = 128
         h = ctypes.c_void_p()
         assert lib.gpugo_problem_create(p.module.handle, ctypes.byref(d2), ctypes.byref(h)) == _lib.E_INVALID
         assert b"unknown flags" in lib.gpugo_last_error()
-        d2.flags = _lib.FLAG_NO_TABLES
-        assert lib.gpugo_problem_create(p.module.handle, ctypes.byref(d2), ctypes.byref(h)) == _lib.OK
-        lib.gpugo_problem_destroy(h)
+        for f in (_lib.FLAG_NO_TABLES, _lib.FLAG_NO_BAND):
+            d2.flags = f
+            assert lib.gpugo_problem_create(p.module.handle, ctypes.byref(d2), ctypes.byref(h)) == _lib.OK
+            lib.gpugo_problem_destroy(h)
         assert p.solve()["status_name"] == "certified"
 
 
@@ -336,3 +345,52 @@ def test_lazy_queue_matches_eager(gpu, spec, grid, kw):
     for key in LAZY_KEYS:
         assert a[key] == b[key], (key, a[key], b[key])
     assert a["queue_peak"] >= b["queue_peak"]
+
+
+BAND_KEYS = ("lb", "ub", "width", "iterations", "boxes_bounded", "queue_size", "queue_peak", "pruned_value",
+             "pruned_kkt", "status")
+
+
+@pytest.mark.parametrize("grid,k,kw", [
+    ((3, 3), 64, {}),
+    ((2, 5), 256, {}),
+    ((3, 4), 1024, {}),            # 82,944 appends a step: the band outgrows its 16384 entries
+    ((2, 5), 256, {"mean_value": True}),
+    ((4, 5), 32, {"split": 4, "persistent": True, "mean_value": True}),
+], ids=lambda v: str(v))
+def test_band_select_matches_full_pass(gpu, grid, k, kw):
+    """k_select > 1: the select takes the K smallest boxes from the priority
+    band the compaction keeps (entries below a threshold) while it holds at
+    least K, instead of passing over the whole queue.  Same selections, so
+    the same results after every number of steps (the plain select between
+    step calls, the fused one within them), after a reset, and the same live
+    boxes, as with the band switched off (GPUGO_FLAG_NO_BAND)."""
+    spec = pb.chain10()
+    obj = pb.objective(spec.family, spec.d)
+    if kw.get("split"):
+        kw = dict(kw, module=gpu.get_module(obj, roundings=("fast",), split=kw["split"]))
+    runs = []
+    for band in (True, False):
+        out = []
+        with gpu.Problem(obj, spec.params, spec.lo, spec.hi, tol=1e-6, grid=grid, k_select=k, band=band,
+                         max_iters=20000, **kw) as p:
+            for n in (1, 2, 5, 17):
+                out.append(p.step(n))
+            lo, hi, lb, nb = p.boxes(1 << 18)
+            assert nb <= 1 << 18
+            order = np.lexsort((lb, *hi.T, *lo.T))
+            out.append((nb, lo[order], hi[order], lb[order]))
+            out.append(p.solve())
+            p.reset()
+            out.append(p.solve())
+        runs.append(out)
+    a, b = runs
+    assert a[-1]["status_name"] == "certified", a[-1]
+    # the queue outgrew the band's target: thresholds below the largest bound
+    assert a[5]["queue_peak"] > 2048
+    for ra, rb in zip(a[:4] + a[5:], b[:4] + b[5:]):
+        for key in BAND_KEYS:
+            assert ra[key] == rb[key], (key, ra[key], rb[key])
+    assert a[4][0] == b[4][0]
+    for x, y in zip(a[4][1:], b[4][1:]):
+        assert np.array_equal(x, y)
