@@ -105,7 +105,8 @@ __device__ __forceinline__ void child_coord(const double *plo, const double *phi
 // of passing over the whole queue whenever it holds at least K entries:
 // every entry outside it has lb >= g, above all of them.  The select of a
 // step resets n and sets g for its compaction from the histogram of a full
-// pass (about BAND_TARGET entries below g).
+// pass (about BAND_TARGET entries below g); a queue of at most BAND_MIN
+// entries keeps no band (g = -inf) and is passed over whole.
 struct BandState {
   double g;                 // threshold (-inf: no band)
   unsigned long long n;     // entries appended by the last compaction
@@ -117,6 +118,7 @@ struct BandView {
   long long cap;
 };
 #define BAND_TARGET 2048
+#define BAND_MIN 32768
 
 typedef unsigned __int128 u128;
 
@@ -574,6 +576,10 @@ __device__ void select_body(const SelectArgs &a, const BandView *band = nullptr)
       unsigned tot;
       unsigned cum = block_excl_scan(ts, scan_sh, &tot);
       const unsigned gt = (unsigned)BAND_TARGET > (unsigned)K ? (unsigned)BAND_TARGET : (unsigned)K;
+      // a new threshold only for a queue larger than BAND_MIN (> gt); a
+      // smaller one keeps no band: its passes are short, and copying all
+      // of it to the band would cost the compaction more than they do
+      const bool set_g = band && !use_band && tot > (unsigned)BAND_MIN;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if (cum < (unsigned)K && (unsigned)K <= cum + h[q]) {
@@ -581,12 +587,12 @@ __device__ void select_body(const SelectArgs &a, const BandView *band = nullptr)
           s_cb = cum;
           s_region = h[q];
         }
-        if (band && !use_band && cum < gt && gt <= cum + h[q])
+        if (set_g && cum < gt && gt <= cum + h[q])
           s_gnext = (4 * tid + q + 1 == SEL_BINS) ? __builtin_huge_val()
                                                    : lmin + (double)(4 * tid + q + 1) * (span / (double)SEL_BINS);
         cum += h[q];
       }
-      if (band && !use_band && tid == 0 && tot < gt) s_gnext = __builtin_huge_val();
+      if (band && !use_band && tid == 0 && !set_g) s_gnext = -__builtin_huge_val();
       __syncthreads();
       if (s_region <= SEL_FINAL || s_region == (unsigned)K - s_cb) {
         if (tid == 0) {
@@ -916,7 +922,7 @@ __device__ __forceinline__ void compact_body(const CompactArgs &a, int blk, int 
       const long long i = base + (long long)u * bsz + threadIdx.x;
       lb[u] = i < n ? a.lb_in[i] : 0.0;
     }
-    unsigned kept = 0u, pre[CMP_U];
+    unsigned kept = 0u, bbits = 0u, pre[CMP_U];
 #pragma unroll
     for (int u = 0; u < CMP_U; ++u) {
       const long long i = base + (long long)u * bsz + threadIdx.x;
@@ -926,10 +932,22 @@ __device__ __forceinline__ void compact_body(const CompactArgs &a, int blk, int 
         kmin = k < kmin ? k : kmin;
         kmax = k > kmax ? k : kmax;
         kept |= 1u << u;
+        if (lb[u] < bg) bbits |= 1u << u;
       }
       const unsigned long long mask = __ballot(keep);
       pre[u] = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
       if (lane == 0) wcnt[u * nw + w] = (unsigned)__popcll(mask);
+    }
+    // band slots of this pass: one atomic per wavefront; the lane's entries
+    // follow the lanes below it, in slab order
+    unsigned long long bbase = 0;
+    if (__ballot(bbits != 0u)) {
+      const unsigned nb = (unsigned)__popc(bbits);
+      const unsigned incl = wave_incl_scan(nb);
+      const unsigned tot = __shfl(incl, 63, 64);
+      unsigned long long b0 = 0;
+      if (lane == 0) b0 = atomicAdd(&bst->n, (unsigned long long)tot);
+      bbase = (unsigned long long)__shfl((long long)b0, 0, 64) + (incl - nb);
     }
     __syncthreads();
     if (w == 0) {
@@ -947,13 +965,6 @@ __device__ __forceinline__ void compact_body(const CompactArgs &a, int blk, int 
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < CMP_U; ++u) {
-      // band slots of this wavefront's entries below the threshold (one
-      // atomic per wavefront and slab; none when no band is kept)
-      const bool inband = ((kept >> u) & 1u) && lb[u] < bg;
-      const unsigned long long bmask = __ballot(inband);
-      unsigned long long bbase = 0;
-      if (bmask && lane == 0) bbase = atomicAdd(&bst->n, (unsigned long long)__popcll(bmask));
-      if (bmask) bbase = (unsigned long long)__shfl((long long)bbase, 0, 64);
       if (kept & (1u << u)) {
         const long long i = base + (long long)u * bsz + threadIdx.x;
         const long long o = (long long)s_base + wcnt[u * nw + w] + pre[u];
@@ -961,9 +972,8 @@ __device__ __forceinline__ void compact_body(const CompactArgs &a, int blk, int 
         a.lb_out[o] = lb[u];
         a.par_out[o] = par;
         a.pos_out[o] = pos;
-        if (inband) {
-          const unsigned long long bo =
-              bbase + __builtin_amdgcn_mbcnt_hi((unsigned)(bmask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bmask, 0u));
+        if ((bbits >> u) & 1u) {
+          const unsigned long long bo = bbase + (unsigned)__popc(bbits & ((1u << u) - 1u));
           if (bo < (unsigned long long)a.band.cap) {
             const_cast<double *>(a.band.lb)[bo] = lb[u];
             const_cast<unsigned *>(a.band.par)[bo] = par;

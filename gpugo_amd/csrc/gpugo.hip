@@ -1006,7 +1006,9 @@ __global__ __launch_bounds__(256) void k_materialize(const double *plo, const do
 // Workgroups of the two compaction launches; each leaves its smallest key
 // in p->wmin for a one-box select.
 #define GPUGO_COMPACT_WG 512   // gpugo_compact, 256 threads
+#ifndef GPUGO_FUSED_WG
 #define GPUGO_FUSED_WG 128     // gpugo_compact_select, 1024 threads
+#endif
 #define GPUGO_LAZY_FULL_WG 512 // the same as the full compaction of a lazy (one box per step) queue:
                                // two workgroups per CU for the queues that grow large
 #define GPUGO_LAZY_WG 256      // gpugo_compact_select as the lazy tail, 1024 threads

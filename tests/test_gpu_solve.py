@@ -351,43 +351,45 @@ BAND_KEYS = ("lb", "ub", "width", "iterations", "boxes_bounded", "queue_size", "
              "pruned_kkt", "status")
 
 
-@pytest.mark.parametrize("grid,k,kw", [
-    ((3, 3), 64, {}),
-    ((2, 5), 256, {}),
-    ((3, 4), 1024, {}),            # 82,944 appends a step: the band outgrows its 16384 entries
-    ((2, 5), 256, {"mean_value": True}),
-    ((4, 5), 32, {"split": 4, "persistent": True, "mean_value": True}),
-], ids=lambda v: str(v))
-def test_band_select_matches_full_pass(gpu, grid, k, kw):
+@pytest.mark.parametrize("spec,grid,k,kw,big", [
+    (pb.chain10(), (3, 3), 64, {}, False),
+    (pb.chain10(), (2, 5), 256, {"mean_value": True}, False),
+    (pb.batch_problem(6), (3, 3), 64, {}, True),  # d = 20: 4475 steps, queue up to 1.1e5 entries
+    (pb.batch_problem(6), (3, 3), 1024, {}, True),  # 27,648 appends a step: the band outgrows its 16384 entries
+    (pb.batch_problem(6), (3, 3), 64, {"split": 4, "persistent": True}, True),
+], ids=lambda v: getattr(v, "name", str(v)))
+def test_band_select_matches_full_pass(gpu, spec, grid, k, kw, big):
     """k_select > 1: the select takes the K smallest boxes from the priority
     band the compaction keeps (entries below a threshold) while it holds at
-    least K, instead of passing over the whole queue.  Same selections, so
-    the same results after every number of steps (the plain select between
-    step calls, the fused one within them), after a reset, and the same live
-    boxes, as with the band switched off (GPUGO_FLAG_NO_BAND)."""
-    spec = pb.chain10()
+    least K, instead of passing over the whole queue; queues of at most 32768
+    entries keep no band.  Same selections, so the same results after every
+    number of steps (the plain select between step calls, the fused one
+    within them), after a reset, and the same live boxes, as with the band
+    switched off (GPUGO_FLAG_NO_BAND)."""
     obj = pb.objective(spec.family, spec.d)
     if kw.get("split"):
         kw = dict(kw, module=gpu.get_module(obj, roundings=("fast",), split=kw["split"]))
     runs = []
     for band in (True, False):
         out = []
-        with gpu.Problem(obj, spec.params, spec.lo, spec.hi, tol=1e-6, grid=grid, k_select=k, band=band,
+        with gpu.Problem(obj, spec.params, spec.lo, spec.hi, tol=spec.tol, grid=grid, k_select=k, band=band,
                          max_iters=20000, **kw) as p:
             for n in (1, 2, 5, 17):
                 out.append(p.step(n))
             lo, hi, lb, nb = p.boxes(1 << 18)
-            assert nb <= 1 << 18
-            order = np.lexsort((lb, *hi.T, *lo.T))
-            out.append((nb, lo[order], hi[order], lb[order]))
+            if nb <= 1 << 18:  # the live boxes as a set (queue order is not fixed)
+                order = np.lexsort((lb, *hi.T, *lo.T))
+                out.append((nb, lo[order], hi[order], lb[order]))
+            else:
+                out.append((nb,))
             out.append(p.solve())
             p.reset()
             out.append(p.solve())
         runs.append(out)
     a, b = runs
     assert a[-1]["status_name"] == "certified", a[-1]
-    # the queue outgrew the band's target: thresholds below the largest bound
-    assert a[5]["queue_peak"] > 2048
+    if big:  # a queue above 32768 entries: the band holds the entries below a threshold
+        assert a[5]["queue_peak"] > 32768, a[5]["queue_peak"]
     for ra, rb in zip(a[:4] + a[5:], b[:4] + b[5:]):
         for key in BAND_KEYS:
             assert ra[key] == rb[key], (key, ra[key], rb[key])
