@@ -75,7 +75,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip every CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=40.0,
                     help="seconds of host work per CPU baseline (complete solves when they fit)")
-    ap.add_argument("--legs", default="c2_forms,c2_s10,c1,c1_tuned,c0,cold,large_d,xl_d,batch,served",
+    ap.add_argument("--legs", default="c2_forms,c2_s10,c1,c1_tuned,c0,cold,large_d,xl_d,batch,served,band",
                     help="comma list besides the headline; 'none' for the headline only")
     ap.add_argument("--batch-n", type=int, default=4096)
     ap.add_argument("--seq-sample", type=int, default=256,
@@ -546,6 +546,34 @@ def leg_batch(G, pb, n, seq_sample, use_graph=True):
             "speedup_certified_per_s": (cert / el) / max(seq_cert / t_seq, 1e-12)}
 
 
+def leg_band(G, pb, k=44, grid=(3, 3), k_select=64, steps=3000):
+    """A problem of the batched workload whose queue grows past 10^6 entries
+    (sepcos_chain d = 30), solved alone for a fixed number of steps with K
+    boxes per step: the select ranks the priority band, or (band=False) the
+    whole queue.  Same steps and results either way; per-step time."""
+    s = pb.batch_problem(k)
+    obj = pb.objective(s.family, s.d)
+    m = G.get_module(obj, roundings=("fast",))
+    out = {"workload": f"batch problem {k}: {s.family} d={s.d}, box [-{s.hi[0]:.2f},{s.hi[0]:.2f}]^{s.d}, "
+                       f"tol {s.tol:g}, grid {grid[0]}^{grid[1]}, {k_select} boxes per step, {steps} steps"}
+    res = {}
+    for band in (True, False):
+        with G.Problem(obj, s.params, s.lo, s.hi, tol=s.tol, grid=grid, k_select=k_select, max_iters=steps,
+                       max_queue=1 << 26, max_parents=1 << 22, module=m, band=band) as p:
+            t0 = time.perf_counter()
+            r = p.solve()
+            el = time.perf_counter() - t0
+        res[band] = r
+        out["band" if band else "whole_queue"] = {
+            "seconds": el, "us_per_step": el / max(r["iterations"], 1) * 1e6, "status": r["status_name"],
+            "iterations": r["iterations"], "boxes_bounded": r["boxes_bounded"], "queue_peak": r["queue_peak"],
+            "width": r["width"]}
+    out["identical"] = all(res[True][x] == res[False][x]
+                           for x in ("lb", "ub", "iterations", "boxes_bounded", "queue_size", "queue_peak"))
+    out["speedup"] = out["whole_queue"]["seconds"] / out["band"]["seconds"]
+    return out
+
+
 def leg_served(pb, lams=(100.0, 1000.0), seconds=3.0):
     """Served workload: Poisson arrivals, continuous batching, p50/p99 latency."""
     from gpugo_amd.serve import Server, latency_summary
@@ -672,6 +700,9 @@ def run_legs(G, pb, args, legs, spec, out, head):
     if "served" in legs:
         print("# leg served", file=sys.stderr, flush=True)
         out["served"] = leg_served(pb)
+    if "band" in legs:
+        print("# leg band", file=sys.stderr, flush=True)
+        out["band"] = leg_band(G, pb)
 
 
 if __name__ == "__main__":
