@@ -24,7 +24,7 @@ enum gpugo_status {
 
 // layout version of the structs below; modules report it and the library
 // refuses a module built against another layout
-#define GPUGO_MODULE_ABI_V 9
+#define GPUGO_MODULE_ABI_V 10
 
 namespace gpugo {
 

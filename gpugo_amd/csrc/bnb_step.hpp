@@ -1248,14 +1248,21 @@ __device__ __forceinline__ void root_body(const RootArgs &a) {
 #endif
 // bit of the polled status word: the stop's final compaction has not run
 #define GPUGO_POLL_FINAL 0x100
+// bit of the polled status word: the problem has stopped and the pinned copy
+// of its final state (pstate) was written before the word
+#define GPUGO_POLL_STATE 0x200
 
 namespace gpugo {
 // Compaction of this step fused with the select of the next one: every
 // workgroup compacts its share; the last to finish (ticket in st->cblocks,
 // each workgroup fences its writes before taking one, the last fences
 // before reading) selects from the compacted queue.  Saves one launch and
-// its gap per step.  `poll` (or null) receives the status after the select.
-__device__ __forceinline__ void compact_select_body(const CompactArgs &ca, const SelectArgs &sa, int *poll) {
+// its gap per step.  `poll` (or null) receives the status after the select;
+// once the problem has stopped and its final compaction has run, `pstate`
+// (host-mapped pinned memory, or null) first receives the whole final state,
+// so the host needs no copy back.
+__device__ __forceinline__ void compact_select_body(const CompactArgs &ca, const SelectArgs &sa, int *poll,
+                                                    DevState *pstate) {
   __shared__ int s_last;
   if (ca.cmin) lazy_chunks_body(ca, blockIdx.x, gridDim.x);
   else compact_body(ca, blockIdx.x, gridDim.x);
@@ -1310,16 +1317,38 @@ __device__ __forceinline__ void compact_select_body(const CompactArgs &ca, const
   }
   // the host reads the status, with GPUGO_POLL_FINAL if the stop's final
   // compaction is still to run
-  if (poll && threadIdx.x == 0)
-    __hip_atomic_store(poll, ca.st->status | (ca.st->compact_final ? GPUGO_POLL_FINAL : 0), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_SYSTEM);
+  if (poll && threadIdx.x < 64) {
+    // wavefront 0: its lanes read the state words together
+    const int status = __hip_atomic_load(&ca.st->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int cfinal = __hip_atomic_load(&ca.st->compact_final, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int word = status | (cfinal ? GPUGO_POLL_FINAL : 0);
+    if (pstate && status != GPUGO_ST_RUNNING && !cfinal) {
+      // the state no longer changes: every workgroup has finished its pass.
+      // Lane i copies word i: device-scope loads (this CU's L1 may hold
+      // words that other workgroups, or this one's atomics, changed since),
+      // system-scope stores to the pinned copy.
+      constexpr int NW = (int)(sizeof(DevState) / 8);
+      static_assert(sizeof(DevState) % 8 == 0 && NW <= 64, "DevState is copied in 8-byte words by one wavefront");
+      const int i = (int)threadIdx.x;
+      if (i < NW) {
+        unsigned long long *src = reinterpret_cast<unsigned long long *>(ca.st);
+        unsigned long long *dst = reinterpret_cast<unsigned long long *>(pstate);
+        __hip_atomic_store(dst + i, __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+      word |= GPUGO_POLL_STATE;
+    }
+    // release (the wavefront's stores above land first), then the word the
+    // host waits for
+    if (threadIdx.x == 0) __hip_atomic_store(poll, word, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 }  // namespace gpugo
 
 #ifdef GPUGO_MODULE_STEP
 extern "C" __global__ __launch_bounds__(SEL_T) void gpugo_compact_select(gpugo::CompactArgs ca, gpugo::SelectArgs sa,
-                                                                         int *poll) {
-  gpugo::compact_select_body(ca, sa, poll);
+                                                                         int *poll, gpugo::DevState *pstate) {
+  gpugo::compact_select_body(ca, sa, poll, pstate);
 }
 extern "C" __global__ __launch_bounds__(SEL_T) void gpugo_select(gpugo::SelectArgs a) { gpugo::select_body(a); }
 extern "C" __global__ __launch_bounds__(256) void gpugo_compact(gpugo::CompactArgs a) {

@@ -1014,6 +1014,7 @@ __global__ __launch_bounds__(256) void k_materialize(const double *plo, const do
 #define GPUGO_LAZY_WG 256      // gpugo_compact_select as the lazy tail, 1024 threads
 #define GPUGO_WMIN_MAX GPUGO_COMPACT_WG
 #define GPUGO_SBAND_CAP 16384  // priority band of a single solve with k_select > 1 (entries)
+#define GPUGO_POLL_STATE_OFF 64  // bytes from the status words to the pinned state copies
 #define GPUGO_CHUNK_LOG2 13    // lazy queue: 8192 entries per chunk minimum
 #define GPUGO_LAZY_MAX_EVERY 256
 #define GPUGO_TAB_MAX (1 << 20) // term-table entries per parent record, at most
@@ -1071,7 +1072,9 @@ struct gpugo_problem_s {
   int last_nwg = GPUGO_COMPACT_WG;  // workgroups of the last compaction (its wmin entries)
   DevState hs;  // last state read back
   // state polling: pinned copies of DevState and an event per slot
-  int *hpoll = nullptr;   // [2] pinned, written by the compaction of a batch's last step
+  // [2] status words, then at byte GPUGO_POLL_STATE_OFF [2] copies of the
+  // final state; pinned, written by the compaction of a batch's last step
+  int *hpoll = nullptr;
   int *dpoll = nullptr;   // its device address
   hipEvent_t poll_ev[2] = {nullptr, nullptr};
   // optional timing of the split+bound launches
@@ -1325,8 +1328,8 @@ extern "C" int gpugo_problem_create(gpugo_module_t m, const gpugo_problem_desc *
   }
   // coherent (fine-grained): the compaction's system-scope store of the
   // status reaches host memory without depending on HIP_HOST_COHERENT
-  if ((e = hipHostMalloc((void **)&p->hpoll, 2 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent)) !=
-      hipSuccess)
+  if ((e = hipHostMalloc((void **)&p->hpoll, GPUGO_POLL_STATE_OFF + 2 * sizeof(DevState),
+                         hipHostMallocMapped | hipHostMallocCoherent)) != hipSuccess)
     return fail(set_err(GPUGO_E_NOMEM, "pinned poll buffer: %s", hipGetErrorString(e)));
   if ((e = hipHostGetDevicePointer((void **)&p->dpoll, p->hpoll, 0)) != hipSuccess)
     return fail(set_err(GPUGO_E_HIP, "poll buffer device address: %s", hipGetErrorString(e)));
@@ -1427,6 +1430,14 @@ static int root_init(gpugo_problem_t p) {
   return GPUGO_OK;
 }
 
+// Pinned copy k of a stopped problem's final state (host and device address).
+static DevState *poll_state_h(gpugo_problem_t p, int k) {
+  return reinterpret_cast<DevState *>(reinterpret_cast<char *>(p->hpoll) + GPUGO_POLL_STATE_OFF) + k;
+}
+static DevState *poll_state_d(gpugo_problem_t p, int k) {
+  return reinterpret_cast<DevState *>(reinterpret_cast<char *>(p->dpoll) + GPUGO_POLL_STATE_OFF) + k;
+}
+
 // The problem's priority band (st null when it keeps none).
 static BandView band_view(gpugo_problem_t p) {
   BandView v{};
@@ -1492,6 +1503,8 @@ static SelectArgs select_args(gpugo_problem_t p, const Queue &q, int nwg) {
 // (the select, rest sums and split+bound of a stopped problem do nothing).
 static int enqueue_step(gpugo_problem_t p, int *poll, bool more, bool compact_only = false) {
   const int d = p->desc.d;
+  // where a fused compaction + select that stops the problem copies its state
+  DevState *pstate = poll ? poll_state_d(p, (int)(poll - p->dpoll)) : nullptr;
   // every compaction moves the queue to the other buffer, a lazy tail does not
   const int cur = (int)((p->steps_enqueued - p->lazy_steps) & 1);
   Queue &A = p->q[cur], &B = p->q[cur ^ 1];
@@ -1624,7 +1637,7 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more, bool compact_on
     ca.cmin = p->cmin.as<unsigned long long>();
     SelectArgs sa = select_args(p, A, nwg);
     ca.poll = nullptr;
-    void *fargs[] = {&ca, &sa, &poll};
+    void *fargs[] = {&ca, &sa, &poll, &pstate};
     HIPCHECK(hipModuleLaunchKernel(p->m->compact_select_fn, nwg, 1, 1, SEL_T, 1, 1, 0, p->stream, fargs,
                                    nullptr));
     p->selected_ahead = true;
@@ -1639,7 +1652,7 @@ static int enqueue_step(gpugo_problem_t p, int *poll, bool more, bool compact_on
     const int nwg = grid_for(8LL * SEL_T, p->lazy ? GPUGO_LAZY_FULL_WG : GPUGO_FUSED_WG);
     SelectArgs sa = select_args(p, B, nwg);
     ca.poll = nullptr;
-    void *fargs[] = {&ca, &sa, &poll};
+    void *fargs[] = {&ca, &sa, &poll, &pstate};
     HIPCHECK(hipModuleLaunchKernel(p->m->compact_select_fn, nwg, 1, 1, SEL_T, 1, 1, 0, p->stream, fargs, nullptr));
     p->selected_ahead = true;
   } else {
@@ -1843,6 +1856,7 @@ static int run_steps(gpugo_problem_t p, int64_t iters, gpugo_result *res, bool s
     p->started = true;
   }
   const long long it0 = p->hs.iterations;
+  int final_copy = -1;  // slot whose pinned state copy holds the stopped problem's final state
   if (p->hs.status == GPUGO_ST_RUNNING && (p->desc.flags & GPUGO_FLAG_PERSIST) && iters > 0) {
     int rc = persist_steps(p, iters);
     if (rc) return rc;
@@ -1899,6 +1913,16 @@ static int run_steps(gpugo_problem_t p, int64_t iters, gpugo_result *res, bool s
           if (pending == 0 && (pv & GPUGO_POLL_FINAL)) {
             int rc = enqueue_step(p, nullptr, false, true);
             if (rc) return rc;
+          } else if (pending == 0 && (pv & GPUGO_POLL_STATE)) {
+            final_copy = old;
+          } else if (pending == 1) {
+            // the newer batch (steps that find the problem stopped, or the
+            // one that runs its final compaction) ends with a word too
+            int pv2;
+            int rc = wait_poll(p, slot, &pv2);
+            if (rc) return rc;
+            --pending;
+            if (pv2 & GPUGO_POLL_STATE) final_copy = slot;
           }
           break;
         }
@@ -1906,8 +1930,16 @@ static int run_steps(gpugo_problem_t p, int64_t iters, gpugo_result *res, bool s
       slot ^= 1;
     }
   }
-  int rc = read_state(p);
-  if (rc) return rc;
+  int rc;
+  if (final_copy >= 0) {
+    // the device wrote the final state before the status word; nothing is
+    // left in flight but the end of that launch
+    memcpy(&p->hs, poll_state_h(p, final_copy), sizeof(DevState));
+    HIPCHECK(hipStreamSynchronize(p->stream));
+  } else {
+    rc = read_state(p);
+    if (rc) return rc;
+  }
   if (p->timing) {
     // working launches come first in the recorded order
     long long nwork = p->hs.iterations - it0;

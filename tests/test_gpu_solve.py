@@ -396,3 +396,27 @@ def test_band_select_matches_full_pass(gpu, spec, grid, k, kw, big):
     assert a[4][0] == b[4][0]
     for x, y in zip(a[4][1:], b[4][1:]):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("spec,grid,k", [
+    (pb.sepcos2(), (64, 2), 1),       # one box per step: lazy queue, stop in a lazy tail
+    (pb.sepcos2(), (2, 2), 64),
+    (pb.rosenbrock(4), (2, 4), 64),   # a few hundred steps: the stop is seen with a later batch in flight
+    (pb.ackley(4), (3, 4), 16),
+    (pb.chain10(), (2, 5), 256),
+], ids=lambda v: getattr(v, "name", str(v)))
+def test_pinned_final_state_equals_readback(gpu, spec, grid, k):
+    """A solve that stops in a fused compaction + select gets its final state
+    from the pinned copy the device writes before the status word; the
+    static-arena solve (no pinned words) reads it back with a copy.  Every
+    result field is the same."""
+    from tests.test_zz_no_alloc import KEYS as SKEYS, desc_for, solve_created, solve_static
+    keep = []
+    m = gpu.get_module(pb.objective(spec.family, spec.d), roundings=("fast",))
+    desc = desc_for(spec, grid, k, 1 << 20, 1 << 14, keep)
+    rc, r = solve_static(m, desc)
+    assert rc == 0, rc
+    c = solve_created(m, desc)
+    assert c["status_name"] == "certified", c
+    for key in SKEYS:
+        assert r[key] == c[key], (key, r[key], c[key])
