@@ -408,6 +408,10 @@ __device__ long long collect_parents(const SelectArgs &a, long long N, long long
 
 __device__ void select_body(const SelectArgs &a, const BandView *band = nullptr) {
   if (!band && a.band.st) band = &a.band;
+  // the band's state, loaded with the problem's: the select is one
+  // workgroup, and a load issued later would add a round trip to every step
+  const long long band_n = band ? (long long)band->st->n : -1;
+  const double band_g = band ? band->st->g : -__builtin_huge_val();
   __shared__ unsigned hist[SEL_BINS];
   __shared__ u128 skey[SEL_KMAX];
   __shared__ unsigned sidx[SEL_KMAX];
@@ -529,8 +533,8 @@ __device__ void select_body(const SelectArgs &a, const BandView *band = nullptr)
   } else {
     // Source of the selection: the slot's band when it is complete and holds
     // at least K entries, else the whole queue.
-    const long long bn = band ? (long long)band->st->n : -1;
-    const double bg = band ? band->st->g : -__builtin_huge_val();
+    const long long bn = band_n;
+    const double bg = band_g;
     const bool use_band = bn >= (long long)K && bn <= band->cap && bg > -__builtin_huge_val();
     SelectArgs sv = a;
     long long Nsel = N;
